@@ -1,0 +1,160 @@
+"""Throughput of the sparse statevector engine on one MI355X.
+
+Workload (one step): the 48-qubit state of 2^20 distinct uniform keys with
+normalised complex-normal amplitudes, taken through the seeded 2000-gate circuit
+(60% diagonal RZ/PHASE/CZ, 35% permutation X/CX/CCX, 5% branching H/RY on 4
+fixed qubits; nonzeros grow from 2^20 toward 2^24), FP64 amplitudes, prune
+threshold 1e-14.  The state is resident in HBM before timing; each step restores
+the device-side snapshot of the initial state and applies the circuit.
+
+    python bench.py [--steps K] [--warmup W] [--prec f64|f32] [--fuse-window N]
+
+Prints one JSON line: value = nonzero-amplitude gate updates per second
+(sum over gates of the nonzeros entering each gate / wall time), with the
+roofline of the dominant kernel (the fused local sweep, timed with HIP events on
+the engine's stream) and the CPU oracle's rate on a bounded sample of the same
+workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E bytes/s
+KEY_B = 8
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--prec", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--fuse-window", type=int, default=0)
+    ap.add_argument("--bucket-bits", type=int, default=-1)
+    ap.add_argument("--n-init-log2", type=int, default=20)
+    ap.add_argument("--gates", type=int, default=2000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(w, seconds):
+    """Oracle (FP64 sparse map, OpenMP) on the first gates of the same workload."""
+    from oracle import oracle as O
+    try:
+        cores = min(16, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        cores = min(16, os.cpu_count() or 1)
+    O.set_threads(cores)
+    s = O.Sparse(w["nq"])
+    s.load(w["keys"], w["re"], w["im"])
+    gl = w["gates"]
+    t0 = time.perf_counter()
+    done = 0
+    while done < len(gl) and time.perf_counter() - t0 < seconds:
+        s.apply(gl[done:done + 25])
+        done = min(len(gl), done + 25)
+    dt = time.perf_counter() - t0
+    return {"value": s.updates() / dt, "unit": "updates/s", "cores": cores, "kind": "port",
+            "sample": "CPU oracle (FP64 sparse map) on the first %d of %d gates of the same "
+                      "workload from the same 2^%d-key state, %.1f s" % (
+                          done, len(gl), int(round(__import__("math").log2(w["keys"].size))), dt)}
+
+
+def traffic_from_profiles(prec, window):
+    """HBM bytes per launch measured by rocprofv3 counters, when committed."""
+    p = os.path.join(ROOT, "profiles", "sweep_traffic.json")
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return d.get("%s_w%d" % (prec, window))
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from sparqc_amd import engine as E
+    from sparqc_amd import gen
+
+    w = gen.circuit_large(a.seed, nq=48, n_init=1 << a.n_init_log2, n_gates=a.gates)
+    eng = E.Engine(48, prec=a.prec, timing=True, fuse_window=a.fuse_window, bucket_bits=a.bucket_bits,
+                   renorm_every=100 if a.prec == "f32" else 0)
+    eng.load(w["keys"], w["re"], w["im"])
+    eng.prepare(w["gates"])
+    eng.snapshot_save()
+    nnz0 = eng.nnz()
+    for _ in range(a.warmup):
+        eng.snapshot_restore()
+        eng.apply(w["gates"])
+        eng.sync()
+    s0 = eng.stats()
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        eng.snapshot_restore()
+        eng.apply(w["gates"])
+        eng.sync()
+    t1 = time.perf_counter()
+    s1 = eng.stats()
+    nnz1 = eng.nnz()
+    dt = t1 - t0
+    upd = s1["updates"] - s0["updates"]
+    amp_b = 16 if a.prec == "f64" else 8
+    sweeps = s1["sweeps"] - s0["sweeps"]
+    sweep_s = (s1["sweep_ms"] - s0["sweep_ms"]) / 1e3
+    moved = s1["moved"] - s0["moved"]
+    alg_bytes = moved * (KEY_B + amp_b)
+    per_launch_bytes = alg_bytes / max(sweeps, 1)
+    per_launch_s = sweep_s / max(sweeps, 1)
+    achieved = per_launch_bytes / per_launch_s if per_launch_s > 0 else 0.0
+    traffic = traffic_from_profiles(a.prec, a.fuse_window)
+    out = {
+        "metric": "nonzero-amp gate updates/s",
+        "value": upd / dt,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": a.prec,
+        "data": "synthetic (seeded splitmix64 circuit and state)",
+        "config": {"workload": "sparse statevector, 48 qubits, 2^%d initial keys, %d-gate mix "
+                               "60%% diag / 35%% perm / 5%% branch on 4 qubits, prune 1e-14"
+                               % (a.n_init_log2, a.gates),
+                   "nq": 48, "n_init": 1 << a.n_init_log2, "gates": a.gates, "seed": a.seed,
+                   "prec": a.prec, "fuse_window": a.fuse_window},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "kernel": "k_sweep (fused local sweep)",
+                     "algorithmic_bytes_per_launch": per_launch_bytes,
+                     "avg_launch_ms": per_launch_s * 1e3, "launches": sweeps},
+        "detail": {"updates_per_step": upd / a.steps, "nnz_initial": nnz0, "nnz_final": nnz1,
+                   "buckets": s1["buckets"], "bucket_bits": s1["bucket_bits"],
+                   "cross_bucket_launches": s1["pairs"] - s0["pairs"],
+                   "sweep_kernel_ms_per_step": sweep_s / a.steps * 1e3,
+                   "device_bytes": s1["bytes_device"]},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,174 @@
+/* sqe.h -- C ABI of the MI355X sparse statevector engine (libsqe.so).
+ *
+ * Drop-in replacement for the GPU path behind the reference's circuit API
+ * (ivanNovikau/SparQC, emulator/).  Each entry point names the reference
+ * interface it replaces; a C++ class with the reference's method names sits on
+ * top of this ABI in include/sparqc_circuit.hpp.
+ *
+ * Conventions
+ *   - Qubit t is bit t of a uint64 basis key, qubit 0 the least significant bit;
+ *     |0...0> is key 0 (emulator/device/device_table_FC.cuh:31-36,42-43).
+ *   - Amplitudes are exchanged as separate FP64 real / imaginary arrays.
+ *   - The engine owns all device memory; the caller owns host arrays, which
+ *     sqe_load / sqe_export copy.
+ *   - Calls enqueue work on the engine's own HIP stream and return; sqe_nnz,
+ *     sqe_norm2, sqe_export and sqe_sync wait for it.  One engine is used by
+ *     one host thread at a time.
+ *   - Errors are status codes; no C++ exception crosses the ABI.
+ *     sqe_last_error() returns the calling thread's last message.
+ */
+#ifndef SQE_H
+#define SQE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sqe_engine sqe_engine;
+
+typedef enum {
+    SQE_OK = 0,
+    SQE_EINVAL = -1,       /* bad argument (qubit index, duplicate key, ...)      */
+    SQE_ENOMEM = -2,       /* device or host allocation failed                    */
+    SQE_ETABLE_FULL = -3,  /* the state would exceed the configured HBM budget    */
+    SQE_EHIP = -4,         /* HIP runtime error or kernel-reported fault          */
+    SQE_EDEVICE = -5       /* no usable GPU                                       */
+} sqe_status;
+
+typedef enum { SQE_F64 = 0, SQE_F32 = 1 } sqe_prec;
+
+/* gate classes: diagonal (keys unchanged, never pruned), permutation (keys
+ * flipped, never pruned), branching (pairs mixed, outputs pruned) */
+typedef enum { SQE_DIAG = 0, SQE_PERM = 1, SQE_BRANCH = 2 } sqe_class;
+
+/* gate kinds and their coefficient conventions (p[]):
+ *   X, Y, Z       none
+ *   H             p[0] = 1/sqrt(2)
+ *   PHASE         p[0..1] = cos(theta), sin(theta)           diag(1, e^{i theta})
+ *   RZ, RX, RY    p[0..1] = cos(theta/2), sin(theta/2)
+ *   U             p[0..7] = u00, u01, u10, u11 (re, im)       any 2x2, branching
+ *   DIAG          p[0..3] = d0, d1 (re, im)                   diag(d0, d1)
+ *   PERM          p[0..3] = u01, u10 (re, im)                 [[0,u01],[u10,0]]
+ * (gate matrices: emulator/include/QLib.h:57-157; angle halving:
+ *  emulator/device/table_manager.cuh:333-336) */
+typedef enum {
+    SQE_GATE_X = 0, SQE_GATE_Y = 1, SQE_GATE_Z = 2, SQE_GATE_H = 3, SQE_GATE_PHASE = 4,
+    SQE_GATE_RZ = 5, SQE_GATE_RX = 6, SQE_GATE_RY = 7, SQE_GATE_U = 8, SQE_GATE_DIAG = 9,
+    SQE_GATE_PERM = 10
+} sqe_gate_kind;
+
+/* One gate (88 bytes).  It acts on keys k with (k & ctrl_mask) == ctrl_val;
+ * ctrl_mask must not contain the target and ctrl_val must be a subset of it. */
+typedef struct sqe_gate {
+    uint32_t kind;
+    uint32_t target;
+    uint64_t ctrl_mask;
+    uint64_t ctrl_val;
+    double p[8];
+} sqe_gate;
+
+typedef struct sqe_config {
+    uint32_t nq;                /* qubits, 1..63                                       */
+    uint32_t prec;              /* sqe_prec                                            */
+    double prune_thr;           /* 0: 1e-14 (F64), 1e-7 (F32)                          */
+    double max_load;            /* hash index load factor, 0: 0.5                      */
+    uint64_t hbm_budget_bytes;  /* 0: 90% of the device's free memory                  */
+    int32_t device;             /* HIP device ordinal                                  */
+    int32_t bucket_bits;        /* -1: planner chooses; else 0..4 (F64) / 0..5 (F32)   */
+    uint32_t renorm_every;      /* renormalise in FP64 every N gates, 0: never         */
+    uint32_t fuse_window;       /* max gates per fused sweep, 0: unlimited             */
+    uint32_t timing;            /* 1: time every kernel launch with HIP events         */
+    uint32_t reserved[7];
+} sqe_config;
+
+typedef struct sqe_stats {
+    uint64_t gates;             /* gates applied                                       */
+    uint64_t updates;           /* sum over gates of nonzeros entering the gate        */
+    uint64_t nnz;               /* nonzero amplitudes (after the last sync)            */
+    uint64_t buckets;           /* live buckets                                        */
+    uint64_t capacity;          /* bucket capacity of one store                        */
+    uint64_t bytes_device;      /* device bytes held by the engine                     */
+    int32_t bucket_bits;        /* slots per bucket = 2^bucket_bits                    */
+    uint32_t pad;
+    uint64_t bucket_mask;       /* bucket qubits                                       */
+    uint64_t sweeps;            /* fused local sweep launches                          */
+    uint64_t sweep_ops;         /* gates executed inside sweeps                        */
+    uint64_t pairs;             /* cross-bucket gate launches                          */
+    uint64_t remaps;            /* re-bucketing passes                                 */
+    uint64_t moved;             /* sum over sweep / cross-bucket launches of nonzeros
+                                   read plus nonzeros written (algorithmic traffic)    */
+    double sweep_ms;            /* summed sweep kernel time (timing = 1)               */
+    double pair_ms;             /* summed cross-bucket kernel time (timing = 1)        */
+    double other_ms;            /* summed time of all other kernels (timing = 1)       */
+} sqe_stats;
+
+/* Fill *cfg with defaults for nq qubits in FP64. */
+void sqe_config_default(sqe_config* cfg, uint32_t nq);
+
+/* Replaces TableManager__::TableManager__ / device_analysis / reserve_on_device
+ * (emulator/device/table_manager.cuh:84-209,227-288; emulator/device/table.cuh:30-45)
+ * and the Mapping__ selection (emulator/device/table_manager.cuh:150-176). */
+sqe_status sqe_create(const sqe_config* cfg, sqe_engine** out);
+
+/* Replaces ~TableManager__ (emulator/device/table_manager.cuh:212-224). */
+sqe_status sqe_destroy(sqe_engine* e);
+
+/* Basis state |key> with amplitude 1.  Replaces Mapping__::init +
+ * FC__::set_to_zero_quantum_state (emulator/device/mapping_GHT.cuh:28-32,
+ * emulator/device/device_table_FC.cuh:39-49); key 0 gives |0...0>. */
+sqe_status sqe_set_basis(sqe_engine* e, uint64_t key);
+
+/* Bulk insert of n distinct keys with amplitudes (zero amplitudes are
+ * skipped).  No reference counterpart (the reference can only start at
+ * |0...0>); it is the insert/rebuild entry of the hash table. */
+sqe_status sqe_load(sqe_engine* e, const uint64_t* keys, const double* re, const double* im, uint64_t n);
+
+/* Apply a 2x2 gate u (row-major, re/im interleaved: u00, u01, u10, u11) of the
+ * given class on `target` under controls.  Replaces TableManager__::gate_sq /
+ * gate_sq_par and the Mapping__ virtual gate methods
+ * (emulator/device/table_manager.cuh:311-340; emulator/device/mapping_GHT.cuh:38-48). */
+sqe_status sqe_apply(sqe_engine* e, uint32_t target, uint64_t ctrl_mask, uint64_t ctrl_val, const double u[8],
+                     sqe_class cls);
+
+/* Apply n gate records in order (same replacement as sqe_apply). */
+sqe_status sqe_apply_gates(sqe_engine* e, const sqe_gate* gates, uint64_t n);
+
+/* Number of nonzero amplitudes (the reference's N_, emulator/device/table.cuh:19). */
+sqe_status sqe_nnz(sqe_engine* e, uint64_t* n);
+
+/* Sum of |a|^2 in FP64 (no reference counterpart). */
+sqe_status sqe_norm2(sqe_engine* e, double* out);
+
+/* Copy the non-negligible entries (|re| >= thr or |im| >= thr) to host arrays
+ * of capacity cap, in the engine's bucket order.  *n receives the entry count;
+ * when it exceeds cap nothing is copied and SQE_EINVAL is returned.  Replaces
+ * Table__::get_statevector (emulator/device/table.cuh:47-75). */
+sqe_status sqe_export(sqe_engine* e, uint64_t* keys, double* re, double* im, uint64_t cap, uint64_t* n);
+
+/* Wait for all enqueued work and surface device-side errors. */
+sqe_status sqe_sync(sqe_engine* e);
+
+/* Counters and timings (waits for enqueued work). */
+sqe_status sqe_get_stats(sqe_engine* e, sqe_stats* out);
+
+/* Move the state into the bucket layout the planner picks for this gate list
+ * (without applying it): a staged state is inserted into the table, a resident
+ * one re-bucketed.  Optional; the first flush of queued gates does the same. */
+sqe_status sqe_prepare(sqe_engine* e, const sqe_gate* gates, uint64_t n);
+
+/* Keep a device-resident copy of the current state / return to it.  No
+ * reference counterpart (its _RESTART.hdf5 naming, emulator/include/QLib.h:39-40,
+ * was never wired up). */
+sqe_status sqe_snapshot_save(sqe_engine* e);
+sqe_status sqe_snapshot_restore(sqe_engine* e);
+
+/* Last error message of the calling thread ("" when none). */
+const char* sqe_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SQE_H */

@@ -1,0 +1,439 @@
+// Device-side building blocks of the sparse statevector engine (gfx950).
+//
+// Data model.  The state is a set of buckets.  A bucket holds the amplitudes of
+// the 2^MB basis keys that share every key bit outside the bucket-qubit set M
+// (|M| = MB <= 5); slot s of a bucket is the key whose M-bits, read in
+// increasing qubit order, spell s.  A key absent from the state has amplitude 0.
+// Buckets are located by an open-addressing hash index (base key -> bucket id).
+// With MB = 0 a bucket is a single key and the table is a plain key -> amplitude
+// map, the reference's model (emulator/device/table.cuh:13-26).
+//
+// Layout in HBM (structure of arrays, tiles of 64 buckets = one wavefront):
+//   base[b]                                  uint64 base key, ~0 = padding
+//   amp[((b >> 6) * S + s) * 64 + (b & 63)]  complex (double2 / float2)
+// so that thread-per-bucket sweeps issue fully coalesced 64-lane loads for
+// every slot.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <utility>
+
+namespace sqe {
+
+constexpr uint64_t EMPTY_KEY = ~0ull;
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr int TILE = 64;          // buckets per tile (one wavefront)
+constexpr int BLOCK = 256;        // threads per workgroup for bucket kernels
+
+// gate kinds (sqe_gate.kind); K_RENORM is an engine-internal op
+enum : uint32_t { KX = 0, KY, KZ, KH, KPHASE, KRZ, KRX, KRY, KU, KDIAG, KPERM, K_RENORM = 16 };
+
+__host__ __device__ constexpr bool is_branch(uint32_t k) {
+    return k == KH || k == KRX || k == KRY || k == KU;
+}
+__host__ __device__ constexpr bool is_perm(uint32_t k) { return k == KX || k == KY || k == KPERM; }
+
+// One gate after planning against the bucket-qubit set.
+struct DOp {
+    uint32_t kind;
+    int32_t tslot;      // slot bit of the target when it is a bucket qubit, else -1
+    uint64_t tbase;     // 1 << target when the target is not a bucket qubit, else 0
+    uint64_t cmB, cvB;  // controls on base bits: (base & cmB) == cvB
+    uint32_t cmS, cvS;  // controls on slot bits: (s & cmS) == cvS
+    double p[8];        // FP64 coefficients (host computed)
+    float pf[8];        // the same rounded to FP32
+};
+
+struct Slot {           // hash index slot: 16 B, 8 slots = one 128 B line
+    uint64_t key;
+    uint64_t val;
+};
+
+struct QMap {           // bucket qubits, increasing
+    uint64_t mask;
+    int32_t mb;
+    uint8_t q[8];
+};
+
+// Per-launch statistics with a self-resetting last-block reduction.
+struct Stats {
+    unsigned long long updates;      // sum over gates of nonzeros entering the gate (monotonic)
+    unsigned long long moved;        // sum over launches of nonzeros read + nonzeros written (monotonic)
+    unsigned long long acc_nnz;      // accumulators (reset by the last block)
+    unsigned long long acc_empty;
+    unsigned int done;
+    unsigned int pad;
+    unsigned long long nnz;          // nonzero slots after the last launch
+    unsigned long long empty;        // all-zero live buckets after the last launch
+    unsigned long long nb_out;       // buckets written by the last compacting launch
+    unsigned long long err;          // sticky error bits
+};
+
+enum : unsigned long long { ERR_PROBE = 1, ERR_DUP = 2, ERR_MISSING = 4, ERR_CAP = 8, ERR_SCAN = 16 };
+
+template <class R> struct CT;
+template <> struct CT<double> { using T = double2; };
+template <> struct CT<float> { using T = float2; };
+
+__device__ __forceinline__ uint64_t hmix(uint64_t k) {
+    k ^= k >> 33; k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33; return k;
+}
+
+__device__ __forceinline__ uint32_t pext_q(uint64_t key, const QMap& qm) {
+    uint32_t s = 0;
+    for (int j = 0; j < qm.mb; ++j) s |= (uint32_t)((key >> qm.q[j]) & 1ull) << j;
+    return s;
+}
+__device__ __forceinline__ uint64_t pdep_q(uint32_t s, const QMap& qm) {
+    uint64_t k = 0;
+    for (int j = 0; j < qm.mb; ++j) k |= (uint64_t)((s >> j) & 1u) << qm.q[j];
+    return k;
+}
+
+__device__ __forceinline__ size_t amp_index(uint64_t b, int s, int S) {
+    return ((size_t)(b >> 6) * (size_t)S + (size_t)s) * TILE + (size_t)(b & 63);
+}
+
+template <int N, class F, int... I>
+__device__ __forceinline__ void static_for_impl(F& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl<N>(f, std::make_integer_sequence<int, N>{});
+}
+
+// ---------------------------------------------------------------------------
+// Gate arithmetic.  FP64 follows the reference's per-gate formulas and their
+// operation order (emulator/device/device_data.cuh:159-319,
+// emulator/include/data_t.h:63-80); the build uses -ffp-contract=off so no
+// multiply-add is fused.  FP32 uses explicit fused multiply-adds.
+// v0 / v1 are the amplitudes of the keys with target bit 0 / 1.
+// ---------------------------------------------------------------------------
+template <uint32_t K>
+__device__ __forceinline__ void pair64(const double* p, double2 v0, double2 v1, double2& n0, double2& n1) {
+    if constexpr (K == KX) { n0 = v1; n1 = v0; }
+    else if constexpr (K == KY) { n0 = make_double2(v1.y, -v1.x); n1 = make_double2(-v0.y, v0.x); }
+    else if constexpr (K == KZ) { n0 = v0; n1 = make_double2(-v1.x, -v1.y); }
+    else if constexpr (K == KH) {
+        const double f = p[0];
+        n0 = make_double2(f * (v0.x + v1.x), f * (v0.y + v1.y));
+        n1 = make_double2(f * (v0.x - v1.x), f * (v0.y - v1.y));
+    } else if constexpr (K == KPHASE) {
+        const double c = p[0], s = p[1];
+        n0 = v0;
+        n1 = make_double2(c * v1.x - s * v1.y, c * v1.y + s * v1.x);
+    } else if constexpr (K == KRZ) {
+        const double c = p[0], s = p[1];
+        n0 = make_double2(c * v0.x + s * v0.y, c * v0.y - s * v0.x);
+        n1 = make_double2(c * v1.x - s * v1.y, c * v1.y + s * v1.x);
+    } else if constexpr (K == KRX) {
+        const double c = p[0], s = p[1];
+        n0 = make_double2(c * v0.x + s * v1.y, c * v0.y - s * v1.x);
+        n1 = make_double2(c * v1.x + s * v0.y, c * v1.y - s * v0.x);
+    } else if constexpr (K == KRY) {
+        const double c = p[0], s = p[1];
+        n0 = make_double2(c * v0.x - s * v1.x, c * v0.y - s * v1.y);
+        n1 = make_double2(c * v1.x + s * v0.x, c * v1.y + s * v0.y);
+    } else if constexpr (K == KU) {
+        n0 = make_double2((p[0] * v0.x - p[1] * v0.y) + (p[2] * v1.x - p[3] * v1.y),
+                          (p[0] * v0.y + p[1] * v0.x) + (p[2] * v1.y + p[3] * v1.x));
+        n1 = make_double2((p[4] * v0.x - p[5] * v0.y) + (p[6] * v1.x - p[7] * v1.y),
+                          (p[4] * v0.y + p[5] * v0.x) + (p[6] * v1.y + p[7] * v1.x));
+    } else if constexpr (K == KDIAG) {
+        n0 = make_double2(p[0] * v0.x - p[1] * v0.y, p[0] * v0.y + p[1] * v0.x);
+        n1 = make_double2(p[2] * v1.x - p[3] * v1.y, p[2] * v1.y + p[3] * v1.x);
+    } else if constexpr (K == KPERM) {
+        n0 = make_double2(p[0] * v1.x - p[1] * v1.y, p[0] * v1.y + p[1] * v1.x);
+        n1 = make_double2(p[2] * v0.x - p[3] * v0.y, p[2] * v0.y + p[3] * v0.x);
+    } else { n0 = v0; n1 = v1; }
+}
+
+template <uint32_t K>
+__device__ __forceinline__ void pair32(const float* p, float2 v0, float2 v1, float2& n0, float2& n1) {
+    if constexpr (K == KX) { n0 = v1; n1 = v0; }
+    else if constexpr (K == KY) { n0 = make_float2(v1.y, -v1.x); n1 = make_float2(-v0.y, v0.x); }
+    else if constexpr (K == KZ) { n0 = v0; n1 = make_float2(-v1.x, -v1.y); }
+    else if constexpr (K == KH) {
+        const float f = p[0];
+        n0 = make_float2(f * (v0.x + v1.x), f * (v0.y + v1.y));
+        n1 = make_float2(f * (v0.x - v1.x), f * (v0.y - v1.y));
+    } else if constexpr (K == KPHASE) {
+        const float c = p[0], s = p[1];
+        n0 = v0;
+        n1 = make_float2(__fmaf_rn(c, v1.x, -(s * v1.y)), __fmaf_rn(c, v1.y, s * v1.x));
+    } else if constexpr (K == KRZ) {
+        const float c = p[0], s = p[1];
+        n0 = make_float2(__fmaf_rn(c, v0.x, s * v0.y), __fmaf_rn(c, v0.y, -(s * v0.x)));
+        n1 = make_float2(__fmaf_rn(c, v1.x, -(s * v1.y)), __fmaf_rn(c, v1.y, s * v1.x));
+    } else if constexpr (K == KRX) {
+        const float c = p[0], s = p[1];
+        n0 = make_float2(__fmaf_rn(c, v0.x, s * v1.y), __fmaf_rn(c, v0.y, -(s * v1.x)));
+        n1 = make_float2(__fmaf_rn(c, v1.x, s * v0.y), __fmaf_rn(c, v1.y, -(s * v0.x)));
+    } else if constexpr (K == KRY) {
+        const float c = p[0], s = p[1];
+        n0 = make_float2(__fmaf_rn(c, v0.x, -(s * v1.x)), __fmaf_rn(c, v0.y, -(s * v1.y)));
+        n1 = make_float2(__fmaf_rn(c, v1.x, s * v0.x), __fmaf_rn(c, v1.y, s * v0.y));
+    } else if constexpr (K == KU) {
+        float t;
+        t = p[0] * v0.x; t = __fmaf_rn(-p[1], v0.y, t); t = __fmaf_rn(p[2], v1.x, t); n0.x = __fmaf_rn(-p[3], v1.y, t);
+        t = p[0] * v0.y; t = __fmaf_rn(p[1], v0.x, t); t = __fmaf_rn(p[2], v1.y, t); n0.y = __fmaf_rn(p[3], v1.x, t);
+        t = p[4] * v0.x; t = __fmaf_rn(-p[5], v0.y, t); t = __fmaf_rn(p[6], v1.x, t); n1.x = __fmaf_rn(-p[7], v1.y, t);
+        t = p[4] * v0.y; t = __fmaf_rn(p[5], v0.x, t); t = __fmaf_rn(p[6], v1.y, t); n1.y = __fmaf_rn(p[7], v1.x, t);
+    } else if constexpr (K == KDIAG) {
+        n0 = make_float2(__fmaf_rn(p[0], v0.x, -(p[1] * v0.y)), __fmaf_rn(p[0], v0.y, p[1] * v0.x));
+        n1 = make_float2(__fmaf_rn(p[2], v1.x, -(p[3] * v1.y)), __fmaf_rn(p[2], v1.y, p[3] * v1.x));
+    } else if constexpr (K == KPERM) {
+        n0 = make_float2(__fmaf_rn(p[0], v1.x, -(p[1] * v1.y)), __fmaf_rn(p[0], v1.y, p[1] * v1.x));
+        n1 = make_float2(__fmaf_rn(p[2], v0.x, -(p[3] * v0.y)), __fmaf_rn(p[2], v0.y, p[3] * v0.x));
+    } else { n0 = v0; n1 = v1; }
+}
+
+__device__ __forceinline__ bool negl64(double2 v, double thr) {
+    return fabs(v.x) < thr && fabs(v.y) < thr;
+}
+__device__ __forceinline__ bool nonzero(double2 v) { return v.x != 0.0 || v.y != 0.0; }
+__device__ __forceinline__ bool nonzero(float2 v) { return v.x != 0.0f || v.y != 0.0f; }
+
+// Apply a branching pair and prune its outputs (componentwise |re|,|im| < thr,
+// emulator/include/data_t.h:50-60).  FP32: decisions for outputs with
+// max(|re|,|im|) in [thr/4, 4 thr) are redone in FP64 from the FP32 inputs.
+template <uint32_t K>
+__device__ __forceinline__ void branch_pair(const DOp& op, double thr, double2 v0, double2 v1, double2& n0, double2& n1) {
+    pair64<K>(op.p, v0, v1, n0, n1);
+    if (negl64(n0, thr)) n0 = make_double2(0.0, 0.0);
+    if (negl64(n1, thr)) n1 = make_double2(0.0, 0.0);
+}
+template <uint32_t K>
+__device__ __forceinline__ void branch_pair(const DOp& op, double thr, float2 v0, float2 v1, float2& n0, float2& n1) {
+    pair32<K>(op.pf, v0, v1, n0, n1);
+    const double m0 = fmax(fabs((double)n0.x), fabs((double)n0.y));
+    const double m1 = fmax(fabs((double)n1.x), fabs((double)n1.y));
+    bool k0 = m0 >= 4.0 * thr, k1 = m1 >= 4.0 * thr;
+    const bool near0 = !k0 && m0 >= 0.25 * thr, near1 = !k1 && m1 >= 0.25 * thr;
+    if (near0 || near1) {
+        double2 w0 = make_double2(v0.x, v0.y), w1 = make_double2(v1.x, v1.y), d0, d1;
+        pair64<K>(op.p, w0, w1, d0, d1);
+        if (near0) k0 = !negl64(d0, thr);
+        if (near1) k1 = !negl64(d1, thr);
+    }
+    if (!k0) n0 = make_float2(0.0f, 0.0f);
+    if (!k1) n1 = make_float2(0.0f, 0.0f);
+}
+
+template <uint32_t K, class C>
+__device__ __forceinline__ void any_pair(const DOp& op, double thr, C v0, C v1, C& n0, C& n1) {
+    if constexpr (is_branch(K)) branch_pair<K>(op, thr, v0, v1, n0, n1);
+    else if constexpr (sizeof(C) == 16) pair64<K>(op.p, v0, v1, n0, n1);
+    else pair32<K>(op.pf, v0, v1, n0, n1);
+}
+
+template <class C>
+__device__ __forceinline__ C csel(bool c, C a, C b) { return c ? a : b; }
+
+// ---------------------------------------------------------------------------
+// Wavefront-cooperative hash probing: 8 lanes probe one key over an aligned
+// window of 8 slots (one 128-byte line) and decide with a 64-lane ballot; the
+// 8 groups of a wavefront serve their 8 lanes' keys in turn.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t group_bits(unsigned long long ball, int g) {
+    return (uint32_t)((ball >> (g * 8)) & 0xffull);
+}
+
+// Returns the value stored for qkey (NONE when absent).  All 64 lanes of the
+// wavefront must call it; `active` marks lanes with a query.
+__device__ __forceinline__ uint64_t coop_find(const Slot* __restrict__ tab, uint64_t mask, uint64_t qkey,
+                                              bool active, unsigned long long* err) {
+    const int lane = threadIdx.x & 63, g = lane >> 3, gl = lane & 7;
+    uint64_t result = NONE;
+    for (int r = 0; r < 8; ++r) {
+        const int src = (g << 3) | r;
+        const uint64_t key = __shfl(qkey, src);
+        const int act = __shfl((int)active, src);
+        if (act) {
+            uint64_t pos = hmix(key) & mask & ~7ull;
+            for (uint64_t it = 0;; ++it) {
+                const Slot sl = tab[(pos + gl) & mask];
+                const uint32_t m = group_bits(__ballot(sl.key == key), g);
+                const uint32_t e = group_bits(__ballot(sl.key == EMPTY_KEY), g);
+                if (m) {
+                    const uint64_t v = __shfl(sl.val, (g << 3) + __ffs(m) - 1);
+                    if (gl == r) result = v;
+                    break;
+                }
+                if (e) break;
+                pos = (pos + 8) & mask;
+                if (it > (mask >> 3)) { if (gl == 0) atomicOr(err, ERR_PROBE); break; }
+            }
+        }
+    }
+    return result;
+}
+
+// Insert qkey if absent.  With `val` != NONE the inserting lane stores it.
+// Returns true when this lane's key was newly inserted.
+__device__ __forceinline__ bool coop_insert(Slot* __restrict__ tab, uint64_t mask, uint64_t qkey, uint64_t val,
+                                            bool active, unsigned long long* err) {
+    const int lane = threadIdx.x & 63, g = lane >> 3, gl = lane & 7;
+    bool inserted = false;
+    for (int r = 0; r < 8; ++r) {
+        const int src = (g << 3) | r;
+        const uint64_t key = __shfl(qkey, src);
+        const uint64_t v = __shfl(val, src);
+        const int act = __shfl((int)active, src);
+        if (act) {
+            uint64_t pos = hmix(key) & mask & ~7ull;
+            for (uint64_t it = 0;; ++it) {
+                const Slot sl = tab[(pos + gl) & mask];
+                const uint32_t m = group_bits(__ballot(sl.key == key), g);
+                if (m) break;                                   // already present
+                const uint32_t e = group_bits(__ballot(sl.key == EMPTY_KEY), g);
+                if (e) {
+                    const int leader = __ffs(e) - 1;
+                    unsigned long long old = 0;
+                    if (gl == leader) {
+                        Slot* s = &tab[(pos + leader) & mask];
+                        old = atomicCAS((unsigned long long*)&s->key, (unsigned long long)EMPTY_KEY,
+                                        (unsigned long long)key);
+                        if (old == EMPTY_KEY && v != NONE) s->val = v;
+                    }
+                    old = __shfl(old, (g << 3) + leader);
+                    if (old == EMPTY_KEY) { if (gl == r) inserted = true; break; }
+                    if (old == key) break;
+                    continue;                                   // lost the slot: re-read the window
+                }
+                pos = (pos + 8) & mask;
+                if (it > (mask >> 3)) { if (gl == 0) atomicOr(err, ERR_PROBE); break; }
+            }
+        }
+    }
+    return inserted;
+}
+
+// ---------------------------------------------------------------------------
+// Single-pass device-wide exclusive scan (decoupled look-back).  Tiles are
+// handed out in start order by a ticket counter that the last tile resets, so
+// a tile only ever waits on tiles that are already running.  Status words
+// carry flag (2 bits) | epoch (22 bits) | value (40 bits); a word from an older
+// launch (other epoch) reads as "not ready".
+// ---------------------------------------------------------------------------
+struct ScanCtl {
+    unsigned long long* status;
+    unsigned int* ticket;
+    unsigned long long* err;   // sticky error word (ERR_SCAN on a look-back that never completes)
+    uint32_t epoch;
+};
+
+constexpr unsigned long long ST_AGG = 1ull << 62, ST_PRE = 2ull << 62;
+constexpr unsigned long long VAL_MASK = (1ull << 40) - 1;
+
+__device__ __forceinline__ unsigned long long st_pack(unsigned long long flag, uint32_t epoch, unsigned long long v) {
+    return flag | ((unsigned long long)(epoch & 0x3fffff) << 40) | (v & VAL_MASK);
+}
+
+// Block-level exclusive scan of `count`; returns this thread's global output
+// offset.  `tile` is the ticket of this block, `ntiles` the grid size.  The
+// last tile writes the total over all tiles to *gtotal; every thread gets the
+// inclusive prefix of its own tile in *ltotal (the total, in the last tile).
+__device__ __forceinline__ unsigned long long block_scan(uint32_t count, ScanCtl sc, uint32_t tile,
+                                                         uint32_t ntiles, unsigned long long* gtotal,
+                                                         unsigned long long* ltotal) {
+    __shared__ unsigned long long s_wave[BLOCK / 64];
+    __shared__ unsigned long long s_prefix;
+    __shared__ unsigned long long s_total;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // inclusive wave scan
+    unsigned long long x = count;
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned long long y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_wave[w] = x;
+    __syncthreads();
+    unsigned long long wave_off = 0, agg = 0;
+    for (int i = 0; i < BLOCK / 64; ++i) {
+        if (i < w) wave_off += s_wave[i];
+        agg += s_wave[i];
+    }
+    if (threadIdx.x == 0) {
+        unsigned long long prefix = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&sc.status[0], st_pack(ST_PRE, sc.epoch, agg), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&sc.status[tile], st_pack(ST_AGG, sc.epoch, agg), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            int j = (int)tile - 1;
+            // every predecessor holds an earlier ticket and is running, so the wait
+            // ends; the bound only turns a broken invariant into a reported error
+            for (unsigned long long spins = 0;; ++spins) {
+                if (spins > (1ull << 24)) { atomicOr(sc.err, ERR_SCAN); break; }
+                const unsigned long long s = __hip_atomic_load(&sc.status[j], __ATOMIC_ACQUIRE,
+                                                               __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t ep = (uint32_t)((s >> 40) & 0x3fffff);
+                const unsigned long long fl = s & (3ull << 62);
+                if (ep != (sc.epoch & 0x3fffff) || fl == 0) continue;   // predecessor not ready yet
+                prefix += s & VAL_MASK;
+                if (fl == ST_PRE) break;
+                --j;
+            }
+            __hip_atomic_store(&sc.status[tile], st_pack(ST_PRE, sc.epoch, prefix + agg), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_prefix = prefix;
+        s_total = prefix + agg;
+        if (tile == ntiles - 1) {
+            *gtotal = prefix + agg;
+            __hip_atomic_store(sc.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    __syncthreads();
+    *ltotal = s_total;
+    return s_prefix + wave_off + (x - count);
+}
+
+__device__ __forceinline__ uint32_t take_ticket(ScanCtl sc) {
+    __shared__ uint32_t s_tile;
+    if (threadIdx.x == 0) s_tile = atomicAdd(sc.ticket, 1u);
+    __syncthreads();
+    return s_tile;
+}
+
+// Wave then block reduction of the counters into Stats accumulators; the last
+// block publishes nnz / empty and resets the accumulators for the next launch.
+// `moved` (entries read + entries written by this thread) feeds the
+// algorithmic-bytes counter of the roofline.
+__device__ __forceinline__ void finish_stats(Stats* st, unsigned long long upd, unsigned long long nnz,
+                                             unsigned long long empty, uint32_t nblocks,
+                                             unsigned long long moved = 0) {
+    for (int d = 32; d >= 1; d >>= 1) {
+        upd += __shfl_xor(upd, d);
+        nnz += __shfl_xor(nnz, d);
+        empty += __shfl_xor(empty, d);
+        moved += __shfl_xor(moved, d);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (upd) atomicAdd(&st->updates, upd);
+        if (moved) atomicAdd(&st->moved, moved);
+        if (nnz) atomicAdd(&st->acc_nnz, nnz);
+        if (empty) atomicAdd(&st->acc_empty, empty);
+    }
+    __threadfence();
+    __syncthreads();
+    __shared__ bool s_last;
+    if (threadIdx.x == 0) s_last = atomicAdd(&st->done, 1u) == nblocks - 1;
+    __syncthreads();
+    if (s_last && threadIdx.x == 0) {
+        __threadfence();
+        st->nnz = __hip_atomic_load(&st->acc_nnz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        st->empty = __hip_atomic_load(&st->acc_empty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&st->acc_nnz, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&st->acc_empty, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&st->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+}  // namespace sqe

@@ -1,0 +1,1071 @@
+// Host side of the sparse statevector engine: C ABI, planner, memory and
+// launch sequencing.  Replaces the reference's TableManager__ / Mapping__ /
+// Table__ host layer (ivanNovikau/SparQC emulator/device/table_manager.cuh,
+// mapping_GHT.cuh, table.cuh).
+//
+// Execution model.  Gates are queued on the host.  When the state is observed
+// (nnz / norm / export / sync) or the queue is long, the planner walks the
+// queue against the bucket-qubit set M: gates that act inside a bucket are
+// batched into fused sweeps (one launch applies the whole run in registers);
+// a branching gate on a non-bucket qubit, or a permutation whose controls
+// include bucket qubits, becomes one cross-bucket launch through the hash
+// index.  M is chosen from the branching targets of the first queued batch and
+// re-chosen (re-bucketing through the insert path) when later batches would
+// otherwise be dominated by cross-bucket gates.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sqe.h"
+#include "sqe_launch.hpp"
+
+using namespace sqe;
+
+namespace {
+
+thread_local std::string g_err;
+
+sqe_status fail(sqe_status s, const std::string& msg) {
+    g_err = msg;
+    return s;
+}
+
+#define HIPCK(x)                                                                         \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            return fail(SQE_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+#define SQCK(x)                              \
+    do {                                     \
+        sqe_status s_ = (x);                 \
+        if (s_ != SQE_OK) return s_;         \
+    } while (0)
+
+inline uint64_t round64(uint64_t x) { return (x + 63) & ~63ull; }
+inline uint64_t pow2_at_least(uint64_t x) {
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+inline uint32_t tiles_of(uint64_t n) { return (uint32_t)((n + BLOCK - 1) / BLOCK); }
+
+struct Store {
+    uint64_t* base = nullptr;
+    void* amp = nullptr;
+    uint64_t cap = 0;   // buckets (multiple of 64)
+    size_t abytes = 0;  // bytes of amp
+    int mb = -1;        // slot bits the amp buffer was sized for
+};
+
+struct Timed {
+    hipEvent_t a, b;
+    int cls;            // 0 sweep, 1 pair, 2 other
+};
+
+enum StepKind { ST_SWEEP, ST_PAIR, ST_NORM };
+struct Step {
+    StepKind kind;
+    uint64_t off, n;    // sweep: ops [off, off+n) of the uploaded program
+    DOp op;             // pair
+};
+
+}  // namespace
+
+struct sqe_engine {
+    sqe_config cfg;
+    bool f32 = false;
+    double thr = 1e-14;
+    hipStream_t st = nullptr;
+
+    bool have_state = false;
+    bool on_device = false;
+    QMap qm{};
+    Store s[2];
+    int cur = 0;
+    Store snap;
+    bool snap_valid = false;
+    QMap snap_qm{};
+
+    unsigned long long* meta = nullptr;     // [0],[1] bucket counts of the stores, [2] export total, [3] snapshot count
+    Stats* stats = nullptr;
+    double* dscal = nullptr;                // [0] norm2, [1] 1/sqrt(norm2)
+    double* partials = nullptr;
+    uint64_t partials_cap = 0;
+    Slot* tab = nullptr;
+    uint64_t tcap = 0;
+    bool tab_valid = false;
+    unsigned long long* scan_status = nullptr;
+    uint64_t scan_cap = 0;
+    unsigned int* ticket = nullptr;
+    uint32_t epoch = 0;
+    DOp* dops = nullptr;
+    uint64_t dops_cap = 0;
+    DOp* hops = nullptr;                    // pinned host staging of programs
+    uint64_t hops_cap = 0;
+    hipEvent_t upload_done = nullptr;
+    bool upload_pending = false;
+    unsigned int* occ = nullptr;
+    uint64_t occ_cap = 0;
+    uint64_t* xkeys = nullptr;
+    double* xre = nullptr;
+    double* xim = nullptr;
+    uint64_t xcap = 0;
+
+    std::vector<uint64_t> hkeys;
+    std::vector<double> hre, him;
+    std::vector<sqe_gate> pending;
+    uint64_t nb_upper = 0;
+    uint64_t gates = 0;
+    uint64_t bytes = 0;
+    bool renorm_pending = false;
+
+    uint64_t n_sweeps = 0, n_sweep_ops = 0, n_pairs = 0, n_remaps = 0;
+    std::vector<Timed> timed;
+    std::vector<hipEvent_t> ev_pool;
+    double t_ms[3] = {0, 0, 0};
+
+    int S() const { return 1 << qm.mb; }
+    size_t csize() const { return f32 ? 8 : 16; }
+    size_t amp_bytes(uint64_t cap) const { return (size_t)cap * (size_t)S() * csize(); }
+};
+
+namespace {
+
+sqe_status dalloc(sqe_engine* e, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return SQE_OK;
+    if (e->bytes + bytes > e->cfg.hbm_budget_bytes)
+        return fail(SQE_ETABLE_FULL, "state needs " + std::to_string(e->bytes + bytes) +
+                                         " device bytes, budget is " + std::to_string(e->cfg.hbm_budget_bytes));
+    hipError_t r = hipMalloc(p, bytes);
+    if (r != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(SQE_ENOMEM, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(r));
+    }
+    e->bytes += bytes;
+    return SQE_OK;
+}
+
+void dfree(sqe_engine* e, void* p, size_t bytes) {
+    if (p) {
+        (void)hipFree(p);
+        e->bytes -= bytes;
+    }
+}
+
+void free_store(sqe_engine* e, Store& s) {
+    dfree(e, s.base, s.cap * 8);
+    dfree(e, s.amp, s.abytes);
+    s = Store{};
+}
+
+sqe_status alloc_store(sqe_engine* e, Store& s, uint64_t cap) {
+    cap = round64(std::max<uint64_t>(cap, 64));
+    Store n;
+    n.cap = cap;
+    n.abytes = e->amp_bytes(cap);
+    n.mb = e->qm.mb;
+    SQCK(dalloc(e, (void**)&n.base, cap * 8));
+    sqe_status r = dalloc(e, &n.amp, n.abytes);
+    if (r != SQE_OK) {
+        dfree(e, n.base, cap * 8);
+        return r;
+    }
+    s = n;
+    return SQE_OK;
+}
+
+// ---- timing -----------------------------------------------------------------
+hipEvent_t get_event(sqe_engine* e) {
+    if (!e->ev_pool.empty()) {
+        hipEvent_t ev = e->ev_pool.back();
+        e->ev_pool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    return ev;
+}
+
+struct TimeScope {
+    sqe_engine* e;
+    Timed t{};
+    bool on;
+    TimeScope(sqe_engine* e_, int cls) : e(e_), on(e_->cfg.timing != 0) {
+        if (!on) return;
+        t.cls = cls;
+        t.a = get_event(e);
+        t.b = get_event(e);
+        if (!t.a || !t.b) { on = false; return; }
+        (void)hipEventRecord(t.a, e->st);
+    }
+    ~TimeScope() {
+        if (!on) return;
+        (void)hipEventRecord(t.b, e->st);
+        e->timed.push_back(t);
+    }
+};
+
+void collect_times(sqe_engine* e) {
+    for (auto& t : e->timed) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) e->t_ms[t.cls] += ms;
+        e->ev_pool.push_back(t.a);
+        e->ev_pool.push_back(t.b);
+    }
+    e->timed.clear();
+}
+
+// ---- scan control -------------------------------------------------------------
+sqe_status next_scan(sqe_engine* e, uint32_t ntiles, ScanCtl* out) {
+    if (ntiles > e->scan_cap) {
+        HIPCK(hipStreamSynchronize(e->st));
+        dfree(e, e->scan_status, e->scan_cap * 8);
+        e->scan_status = nullptr;
+        const uint64_t cap = std::max<uint64_t>(1024, pow2_at_least(ntiles));
+        SQCK(dalloc(e, (void**)&e->scan_status, cap * 8));
+        e->scan_cap = cap;
+        HIPCK(launch_fill_u64(e->st, (uint64_t*)e->scan_status, cap, 0));
+        e->epoch = 0;
+    }
+    if (++e->epoch >= (1u << 22) - 1) {
+        HIPCK(launch_fill_u64(e->st, (uint64_t*)e->scan_status, e->scan_cap, 0));
+        e->epoch = 1;
+    }
+    out->status = e->scan_status;
+    out->ticket = e->ticket;
+    out->err = &e->stats->err;
+    out->epoch = e->epoch;
+    return SQE_OK;
+}
+
+sqe_status read_stats(sqe_engine* e, Stats* h) {
+    HIPCK(hipMemcpyAsync(h, e->stats, sizeof(Stats), hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
+    return SQE_OK;
+}
+
+sqe_status check_device_errors(sqe_engine* e) {
+    Stats h;
+    SQCK(read_stats(e, &h));
+    if (h.err) {
+        const unsigned long long zero = 0;
+        HIPCK(hipMemcpy(&e->stats->err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        std::string m = "device reported:";
+        if (h.err & ERR_DUP) m += " duplicate key in load;";
+        if (h.err & ERR_MISSING) m += " key missing from index;";
+        if (h.err & ERR_PROBE) m += " hash probe exhausted;";
+        if (h.err & ERR_CAP) m += " output capacity exceeded;";
+        if (h.err & ERR_SCAN) m += " scan look-back did not complete;";
+        return fail((h.err & ERR_DUP) ? SQE_EINVAL : SQE_EHIP, m);
+    }
+    return SQE_OK;
+}
+
+sqe_status read_nb(sqe_engine* e, uint64_t* nb) {
+    unsigned long long v = 0;
+    HIPCK(hipMemcpyAsync(&v, &e->meta[e->cur], sizeof(v), hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
+    *nb = v;
+    return SQE_OK;
+}
+
+// ---- planner -------------------------------------------------------------------
+bool kind_is_branch(uint32_t k) { return k == SQE_GATE_H || k == SQE_GATE_RX || k == SQE_GATE_RY || k == SQE_GATE_U; }
+bool kind_is_perm(uint32_t k) { return k == SQE_GATE_X || k == SQE_GATE_Y || k == SQE_GATE_PERM; }
+
+uint32_t pext_host(uint64_t v, const QMap& qm) {
+    uint32_t s = 0;
+    for (int j = 0; j < qm.mb; ++j) s |= (uint32_t)((v >> qm.q[j]) & 1ull) << j;
+    return s;
+}
+
+QMap make_qmap(const std::vector<int>& qs) {
+    QMap m{};
+    std::vector<int> q(qs);
+    std::sort(q.begin(), q.end());
+    m.mb = (int)q.size();
+    m.mask = 0;
+    for (size_t j = 0; j < q.size(); ++j) {
+        m.q[j] = (uint8_t)q[j];
+        m.mask |= 1ull << q[j];
+    }
+    return m;
+}
+
+bool gate_local(const sqe_gate& g, const QMap& qm) {
+    if ((qm.mask >> g.target) & 1ull) return true;
+    if (kind_is_branch(g.kind)) return false;
+    if (kind_is_perm(g.kind)) return (g.ctrl_mask & qm.mask) == 0;
+    return true;
+}
+
+uint64_t count_cross(const std::vector<sqe_gate>& gl, const QMap& qm) {
+    uint64_t c = 0;
+    for (auto& g : gl) c += gate_local(g, qm) ? 0 : 1;
+    return c;
+}
+
+// Bucket qubits: the most frequent branching targets (ties: lower qubit).
+QMap choose_map(const sqe_engine* e, const std::vector<sqe_gate>& gl) {
+    const int nq = (int)e->cfg.nq;
+    const int mmax = e->f32 ? 5 : 4;
+    int want = e->cfg.bucket_bits;
+    std::vector<uint64_t> cnt(nq, 0);
+    for (auto& g : gl)
+        if (kind_is_branch(g.kind)) cnt[g.target]++;
+    std::vector<int> order(nq);
+    for (int i = 0; i < nq; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+    std::vector<int> pick;
+    if (want < 0) {
+        for (int i = 0; i < nq && (int)pick.size() < mmax; ++i)
+            if (cnt[order[i]] > 0) pick.push_back(order[i]);
+    } else {
+        want = std::min(want, std::min(mmax, nq));
+        for (int i = 0; i < nq && (int)pick.size() < want; ++i) pick.push_back(order[i]);
+    }
+    return make_qmap(pick);
+}
+
+DOp plan_op(const sqe_gate& g, const QMap& qm) {
+    DOp d{};
+    d.kind = g.kind;
+    const bool tin = (qm.mask >> g.target) & 1ull;
+    d.tslot = -1;
+    d.tbase = 0;
+    if (tin) {
+        for (int j = 0; j < qm.mb; ++j)
+            if (qm.q[j] == g.target) d.tslot = j;
+    } else {
+        d.tbase = 1ull << g.target;
+    }
+    d.cmB = g.ctrl_mask & ~qm.mask;
+    d.cvB = g.ctrl_val & ~qm.mask;
+    d.cmS = pext_host(g.ctrl_mask, qm);
+    d.cvS = pext_host(g.ctrl_val, qm);
+    for (int k = 0; k < 8; ++k) {
+        d.p[k] = g.p[k];
+        d.pf[k] = (float)g.p[k];
+    }
+    return d;
+}
+
+DOp renorm_op() {
+    DOp d{};
+    d.kind = K_RENORM;
+    d.tslot = -1;
+    return d;
+}
+
+// ---- state transfer ---------------------------------------------------------------
+sqe_status ensure_staging(sqe_engine* e, uint64_t n) {
+    if (n <= e->xcap) return SQE_OK;
+    HIPCK(hipStreamSynchronize(e->st));
+    dfree(e, e->xkeys, e->xcap * 8);
+    dfree(e, e->xre, e->xcap * 8);
+    dfree(e, e->xim, e->xcap * 8);
+    e->xkeys = nullptr; e->xre = nullptr; e->xim = nullptr; e->xcap = 0;
+    const uint64_t cap = std::max<uint64_t>(n, 1024);
+    SQCK(dalloc(e, (void**)&e->xkeys, cap * 8));
+    SQCK(dalloc(e, (void**)&e->xre, cap * 8));
+    SQCK(dalloc(e, (void**)&e->xim, cap * 8));
+    e->xcap = cap;
+    return SQE_OK;
+}
+
+sqe_status ensure_index(sqe_engine* e, uint64_t nkeys) {
+    const double ml = e->cfg.max_load > 0.0 && e->cfg.max_load < 0.95 ? e->cfg.max_load : 0.5;
+    const uint64_t want = std::max<uint64_t>(1024, pow2_at_least((uint64_t)std::ceil((double)nkeys / ml) + 8));
+    if (want > e->tcap || want * 8 < e->tcap) {
+        HIPCK(hipStreamSynchronize(e->st));
+        dfree(e, e->tab, e->tcap * sizeof(Slot));
+        e->tab = nullptr;
+        e->tcap = 0;
+        SQCK(dalloc(e, (void**)&e->tab, want * sizeof(Slot)));
+        e->tcap = want;
+    }
+    return SQE_OK;
+}
+
+sqe_status ensure_stores(sqe_engine* e, uint64_t cap) {
+    cap = round64(std::max<uint64_t>(cap, 256));
+    for (int i = 0; i < 2; ++i) {
+        if (e->s[i].cap >= cap && e->s[i].mb == e->qm.mb) continue;
+        HIPCK(hipStreamSynchronize(e->st));
+        free_store(e, e->s[i]);
+        SQCK(alloc_store(e, e->s[i], cap));
+    }
+    return SQE_OK;
+}
+
+// Build the bucket store from n FP64 entries staged in xkeys/xre/xim.
+sqe_status load_from_staging(sqe_engine* e, uint64_t n) {
+    const uint64_t S = (uint64_t)e->S();
+    // buckets <= n; leave headroom for growth by cross-bucket gates
+    const uint64_t cap = round64(std::max<uint64_t>(n + n / 2, 256));
+    if (e->s[0].cap < cap || e->s[1].cap < cap || e->s[0].cap > 4 * cap || e->s[0].mb != e->qm.mb ||
+        e->s[1].mb != e->qm.mb) {
+        HIPCK(hipStreamSynchronize(e->st));
+        free_store(e, e->s[0]);
+        free_store(e, e->s[1]);
+        SQCK(alloc_store(e, e->s[0], cap));
+        SQCK(alloc_store(e, e->s[1], cap));
+    }
+    e->cur = 0;
+    Store& o = e->s[0];
+    if (o.cap > e->occ_cap) {
+        HIPCK(hipStreamSynchronize(e->st));
+        dfree(e, e->occ, e->occ_cap * 4);
+        e->occ = nullptr;
+        e->occ_cap = 0;
+        SQCK(dalloc(e, (void**)&e->occ, o.cap * 4));
+        e->occ_cap = o.cap;
+    }
+    SQCK(ensure_index(e, n));
+    HIPCK(launch_fill_slots(e->st, e->tab, e->tcap));
+    HIPCK(launch_fill_u64(e->st, (uint64_t*)o.amp, e->amp_bytes(o.cap) / 8, 0));
+    HIPCK(launch_fill_u64(e->st, (uint64_t*)e->occ, o.cap / 2, 0));
+    LoadArgs la{};
+    la.keys = e->xkeys; la.re = e->xre; la.im = e->xim; la.n = n; la.qm = e->qm;
+    la.tab = e->tab; la.tmask = e->tcap - 1; la.amp = o.amp; la.occ = e->occ; la.stats = e->stats;
+    EnumArgs ea{};
+    ea.tab = e->tab; ea.tcap = e->tcap; ea.out_base = o.base; ea.out_nb = &e->meta[0];
+    ea.out_cap = o.cap; ea.stats = e->stats;
+    const uint32_t eg = tiles_of(e->tcap);
+    SQCK(next_scan(e, eg, &ea.scan));
+    {
+        TimeScope ts(e, 2);
+        HIPCK(launch_load(e->f32, e->st, la, ea, eg));
+    }
+    SQCK(check_device_errors(e));
+    uint64_t nb = 0;
+    SQCK(read_nb(e, &nb));
+    e->nb_upper = nb;
+    e->tab_valid = true;
+    e->on_device = true;
+    (void)S;
+    return SQE_OK;
+}
+
+sqe_status materialize(sqe_engine* e, const std::vector<sqe_gate>& gl) {
+    e->qm = choose_map(e, gl);
+    const uint64_t n = e->hkeys.size();
+    SQCK(ensure_staging(e, std::max<uint64_t>(n, 1)));
+    if (n) {
+        HIPCK(hipMemcpyAsync(e->xkeys, e->hkeys.data(), n * 8, hipMemcpyHostToDevice, e->st));
+        HIPCK(hipMemcpyAsync(e->xre, e->hre.data(), n * 8, hipMemcpyHostToDevice, e->st));
+        HIPCK(hipMemcpyAsync(e->xim, e->him.data(), n * 8, hipMemcpyHostToDevice, e->st));
+    }
+    SQCK(load_from_staging(e, n));
+    std::vector<uint64_t>().swap(e->hkeys);
+    std::vector<double>().swap(e->hre);
+    std::vector<double>().swap(e->him);
+    return SQE_OK;
+}
+
+// Export the current store into the FP64 staging arrays; returns the count.
+sqe_status export_to_staging(sqe_engine* e, int filter, uint64_t* count) {
+    uint64_t nb = 0;
+    SQCK(read_nb(e, &nb));
+    const uint64_t bound = std::max<uint64_t>(nb * (uint64_t)e->S(), 1);
+    SQCK(ensure_staging(e, bound));
+    ExportArgs xa{};
+    const Store& o = e->s[e->cur];
+    xa.base = o.base; xa.amp = o.amp; xa.nb = &e->meta[e->cur]; xa.qm = e->qm; xa.thr = e->thr;
+    xa.filter = filter; xa.keys = e->xkeys; xa.re = e->xre; xa.im = e->xim; xa.cap = e->xcap;
+    xa.total = &e->meta[2];
+    const uint32_t g = std::max<uint32_t>(1, tiles_of(round64(nb)));
+    SQCK(next_scan(e, g, &xa.scan));
+    {
+        TimeScope ts(e, 2);
+        HIPCK(launch_export(e->qm.mb, e->f32, e->st, g, xa));
+    }
+    unsigned long long tot = 0;
+    HIPCK(hipMemcpyAsync(&tot, &e->meta[2], sizeof(tot), hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
+    *count = tot;
+    return SQE_OK;
+}
+
+sqe_status remap(sqe_engine* e, const QMap& nm) {
+    uint64_t n = 0;
+    SQCK(export_to_staging(e, 0, &n));
+    e->qm = nm;
+    ++e->n_remaps;
+    return load_from_staging(e, n);
+}
+
+sqe_status rebuild_index(sqe_engine* e) {
+    SQCK(ensure_index(e, e->nb_upper));
+    HIPCK(launch_fill_slots(e->st, e->tab, e->tcap));
+    IndexArgs ia{};
+    const Store& o = e->s[e->cur];
+    ia.base = o.base; ia.nb = &e->meta[e->cur]; ia.tab = e->tab; ia.tmask = e->tcap - 1; ia.stats = e->stats;
+    {
+        TimeScope ts(e, 2);
+        HIPCK(launch_index_build(e->st, tiles_of(round64(e->nb_upper)), ia));
+    }
+    e->tab_valid = true;
+    return SQE_OK;
+}
+
+// Grow both stores so that `need` buckets fit, keeping the current state.
+// `need` = 0 asks for room for one cross-bucket gate (2 * buckets + 64).
+sqe_status grow(sqe_engine* e, uint64_t need) {
+    uint64_t nb = 0;
+    SQCK(read_nb(e, &nb));
+    e->nb_upper = nb;
+    if (!need) need = 2 * nb + 64;
+    if (need <= e->s[0].cap && need <= e->s[1].cap) return SQE_OK;
+    const uint64_t cap = round64(std::max<uint64_t>(need, e->s[0].cap * 2));
+    const int other = e->cur ^ 1;
+    free_store(e, e->s[other]);
+    SQCK(alloc_store(e, e->s[other], cap));
+    const uint64_t nbp = round64(nb);
+    HIPCK(hipMemcpyAsync(e->s[other].base, e->s[e->cur].base, nbp * 8, hipMemcpyDeviceToDevice, e->st));
+    HIPCK(hipMemcpyAsync(e->s[other].amp, e->s[e->cur].amp, e->amp_bytes(nbp), hipMemcpyDeviceToDevice, e->st));
+    HIPCK(hipMemcpyAsync(&e->meta[other], &e->meta[e->cur], 8, hipMemcpyDeviceToDevice, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
+    free_store(e, e->s[e->cur]);
+    SQCK(alloc_store(e, e->s[e->cur], cap));
+    e->cur = other;
+    return SQE_OK;
+}
+
+sqe_status run_sweep(sqe_engine* e, const DOp* dops, uint64_t nops, int store) {
+    SweepArgs a{};
+    Store& o = e->s[e->cur];
+    a.base = o.base; a.amp = o.amp; a.nb = &e->meta[e->cur]; a.ops = dops; a.nops = (uint32_t)nops;
+    a.store = store; a.thr = e->thr; a.scale = &e->dscal[1]; a.stats = e->stats;
+    const uint32_t g = std::max<uint32_t>(1, tiles_of(round64(e->nb_upper)));
+    TimeScope ts(e, 0);
+    HIPCK(launch_sweep(e->qm.mb, e->f32, e->st, g, a));
+    return SQE_OK;
+}
+
+sqe_status run_pair(sqe_engine* e, const DOp& op) {
+    if (!e->tab_valid) SQCK(rebuild_index(e));
+    const int other = e->cur ^ 1;
+    if (2 * e->nb_upper + 64 > e->s[other].cap) SQCK(grow(e, 0));
+    PairArgs a{};
+    const Store& i = e->s[e->cur];
+    const Store& o = e->s[other];
+    a.in_base = i.base; a.in_amp = i.amp; a.in_nb = &e->meta[e->cur]; a.tab = e->tab; a.tmask = e->tcap - 1;
+    a.op = op; a.thr = e->thr; a.out_base = o.base; a.out_amp = o.amp; a.out_nb = &e->meta[other];
+    a.out_cap = o.cap; a.stats = e->stats;
+    const uint32_t g = std::max<uint32_t>(1, tiles_of(round64(e->nb_upper)));
+    SQCK(next_scan(e, g, &a.scan));
+    {
+        TimeScope ts(e, 1);
+        HIPCK(launch_pair(e->qm.mb, e->f32, op.kind, e->st, g, a));
+    }
+    e->cur = other;
+    e->nb_upper = std::min<uint64_t>(2 * e->nb_upper, o.cap);
+    e->tab_valid = false;
+    ++e->n_pairs;
+    return SQE_OK;
+}
+
+sqe_status run_norm(sqe_engine* e) {
+    const uint32_t g = std::max<uint32_t>(1, tiles_of(round64(e->nb_upper)));
+    if (g > e->partials_cap) {
+        HIPCK(hipStreamSynchronize(e->st));
+        dfree(e, e->partials, e->partials_cap * 8);
+        e->partials = nullptr;
+        e->partials_cap = 0;
+        SQCK(dalloc(e, (void**)&e->partials, (uint64_t)g * 8));
+        e->partials_cap = g;
+    }
+    NormArgs a{};
+    a.amp = e->s[e->cur].amp; a.nb = &e->meta[e->cur]; a.partials = e->partials;
+    TimeScope ts(e, 2);
+    HIPCK(launch_norm(e->qm.mb, e->f32, e->st, g, a, &e->dscal[0], &e->dscal[1]));
+    return SQE_OK;
+}
+
+sqe_status compact(sqe_engine* e) {
+    const int other = e->cur ^ 1;
+    CompactArgs a{};
+    const Store& i = e->s[e->cur];
+    const Store& o = e->s[other];
+    a.in_base = i.base; a.in_amp = i.amp; a.in_nb = &e->meta[e->cur]; a.out_base = o.base; a.out_amp = o.amp;
+    a.out_nb = &e->meta[other]; a.out_cap = o.cap; a.stats = e->stats;
+    const uint32_t g = std::max<uint32_t>(1, tiles_of(round64(e->nb_upper)));
+    SQCK(next_scan(e, g, &a.scan));
+    {
+        TimeScope ts(e, 2);
+        HIPCK(launch_compact(e->qm.mb, e->f32, e->st, g, a));
+    }
+    e->cur = other;
+    e->tab_valid = false;
+    return SQE_OK;
+}
+
+// Plan and launch the queued gates.
+sqe_status flush(sqe_engine* e) {
+    if (e->pending.empty()) return SQE_OK;
+    if (!e->have_state) return fail(SQE_EINVAL, "no state: call sqe_set_basis or sqe_load first");
+    std::vector<sqe_gate> gl;
+    gl.swap(e->pending);
+    if (!e->on_device) {
+        SQCK(materialize(e, gl));
+    } else if (e->cfg.bucket_bits < 0) {
+        const QMap best = choose_map(e, gl);
+        const uint64_t now = count_cross(gl, e->qm), then = count_cross(gl, best);
+        if (best.mask != e->qm.mask && now >= 8 && then * 2 < now) SQCK(remap(e, best));
+    }
+    // plan into steps over one uploaded program
+    std::vector<DOp> prog;
+    std::vector<Step> steps;
+    const uint64_t W = e->cfg.fuse_window ? e->cfg.fuse_window : ~0ull;
+    uint64_t run_start = 0;
+    auto close_run = [&]() {
+        uint64_t n = prog.size() - run_start;
+        uint64_t off = run_start;
+        while (n) {
+            const uint64_t k = std::min(n, W);
+            steps.push_back(Step{ST_SWEEP, off, k, DOp{}});
+            off += k;
+            n -= k;
+        }
+        run_start = prog.size();
+    };
+    auto open_run = [&]() {
+        if (e->renorm_pending) {
+            prog.push_back(renorm_op());
+            e->renorm_pending = false;
+        }
+    };
+    for (const auto& g : gl) {
+        if (gate_local(g, e->qm)) {
+            if (prog.size() == run_start) open_run();
+            prog.push_back(plan_op(g, e->qm));
+        } else {
+            if (e->renorm_pending) open_run();
+            close_run();
+            steps.push_back(Step{ST_PAIR, 0, 0, plan_op(g, e->qm)});
+        }
+        ++e->gates;
+        if (e->cfg.renorm_every && e->gates % e->cfg.renorm_every == 0) {
+            close_run();
+            steps.push_back(Step{ST_NORM, 0, 0, DOp{}});
+            e->renorm_pending = true;
+        }
+    }
+    if (e->renorm_pending) open_run();
+    close_run();
+    // upload the program
+    if (!prog.empty()) {
+        if (e->upload_pending) {
+            HIPCK(hipEventSynchronize(e->upload_done));
+            e->upload_pending = false;
+        }
+        if (prog.size() > e->hops_cap) {
+            if (e->hops) (void)hipHostFree(e->hops);
+            e->hops = nullptr;
+            e->hops_cap = 0;
+            const uint64_t cap = pow2_at_least(prog.size());
+            if (hipHostMalloc((void**)&e->hops, cap * sizeof(DOp), hipHostMallocDefault) != hipSuccess)
+                return fail(SQE_ENOMEM, "pinned program buffer");
+            e->hops_cap = cap;
+        }
+        if (prog.size() > e->dops_cap) {
+            HIPCK(hipStreamSynchronize(e->st));
+            dfree(e, e->dops, e->dops_cap * sizeof(DOp));
+            e->dops = nullptr;
+            e->dops_cap = 0;
+            const uint64_t cap = pow2_at_least(prog.size());
+            SQCK(dalloc(e, (void**)&e->dops, cap * sizeof(DOp)));
+            e->dops_cap = cap;
+        }
+        std::memcpy(e->hops, prog.data(), prog.size() * sizeof(DOp));
+        HIPCK(hipMemcpyAsync(e->dops, e->hops, prog.size() * sizeof(DOp), hipMemcpyHostToDevice, e->st));
+        HIPCK(hipEventRecord(e->upload_done, e->st));
+        e->upload_pending = true;
+    }
+    for (const auto& s : steps) {
+        switch (s.kind) {
+        case ST_SWEEP: {
+            SQCK(run_sweep(e, e->dops + s.off, s.n, 1));
+            ++e->n_sweeps;
+            for (uint64_t i = s.off; i < s.off + s.n; ++i) {
+                const DOp& d = prog[i];
+                if (d.kind != K_RENORM) ++e->n_sweep_ops;
+                if (d.tbase && kind_is_perm(d.kind)) e->tab_valid = false;
+            }
+        } break;
+        case ST_PAIR: SQCK(run_pair(e, s.op)); break;
+        case ST_NORM: SQCK(run_norm(e)); break;
+        }
+    }
+    return SQE_OK;
+}
+
+sqe_status enter(sqe_engine* e) {
+    if (!e) return fail(SQE_EINVAL, "null engine");
+    HIPCK(hipSetDevice(e->cfg.device));
+    return SQE_OK;
+}
+
+sqe_status full_sync(sqe_engine* e) {
+    SQCK(flush(e));
+    HIPCK(hipStreamSynchronize(e->st));
+    SQCK(check_device_errors(e));
+    collect_times(e);
+    return SQE_OK;
+}
+
+sqe_status count_now(sqe_engine* e, Stats* h) {
+    SQCK(full_sync(e));
+    if (!e->on_device) {
+        std::memset(h, 0, sizeof(*h));
+        h->nnz = e->hkeys.size();
+        return SQE_OK;
+    }
+    SQCK(run_sweep(e, nullptr, 0, 0));
+    SQCK(read_stats(e, h));
+    collect_times(e);
+    return SQE_OK;
+}
+
+void destroy_all(sqe_engine* e) {
+    if (e->st) (void)hipStreamSynchronize(e->st);
+    free_store(e, e->s[0]);
+    free_store(e, e->s[1]);
+    free_store(e, e->snap);
+    dfree(e, e->meta, 64);
+    dfree(e, e->stats, sizeof(Stats));
+    dfree(e, e->dscal, 16);
+    dfree(e, e->partials, e->partials_cap * 8);
+    dfree(e, e->tab, e->tcap * sizeof(Slot));
+    dfree(e, e->scan_status, e->scan_cap * 8);
+    dfree(e, e->ticket, 64);
+    dfree(e, e->dops, e->dops_cap * sizeof(DOp));
+    dfree(e, e->occ, e->occ_cap * 4);
+    dfree(e, e->xkeys, e->xcap * 8);
+    dfree(e, e->xre, e->xcap * 8);
+    dfree(e, e->xim, e->xcap * 8);
+    if (e->hops) (void)hipHostFree(e->hops);
+    for (auto& t : e->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
+    if (e->upload_done) (void)hipEventDestroy(e->upload_done);
+    if (e->st) (void)hipStreamDestroy(e->st);
+}
+
+}  // namespace
+
+extern "C" {
+
+void sqe_config_default(sqe_config* c, uint32_t nq) {
+    std::memset(c, 0, sizeof(*c));
+    c->nq = nq;
+    c->prec = SQE_F64;
+    c->prune_thr = 0.0;
+    c->max_load = 0.5;
+    c->hbm_budget_bytes = 0;
+    c->device = 0;
+    c->bucket_bits = -1;
+    c->renorm_every = 0;
+    c->fuse_window = 0;
+    c->timing = 0;
+}
+
+sqe_status sqe_create(const sqe_config* cfg, sqe_engine** out) {
+    if (!cfg || !out) return fail(SQE_EINVAL, "null argument");
+    *out = nullptr;
+    if (cfg->nq < 1 || cfg->nq > 63) return fail(SQE_EINVAL, "nq must be in 1..63");
+    if (cfg->prec != SQE_F64 && cfg->prec != SQE_F32) return fail(SQE_EINVAL, "bad precision");
+    if (cfg->bucket_bits > (cfg->prec == SQE_F32 ? 5 : 4)) return fail(SQE_EINVAL, "bucket_bits too large");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return fail(SQE_EDEVICE, "no HIP device");
+    }
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(SQE_EINVAL, "bad device ordinal");
+    HIPCK(hipSetDevice(cfg->device));
+    sqe_engine* e = new (std::nothrow) sqe_engine();
+    if (!e) return fail(SQE_ENOMEM, "host allocation");
+    e->cfg = *cfg;
+    e->f32 = cfg->prec == SQE_F32;
+    e->thr = cfg->prune_thr > 0.0 ? cfg->prune_thr : (e->f32 ? 1e-7 : 1e-14);
+    if (e->cfg.hbm_budget_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete e; return fail(SQE_EHIP, "hipMemGetInfo"); }
+        e->cfg.hbm_budget_bytes = (uint64_t)(0.9 * (double)fr);
+    }
+    sqe_status r = SQE_OK;
+    if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) r = fail(SQE_EHIP, "stream");
+    if (r == SQE_OK && hipEventCreateWithFlags(&e->upload_done, hipEventDisableTiming) != hipSuccess)
+        r = fail(SQE_EHIP, "event");
+    if (r == SQE_OK) r = dalloc(e, (void**)&e->meta, 64);
+    if (r == SQE_OK) r = dalloc(e, (void**)&e->stats, sizeof(Stats));
+    if (r == SQE_OK) r = dalloc(e, (void**)&e->dscal, 16);
+    if (r == SQE_OK) r = dalloc(e, (void**)&e->ticket, 64);
+    if (r == SQE_OK && hipMemsetAsync(e->meta, 0, 64, e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
+    if (r == SQE_OK && hipMemsetAsync(e->stats, 0, sizeof(Stats), e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
+    if (r == SQE_OK && hipMemsetAsync(e->dscal, 0, 16, e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
+    if (r == SQE_OK && hipMemsetAsync(e->ticket, 0, 64, e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
+    if (r == SQE_OK && hipStreamSynchronize(e->st) != hipSuccess) r = fail(SQE_EHIP, "sync");
+    if (r != SQE_OK) {
+        destroy_all(e);
+        delete e;
+        return r;
+    }
+    *out = e;
+    return SQE_OK;
+}
+
+sqe_status sqe_destroy(sqe_engine* e) {
+    if (!e) return SQE_OK;
+    (void)hipSetDevice(e->cfg.device);
+    destroy_all(e);
+    delete e;
+    return SQE_OK;
+}
+
+sqe_status sqe_load(sqe_engine* e, const uint64_t* keys, const double* re, const double* im, uint64_t n) {
+    SQCK(enter(e));
+    if (n && (!keys || !re || !im)) return fail(SQE_EINVAL, "null array");
+    const uint64_t lim = (e->cfg.nq >= 64) ? ~0ull : ((1ull << e->cfg.nq) - 1);
+    std::vector<uint64_t> k;
+    std::vector<double> r, i;
+    k.reserve(n); r.reserve(n); i.reserve(n);
+    for (uint64_t j = 0; j < n; ++j) {
+        if (keys[j] & ~lim) return fail(SQE_EINVAL, "key has bits beyond nq");
+        if (re[j] == 0.0 && im[j] == 0.0) continue;
+        k.push_back(keys[j]); r.push_back(re[j]); i.push_back(im[j]);
+    }
+    // queued gates belonged to the state being replaced
+    e->pending.clear();
+    HIPCK(hipStreamSynchronize(e->st));
+    e->hkeys.swap(k);
+    e->hre.swap(r);
+    e->him.swap(i);
+    e->have_state = true;
+    e->on_device = false;
+    e->tab_valid = false;
+    e->renorm_pending = false;
+    return SQE_OK;
+}
+
+sqe_status sqe_set_basis(sqe_engine* e, uint64_t key) {
+    const double one = 1.0, zero = 0.0;
+    return sqe_load(e, &key, &one, &zero, 1);
+}
+
+sqe_status sqe_apply_gates(sqe_engine* e, const sqe_gate* g, uint64_t n) {
+    SQCK(enter(e));
+    if (n && !g) return fail(SQE_EINVAL, "null gates");
+    const uint64_t lim = (1ull << e->cfg.nq) - 1;
+    for (uint64_t j = 0; j < n; ++j) {
+        const sqe_gate& x = g[j];
+        if (x.kind > SQE_GATE_PERM) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": bad kind");
+        if (x.target >= e->cfg.nq) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": target out of range");
+        if (x.ctrl_mask & ~lim) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": control out of range");
+        if ((x.ctrl_mask >> x.target) & 1ull) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": target is a control");
+        if (x.ctrl_val & ~x.ctrl_mask) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": ctrl_val outside ctrl_mask");
+    }
+    e->pending.insert(e->pending.end(), g, g + n);
+    if (e->pending.size() >= (1u << 20)) return flush(e);
+    return SQE_OK;
+}
+
+sqe_status sqe_apply(sqe_engine* e, uint32_t target, uint64_t ctrl_mask, uint64_t ctrl_val, const double u[8],
+                     sqe_class cls) {
+    if (!u) return fail(SQE_EINVAL, "null matrix");
+    sqe_gate g{};
+    g.target = target;
+    g.ctrl_mask = ctrl_mask;
+    g.ctrl_val = ctrl_val;
+    switch (cls) {
+    case SQE_DIAG:
+        g.kind = SQE_GATE_DIAG;
+        g.p[0] = u[0]; g.p[1] = u[1]; g.p[2] = u[6]; g.p[3] = u[7];
+        break;
+    case SQE_PERM:
+        g.kind = SQE_GATE_PERM;
+        g.p[0] = u[2]; g.p[1] = u[3]; g.p[2] = u[4]; g.p[3] = u[5];
+        break;
+    case SQE_BRANCH:
+        g.kind = SQE_GATE_U;
+        for (int k = 0; k < 8; ++k) g.p[k] = u[k];
+        break;
+    default:
+        return fail(SQE_EINVAL, "bad gate class");
+    }
+    return sqe_apply_gates(e, &g, 1);
+}
+
+sqe_status sqe_sync(sqe_engine* e) {
+    SQCK(enter(e));
+    SQCK(full_sync(e));
+    if (e->on_device) {
+        // drop buckets left all-zero by pruning once they are a quarter of the store
+        Stats h;
+        SQCK(read_stats(e, &h));
+        uint64_t nb = 0;
+        SQCK(read_nb(e, &nb));
+        if (h.empty * 4 > nb && nb > 1024) {
+            SQCK(compact(e));
+            HIPCK(hipStreamSynchronize(e->st));
+            SQCK(read_nb(e, &e->nb_upper));
+            collect_times(e);
+        }
+    }
+    return SQE_OK;
+}
+
+sqe_status sqe_nnz(sqe_engine* e, uint64_t* n) {
+    SQCK(enter(e));
+    if (!n) return fail(SQE_EINVAL, "null output");
+    Stats h;
+    SQCK(count_now(e, &h));
+    *n = h.nnz;
+    return SQE_OK;
+}
+
+sqe_status sqe_norm2(sqe_engine* e, double* out) {
+    SQCK(enter(e));
+    if (!out) return fail(SQE_EINVAL, "null output");
+    SQCK(full_sync(e));
+    if (!e->on_device) {
+        double s = 0.0;
+        for (size_t i = 0; i < e->hkeys.size(); ++i) s += e->hre[i] * e->hre[i] + e->him[i] * e->him[i];
+        *out = s;
+        return SQE_OK;
+    }
+    SQCK(run_norm(e));
+    HIPCK(hipMemcpyAsync(out, &e->dscal[0], 8, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
+    collect_times(e);
+    return SQE_OK;
+}
+
+sqe_status sqe_export(sqe_engine* e, uint64_t* keys, double* re, double* im, uint64_t cap, uint64_t* n) {
+    SQCK(enter(e));
+    if (!n) return fail(SQE_EINVAL, "null output");
+    SQCK(full_sync(e));
+    if (!e->have_state) { *n = 0; return SQE_OK; }
+    if (!e->on_device) {
+        // nothing applied yet: filter the staged host copy
+        uint64_t c = 0;
+        for (size_t i = 0; i < e->hkeys.size(); ++i)
+            if (!(std::fabs(e->hre[i]) < e->thr && std::fabs(e->him[i]) < e->thr)) ++c;
+        *n = c;
+        if (c > cap) return fail(SQE_EINVAL, "export capacity too small");
+        uint64_t j = 0;
+        for (size_t i = 0; i < e->hkeys.size(); ++i) {
+            if (std::fabs(e->hre[i]) < e->thr && std::fabs(e->him[i]) < e->thr) continue;
+            double r = e->hre[i], m = e->him[i];
+            if (e->f32) { r = (double)(float)r; m = (double)(float)m; }
+            keys[j] = e->hkeys[i]; re[j] = r; im[j] = m; ++j;
+        }
+        return SQE_OK;
+    }
+    uint64_t tot = 0;
+    SQCK(export_to_staging(e, 1, &tot));
+    *n = tot;
+    if (tot > cap) return fail(SQE_EINVAL, "export capacity too small (" + std::to_string(tot) + " needed)");
+    if (tot) {
+        if (!keys || !re || !im) return fail(SQE_EINVAL, "null array");
+        HIPCK(hipMemcpyAsync(keys, e->xkeys, tot * 8, hipMemcpyDeviceToHost, e->st));
+        HIPCK(hipMemcpyAsync(re, e->xre, tot * 8, hipMemcpyDeviceToHost, e->st));
+        HIPCK(hipMemcpyAsync(im, e->xim, tot * 8, hipMemcpyDeviceToHost, e->st));
+        HIPCK(hipStreamSynchronize(e->st));
+    }
+    collect_times(e);
+    return SQE_OK;
+}
+
+sqe_status sqe_get_stats(sqe_engine* e, sqe_stats* out) {
+    SQCK(enter(e));
+    if (!out) return fail(SQE_EINVAL, "null output");
+    Stats h;
+    SQCK(count_now(e, &h));
+    std::memset(out, 0, sizeof(*out));
+    out->gates = e->gates;
+    out->updates = h.updates;
+    out->nnz = h.nnz;
+    uint64_t nb = 0;
+    if (e->on_device) SQCK(read_nb(e, &nb));
+    out->buckets = nb;
+    out->capacity = e->s[e->cur].cap;
+    out->bytes_device = e->bytes;
+    out->bucket_bits = e->qm.mb;
+    out->bucket_mask = e->qm.mask;
+    out->sweeps = e->n_sweeps;
+    out->sweep_ops = e->n_sweep_ops;
+    out->pairs = e->n_pairs;
+    out->remaps = e->n_remaps;
+    out->moved = h.moved;
+    out->sweep_ms = e->t_ms[0];
+    out->pair_ms = e->t_ms[1];
+    out->other_ms = e->t_ms[2];
+    return SQE_OK;
+}
+
+sqe_status sqe_prepare(sqe_engine* e, const sqe_gate* g, uint64_t n) {
+    SQCK(enter(e));
+    if (n && !g) return fail(SQE_EINVAL, "null gates");
+    SQCK(full_sync(e));
+    if (!e->have_state) return fail(SQE_EINVAL, "no state");
+    std::vector<sqe_gate> gl(g, g + n);
+    if (!e->on_device) return materialize(e, gl);
+    const QMap best = choose_map(e, gl);
+    if (best.mask != e->qm.mask) SQCK(remap(e, best));
+    return SQE_OK;
+}
+
+sqe_status sqe_snapshot_save(sqe_engine* e) {
+    SQCK(enter(e));
+    SQCK(full_sync(e));
+    if (!e->have_state) return fail(SQE_EINVAL, "no state");
+    if (!e->on_device) SQCK(materialize(e, std::vector<sqe_gate>()));
+    uint64_t nb = 0;
+    SQCK(read_nb(e, &nb));
+    const uint64_t nbp = round64(std::max<uint64_t>(nb, 1));
+    if (e->snap.cap < nbp || e->snap.mb != e->qm.mb) {
+        free_store(e, e->snap);
+        SQCK(alloc_store(e, e->snap, nbp));
+    }
+    HIPCK(hipMemcpyAsync(e->snap.base, e->s[e->cur].base, nbp * 8, hipMemcpyDeviceToDevice, e->st));
+    HIPCK(hipMemcpyAsync(e->snap.amp, e->s[e->cur].amp, e->amp_bytes(nbp), hipMemcpyDeviceToDevice, e->st));
+    HIPCK(hipMemcpyAsync(&e->meta[3], &e->meta[e->cur], 8, hipMemcpyDeviceToDevice, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
+    e->snap_qm = e->qm;
+    e->snap_valid = true;
+    return SQE_OK;
+}
+
+sqe_status sqe_snapshot_restore(sqe_engine* e) {
+    SQCK(enter(e));
+    if (!e->snap_valid) return fail(SQE_EINVAL, "no snapshot");
+    e->pending.clear();
+    HIPCK(hipStreamSynchronize(e->st));
+    unsigned long long nb = 0;
+    HIPCK(hipMemcpy(&nb, &e->meta[3], 8, hipMemcpyDeviceToHost));
+    const uint64_t nbp = round64(std::max<uint64_t>(nb, 1));
+    e->qm = e->snap_qm;
+    SQCK(ensure_stores(e, nbp));
+    HIPCK(hipMemcpyAsync(e->s[e->cur].base, e->snap.base, nbp * 8, hipMemcpyDeviceToDevice, e->st));
+    HIPCK(hipMemcpyAsync(e->s[e->cur].amp, e->snap.amp, e->amp_bytes(nbp), hipMemcpyDeviceToDevice, e->st));
+    HIPCK(hipMemcpyAsync(&e->meta[e->cur], &e->meta[3], 8, hipMemcpyDeviceToDevice, e->st));
+    e->nb_upper = nb;
+    e->tab_valid = false;
+    e->renorm_pending = false;
+    e->on_device = true;
+    e->have_state = true;
+    return SQE_OK;
+}
+
+const char* sqe_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

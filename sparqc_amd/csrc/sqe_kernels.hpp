@@ -1,0 +1,524 @@
+// Kernel templates of the sparse statevector engine (gfx950); instantiated
+// per precision in sqe_kernels_f64.hip / sqe_kernels_f32.hip.
+//
+// Reference path replaced (ivanNovikau/SparQC, emulator/device/):
+//   FC__::gate_sq / gate_sq_par   device_table_FC.cuh:113-205  -> k_sweep, k_pair
+//   *_core gate arithmetic         device_data.cuh:159-319      -> pair64 / pair32
+//   update_ah_sup prune+append     device_data.cuh:134-156      -> branch_pair + block_scan
+//   device_zero_arrays rebuild     device_data.cuh:62-105       -> out-of-place stores, k_compact
+//   Table__::get_statevector       table.cuh:47-75              -> k_export
+// Norm reduction, controlled gates, hashing of 64-bit keys and the FP32 form
+// have no reference counterpart.
+#pragma once
+#include "sqe_device.hpp"
+#include "sqe_launch.hpp"
+
+namespace sqe {
+
+// ---------------------------------------------------------------------------
+// Fused local sweep: one thread owns one bucket, keeps its 2^MB amplitudes in
+// registers and applies a whole run of bucket-local gates before writing back.
+// ---------------------------------------------------------------------------
+template <int MB, class R, uint32_t K, int J>
+__device__ __forceinline__ void slot_gate(const DOp& op, typename CT<R>::T (&a)[1 << MB], bool pred, double thr) {
+    using C = typename CT<R>::T;
+    static_for<(1 << MB) / 2>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int s0 = ((q >> J) << (J + 1)) | (q & ((1 << J) - 1));
+        constexpr int s1 = s0 | (1 << J);
+        if (((uint32_t)s0 & op.cmS) == op.cvS) {
+            C n0, n1;
+            any_pair<K>(op, thr, a[s0], a[s1], n0, n1);
+            a[s0] = csel(pred, n0, a[s0]);
+            a[s1] = csel(pred, n1, a[s1]);
+        }
+    });
+}
+
+template <int MB, class R, uint32_t K>
+__device__ __forceinline__ void slot_dispatch_j(const DOp& op, typename CT<R>::T (&a)[1 << MB], bool pred, double thr) {
+    switch (op.tslot) {
+    case 0: if constexpr (MB > 0) slot_gate<MB, R, K, 0>(op, a, pred, thr); break;
+    case 1: if constexpr (MB > 1) slot_gate<MB, R, K, 1>(op, a, pred, thr); break;
+    case 2: if constexpr (MB > 2) slot_gate<MB, R, K, 2>(op, a, pred, thr); break;
+    case 3: if constexpr (MB > 3) slot_gate<MB, R, K, 3>(op, a, pred, thr); break;
+    case 4: if constexpr (MB > 4) slot_gate<MB, R, K, 4>(op, a, pred, thr); break;
+    default: break;
+    }
+}
+
+template <int MB, class R>
+__device__ __forceinline__ void slot_op(const DOp& op, typename CT<R>::T (&a)[1 << MB], bool pred, double thr) {
+    switch (op.kind) {
+    case KX: slot_dispatch_j<MB, R, KX>(op, a, pred, thr); break;
+    case KY: slot_dispatch_j<MB, R, KY>(op, a, pred, thr); break;
+    case KZ: slot_dispatch_j<MB, R, KZ>(op, a, pred, thr); break;
+    case KH: slot_dispatch_j<MB, R, KH>(op, a, pred, thr); break;
+    case KPHASE: slot_dispatch_j<MB, R, KPHASE>(op, a, pred, thr); break;
+    case KRZ: slot_dispatch_j<MB, R, KRZ>(op, a, pred, thr); break;
+    case KRX: slot_dispatch_j<MB, R, KRX>(op, a, pred, thr); break;
+    case KRY: slot_dispatch_j<MB, R, KRY>(op, a, pred, thr); break;
+    case KU: slot_dispatch_j<MB, R, KU>(op, a, pred, thr); break;
+    case KDIAG: slot_dispatch_j<MB, R, KDIAG>(op, a, pred, thr); break;
+    case KPERM: slot_dispatch_j<MB, R, KPERM>(op, a, pred, thr); break;
+    default: break;
+    }
+}
+
+// Complex rotation by (c, s): (c x - s y, c y + s x), the reference's phase
+// formula (emulator/device/device_data.cuh:221-224); FP32 with fused multiply-add.
+__device__ __forceinline__ double2 rot(double c, double s, double2 v) {
+    return make_double2(c * v.x - s * v.y, c * v.y + s * v.x);
+}
+__device__ __forceinline__ float2 rot(float c, float s, float2 v) {
+    return make_float2(__fmaf_rn(c, v.x, -(s * v.y)), __fmaf_rn(c, v.y, s * v.x));
+}
+
+// Gate whose target is a base (non-bucket) qubit: diagonal gates scale the
+// whole bucket by the factor of its base bit; X / Y move the whole bucket to
+// base ^ (1 << target).  The planner never sends a branching gate, or a
+// permutation with slot controls, down this path.
+template <int MB, class R>
+__device__ __forceinline__ void base_op(const DOp& op, typename CT<R>::T (&a)[1 << MB], uint64_t& base, bool pred) {
+    using C = typename CT<R>::T;
+    constexpr int S = 1 << MB;
+    const bool b = (base & op.tbase) != 0;
+    const bool f64 = sizeof(R) == 8;
+    switch (op.kind) {
+    case KZ:
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            if (((uint32_t)s & op.cmS) == op.cvS && pred && b) { a[s].x = -a[s].x; a[s].y = -a[s].y; }
+        });
+        break;
+    case KPHASE: {
+        const R c = f64 ? (R)op.p[0] : (R)op.pf[0], sn = f64 ? (R)op.p[1] : (R)op.pf[1];
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            if (((uint32_t)s & op.cmS) == op.cvS) a[s] = csel(pred && b, rot(c, sn, a[s]), a[s]);
+        });
+    } break;
+    case KRZ: {
+        const R c = f64 ? (R)op.p[0] : (R)op.pf[0], s0 = f64 ? (R)op.p[1] : (R)op.pf[1];
+        const R sn = b ? s0 : -s0;      // bit 0: e^{-i theta/2}, bit 1: e^{+i theta/2}
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            if (((uint32_t)s & op.cmS) == op.cvS) a[s] = csel(pred, rot(c, sn, a[s]), a[s]);
+        });
+    } break;
+    case KDIAG: {
+        const int o = b ? 2 : 0;
+        const R dr = f64 ? (R)op.p[o] : (R)op.pf[o], di = f64 ? (R)op.p[o + 1] : (R)op.pf[o + 1];
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            if (((uint32_t)s & op.cmS) == op.cvS) a[s] = csel(pred, rot(dr, di, a[s]), a[s]);
+        });
+    } break;
+    case KX:
+        if (pred) base ^= op.tbase;
+        break;
+    case KY:
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            C v = a[s];
+            C w;
+            if (b) { w.x = v.y; w.y = -v.x; } else { w.x = -v.y; w.y = v.x; }
+            a[s] = csel(pred, w, v);
+        });
+        if (pred) base ^= op.tbase;
+        break;
+    case KPERM: {
+        // bit 0 moves to bit 1 with factor u10, bit 1 moves to bit 0 with factor u01
+        const int o = b ? 0 : 2;
+        const R dr = f64 ? (R)op.p[o] : (R)op.pf[o], di = f64 ? (R)op.p[o + 1] : (R)op.pf[o + 1];
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            a[s] = csel(pred, rot(dr, di, a[s]), a[s]);
+        });
+        if (pred) base ^= op.tbase;
+    } break;
+    default:
+        break;
+    }
+}
+
+template <int MB, class R>
+__device__ __forceinline__ uint32_t count_nz(const typename CT<R>::T (&a)[1 << MB]) {
+    uint32_t c = 0;
+    static_for<(1 << MB)>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        c += nonzero(a[s]) ? 1u : 0u;
+    });
+    return c;
+}
+
+template <int MB, class R>
+__global__ __launch_bounds__(BLOCK) void k_sweep(SweepArgs A) {
+    using C = typename CT<R>::T;
+    constexpr int S = 1 << MB;
+    const uint64_t b = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t nb = *A.nb;
+    const uint64_t nb_pad = (nb + 63) & ~63ull;
+    const bool live = b < nb_pad;                     // wavefront-uniform
+    C* __restrict__ amp = (C*)A.amp;
+    C a[S];
+    uint64_t base = EMPTY_KEY;
+    if (live) {
+        base = A.base[b];
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            a[s] = amp[amp_index(b, s, S)];
+        });
+    } else {
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            a[s].x = 0; a[s].y = 0;
+        });
+    }
+    const bool valid = live && base != EMPTY_KEY;
+    uint32_t cnt = valid ? count_nz<MB, R>(a) : 0u;
+    const uint32_t cnt_in = cnt;
+    unsigned long long upd = 0;
+    const uint64_t base0 = base;
+    if (live) {
+        for (uint32_t i = 0; i < A.nops; ++i) {
+            const DOp& op = A.ops[i];
+            if (op.kind == K_RENORM) {
+                const double f = *A.scale;
+                static_for<S>([&](auto sc) {
+                    constexpr int s = decltype(sc)::value;
+                    a[s].x = (R)((double)a[s].x * f);
+                    a[s].y = (R)((double)a[s].y * f);
+                });
+                continue;
+            }
+            upd += cnt;
+            const bool pred = valid && ((base & op.cmB) == op.cvB);
+            if (op.tslot >= 0) {
+                if constexpr (MB > 0) {
+                    slot_op<MB, R>(op, a, pred, A.thr);
+                    if (is_branch(op.kind)) cnt = valid ? count_nz<MB, R>(a) : 0u;
+                }
+            } else {
+                base_op<MB, R>(op, a, base, valid && pred);
+            }
+        }
+        if (A.store) {
+            static_for<S>([&](auto sc) {
+                constexpr int s = decltype(sc)::value;
+                amp[amp_index(b, s, S)] = a[s];
+            });
+            if (valid && base != base0) A.base[b] = base;
+        }
+    }
+    finish_stats(A.stats, upd, cnt, (valid && cnt == 0) ? 1ull : 0ull, gridDim.x,
+                 A.store ? (unsigned long long)cnt_in + cnt : 0ull);
+}
+
+// ---------------------------------------------------------------------------
+// Cross-bucket gate (target is a base qubit): pairs bucket `base` with bucket
+// `base ^ (1 << target)` found through the hash index.  The bucket with target
+// bit 0 owns the pair when it exists, otherwise the bit-1 bucket does
+// (the reference's ownership rule, emulator/device/device_table_FC.cuh:95-108).
+// Outputs are compacted into the other store with a single-pass scan;
+// all-zero buckets are dropped.
+// ---------------------------------------------------------------------------
+template <int MB, class R, uint32_t K>
+__global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
+    using C = typename CT<R>::T;
+    constexpr int S = 1 << MB;
+    const uint32_t tile = take_ticket(A.scan);
+    const uint64_t b = (uint64_t)tile * BLOCK + threadIdx.x;
+    const uint64_t nb = *A.in_nb;
+    const uint64_t nb_pad = (nb + 63) & ~63ull;
+    const bool live = b < nb_pad;
+    const C* __restrict__ in_amp = (const C*)A.in_amp;
+    C* __restrict__ out_amp = (C*)A.out_amp;
+    const DOp& op = A.op;
+
+    const uint64_t base = live ? A.in_base[b] : EMPTY_KEY;
+    const bool valid = base != EMPTY_KEY;
+    const bool ctrl_ok = valid && ((base & op.cmB) == op.cvB);
+    const uint64_t pbase = base ^ op.tbase;
+    const uint64_t j = coop_find(A.tab, A.tmask, pbase, ctrl_ok, &A.stats->err);
+    const bool bit = (base & op.tbase) != 0;
+    const bool owner = ctrl_ok && !(bit && j != NONE);
+
+    C own[S], par[S];
+    static_for<S>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        own[s].x = 0; own[s].y = 0; par[s].x = 0; par[s].y = 0;
+    });
+    if (valid) {
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            own[s] = in_amp[amp_index(b, s, S)];
+        });
+    }
+    const uint32_t nz_in = valid ? count_nz<MB, R>(own) : 0u;
+    if (owner && j != NONE) {
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            par[s] = in_amp[amp_index(j, s, S)];
+        });
+    }
+    uint32_t count = 0;
+    bool lo_nz = false, hi_nz = false;
+    if (owner) {
+        // own/par -> lo (target bit 0) / hi (target bit 1), in place
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            const C x = own[s], y = par[s];
+            C v0 = bit ? y : x, v1 = bit ? x : y;
+            if (((uint32_t)s & op.cmS) == op.cvS) {
+                C n0, n1;
+                any_pair<K>(op, A.thr, v0, v1, n0, n1);
+                v0 = n0; v1 = n1;
+            }
+            own[s] = v0; par[s] = v1;
+            lo_nz |= nonzero(v0);
+            hi_nz |= nonzero(v1);
+        });
+        count = (lo_nz ? 1u : 0u) + (hi_nz ? 1u : 0u);
+    } else if (valid && !ctrl_ok) {
+        count = nz_in ? 1u : 0u;      // untouched bucket is carried over (dropped if all zero)
+    }
+    unsigned long long total = 0;
+    unsigned long long off = block_scan(count, A.scan, tile, gridDim.x, A.out_nb, &total);
+    uint32_t nz_out = 0;
+    if (count) {
+        if (owner) {
+            if (lo_nz) {
+                if (off < A.out_cap) {
+                    A.out_base[off] = base & ~op.tbase;
+                    static_for<S>([&](auto sc) {
+                        constexpr int s = decltype(sc)::value;
+                        out_amp[amp_index(off, s, S)] = own[s];
+                    });
+                }
+                ++off;
+            }
+            if (hi_nz && off < A.out_cap) {
+                A.out_base[off] = base | op.tbase;
+                static_for<S>([&](auto sc) {
+                    constexpr int s = decltype(sc)::value;
+                    out_amp[amp_index(off, s, S)] = par[s];
+                });
+            }
+            nz_out = (lo_nz ? count_nz<MB, R>(own) : 0u) + (hi_nz ? count_nz<MB, R>(par) : 0u);
+        } else if (off < A.out_cap) {
+            A.out_base[off] = base;
+            static_for<S>([&](auto sc) {
+                constexpr int s = decltype(sc)::value;
+                out_amp[amp_index(off, s, S)] = own[s];
+            });
+            nz_out = nz_in;
+        }
+    }
+    // the last tile pads the output up to a whole tile of buckets
+    if (tile == gridDim.x - 1) {
+        if (total > A.out_cap) {
+            if (threadIdx.x == 0) atomicOr(&A.stats->err, ERR_CAP);
+        } else {
+            const uint64_t pad_end = (total + 63) & ~63ull;
+            for (uint64_t p = total + threadIdx.x; p < pad_end && p < A.out_cap; p += BLOCK) {
+                A.out_base[p] = EMPTY_KEY;
+                static_for<S>([&](auto sc) {
+                    constexpr int s = decltype(sc)::value;
+                    C z; z.x = 0; z.y = 0;
+                    out_amp[amp_index(p, s, S)] = z;
+                });
+            }
+        }
+    }
+    finish_stats(A.stats, nz_in, nz_out, 0ull, gridDim.x, (unsigned long long)nz_in + nz_out);
+    (void)nb;
+}
+
+// Drop all-zero buckets (threshold pruning leaves them behind) with a scan.
+template <int MB, class R>
+__global__ __launch_bounds__(BLOCK) void k_compact(CompactArgs A) {
+    using C = typename CT<R>::T;
+    constexpr int S = 1 << MB;
+    const uint32_t tile = take_ticket(A.scan);
+    const uint64_t b = (uint64_t)tile * BLOCK + threadIdx.x;
+    const uint64_t nb_pad = (*A.in_nb + 63) & ~63ull;
+    const bool live = b < nb_pad;
+    const C* __restrict__ in_amp = (const C*)A.in_amp;
+    C* __restrict__ out_amp = (C*)A.out_amp;
+    const uint64_t base = live ? A.in_base[b] : EMPTY_KEY;
+    const bool valid = base != EMPTY_KEY;
+    C a[S];
+    static_for<S>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        if (valid) a[s] = in_amp[amp_index(b, s, S)]; else { a[s].x = 0; a[s].y = 0; }
+    });
+    const uint32_t nz = count_nz<MB, R>(a);
+    const uint32_t count = (valid && nz) ? 1u : 0u;
+    unsigned long long total = 0;
+    const unsigned long long off = block_scan(count, A.scan, tile, gridDim.x, A.out_nb, &total);
+    if (count && off < A.out_cap) {
+        A.out_base[off] = base;
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            out_amp[amp_index(off, s, S)] = a[s];
+        });
+    }
+    if (tile == gridDim.x - 1) {
+        const uint64_t pad_end = (total + 63) & ~63ull;
+        for (uint64_t p = total + threadIdx.x; p < pad_end && p < A.out_cap; p += BLOCK) {
+            A.out_base[p] = EMPTY_KEY;
+            static_for<S>([&](auto sc) {
+                constexpr int s = decltype(sc)::value;
+                C z; z.x = 0; z.y = 0;
+                out_amp[amp_index(p, s, S)] = z;
+            });
+        }
+    }
+    finish_stats(A.stats, 0ull, count ? nz : 0u, 0ull, gridDim.x);
+}
+
+// Export non-negligible (or all nonzero) entries as FP64 (key, re, im) arrays
+// in bucket order (the reference's readout filter, emulator/device/table.cuh:58-73).
+template <int MB, class R>
+__global__ __launch_bounds__(BLOCK) void k_export(ExportArgs A) {
+    using C = typename CT<R>::T;
+    constexpr int S = 1 << MB;
+    const uint32_t tile = take_ticket(A.scan);
+    const uint64_t b = (uint64_t)tile * BLOCK + threadIdx.x;
+    const uint64_t nb_pad = (*A.nb + 63) & ~63ull;
+    const bool live = b < nb_pad;
+    const C* __restrict__ amp = (const C*)A.amp;
+    const uint64_t base = live ? A.base[b] : EMPTY_KEY;
+    const bool valid = base != EMPTY_KEY;
+    C a[S];
+    uint32_t keep = 0, count = 0;
+    static_for<S>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        if (valid) a[s] = amp[amp_index(b, s, S)]; else { a[s].x = 0; a[s].y = 0; }
+        const double2 v = make_double2((double)a[s].x, (double)a[s].y);
+        const bool k = A.filter ? !negl64(v, A.thr) : nonzero(v);
+        if (valid && k) { keep |= 1u << s; ++count; }
+    });
+    unsigned long long total = 0;
+    unsigned long long off = block_scan(count, A.scan, tile, gridDim.x, A.total, &total);
+    if (count) {
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            if ((keep >> s) & 1u) {
+                if (off < A.cap) {
+                    A.keys[off] = base | pdep_q((uint32_t)s, A.qm);
+                    A.re[off] = (double)a[s].x;
+                    A.im[off] = (double)a[s].y;
+                }
+                ++off;
+            }
+        });
+    }
+}
+
+// Norm: per-block FP64 partial sums in a fixed order, then one block.
+template <int MB, class R>
+__global__ __launch_bounds__(BLOCK) void k_norm_partial(NormArgs A) {
+    using C = typename CT<R>::T;
+    constexpr int S = 1 << MB;
+    __shared__ double s_w[BLOCK / 64];
+    const uint64_t b = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t nb_pad = (*A.nb + 63) & ~63ull;
+    const C* __restrict__ amp = (const C*)A.amp;
+    double acc = 0.0;
+    if (b < nb_pad) {
+        static_for<S>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            const C v = amp[amp_index(b, s, S)];
+            acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+        });
+    }
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < BLOCK / 64; ++i) t += s_w[i];
+        A.partials[blockIdx.x] = t;
+    }
+}
+
+// Scatter each input amplitude into its bucket slot; a second write to the
+// same (bucket, slot) flags a duplicate key.
+template <class R>
+__global__ __launch_bounds__(BLOCK) void k_load_scatter(LoadArgs A) {
+    using C = typename CT<R>::T;
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const bool act = i < A.n;
+    const uint64_t key = act ? A.keys[i] : EMPTY_KEY;
+    const uint64_t j = coop_find(A.tab, A.tmask, key & ~A.qm.mask, act, &A.stats->err);
+    if (!act) return;
+    if (j == NONE) { atomicOr(&A.stats->err, ERR_MISSING); return; }
+    const uint32_t s = pext_q(key, A.qm);
+    const uint32_t old = atomicOr(&A.occ[j], 1u << s);
+    if (old & (1u << s)) { atomicOr(&A.stats->err, ERR_DUP); return; }
+    C v;
+    v.x = (R)A.re[i];
+    v.y = (R)A.im[i];
+    ((C*)A.amp)[amp_index(j, (int)s, 1 << A.qm.mb)] = v;
+}
+
+
+// ---------------------------------------------------------------------------
+// per-precision launchers (MB = 5 with FP64 is not instantiated: register budget)
+// ---------------------------------------------------------------------------
+#define SQE_MB_SWITCH(mb, ...)                                                     \
+    switch (mb) {                                                                  \
+    case 0: { constexpr int MB_ = 0; __VA_ARGS__; } break;                         \
+    case 1: { constexpr int MB_ = 1; __VA_ARGS__; } break;                         \
+    case 2: { constexpr int MB_ = 2; __VA_ARGS__; } break;                         \
+    case 3: { constexpr int MB_ = 3; __VA_ARGS__; } break;                         \
+    case 4: { constexpr int MB_ = 4; __VA_ARGS__; } break;                         \
+    case 5: { constexpr int MB_ = (sizeof(R) == 8) ? 4 : 5;                        \
+              if (sizeof(R) == 8) return hipErrorInvalidValue; __VA_ARGS__; } break;  \
+    default: return hipErrorInvalidValue;                                          \
+    }
+
+template <class R>
+struct Launch {
+    static hipError_t sweep(int mb, hipStream_t st, uint32_t grid, const SweepArgs& a) {
+        SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_sweep<MB_, R>), dim3(grid), dim3(BLOCK), 0, st, a));
+        return hipGetLastError();
+    }
+    template <int MB>
+    static void pair_kind(uint32_t kind, hipStream_t st, uint32_t grid, const PairArgs& a) {
+        switch (kind) {
+        case KX: hipLaunchKernelGGL((k_pair<MB, R, KX>), dim3(grid), dim3(BLOCK), 0, st, a); break;
+        case KY: hipLaunchKernelGGL((k_pair<MB, R, KY>), dim3(grid), dim3(BLOCK), 0, st, a); break;
+        case KPERM: hipLaunchKernelGGL((k_pair<MB, R, KPERM>), dim3(grid), dim3(BLOCK), 0, st, a); break;
+        case KH: hipLaunchKernelGGL((k_pair<MB, R, KH>), dim3(grid), dim3(BLOCK), 0, st, a); break;
+        case KRX: hipLaunchKernelGGL((k_pair<MB, R, KRX>), dim3(grid), dim3(BLOCK), 0, st, a); break;
+        case KRY: hipLaunchKernelGGL((k_pair<MB, R, KRY>), dim3(grid), dim3(BLOCK), 0, st, a); break;
+        case KU: hipLaunchKernelGGL((k_pair<MB, R, KU>), dim3(grid), dim3(BLOCK), 0, st, a); break;
+        default: break;
+        }
+    }
+    static hipError_t pair(int mb, uint32_t kind, hipStream_t st, uint32_t grid, const PairArgs& a) {
+        SQE_MB_SWITCH(mb, pair_kind<MB_>(kind, st, grid, a));
+        return hipGetLastError();
+    }
+    static hipError_t compact(int mb, hipStream_t st, uint32_t grid, const CompactArgs& a) {
+        SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_compact<MB_, R>), dim3(grid), dim3(BLOCK), 0, st, a));
+        return hipGetLastError();
+    }
+    static hipError_t exportk(int mb, hipStream_t st, uint32_t grid, const ExportArgs& a) {
+        SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_export<MB_, R>), dim3(grid), dim3(BLOCK), 0, st, a));
+        return hipGetLastError();
+    }
+    static hipError_t norm_partial(int mb, hipStream_t st, uint32_t grid, const NormArgs& a) {
+        SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_norm_partial<MB_, R>), dim3(grid), dim3(BLOCK), 0, st, a));
+        return hipGetLastError();
+    }
+    static hipError_t scatter(hipStream_t st, uint32_t grid, const LoadArgs& a) {
+        hipLaunchKernelGGL(k_load_scatter<R>, dim3(grid), dim3(BLOCK), 0, st, a);
+        return hipGetLastError();
+    }
+};
+
+}  // namespace sqe

@@ -1,0 +1,151 @@
+// Non-templated kernels (fills, hash index, load path, norm finish) and the
+// public launchers of the sparse statevector engine (gfx950).
+//
+// Reference path replaced (ivanNovikau/SparQC, emulator/device/):
+//   device_init_arrays / set zero  device_data.cuh:48-59,
+//                                  device_table_FC.cuh:39-49    -> k_fill_*, load path
+#include "sqe_kernels.hpp"
+
+namespace sqe {
+
+extern template struct Launch<double>;
+extern template struct Launch<float>;
+
+// ---------------------------------------------------------------------------
+// fills (used instead of memset so that captured graphs never hold memset nodes)
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u64(uint64_t* __restrict__ p, size_t n, uint64_t v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_slots(Slot* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        Slot s;
+        s.key = EMPTY_KEY;
+        s.val = NONE;
+        p[i] = s;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_norm_final(const double* __restrict__ partials, uint32_t n,
+                                                      double* __restrict__ out_norm2, double* __restrict__ out_scale) {
+    __shared__ double s_w[BLOCK / 64];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) acc += partials[i];
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < BLOCK / 64; ++i) t += s_w[i];
+        *out_norm2 = t;
+        *out_scale = t > 0.0 ? 1.0 / sqrt(t) : 1.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hash index and load path
+// ---------------------------------------------------------------------------
+// (base of bucket b) -> b for every live bucket of a store.
+__global__ __launch_bounds__(BLOCK) void k_index_build(IndexArgs A) {
+    const uint64_t b = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t nb_pad = (*A.nb + 63) & ~63ull;
+    const uint64_t base = b < nb_pad ? A.base[b] : EMPTY_KEY;
+    coop_insert(A.tab, A.tmask, base, b, base != EMPTY_KEY, &A.stats->err);
+}
+
+// Insert the base (key with bucket bits cleared) of every input key.
+__global__ __launch_bounds__(BLOCK) void k_load_insert(LoadArgs A) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const bool act = i < A.n;
+    const uint64_t key = act ? A.keys[i] : EMPTY_KEY;
+    coop_insert(A.tab, A.tmask, key & ~A.qm.mask, NONE, act, &A.stats->err);
+}
+
+// Number the occupied index slots in slot order (scan) -> bucket ids.
+__global__ __launch_bounds__(BLOCK) void k_index_enumerate(EnumArgs A) {
+    const uint32_t tile = take_ticket(A.scan);
+    const uint64_t i = (uint64_t)tile * BLOCK + threadIdx.x;
+    const bool in = i < A.tcap;
+    const uint64_t key = in ? A.tab[i].key : EMPTY_KEY;
+    const uint32_t count = key != EMPTY_KEY ? 1u : 0u;
+    unsigned long long total = 0;
+    const unsigned long long id = block_scan(count, A.scan, tile, gridDim.x, A.out_nb, &total);
+    if (count) {
+        A.tab[i].val = id;
+        if (id < A.out_cap) A.out_base[id] = key; else atomicOr(&A.stats->err, ERR_CAP);
+    }
+    if (tile == gridDim.x - 1) {
+        const uint64_t pad_end = (total + 63) & ~63ull;
+        for (uint64_t p = total + threadIdx.x; p < pad_end && p < A.out_cap; p += BLOCK) A.out_base[p] = EMPTY_KEY;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// public launchers
+// ---------------------------------------------------------------------------
+static inline uint32_t fill_grid(size_t n) {
+    size_t g = (n + BLOCK - 1) / BLOCK;
+    if (g > 4096) g = 4096;
+    if (g == 0) g = 1;
+    return (uint32_t)g;
+}
+
+hipError_t launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u64, dim3(fill_grid(n)), dim3(BLOCK), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_slots(hipStream_t st, Slot* p, size_t n) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_slots, dim3(fill_grid(n)), dim3(BLOCK), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep(int mb, bool f32, hipStream_t st, uint32_t grid, const SweepArgs& a) {
+    if (!grid) return hipSuccess;
+    return f32 ? Launch<float>::sweep(mb, st, grid, a) : Launch<double>::sweep(mb, st, grid, a);
+}
+
+hipError_t launch_pair(int mb, bool f32, uint32_t kind, hipStream_t st, uint32_t grid, const PairArgs& a) {
+    if (!grid) return hipSuccess;
+    if (!(is_perm(kind) || is_branch(kind))) return hipErrorInvalidValue;
+    return f32 ? Launch<float>::pair(mb, kind, st, grid, a) : Launch<double>::pair(mb, kind, st, grid, a);
+}
+
+hipError_t launch_compact(int mb, bool f32, hipStream_t st, uint32_t grid, const CompactArgs& a) {
+    if (!grid) return hipSuccess;
+    return f32 ? Launch<float>::compact(mb, st, grid, a) : Launch<double>::compact(mb, st, grid, a);
+}
+
+hipError_t launch_export(int mb, bool f32, hipStream_t st, uint32_t grid, const ExportArgs& a) {
+    if (!grid) return hipSuccess;
+    return f32 ? Launch<float>::exportk(mb, st, grid, a) : Launch<double>::exportk(mb, st, grid, a);
+}
+
+hipError_t launch_norm(int mb, bool f32, hipStream_t st, uint32_t grid, const NormArgs& a, double* norm2, double* scale) {
+    if (!grid) return hipSuccess;
+    hipError_t r = f32 ? Launch<float>::norm_partial(mb, st, grid, a) : Launch<double>::norm_partial(mb, st, grid, a);
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(BLOCK), 0, st, (const double*)a.partials, grid, norm2, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_index_build(hipStream_t st, uint32_t grid, const IndexArgs& a) {
+    if (!grid) return hipSuccess;
+    hipLaunchKernelGGL(k_index_build, dim3(grid), dim3(BLOCK), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_load(bool f32, hipStream_t st, const LoadArgs& a, const EnumArgs& e, uint32_t enum_grid) {
+    const uint32_t grid = (uint32_t)((a.n + BLOCK - 1) / BLOCK);
+    if (grid) hipLaunchKernelGGL(k_load_insert, dim3(grid), dim3(BLOCK), 0, st, a);
+    hipLaunchKernelGGL(k_index_enumerate, dim3(enum_grid), dim3(BLOCK), 0, st, e);
+    hipError_t r = hipGetLastError();
+    if (r != hipSuccess || !grid) return r;
+    return f32 ? Launch<float>::scatter(st, grid, a) : Launch<double>::scatter(st, grid, a);
+}
+
+}  // namespace sqe

@@ -1,0 +1,114 @@
+// Kernel argument blocks and launchers shared by the engine host code and the
+// kernels (the only interface between sqe_engine.cpp and sqe_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sqe_device.hpp"
+
+namespace sqe {
+
+struct SweepArgs {
+    uint64_t* base;
+    void* amp;
+    const unsigned long long* nb;
+    const DOp* ops;
+    uint32_t nops;
+    int32_t store;
+    double thr;
+    const double* scale;
+    Stats* stats;
+};
+
+struct PairArgs {
+    const uint64_t* in_base;
+    const void* in_amp;
+    const unsigned long long* in_nb;
+    const Slot* tab;
+    uint64_t tmask;
+    DOp op;
+    double thr;
+    uint64_t* out_base;
+    void* out_amp;
+    unsigned long long* out_nb;
+    uint64_t out_cap;
+    ScanCtl scan;
+    Stats* stats;
+};
+
+struct CompactArgs {
+    const uint64_t* in_base;
+    const void* in_amp;
+    const unsigned long long* in_nb;
+    uint64_t* out_base;
+    void* out_amp;
+    unsigned long long* out_nb;
+    uint64_t out_cap;
+    ScanCtl scan;
+    Stats* stats;
+};
+
+struct ExportArgs {
+    const uint64_t* base;
+    const void* amp;
+    const unsigned long long* nb;
+    QMap qm;
+    double thr;
+    int32_t filter;
+    uint64_t* keys;
+    double* re;
+    double* im;
+    uint64_t cap;
+    unsigned long long* total;
+    ScanCtl scan;
+};
+
+struct NormArgs {
+    const void* amp;
+    const unsigned long long* nb;
+    double* partials;
+};
+
+struct IndexArgs {
+    const uint64_t* base;
+    const unsigned long long* nb;
+    Slot* tab;
+    uint64_t tmask;
+    Stats* stats;
+};
+
+struct LoadArgs {
+    const uint64_t* keys;
+    const double* re;
+    const double* im;
+    uint64_t n;
+    QMap qm;
+    Slot* tab;
+    uint64_t tmask;
+    void* amp;
+    unsigned int* occ;
+    Stats* stats;
+};
+
+struct EnumArgs {
+    Slot* tab;
+    uint64_t tcap;
+    ScanCtl scan;
+    uint64_t* out_base;
+    unsigned long long* out_nb;
+    uint64_t out_cap;
+    Stats* stats;
+};
+
+hipError_t launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v);
+hipError_t launch_fill_slots(hipStream_t st, Slot* p, size_t n);
+hipError_t launch_sweep(int mb, bool f32, hipStream_t st, uint32_t grid, const SweepArgs& a);
+hipError_t launch_pair(int mb, bool f32, uint32_t kind, hipStream_t st, uint32_t grid, const PairArgs& a);
+hipError_t launch_compact(int mb, bool f32, hipStream_t st, uint32_t grid, const CompactArgs& a);
+hipError_t launch_export(int mb, bool f32, hipStream_t st, uint32_t grid, const ExportArgs& a);
+hipError_t launch_norm(int mb, bool f32, hipStream_t st, uint32_t grid, const NormArgs& a, double* norm2,
+                       double* scale);
+hipError_t launch_index_build(hipStream_t st, uint32_t grid, const IndexArgs& a);
+hipError_t launch_load(bool f32, hipStream_t st, const LoadArgs& a, const EnumArgs& e, uint32_t enum_grid);
+
+}  // namespace sqe

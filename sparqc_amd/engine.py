@@ -1,0 +1,232 @@
+"""Python binding of the C ABI in include/sqe.h (libsqe.so, built in-tree).
+
+The engine never falls back to a CPU path: if libsqe.so is missing this module
+raises, and every call that needs a GPU reports the engine's own error.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import gates as G
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libsqe.so")
+
+SQE_OK, SQE_EINVAL, SQE_ENOMEM, SQE_ETABLE_FULL, SQE_EHIP, SQE_EDEVICE = 0, -1, -2, -3, -4, -5
+STATUS_NAMES = {0: "SQE_OK", -1: "SQE_EINVAL", -2: "SQE_ENOMEM", -3: "SQE_ETABLE_FULL",
+                -4: "SQE_EHIP", -5: "SQE_EDEVICE"}
+F64, F32 = 0, 1
+
+# every entry point declared in include/sqe.h
+EXPORTS = ["sqe_config_default", "sqe_create", "sqe_destroy", "sqe_set_basis", "sqe_load",
+           "sqe_apply", "sqe_apply_gates", "sqe_nnz", "sqe_norm2", "sqe_export", "sqe_sync",
+           "sqe_get_stats", "sqe_prepare", "sqe_snapshot_save", "sqe_snapshot_restore",
+           "sqe_last_error"]
+
+
+class SqeError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS_NAMES.get(status, status), msg))
+        self.status = status
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("nq", ctypes.c_uint32), ("prec", ctypes.c_uint32), ("prune_thr", ctypes.c_double),
+                ("max_load", ctypes.c_double), ("hbm_budget_bytes", ctypes.c_uint64),
+                ("device", ctypes.c_int32), ("bucket_bits", ctypes.c_int32),
+                ("renorm_every", ctypes.c_uint32), ("fuse_window", ctypes.c_uint32),
+                ("timing", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 7)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("gates", ctypes.c_uint64), ("updates", ctypes.c_uint64), ("nnz", ctypes.c_uint64),
+                ("buckets", ctypes.c_uint64), ("capacity", ctypes.c_uint64),
+                ("bytes_device", ctypes.c_uint64), ("bucket_bits", ctypes.c_int32),
+                ("pad", ctypes.c_uint32), ("bucket_mask", ctypes.c_uint64), ("sweeps", ctypes.c_uint64),
+                ("sweep_ops", ctypes.c_uint64), ("pairs", ctypes.c_uint64), ("remaps", ctypes.c_uint64),
+                ("moved", ctypes.c_uint64), ("sweep_ms", ctypes.c_double), ("pair_ms", ctypes.c_double), ("other_ms", ctypes.c_double)]
+
+
+assert ctypes.sizeof(Config) == 80
+assert ctypes.sizeof(Stats) == 128
+
+_lib = None
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    """Load libsqe.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libsqe.so not built: run `python -m sparqc_amd.build` (%s)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        st = ctypes.c_int
+        L.sqe_config_default.argtypes = [ctypes.POINTER(Config), ctypes.c_uint32]
+        L.sqe_config_default.restype = None
+        L.sqe_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
+        L.sqe_create.restype = st
+        L.sqe_destroy.argtypes = [vp]
+        L.sqe_destroy.restype = st
+        L.sqe_set_basis.argtypes = [vp, ctypes.c_uint64]
+        L.sqe_set_basis.restype = st
+        L.sqe_load.argtypes = [vp, _u64p, _f64p, _f64p, ctypes.c_uint64]
+        L.sqe_load.restype = st
+        L.sqe_apply.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, _f64p, ctypes.c_int]
+        L.sqe_apply.restype = st
+        L.sqe_apply_gates.argtypes = [vp, vp, ctypes.c_uint64]
+        L.sqe_apply_gates.restype = st
+        L.sqe_prepare.argtypes = [vp, vp, ctypes.c_uint64]
+        L.sqe_prepare.restype = st
+        L.sqe_nnz.argtypes = [vp, _u64p]
+        L.sqe_nnz.restype = st
+        L.sqe_norm2.argtypes = [vp, _f64p]
+        L.sqe_norm2.restype = st
+        L.sqe_export.argtypes = [vp, _u64p, _f64p, _f64p, ctypes.c_uint64, _u64p]
+        L.sqe_export.restype = st
+        L.sqe_sync.argtypes = [vp]
+        L.sqe_sync.restype = st
+        L.sqe_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.sqe_get_stats.restype = st
+        L.sqe_snapshot_save.argtypes = [vp]
+        L.sqe_snapshot_save.restype = st
+        L.sqe_snapshot_restore.argtypes = [vp]
+        L.sqe_snapshot_restore.restype = st
+        L.sqe_last_error.argtypes = []
+        L.sqe_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != SQE_OK:
+        raise SqeError(rc, lib().sqe_last_error().decode())
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Engine:
+    """One sparse statevector on one GPU (owns its device memory and stream)."""
+
+    def __init__(self, nq, prec="f64", prune_thr=0.0, bucket_bits=-1, renorm_every=0,
+                 fuse_window=0, timing=False, device=0, hbm_budget_bytes=0, max_load=0.5):
+        L = lib()
+        cfg = Config()
+        L.sqe_config_default(ctypes.byref(cfg), nq)
+        cfg.prec = F32 if prec in ("f32", F32) else F64
+        cfg.prune_thr = prune_thr
+        cfg.bucket_bits = bucket_bits
+        cfg.renorm_every = renorm_every
+        cfg.fuse_window = fuse_window
+        cfg.timing = 1 if timing else 0
+        cfg.device = device
+        cfg.hbm_budget_bytes = hbm_budget_bytes
+        cfg.max_load = max_load
+        self.cfg = cfg
+        self.nq = nq
+        self.prec = "f32" if cfg.prec == F32 else "f64"
+        h = ctypes.c_void_p()
+        _check(L.sqe_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sqe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_basis(self, key=0):
+        _check(lib().sqe_set_basis(self.h, int(key)))
+
+    def load(self, keys, re, im):
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        r = np.ascontiguousarray(re, dtype=np.float64)
+        i = np.ascontiguousarray(im, dtype=np.float64)
+        if not (k.size == r.size == i.size):
+            raise ValueError("keys/re/im sizes differ")
+        _check(lib().sqe_load(self.h, _ptr(k, _u64p), _ptr(r, _f64p), _ptr(i, _f64p), k.size))
+
+    def apply(self, gates):
+        g = G.gate_array(gates)
+        _check(lib().sqe_apply_gates(self.h, g.ctypes.data, g.size))
+
+    def apply_matrix(self, target, u, cls, controls=(), ctrl_val=None):
+        cm = 0
+        for c in controls:
+            cm |= 1 << int(c)
+        cv = cm if ctrl_val is None else int(ctrl_val)
+        m = np.asarray(u, dtype=np.complex128).reshape(2, 2)
+        flat = np.array([m[0, 0].real, m[0, 0].imag, m[0, 1].real, m[0, 1].imag,
+                         m[1, 0].real, m[1, 0].imag, m[1, 1].real, m[1, 1].imag])
+        _check(lib().sqe_apply(self.h, int(target), cm, cv, _ptr(flat, _f64p), int(cls)))
+
+    def prepare(self, gates):
+        g = G.gate_array(gates)
+        _check(lib().sqe_prepare(self.h, g.ctypes.data, g.size))
+
+    def sync(self):
+        _check(lib().sqe_sync(self.h))
+
+    def nnz(self):
+        n = ctypes.c_uint64()
+        _check(lib().sqe_nnz(self.h, ctypes.byref(n)))
+        return n.value
+
+    def norm2(self):
+        x = ctypes.c_double()
+        _check(lib().sqe_norm2(self.h, ctypes.byref(x)))
+        return x.value
+
+    def export(self, sort=True):
+        """Non-negligible entries as (keys, re, im); sorted by key by default."""
+        n = ctypes.c_uint64()
+        rc = lib().sqe_export(self.h, None, None, None, 0, ctypes.byref(n))
+        if rc not in (SQE_OK, SQE_EINVAL):
+            _check(rc)
+        cap = n.value
+        k = np.zeros(cap, dtype=np.uint64)
+        r = np.zeros(cap)
+        i = np.zeros(cap)
+        _check(lib().sqe_export(self.h, _ptr(k, _u64p), _ptr(r, _f64p), _ptr(i, _f64p), cap,
+                                ctypes.byref(n)))
+        m = n.value
+        k, r, i = k[:m], r[:m], i[:m]
+        if sort:
+            o = np.argsort(k, kind="stable")
+            k, r, i = k[o], r[o], i[o]
+        return k, r, i
+
+    def stats(self):
+        s = Stats()
+        _check(lib().sqe_get_stats(self.h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in Stats._fields_ if f != "pad"}
+
+    def snapshot_save(self):
+        _check(lib().sqe_snapshot_save(self.h))
+
+    def snapshot_restore(self):
+        _check(lib().sqe_snapshot_restore(self.h))
+
+
+def run(w, prec="f64", **kw):
+    """Load a workload dict, apply its gates, return the sorted export."""
+    with Engine(w["nq"], prec=prec, **kw) as e:
+        e.load(w["keys"], w["re"], w["im"])
+        e.apply(w["gates"])
+        return e.export()

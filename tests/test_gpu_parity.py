@@ -1,0 +1,189 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle.
+
+Bar: bit-exact key sets; FP64 amplitudes within 1e-12 (the FP64 kernels use the
+oracle's operation order, so they normally agree to the bit); FP32 against the
+FP32 oracle within 1e-6 with key sets equal away from the prune threshold.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from sparqc_amd import engine as E
+from sparqc_amd import gates as G
+from sparqc_amd import gen
+from tests.helpers import assert_same_state, random_gates, random_state
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_run(O, nq, keys, re, im, gates, f32=False, thr=None, renorm_every=0):
+    s = O.Sparse(nq, f32=f32, thr=(1e-7 if f32 else 1e-14) if thr is None else thr,
+                 renorm_every=renorm_every)
+    s.load(keys, re, im)
+    s.apply(gates)
+    return s.export(), s
+
+
+def gpu_run(nq, keys, re, im, gates, **kw):
+    with E.Engine(nq, **kw) as e:
+        e.load(keys, re, im)
+        e.apply(gates)
+        out = e.export()
+        st = e.stats()
+    return out, st
+
+
+@pytest.mark.parametrize("bits", [0, 1, 2, 3, 4])
+def test_single_gates_every_kind(oracle, bits):
+    """Each gate kind, on bucket and base targets, with and without controls."""
+    nq = 9
+    keys, re, im = random_state(11 + bits, nq, 180)
+    for kind in range(11):
+        for t in (0, 4, 8):
+            for ctrls in ((), (2,), (1, 6)):
+                if t in ctrls:
+                    continue
+                g = G.gate_array([G.make_gate(kind, t, ctrls, theta=1.234,
+                                              matrix=([[0.6, 0.8j], [0.8j, 0.6]] if kind == G.U else
+                                                      [np.exp(0.3j), np.exp(-1.1j)]))])
+                say("kind", G.KIND_NAMES[kind], "t", t, "ctrls", ctrls, "bits", bits)
+                want, _ = oracle_run(oracle, nq, keys, re, im, g)
+                got, _ = gpu_run(nq, keys, re, im, g, bucket_bits=bits)
+                assert_same_state(got, want, atol=0.0, label="%s t=%d c=%s" % (G.KIND_NAMES[kind], t, ctrls))
+
+
+@pytest.mark.parametrize("bits,window", [(-1, 0), (0, 0), (1, 3), (2, 1), (3, 0), (4, 7)])
+def test_random_circuits(oracle, bits, window):
+    """Random mixed circuits with random controls (cross-bucket and local paths)."""
+    nq = 12
+    for seed in (1, 2):
+        keys, re, im = random_state(100 + seed, nq, 64)
+        gl = random_gates(200 + seed, nq, 300)
+        say("random circuit seed", seed, "bits", bits, "window", window)
+        want, orc = oracle_run(oracle, nq, keys, re, im, gl)
+        got, st = gpu_run(nq, keys, re, im, gl, bucket_bits=bits, fuse_window=window)
+        assert_same_state(got, want, atol=1e-12, label="seed %d" % seed)
+        assert st["updates"] == orc.updates()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_small_workload_vs_oracle_and_dense(oracle, seed):
+    """20-qubit, 200-gate workload: oracle (bit-exact) and dense FP64 statevector."""
+    w = gen.circuit_small(seed)
+    say("circuit_small seed", seed)
+    want, _ = oracle_run(oracle, w["nq"], w["keys"], w["re"], w["im"], w["gates"])
+    got, st = gpu_run(w["nq"], w["keys"], w["re"], w["im"], w["gates"])
+    assert_same_state(got, want, atol=1e-12, label="circuit_small %d" % seed)
+    dense = oracle.run_dense(w["nq"], w["keys"], w["re"], w["im"], w["gates"])
+    sp = np.zeros(1 << w["nq"], dtype=complex)
+    sp[got[0].astype(np.int64)] = got[1] + 1j * got[2]
+    assert np.max(np.abs(dense - sp)) <= 1e-12
+
+
+def test_large_scaled_24q_vs_dense(oracle):
+    """The 48-qubit workload's gate mix scaled to 24 qubits (dense check possible)."""
+    w = gen.circuit_large(7, nq=24, n_init=1 << 12, n_gates=2000)
+    say("circuit_large nq=24")
+    want, _ = oracle_run(oracle, 24, w["keys"], w["re"], w["im"], w["gates"])
+    got, st = gpu_run(24, w["keys"], w["re"], w["im"], w["gates"])
+    assert_same_state(got, want, atol=1e-12, label="large-24q")
+    dense = oracle.run_dense(24, w["keys"], w["re"], w["im"], w["gates"])
+    sp = np.zeros(1 << 24, dtype=complex)
+    sp[got[0].astype(np.int64)] = got[1] + 1j * got[2]
+    assert np.max(np.abs(dense - sp)) <= 1e-12
+    assert st["pairs"] == 0          # every gate ran inside fused sweeps
+
+
+def test_large_48q_medium(oracle):
+    w = gen.circuit_large(3, nq=48, n_init=1 << 14, n_gates=600)
+    say("circuit_large nq=48 n_init=2^14")
+    want, orc = oracle_run(oracle, 48, w["keys"], w["re"], w["im"], w["gates"])
+    got, st = gpu_run(48, w["keys"], w["re"], w["im"], w["gates"])
+    assert_same_state(got, want, atol=1e-12, label="large-48q")
+
+
+def test_fp32_against_fp32_oracle_and_fp64(oracle):
+    w = gen.circuit_large(5, nq=48, n_init=1 << 12, n_gates=1000)
+    say("fp32 nq=48")
+    (wk, wr, wi), _ = oracle_run(oracle, 48, w["keys"], w["re"], w["im"], w["gates"], f32=True,
+                                 renorm_every=100)
+    gk, gr, gi = gpu_run(48, w["keys"], w["re"], w["im"], w["gates"], prec="f32", renorm_every=100)[0]
+    common, ia, ib = np.intersect1d(gk, wk, return_indices=True)
+    only = np.setxor1d(gk, wk)
+    assert only.size <= max(2, gk.size // 10000)
+    assert np.max(np.abs(gr[ia] - wr[ib])) <= 1e-6
+    assert np.max(np.abs(gi[ia] - wi[ib])) <= 1e-6
+    (dk, dr, di), _ = oracle_run(oracle, 48, w["keys"], w["re"], w["im"], w["gates"])
+    common, ia, ib = np.intersect1d(gk, dk, return_indices=True)
+    ov = np.sum((dr[ib] - 1j * di[ib]) * (gr[ia] + 1j * gi[ia]))
+    infid = 1.0 - abs(ov) ** 2 / (np.sum(dr ** 2 + di ** 2) * np.sum(gr ** 2 + gi ** 2))
+    say("fp32 infidelity", infid)
+    assert infid <= 1e-6
+
+
+def test_all_h_closed_form():
+    """H on every qubit of |0...0> gives 2^(-n/2) on every key (emulator/src/main.cu:142-143)."""
+    nq = 16
+    w = gen.circuit_all_h(nq)
+    k, r, i = gpu_run(nq, w["keys"], w["re"], w["im"], w["gates"])[0]
+    assert k.size == 1 << nq
+    assert np.array_equal(k, np.arange(1 << nq, dtype=np.uint64))
+    assert np.max(np.abs(r - 2.0 ** (-nq / 2))) <= 1e-15 and np.max(np.abs(i)) == 0.0
+
+
+def test_inverse_circuit_returns_initial_state():
+    w = gen.circuit_large(9, nq=48, n_init=1 << 13, n_gates=800)
+    inv = G.gate_array([G.inverse_gate(g) for g in w["gates"][::-1]])
+    with E.Engine(48) as e:
+        e.load(w["keys"], w["re"], w["im"])
+        e.apply(w["gates"])
+        e.apply(inv)
+        k, r, i = e.export()
+    o = np.argsort(w["keys"])
+    assert np.array_equal(k, w["keys"][o])
+    assert np.max(np.abs(r - w["re"][o])) <= 1e-12 and np.max(np.abs(i - w["im"][o])) <= 1e-12
+
+
+def test_snapshot_restore_repeats_bit_exactly():
+    w = gen.circuit_large(4, nq=48, n_init=1 << 12, n_gates=400)
+    with E.Engine(48) as e:
+        e.load(w["keys"], w["re"], w["im"])
+        e.prepare(w["gates"])
+        e.snapshot_save()
+        e.apply(w["gates"])
+        a = e.export()
+        e.snapshot_restore()
+        e.apply(w["gates"])
+        b = e.export()
+    assert_same_state(b, a, atol=0.0)
+
+
+def test_errors_are_reported():
+    with E.Engine(10) as e:
+        with pytest.raises(E.SqeError) as ex:
+            e.apply(G.gate_array([G.make_gate(G.H, 12)]))
+        assert ex.value.status == E.SQE_EINVAL
+        with pytest.raises(E.SqeError) as ex:
+            e.apply(G.gate_array([G.make_gate(G.H, 3)]))
+            e.sync()
+        assert ex.value.status == E.SQE_EINVAL          # no state loaded
+        e.load(np.array([3, 5, 3], dtype=np.uint64), np.ones(3) / 3 ** 0.5, np.zeros(3))
+        e.apply(G.gate_array([G.make_gate(G.H, 1)]))
+        with pytest.raises(E.SqeError) as ex:
+            e.sync()
+        assert ex.value.status == E.SQE_EINVAL          # duplicate key
+    e = E.Engine(20, hbm_budget_bytes=1 << 16)
+    e.load(np.arange(1 << 14, dtype=np.uint64), np.ones(1 << 14) / 128.0, np.zeros(1 << 14))
+    assert e.nnz() == 1 << 14                           # staged on the host, nothing allocated yet
+    with pytest.raises(E.SqeError) as ex:
+        e.apply(G.gate_array([G.make_gate(G.H, 1)]))
+        e.sync()
+    assert ex.value.status == E.SQE_ETABLE_FULL
+    e.close()

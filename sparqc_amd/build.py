@@ -9,6 +9,10 @@ into sparqc_amd/_build/libsqe.so.  Flags:
                       oracle bit for bit; FP32 kernels fuse explicitly.
   -target-feature -packed-fp32-ops
                       no packed FP32 instructions.
+  -simplifycfg-sink-common=false
+                      keeps the per-bucket amplitude arrays in registers: merging
+                      the per-target-bit variants of a gate otherwise leaves
+                      pointer phis into the array and forces it to scratch.
 """
 import concurrent.futures as cf
 import os
@@ -25,6 +29,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-simplifycfg-sink-common=false",
          "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
 SOURCES = ["sqe_kernels_f64.hip", "sqe_kernels_f32.hip", "sqe_kernels_common.hip", "sqe_engine.cpp"]

@@ -56,25 +56,37 @@ struct QMap {           // bucket qubits, increasing
     uint8_t q[8];
 };
 
-// Per-launch statistics with a self-resetting last-block reduction.
+// Engine counters, folded from the per-block slots by k_stats_reduce.
 struct Stats {
     unsigned long long updates;      // sum over gates of nonzeros entering the gate (monotonic)
     unsigned long long moved;        // sum over launches of nonzeros read + nonzeros written (monotonic)
-    unsigned long long acc_nnz;      // accumulators (reset by the last block)
-    unsigned long long acc_empty;
-    unsigned int done;
-    unsigned int pad;
-    unsigned long long nnz;          // nonzero slots after the last launch
-    unsigned long long empty;        // all-zero live buckets after the last launch
-    unsigned long long nb_out;       // buckets written by the last compacting launch
+    unsigned long long nnz;          // nonzero slots after the last counting launch
+    unsigned long long empty;        // all-zero live buckets after the last counting launch
     unsigned long long err;          // sticky error bits
+    unsigned long long pad[3];
+};
+
+// One slot per workgroup: counting kernels write their block's totals here
+// with plain stores (no same-address atomics across thousands of blocks);
+// upd / moved accumulate across launches until k_stats_reduce folds them.
+struct BlockStats {
+    unsigned long long upd, moved, nnz, empty;
 };
 
 enum : unsigned long long { ERR_PROBE = 1, ERR_DUP = 2, ERR_MISSING = 4, ERR_CAP = 8, ERR_SCAN = 16 };
 
-template <class R> struct CT;
-template <> struct CT<double> { using T = double2; };
-template <> struct CT<float> { using T = float2; };
+// Plain complex value (trivially copyable, so register arrays of it are fully
+// scalarised); 8/16-byte aligned for single 64/128-bit memory accesses.
+template <class R>
+struct alignas(2 * sizeof(R)) Cplx {
+    R x, y;
+};
+using cd = Cplx<double>;
+using cf = Cplx<float>;
+__device__ __forceinline__ cd mkd(double a, double b) { return cd{a, b}; }
+__device__ __forceinline__ cf mkf(float a, float b) { return cf{a, b}; }
+
+template <class R> struct CT { using T = Cplx<R>; };
 
 __device__ __forceinline__ uint64_t hmix(uint64_t k) {
     k ^= k >> 33; k *= 0xff51afd7ed558ccdull;
@@ -114,69 +126,69 @@ __device__ __forceinline__ void static_for(F&& f) {
 // v0 / v1 are the amplitudes of the keys with target bit 0 / 1.
 // ---------------------------------------------------------------------------
 template <uint32_t K>
-__device__ __forceinline__ void pair64(const double* p, double2 v0, double2 v1, double2& n0, double2& n1) {
+__device__ __forceinline__ void pair64(const double* p, cd v0, cd v1, cd& n0, cd& n1) {
     if constexpr (K == KX) { n0 = v1; n1 = v0; }
-    else if constexpr (K == KY) { n0 = make_double2(v1.y, -v1.x); n1 = make_double2(-v0.y, v0.x); }
-    else if constexpr (K == KZ) { n0 = v0; n1 = make_double2(-v1.x, -v1.y); }
+    else if constexpr (K == KY) { n0 = mkd(v1.y, -v1.x); n1 = mkd(-v0.y, v0.x); }
+    else if constexpr (K == KZ) { n0 = v0; n1 = mkd(-v1.x, -v1.y); }
     else if constexpr (K == KH) {
         const double f = p[0];
-        n0 = make_double2(f * (v0.x + v1.x), f * (v0.y + v1.y));
-        n1 = make_double2(f * (v0.x - v1.x), f * (v0.y - v1.y));
+        n0 = mkd(f * (v0.x + v1.x), f * (v0.y + v1.y));
+        n1 = mkd(f * (v0.x - v1.x), f * (v0.y - v1.y));
     } else if constexpr (K == KPHASE) {
         const double c = p[0], s = p[1];
         n0 = v0;
-        n1 = make_double2(c * v1.x - s * v1.y, c * v1.y + s * v1.x);
+        n1 = mkd(c * v1.x - s * v1.y, c * v1.y + s * v1.x);
     } else if constexpr (K == KRZ) {
         const double c = p[0], s = p[1];
-        n0 = make_double2(c * v0.x + s * v0.y, c * v0.y - s * v0.x);
-        n1 = make_double2(c * v1.x - s * v1.y, c * v1.y + s * v1.x);
+        n0 = mkd(c * v0.x + s * v0.y, c * v0.y - s * v0.x);
+        n1 = mkd(c * v1.x - s * v1.y, c * v1.y + s * v1.x);
     } else if constexpr (K == KRX) {
         const double c = p[0], s = p[1];
-        n0 = make_double2(c * v0.x + s * v1.y, c * v0.y - s * v1.x);
-        n1 = make_double2(c * v1.x + s * v0.y, c * v1.y - s * v0.x);
+        n0 = mkd(c * v0.x + s * v1.y, c * v0.y - s * v1.x);
+        n1 = mkd(c * v1.x + s * v0.y, c * v1.y - s * v0.x);
     } else if constexpr (K == KRY) {
         const double c = p[0], s = p[1];
-        n0 = make_double2(c * v0.x - s * v1.x, c * v0.y - s * v1.y);
-        n1 = make_double2(c * v1.x + s * v0.x, c * v1.y + s * v0.y);
+        n0 = mkd(c * v0.x - s * v1.x, c * v0.y - s * v1.y);
+        n1 = mkd(c * v1.x + s * v0.x, c * v1.y + s * v0.y);
     } else if constexpr (K == KU) {
-        n0 = make_double2((p[0] * v0.x - p[1] * v0.y) + (p[2] * v1.x - p[3] * v1.y),
+        n0 = mkd((p[0] * v0.x - p[1] * v0.y) + (p[2] * v1.x - p[3] * v1.y),
                           (p[0] * v0.y + p[1] * v0.x) + (p[2] * v1.y + p[3] * v1.x));
-        n1 = make_double2((p[4] * v0.x - p[5] * v0.y) + (p[6] * v1.x - p[7] * v1.y),
+        n1 = mkd((p[4] * v0.x - p[5] * v0.y) + (p[6] * v1.x - p[7] * v1.y),
                           (p[4] * v0.y + p[5] * v0.x) + (p[6] * v1.y + p[7] * v1.x));
     } else if constexpr (K == KDIAG) {
-        n0 = make_double2(p[0] * v0.x - p[1] * v0.y, p[0] * v0.y + p[1] * v0.x);
-        n1 = make_double2(p[2] * v1.x - p[3] * v1.y, p[2] * v1.y + p[3] * v1.x);
+        n0 = mkd(p[0] * v0.x - p[1] * v0.y, p[0] * v0.y + p[1] * v0.x);
+        n1 = mkd(p[2] * v1.x - p[3] * v1.y, p[2] * v1.y + p[3] * v1.x);
     } else if constexpr (K == KPERM) {
-        n0 = make_double2(p[0] * v1.x - p[1] * v1.y, p[0] * v1.y + p[1] * v1.x);
-        n1 = make_double2(p[2] * v0.x - p[3] * v0.y, p[2] * v0.y + p[3] * v0.x);
+        n0 = mkd(p[0] * v1.x - p[1] * v1.y, p[0] * v1.y + p[1] * v1.x);
+        n1 = mkd(p[2] * v0.x - p[3] * v0.y, p[2] * v0.y + p[3] * v0.x);
     } else { n0 = v0; n1 = v1; }
 }
 
 template <uint32_t K>
-__device__ __forceinline__ void pair32(const float* p, float2 v0, float2 v1, float2& n0, float2& n1) {
+__device__ __forceinline__ void pair32(const float* p, cf v0, cf v1, cf& n0, cf& n1) {
     if constexpr (K == KX) { n0 = v1; n1 = v0; }
-    else if constexpr (K == KY) { n0 = make_float2(v1.y, -v1.x); n1 = make_float2(-v0.y, v0.x); }
-    else if constexpr (K == KZ) { n0 = v0; n1 = make_float2(-v1.x, -v1.y); }
+    else if constexpr (K == KY) { n0 = mkf(v1.y, -v1.x); n1 = mkf(-v0.y, v0.x); }
+    else if constexpr (K == KZ) { n0 = v0; n1 = mkf(-v1.x, -v1.y); }
     else if constexpr (K == KH) {
         const float f = p[0];
-        n0 = make_float2(f * (v0.x + v1.x), f * (v0.y + v1.y));
-        n1 = make_float2(f * (v0.x - v1.x), f * (v0.y - v1.y));
+        n0 = mkf(f * (v0.x + v1.x), f * (v0.y + v1.y));
+        n1 = mkf(f * (v0.x - v1.x), f * (v0.y - v1.y));
     } else if constexpr (K == KPHASE) {
         const float c = p[0], s = p[1];
         n0 = v0;
-        n1 = make_float2(__fmaf_rn(c, v1.x, -(s * v1.y)), __fmaf_rn(c, v1.y, s * v1.x));
+        n1 = mkf(__fmaf_rn(c, v1.x, -(s * v1.y)), __fmaf_rn(c, v1.y, s * v1.x));
     } else if constexpr (K == KRZ) {
         const float c = p[0], s = p[1];
-        n0 = make_float2(__fmaf_rn(c, v0.x, s * v0.y), __fmaf_rn(c, v0.y, -(s * v0.x)));
-        n1 = make_float2(__fmaf_rn(c, v1.x, -(s * v1.y)), __fmaf_rn(c, v1.y, s * v1.x));
+        n0 = mkf(__fmaf_rn(c, v0.x, s * v0.y), __fmaf_rn(c, v0.y, -(s * v0.x)));
+        n1 = mkf(__fmaf_rn(c, v1.x, -(s * v1.y)), __fmaf_rn(c, v1.y, s * v1.x));
     } else if constexpr (K == KRX) {
         const float c = p[0], s = p[1];
-        n0 = make_float2(__fmaf_rn(c, v0.x, s * v1.y), __fmaf_rn(c, v0.y, -(s * v1.x)));
-        n1 = make_float2(__fmaf_rn(c, v1.x, s * v0.y), __fmaf_rn(c, v1.y, -(s * v0.x)));
+        n0 = mkf(__fmaf_rn(c, v0.x, s * v1.y), __fmaf_rn(c, v0.y, -(s * v1.x)));
+        n1 = mkf(__fmaf_rn(c, v1.x, s * v0.y), __fmaf_rn(c, v1.y, -(s * v0.x)));
     } else if constexpr (K == KRY) {
         const float c = p[0], s = p[1];
-        n0 = make_float2(__fmaf_rn(c, v0.x, -(s * v1.x)), __fmaf_rn(c, v0.y, -(s * v1.y)));
-        n1 = make_float2(__fmaf_rn(c, v1.x, s * v0.x), __fmaf_rn(c, v1.y, s * v0.y));
+        n0 = mkf(__fmaf_rn(c, v0.x, -(s * v1.x)), __fmaf_rn(c, v0.y, -(s * v1.y)));
+        n1 = mkf(__fmaf_rn(c, v1.x, s * v0.x), __fmaf_rn(c, v1.y, s * v0.y));
     } else if constexpr (K == KU) {
         float t;
         t = p[0] * v0.x; t = __fmaf_rn(-p[1], v0.y, t); t = __fmaf_rn(p[2], v1.x, t); n0.x = __fmaf_rn(-p[3], v1.y, t);
@@ -184,44 +196,44 @@ __device__ __forceinline__ void pair32(const float* p, float2 v0, float2 v1, flo
         t = p[4] * v0.x; t = __fmaf_rn(-p[5], v0.y, t); t = __fmaf_rn(p[6], v1.x, t); n1.x = __fmaf_rn(-p[7], v1.y, t);
         t = p[4] * v0.y; t = __fmaf_rn(p[5], v0.x, t); t = __fmaf_rn(p[6], v1.y, t); n1.y = __fmaf_rn(p[7], v1.x, t);
     } else if constexpr (K == KDIAG) {
-        n0 = make_float2(__fmaf_rn(p[0], v0.x, -(p[1] * v0.y)), __fmaf_rn(p[0], v0.y, p[1] * v0.x));
-        n1 = make_float2(__fmaf_rn(p[2], v1.x, -(p[3] * v1.y)), __fmaf_rn(p[2], v1.y, p[3] * v1.x));
+        n0 = mkf(__fmaf_rn(p[0], v0.x, -(p[1] * v0.y)), __fmaf_rn(p[0], v0.y, p[1] * v0.x));
+        n1 = mkf(__fmaf_rn(p[2], v1.x, -(p[3] * v1.y)), __fmaf_rn(p[2], v1.y, p[3] * v1.x));
     } else if constexpr (K == KPERM) {
-        n0 = make_float2(__fmaf_rn(p[0], v1.x, -(p[1] * v1.y)), __fmaf_rn(p[0], v1.y, p[1] * v1.x));
-        n1 = make_float2(__fmaf_rn(p[2], v0.x, -(p[3] * v0.y)), __fmaf_rn(p[2], v0.y, p[3] * v0.x));
+        n0 = mkf(__fmaf_rn(p[0], v1.x, -(p[1] * v1.y)), __fmaf_rn(p[0], v1.y, p[1] * v1.x));
+        n1 = mkf(__fmaf_rn(p[2], v0.x, -(p[3] * v0.y)), __fmaf_rn(p[2], v0.y, p[3] * v0.x));
     } else { n0 = v0; n1 = v1; }
 }
 
-__device__ __forceinline__ bool negl64(double2 v, double thr) {
+__device__ __forceinline__ bool negl64(cd v, double thr) {
     return fabs(v.x) < thr && fabs(v.y) < thr;
 }
-__device__ __forceinline__ bool nonzero(double2 v) { return v.x != 0.0 || v.y != 0.0; }
-__device__ __forceinline__ bool nonzero(float2 v) { return v.x != 0.0f || v.y != 0.0f; }
+__device__ __forceinline__ bool nonzero(cd v) { return v.x != 0.0 || v.y != 0.0; }
+__device__ __forceinline__ bool nonzero(cf v) { return v.x != 0.0f || v.y != 0.0f; }
 
 // Apply a branching pair and prune its outputs (componentwise |re|,|im| < thr,
 // emulator/include/data_t.h:50-60).  FP32: decisions for outputs with
 // max(|re|,|im|) in [thr/4, 4 thr) are redone in FP64 from the FP32 inputs.
 template <uint32_t K>
-__device__ __forceinline__ void branch_pair(const DOp& op, double thr, double2 v0, double2 v1, double2& n0, double2& n1) {
+__device__ __forceinline__ void branch_pair(const DOp& op, double thr, cd v0, cd v1, cd& n0, cd& n1) {
     pair64<K>(op.p, v0, v1, n0, n1);
-    if (negl64(n0, thr)) n0 = make_double2(0.0, 0.0);
-    if (negl64(n1, thr)) n1 = make_double2(0.0, 0.0);
+    if (negl64(n0, thr)) n0 = mkd(0.0, 0.0);
+    if (negl64(n1, thr)) n1 = mkd(0.0, 0.0);
 }
 template <uint32_t K>
-__device__ __forceinline__ void branch_pair(const DOp& op, double thr, float2 v0, float2 v1, float2& n0, float2& n1) {
+__device__ __forceinline__ void branch_pair(const DOp& op, double thr, cf v0, cf v1, cf& n0, cf& n1) {
     pair32<K>(op.pf, v0, v1, n0, n1);
     const double m0 = fmax(fabs((double)n0.x), fabs((double)n0.y));
     const double m1 = fmax(fabs((double)n1.x), fabs((double)n1.y));
     bool k0 = m0 >= 4.0 * thr, k1 = m1 >= 4.0 * thr;
     const bool near0 = !k0 && m0 >= 0.25 * thr, near1 = !k1 && m1 >= 0.25 * thr;
     if (near0 || near1) {
-        double2 w0 = make_double2(v0.x, v0.y), w1 = make_double2(v1.x, v1.y), d0, d1;
+        cd w0 = mkd(v0.x, v0.y), w1 = mkd(v1.x, v1.y), d0, d1;
         pair64<K>(op.p, w0, w1, d0, d1);
         if (near0) k0 = !negl64(d0, thr);
         if (near1) k1 = !negl64(d1, thr);
     }
-    if (!k0) n0 = make_float2(0.0f, 0.0f);
-    if (!k1) n1 = make_float2(0.0f, 0.0f);
+    if (!k0) n0 = mkf(0.0f, 0.0f);
+    if (!k1) n1 = mkf(0.0f, 0.0f);
 }
 
 template <uint32_t K, class C>
@@ -402,37 +414,33 @@ __device__ __forceinline__ uint32_t take_ticket(ScanCtl sc) {
     return s_tile;
 }
 
-// Wave then block reduction of the counters into Stats accumulators; the last
-// block publishes nnz / empty and resets the accumulators for the next launch.
-// `moved` (entries read + entries written by this thread) feeds the
-// algorithmic-bytes counter of the roofline.
-__device__ __forceinline__ void finish_stats(Stats* st, unsigned long long upd, unsigned long long nnz,
-                                             unsigned long long empty, uint32_t nblocks,
-                                             unsigned long long moved = 0) {
+// Block reduction of the counters into this workgroup's BlockStats slot.
+__device__ __forceinline__ void block_stats(BlockStats* bs, unsigned long long upd, unsigned long long nnz,
+                                            unsigned long long empty, unsigned long long moved = 0) {
+    __shared__ unsigned long long s_red[4][BLOCK / 64];
     for (int d = 32; d >= 1; d >>= 1) {
         upd += __shfl_xor(upd, d);
         nnz += __shfl_xor(nnz, d);
         empty += __shfl_xor(empty, d);
         moved += __shfl_xor(moved, d);
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        if (upd) atomicAdd(&st->updates, upd);
-        if (moved) atomicAdd(&st->moved, moved);
-        if (nnz) atomicAdd(&st->acc_nnz, nnz);
-        if (empty) atomicAdd(&st->acc_empty, empty);
+        s_red[0][w] = upd;
+        s_red[1][w] = nnz;
+        s_red[2][w] = empty;
+        s_red[3][w] = moved;
     }
-    __threadfence();
     __syncthreads();
-    __shared__ bool s_last;
-    if (threadIdx.x == 0) s_last = atomicAdd(&st->done, 1u) == nblocks - 1;
-    __syncthreads();
-    if (s_last && threadIdx.x == 0) {
-        __threadfence();
-        st->nnz = __hip_atomic_load(&st->acc_nnz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        st->empty = __hip_atomic_load(&st->acc_empty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&st->acc_nnz, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&st->acc_empty, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&st->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+        unsigned long long t[4] = {0, 0, 0, 0};
+        for (int k = 0; k < 4; ++k)
+            for (int i = 0; i < BLOCK / 64; ++i) t[k] += s_red[k][i];
+        BlockStats& b = bs[blockIdx.x];
+        b.upd += t[0];
+        b.moved += t[3];
+        b.nnz = t[1];
+        b.empty = t[2];
     }
 }
 

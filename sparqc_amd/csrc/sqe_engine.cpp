@@ -94,6 +94,10 @@ struct sqe_engine {
 
     unsigned long long* meta = nullptr;     // [0],[1] bucket counts of the stores, [2] export total, [3] snapshot count
     Stats* stats = nullptr;
+    BlockStats* bstats = nullptr;           // per-workgroup counter slots
+    uint64_t bstats_cap = 0;
+    uint32_t acc_hi = 0;                    // slots holding unfolded upd / moved
+    uint32_t last_grid = 0;                 // grid of the last counting launch
     double* dscal = nullptr;                // [0] norm2, [1] 1/sqrt(norm2)
     double* partials = nullptr;
     uint64_t partials_cap = 0;
@@ -246,7 +250,39 @@ sqe_status next_scan(sqe_engine* e, uint32_t ntiles, ScanCtl* out) {
     return SQE_OK;
 }
 
+// Make room for `grid` per-workgroup counter slots (folding what they hold first).
+sqe_status prep_bstats(sqe_engine* e, uint32_t grid) {
+    if (grid <= e->bstats_cap) return SQE_OK;
+    if (e->acc_hi) {
+        HIPCK(launch_stats_reduce(e->st, e->bstats, e->acc_hi, 0, e->stats));
+        e->acc_hi = 0;
+    }
+    HIPCK(hipStreamSynchronize(e->st));
+    dfree(e, e->bstats, e->bstats_cap * sizeof(BlockStats));
+    e->bstats = nullptr;
+    e->bstats_cap = 0;
+    e->last_grid = 0;
+    const uint64_t cap = std::max<uint64_t>(4096, pow2_at_least(grid));
+    SQCK(dalloc(e, (void**)&e->bstats, cap * sizeof(BlockStats)));
+    e->bstats_cap = cap;
+    HIPCK(launch_fill_u64(e->st, (uint64_t*)e->bstats, cap * (sizeof(BlockStats) / 8), 0));
+    return SQE_OK;
+}
+
+void note_counting_launch(sqe_engine* e, uint32_t grid) {
+    e->acc_hi = std::max(e->acc_hi, grid);
+    e->last_grid = grid;
+}
+
+sqe_status reduce_stats(sqe_engine* e) {
+    if (!e->acc_hi && !e->last_grid) return SQE_OK;
+    HIPCK(launch_stats_reduce(e->st, e->bstats, e->acc_hi, e->last_grid, e->stats));
+    e->acc_hi = 0;
+    return SQE_OK;
+}
+
 sqe_status read_stats(sqe_engine* e, Stats* h) {
+    SQCK(reduce_stats(e));
     HIPCK(hipMemcpyAsync(h, e->stats, sizeof(Stats), hipMemcpyDeviceToHost, e->st));
     HIPCK(hipStreamSynchronize(e->st));
     return SQE_OK;
@@ -546,8 +582,13 @@ sqe_status run_sweep(sqe_engine* e, const DOp* dops, uint64_t nops, int store) {
     a.base = o.base; a.amp = o.amp; a.nb = &e->meta[e->cur]; a.ops = dops; a.nops = (uint32_t)nops;
     a.store = store; a.thr = e->thr; a.scale = &e->dscal[1]; a.stats = e->stats;
     const uint32_t g = std::max<uint32_t>(1, tiles_of(round64(e->nb_upper)));
-    TimeScope ts(e, 0);
-    HIPCK(launch_sweep(e->qm.mb, e->f32, e->st, g, a));
+    SQCK(prep_bstats(e, g));
+    a.bstats = e->bstats;
+    {
+        TimeScope ts(e, 0);
+        HIPCK(launch_sweep(e->qm.mb, e->f32, e->st, g, a));
+    }
+    note_counting_launch(e, g);
     return SQE_OK;
 }
 
@@ -563,10 +604,13 @@ sqe_status run_pair(sqe_engine* e, const DOp& op) {
     a.out_cap = o.cap; a.stats = e->stats;
     const uint32_t g = std::max<uint32_t>(1, tiles_of(round64(e->nb_upper)));
     SQCK(next_scan(e, g, &a.scan));
+    SQCK(prep_bstats(e, g));
+    a.bstats = e->bstats;
     {
         TimeScope ts(e, 1);
         HIPCK(launch_pair(e->qm.mb, e->f32, op.kind, e->st, g, a));
     }
+    note_counting_launch(e, g);
     e->cur = other;
     e->nb_upper = std::min<uint64_t>(2 * e->nb_upper, o.cap);
     e->tab_valid = false;
@@ -600,10 +644,13 @@ sqe_status compact(sqe_engine* e) {
     a.out_nb = &e->meta[other]; a.out_cap = o.cap; a.stats = e->stats;
     const uint32_t g = std::max<uint32_t>(1, tiles_of(round64(e->nb_upper)));
     SQCK(next_scan(e, g, &a.scan));
+    SQCK(prep_bstats(e, g));
+    a.bstats = e->bstats;
     {
         TimeScope ts(e, 2);
         HIPCK(launch_compact(e->qm.mb, e->f32, e->st, g, a));
     }
+    note_counting_launch(e, g);
     e->cur = other;
     e->tab_valid = false;
     return SQE_OK;
@@ -706,7 +753,7 @@ sqe_status flush(sqe_engine* e) {
         case ST_NORM: SQCK(run_norm(e)); break;
         }
     }
-    return SQE_OK;
+    return reduce_stats(e);
 }
 
 sqe_status enter(sqe_engine* e) {
@@ -743,6 +790,7 @@ void destroy_all(sqe_engine* e) {
     free_store(e, e->snap);
     dfree(e, e->meta, 64);
     dfree(e, e->stats, sizeof(Stats));
+    dfree(e, e->bstats, e->bstats_cap * sizeof(BlockStats));
     dfree(e, e->dscal, 16);
     dfree(e, e->partials, e->partials_cap * 8);
     dfree(e, e->tab, e->tcap * sizeof(Slot));

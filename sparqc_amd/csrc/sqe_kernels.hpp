@@ -19,123 +19,141 @@ namespace sqe {
 // Fused local sweep: one thread owns one bucket, keeps its 2^MB amplitudes in
 // registers and applies a whole run of bucket-local gates before writing back.
 // ---------------------------------------------------------------------------
-template <int MB, class R, uint32_t K, int J>
+// PALL: the gate has no controls on base qubits, so every bucket takes it and
+// no per-lane select is needed.
+template <int MB, class R, uint32_t K, int J, bool PALL>
 __device__ __forceinline__ void slot_gate(const DOp& op, typename CT<R>::T (&a)[1 << MB], bool pred, double thr) {
     using C = typename CT<R>::T;
-    static_for<(1 << MB) / 2>([&](auto qc) {
+    static_for<(1 << MB) / 2>([&](auto qc) __attribute__((always_inline)) {
         constexpr int q = decltype(qc)::value;
         constexpr int s0 = ((q >> J) << (J + 1)) | (q & ((1 << J) - 1));
         constexpr int s1 = s0 | (1 << J);
         if (((uint32_t)s0 & op.cmS) == op.cvS) {
             C n0, n1;
             any_pair<K>(op, thr, a[s0], a[s1], n0, n1);
-            a[s0] = csel(pred, n0, a[s0]);
-            a[s1] = csel(pred, n1, a[s1]);
+            if constexpr (PALL) {
+                a[s0] = n0;
+                a[s1] = n1;
+            } else {
+                a[s0] = csel(pred, n0, a[s0]);
+                a[s1] = csel(pred, n1, a[s1]);
+            }
         }
     });
 }
 
-template <int MB, class R, uint32_t K>
+template <int MB, class R, uint32_t K, bool PALL>
 __device__ __forceinline__ void slot_dispatch_j(const DOp& op, typename CT<R>::T (&a)[1 << MB], bool pred, double thr) {
     switch (op.tslot) {
-    case 0: if constexpr (MB > 0) slot_gate<MB, R, K, 0>(op, a, pred, thr); break;
-    case 1: if constexpr (MB > 1) slot_gate<MB, R, K, 1>(op, a, pred, thr); break;
-    case 2: if constexpr (MB > 2) slot_gate<MB, R, K, 2>(op, a, pred, thr); break;
-    case 3: if constexpr (MB > 3) slot_gate<MB, R, K, 3>(op, a, pred, thr); break;
-    case 4: if constexpr (MB > 4) slot_gate<MB, R, K, 4>(op, a, pred, thr); break;
+    case 0: if constexpr (MB > 0) slot_gate<MB, R, K, 0, PALL>(op, a, pred, thr); break;
+    case 1: if constexpr (MB > 1) slot_gate<MB, R, K, 1, PALL>(op, a, pred, thr); break;
+    case 2: if constexpr (MB > 2) slot_gate<MB, R, K, 2, PALL>(op, a, pred, thr); break;
+    case 3: if constexpr (MB > 3) slot_gate<MB, R, K, 3, PALL>(op, a, pred, thr); break;
+    case 4: if constexpr (MB > 4) slot_gate<MB, R, K, 4, PALL>(op, a, pred, thr); break;
     default: break;
     }
 }
 
-template <int MB, class R>
+template <int MB, class R, bool PALL>
 __device__ __forceinline__ void slot_op(const DOp& op, typename CT<R>::T (&a)[1 << MB], bool pred, double thr) {
     switch (op.kind) {
-    case KX: slot_dispatch_j<MB, R, KX>(op, a, pred, thr); break;
-    case KY: slot_dispatch_j<MB, R, KY>(op, a, pred, thr); break;
-    case KZ: slot_dispatch_j<MB, R, KZ>(op, a, pred, thr); break;
-    case KH: slot_dispatch_j<MB, R, KH>(op, a, pred, thr); break;
-    case KPHASE: slot_dispatch_j<MB, R, KPHASE>(op, a, pred, thr); break;
-    case KRZ: slot_dispatch_j<MB, R, KRZ>(op, a, pred, thr); break;
-    case KRX: slot_dispatch_j<MB, R, KRX>(op, a, pred, thr); break;
-    case KRY: slot_dispatch_j<MB, R, KRY>(op, a, pred, thr); break;
-    case KU: slot_dispatch_j<MB, R, KU>(op, a, pred, thr); break;
-    case KDIAG: slot_dispatch_j<MB, R, KDIAG>(op, a, pred, thr); break;
-    case KPERM: slot_dispatch_j<MB, R, KPERM>(op, a, pred, thr); break;
+    case KX: slot_dispatch_j<MB, R, KX, PALL>(op, a, pred, thr); break;
+    case KY: slot_dispatch_j<MB, R, KY, PALL>(op, a, pred, thr); break;
+    case KZ: slot_dispatch_j<MB, R, KZ, PALL>(op, a, pred, thr); break;
+    case KH: slot_dispatch_j<MB, R, KH, PALL>(op, a, pred, thr); break;
+    case KPHASE: slot_dispatch_j<MB, R, KPHASE, PALL>(op, a, pred, thr); break;
+    case KRZ: slot_dispatch_j<MB, R, KRZ, PALL>(op, a, pred, thr); break;
+    case KRX: slot_dispatch_j<MB, R, KRX, PALL>(op, a, pred, thr); break;
+    case KRY: slot_dispatch_j<MB, R, KRY, PALL>(op, a, pred, thr); break;
+    case KU: slot_dispatch_j<MB, R, KU, PALL>(op, a, pred, thr); break;
+    case KDIAG: slot_dispatch_j<MB, R, KDIAG, PALL>(op, a, pred, thr); break;
+    case KPERM: slot_dispatch_j<MB, R, KPERM, PALL>(op, a, pred, thr); break;
     default: break;
     }
 }
 
 // Complex rotation by (c, s): (c x - s y, c y + s x), the reference's phase
 // formula (emulator/device/device_data.cuh:221-224); FP32 with fused multiply-add.
-__device__ __forceinline__ double2 rot(double c, double s, double2 v) {
-    return make_double2(c * v.x - s * v.y, c * v.y + s * v.x);
+__device__ __forceinline__ cd rot(double c, double s, cd v) {
+    return mkd(c * v.x - s * v.y, c * v.y + s * v.x);
 }
-__device__ __forceinline__ float2 rot(float c, float s, float2 v) {
-    return make_float2(__fmaf_rn(c, v.x, -(s * v.y)), __fmaf_rn(c, v.y, s * v.x));
+__device__ __forceinline__ cf rot(float c, float s, cf v) {
+    return mkf(__fmaf_rn(c, v.x, -(s * v.y)), __fmaf_rn(c, v.y, s * v.x));
 }
 
 // Gate whose target is a base (non-bucket) qubit: diagonal gates scale the
-// whole bucket by the factor of its base bit; X / Y move the whole bucket to
-// base ^ (1 << target).  The planner never sends a branching gate, or a
-// permutation with slot controls, down this path.
-template <int MB, class R>
-__device__ __forceinline__ void base_op(const DOp& op, typename CT<R>::T (&a)[1 << MB], uint64_t& base, bool pred) {
+// whole bucket by the factor of its base bit; X / Y / PERM move the whole
+// bucket to base ^ (1 << target).  The planner never sends a branching gate,
+// or a permutation with slot controls, down this path.  PALL: no base
+// controls (every live bucket takes the gate; padding lanes hold zeros).
+template <int MB, class R, bool PALL>
+__device__ __forceinline__ void base_op(const DOp& op, typename CT<R>::T (&a)[1 << MB], uint64_t& base, bool pred,
+                                        bool valid) {
     using C = typename CT<R>::T;
     constexpr int S = 1 << MB;
     const bool b = (base & op.tbase) != 0;
     const bool f64 = sizeof(R) == 8;
+    auto coef = [&](int k) __attribute__((always_inline)) { return f64 ? (R)op.p[k] : (R)op.pf[k]; };
+    auto put = [&](C& dst, C v) __attribute__((always_inline)) {
+        if constexpr (PALL) dst = v; else dst = csel(pred, v, dst);
+    };
     switch (op.kind) {
     case KZ:
-        static_for<S>([&](auto sc) {
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
-            if (((uint32_t)s & op.cmS) == op.cvS && pred && b) { a[s].x = -a[s].x; a[s].y = -a[s].y; }
+            if (((uint32_t)s & op.cmS) == op.cvS) {
+                C v = a[s];
+                C w;
+                w.x = b ? -v.x : v.x;
+                w.y = b ? -v.y : v.y;
+                put(a[s], w);
+            }
         });
         break;
     case KPHASE: {
-        const R c = f64 ? (R)op.p[0] : (R)op.pf[0], sn = f64 ? (R)op.p[1] : (R)op.pf[1];
-        static_for<S>([&](auto sc) {
+        // bit 1 rotates by (c, s); bit 0 keeps the value (selected per lane)
+        const R c = coef(0), sn = coef(1);
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
-            if (((uint32_t)s & op.cmS) == op.cvS) a[s] = csel(pred && b, rot(c, sn, a[s]), a[s]);
+            if (((uint32_t)s & op.cmS) == op.cvS) put(a[s], csel(b, rot(c, sn, a[s]), a[s]));
         });
     } break;
     case KRZ: {
-        const R c = f64 ? (R)op.p[0] : (R)op.pf[0], s0 = f64 ? (R)op.p[1] : (R)op.pf[1];
+        const R c = coef(0), s0 = coef(1);
         const R sn = b ? s0 : -s0;      // bit 0: e^{-i theta/2}, bit 1: e^{+i theta/2}
-        static_for<S>([&](auto sc) {
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
-            if (((uint32_t)s & op.cmS) == op.cvS) a[s] = csel(pred, rot(c, sn, a[s]), a[s]);
+            if (((uint32_t)s & op.cmS) == op.cvS) put(a[s], rot(c, sn, a[s]));
         });
     } break;
     case KDIAG: {
-        const int o = b ? 2 : 0;
-        const R dr = f64 ? (R)op.p[o] : (R)op.pf[o], di = f64 ? (R)op.p[o + 1] : (R)op.pf[o + 1];
-        static_for<S>([&](auto sc) {
+        const R dr = b ? coef(2) : coef(0), di = b ? coef(3) : coef(1);
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
-            if (((uint32_t)s & op.cmS) == op.cvS) a[s] = csel(pred, rot(dr, di, a[s]), a[s]);
+            if (((uint32_t)s & op.cmS) == op.cvS) put(a[s], rot(dr, di, a[s]));
         });
     } break;
     case KX:
-        if (pred) base ^= op.tbase;
+        if (valid && pred) base ^= op.tbase;
         break;
     case KY:
-        static_for<S>([&](auto sc) {
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
             C v = a[s];
             C w;
             if (b) { w.x = v.y; w.y = -v.x; } else { w.x = -v.y; w.y = v.x; }
-            a[s] = csel(pred, w, v);
+            put(a[s], w);
         });
-        if (pred) base ^= op.tbase;
+        if (valid && pred) base ^= op.tbase;
         break;
     case KPERM: {
         // bit 0 moves to bit 1 with factor u10, bit 1 moves to bit 0 with factor u01
-        const int o = b ? 0 : 2;
-        const R dr = f64 ? (R)op.p[o] : (R)op.pf[o], di = f64 ? (R)op.p[o + 1] : (R)op.pf[o + 1];
-        static_for<S>([&](auto sc) {
+        const R dr = b ? coef(0) : coef(2), di = b ? coef(1) : coef(3);
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
-            a[s] = csel(pred, rot(dr, di, a[s]), a[s]);
+            put(a[s], rot(dr, di, a[s]));
         });
-        if (pred) base ^= op.tbase;
+        if (valid && pred) base ^= op.tbase;
     } break;
     default:
         break;
@@ -145,17 +163,57 @@ __device__ __forceinline__ void base_op(const DOp& op, typename CT<R>::T (&a)[1 
 template <int MB, class R>
 __device__ __forceinline__ uint32_t count_nz(const typename CT<R>::T (&a)[1 << MB]) {
     uint32_t c = 0;
-    static_for<(1 << MB)>([&](auto sc) {
+    static_for<(1 << MB)>([&](auto sc) __attribute__((always_inline)) {
         constexpr int s = decltype(sc)::value;
         c += nonzero(a[s]) ? 1u : 0u;
     });
     return c;
 }
 
+// FP32 form: a gate on a base qubit that acts on every slot of a bucket alike
+// (diagonal, Y, PERM without slot controls) only multiplies the bucket by one
+// complex factor.  The FP32 sweep keeps that factor per bucket in FP64 and
+// applies it to the slots lazily -- before a branching gate (whose prune
+// decision needs the actual values) and at the end of the sweep -- instead of
+// rescaling every slot per gate.  Returns false when `op` is not such a gate.
+__device__ __forceinline__ bool fold_phase(const DOp& op, uint64_t& base, bool pred, bool valid, double& pr,
+                                           double& pi) {
+    if (op.tslot >= 0 || op.cmS != 0) return false;
+    const bool b = (base & op.tbase) != 0;
+    double fr, fi;
+    switch (op.kind) {
+    case KZ: fr = b ? -1.0 : 1.0; fi = 0.0; break;
+    case KPHASE: fr = b ? op.p[0] : 1.0; fi = b ? op.p[1] : 0.0; break;
+    case KRZ: fr = op.p[0]; fi = b ? op.p[1] : -op.p[1]; break;
+    case KDIAG: fr = b ? op.p[2] : op.p[0]; fi = b ? op.p[3] : op.p[1]; break;
+    case KY: fr = 0.0; fi = b ? -1.0 : 1.0; break;          // bit 0 -> i v, bit 1 -> -i v
+    case KPERM: fr = b ? op.p[0] : op.p[2]; fi = b ? op.p[1] : op.p[3]; break;
+    default: return false;
+    }
+    if (pred) {
+        const double r = pr * fr - pi * fi, i = pr * fi + pi * fr;
+        pr = r;
+        pi = i;
+        if (valid && is_perm(op.kind)) base ^= op.tbase;
+    }
+    return true;
+}
+
+template <int MB>
+__device__ __forceinline__ void apply_phase(Cplx<float> (&a)[1 << MB], double pr, double pi) {
+    static_for<(1 << MB)>([&](auto sc) __attribute__((always_inline)) {
+        constexpr int s = decltype(sc)::value;
+        const double x = a[s].x, y = a[s].y;
+        a[s].x = (float)(pr * x - pi * y);
+        a[s].y = (float)(pr * y + pi * x);
+    });
+}
+
 template <int MB, class R>
 __global__ __launch_bounds__(BLOCK) void k_sweep(SweepArgs A) {
     using C = typename CT<R>::T;
     constexpr int S = 1 << MB;
+    constexpr bool F32 = sizeof(R) == 4;
     const uint64_t b = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     const uint64_t nb = *A.nb;
     const uint64_t nb_pad = (nb + 63) & ~63ull;
@@ -165,12 +223,12 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(SweepArgs A) {
     uint64_t base = EMPTY_KEY;
     if (live) {
         base = A.base[b];
-        static_for<S>([&](auto sc) {
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
             a[s] = amp[amp_index(b, s, S)];
         });
     } else {
-        static_for<S>([&](auto sc) {
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
             a[s].x = 0; a[s].y = 0;
         });
@@ -180,39 +238,67 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(SweepArgs A) {
     const uint32_t cnt_in = cnt;
     unsigned long long upd = 0;
     const uint64_t base0 = base;
+    double pr = 1.0, pi = 0.0;        // FP32 form: pending per-bucket factor
+    bool pending = false;             // wavefront-uniform: a factor is pending
     if (live) {
         for (uint32_t i = 0; i < A.nops; ++i) {
             const DOp& op = A.ops[i];
             if (op.kind == K_RENORM) {
                 const double f = *A.scale;
-                static_for<S>([&](auto sc) {
-                    constexpr int s = decltype(sc)::value;
-                    a[s].x = (R)((double)a[s].x * f);
-                    a[s].y = (R)((double)a[s].y * f);
-                });
+                if constexpr (F32) {
+                    pr *= f;
+                    pi *= f;
+                    pending = true;
+                } else {
+                    static_for<S>([&](auto sc) __attribute__((always_inline)) {
+                        constexpr int s = decltype(sc)::value;
+                        a[s].x = (R)((double)a[s].x * f);
+                        a[s].y = (R)((double)a[s].y * f);
+                    });
+                }
                 continue;
             }
             upd += cnt;
             const bool pred = valid && ((base & op.cmB) == op.cvB);
+            if constexpr (F32) {
+                if (fold_phase(op, base, op.cmB == 0 || pred, valid, pr, pi)) {
+                    pending = true;
+                    continue;
+                }
+                if (pending && op.tslot >= 0 && is_branch(op.kind)) {
+                    apply_phase<MB>(a, pr, pi);
+                    pr = 1.0;
+                    pi = 0.0;
+                    pending = false;
+                }
+            }
             if (op.tslot >= 0) {
                 if constexpr (MB > 0) {
-                    slot_op<MB, R>(op, a, pred, A.thr);
+                    // padding lanes hold zeros, which every gate maps to zeros, so a
+                    // gate without base controls runs unpredicated on all lanes
+                    if (op.cmB == 0) slot_op<MB, R, true>(op, a, true, A.thr);
+                    else slot_op<MB, R, false>(op, a, pred, A.thr);
                     if (is_branch(op.kind)) cnt = valid ? count_nz<MB, R>(a) : 0u;
                 }
+            } else if (op.cmB == 0) {
+                base_op<MB, R, true>(op, a, base, true, valid);
             } else {
-                base_op<MB, R>(op, a, base, valid && pred);
+                base_op<MB, R, false>(op, a, base, pred, valid);
             }
         }
+        if constexpr (F32) {
+            if (pending) apply_phase<MB>(a, pr, pi);
+        }
         if (A.store) {
-            static_for<S>([&](auto sc) {
+            static_for<S>([&](auto sc) __attribute__((always_inline)) {
                 constexpr int s = decltype(sc)::value;
                 amp[amp_index(b, s, S)] = a[s];
             });
             if (valid && base != base0) A.base[b] = base;
         }
     }
-    finish_stats(A.stats, upd, cnt, (valid && cnt == 0) ? 1ull : 0ull, gridDim.x,
-                 A.store ? (unsigned long long)cnt_in + cnt : 0ull);
+    block_stats(A.bstats, upd, cnt, (valid && cnt == 0) ? 1ull : 0ull,
+                A.store ? (unsigned long long)cnt_in + cnt : 0ull);
 }
 
 // ---------------------------------------------------------------------------
@@ -245,19 +331,19 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     const bool owner = ctrl_ok && !(bit && j != NONE);
 
     C own[S], par[S];
-    static_for<S>([&](auto sc) {
+    static_for<S>([&](auto sc) __attribute__((always_inline)) {
         constexpr int s = decltype(sc)::value;
         own[s].x = 0; own[s].y = 0; par[s].x = 0; par[s].y = 0;
     });
     if (valid) {
-        static_for<S>([&](auto sc) {
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
             own[s] = in_amp[amp_index(b, s, S)];
         });
     }
     const uint32_t nz_in = valid ? count_nz<MB, R>(own) : 0u;
     if (owner && j != NONE) {
-        static_for<S>([&](auto sc) {
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
             par[s] = in_amp[amp_index(j, s, S)];
         });
@@ -266,7 +352,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
     bool lo_nz = false, hi_nz = false;
     if (owner) {
         // own/par -> lo (target bit 0) / hi (target bit 1), in place
-        static_for<S>([&](auto sc) {
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
             const C x = own[s], y = par[s];
             C v0 = bit ? y : x, v1 = bit ? x : y;
@@ -291,7 +377,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             if (lo_nz) {
                 if (off < A.out_cap) {
                     A.out_base[off] = base & ~op.tbase;
-                    static_for<S>([&](auto sc) {
+                    static_for<S>([&](auto sc) __attribute__((always_inline)) {
                         constexpr int s = decltype(sc)::value;
                         out_amp[amp_index(off, s, S)] = own[s];
                     });
@@ -300,7 +386,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             }
             if (hi_nz && off < A.out_cap) {
                 A.out_base[off] = base | op.tbase;
-                static_for<S>([&](auto sc) {
+                static_for<S>([&](auto sc) __attribute__((always_inline)) {
                     constexpr int s = decltype(sc)::value;
                     out_amp[amp_index(off, s, S)] = par[s];
                 });
@@ -308,7 +394,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             nz_out = (lo_nz ? count_nz<MB, R>(own) : 0u) + (hi_nz ? count_nz<MB, R>(par) : 0u);
         } else if (off < A.out_cap) {
             A.out_base[off] = base;
-            static_for<S>([&](auto sc) {
+            static_for<S>([&](auto sc) __attribute__((always_inline)) {
                 constexpr int s = decltype(sc)::value;
                 out_amp[amp_index(off, s, S)] = own[s];
             });
@@ -323,7 +409,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             const uint64_t pad_end = (total + 63) & ~63ull;
             for (uint64_t p = total + threadIdx.x; p < pad_end && p < A.out_cap; p += BLOCK) {
                 A.out_base[p] = EMPTY_KEY;
-                static_for<S>([&](auto sc) {
+                static_for<S>([&](auto sc) __attribute__((always_inline)) {
                     constexpr int s = decltype(sc)::value;
                     C z; z.x = 0; z.y = 0;
                     out_amp[amp_index(p, s, S)] = z;
@@ -331,7 +417,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             }
         }
     }
-    finish_stats(A.stats, nz_in, nz_out, 0ull, gridDim.x, (unsigned long long)nz_in + nz_out);
+    block_stats(A.bstats, nz_in, nz_out, 0ull, (unsigned long long)nz_in + nz_out);
     (void)nb;
 }
 
@@ -349,7 +435,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact(CompactArgs A) {
     const uint64_t base = live ? A.in_base[b] : EMPTY_KEY;
     const bool valid = base != EMPTY_KEY;
     C a[S];
-    static_for<S>([&](auto sc) {
+    static_for<S>([&](auto sc) __attribute__((always_inline)) {
         constexpr int s = decltype(sc)::value;
         if (valid) a[s] = in_amp[amp_index(b, s, S)]; else { a[s].x = 0; a[s].y = 0; }
     });
@@ -359,7 +445,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact(CompactArgs A) {
     const unsigned long long off = block_scan(count, A.scan, tile, gridDim.x, A.out_nb, &total);
     if (count && off < A.out_cap) {
         A.out_base[off] = base;
-        static_for<S>([&](auto sc) {
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
             out_amp[amp_index(off, s, S)] = a[s];
         });
@@ -368,14 +454,14 @@ __global__ __launch_bounds__(BLOCK) void k_compact(CompactArgs A) {
         const uint64_t pad_end = (total + 63) & ~63ull;
         for (uint64_t p = total + threadIdx.x; p < pad_end && p < A.out_cap; p += BLOCK) {
             A.out_base[p] = EMPTY_KEY;
-            static_for<S>([&](auto sc) {
+            static_for<S>([&](auto sc) __attribute__((always_inline)) {
                 constexpr int s = decltype(sc)::value;
                 C z; z.x = 0; z.y = 0;
                 out_amp[amp_index(p, s, S)] = z;
             });
         }
     }
-    finish_stats(A.stats, 0ull, count ? nz : 0u, 0ull, gridDim.x);
+    block_stats(A.bstats, 0ull, count ? nz : 0u, 0ull);
 }
 
 // Export non-negligible (or all nonzero) entries as FP64 (key, re, im) arrays
@@ -393,17 +479,17 @@ __global__ __launch_bounds__(BLOCK) void k_export(ExportArgs A) {
     const bool valid = base != EMPTY_KEY;
     C a[S];
     uint32_t keep = 0, count = 0;
-    static_for<S>([&](auto sc) {
+    static_for<S>([&](auto sc) __attribute__((always_inline)) {
         constexpr int s = decltype(sc)::value;
         if (valid) a[s] = amp[amp_index(b, s, S)]; else { a[s].x = 0; a[s].y = 0; }
-        const double2 v = make_double2((double)a[s].x, (double)a[s].y);
+        const cd v = mkd((double)a[s].x, (double)a[s].y);
         const bool k = A.filter ? !negl64(v, A.thr) : nonzero(v);
         if (valid && k) { keep |= 1u << s; ++count; }
     });
     unsigned long long total = 0;
     unsigned long long off = block_scan(count, A.scan, tile, gridDim.x, A.total, &total);
     if (count) {
-        static_for<S>([&](auto sc) {
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
             if ((keep >> s) & 1u) {
                 if (off < A.cap) {
@@ -428,7 +514,7 @@ __global__ __launch_bounds__(BLOCK) void k_norm_partial(NormArgs A) {
     const C* __restrict__ amp = (const C*)A.amp;
     double acc = 0.0;
     if (b < nb_pad) {
-        static_for<S>([&](auto sc) {
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
             const C v = amp[amp_index(b, s, S)];
             acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;

@@ -82,6 +82,49 @@ __global__ __launch_bounds__(BLOCK) void k_index_enumerate(EnumArgs A) {
     }
 }
 
+// Fold the per-block slots: upd / moved over every slot used since the last
+// fold (then cleared), nnz / empty over the slots of the last counting launch.
+__global__ __launch_bounds__(BLOCK) void k_stats_reduce(BlockStats* __restrict__ bs, uint32_t n_acc,
+                                                        uint32_t n_last, Stats* __restrict__ st) {
+    __shared__ unsigned long long s_red[4][BLOCK / 64];
+    unsigned long long u = 0, m = 0, z = 0, e = 0;
+    for (uint32_t i = threadIdx.x; i < n_acc; i += BLOCK) {
+        u += bs[i].upd;
+        m += bs[i].moved;
+        bs[i].upd = 0;
+        bs[i].moved = 0;
+    }
+    for (uint32_t i = threadIdx.x; i < n_last; i += BLOCK) {
+        z += bs[i].nnz;
+        e += bs[i].empty;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        u += __shfl_xor(u, d);
+        m += __shfl_xor(m, d);
+        z += __shfl_xor(z, d);
+        e += __shfl_xor(e, d);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_red[0][w] = u;
+        s_red[1][w] = m;
+        s_red[2][w] = z;
+        s_red[3][w] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t[4] = {0, 0, 0, 0};
+        for (int k = 0; k < 4; ++k)
+            for (int i = 0; i < BLOCK / 64; ++i) t[k] += s_red[k][i];
+        st->updates += t[0];
+        st->moved += t[1];
+        if (n_last) {
+            st->nnz = t[2];
+            st->empty = t[3];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // public launchers
 // ---------------------------------------------------------------------------
@@ -130,6 +173,11 @@ hipError_t launch_norm(int mb, bool f32, hipStream_t st, uint32_t grid, const No
     hipError_t r = f32 ? Launch<float>::norm_partial(mb, st, grid, a) : Launch<double>::norm_partial(mb, st, grid, a);
     if (r != hipSuccess) return r;
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(BLOCK), 0, st, (const double*)a.partials, grid, norm2, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_stats_reduce(hipStream_t st, BlockStats* bs, uint32_t n_acc, uint32_t n_last, Stats* stats) {
+    hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(BLOCK), 0, st, bs, n_acc, n_last, stats);
     return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@ struct SweepArgs {
     double thr;
     const double* scale;
     Stats* stats;
+    BlockStats* bstats;
 };
 
 struct PairArgs {
@@ -34,6 +35,7 @@ struct PairArgs {
     uint64_t out_cap;
     ScanCtl scan;
     Stats* stats;
+    BlockStats* bstats;
 };
 
 struct CompactArgs {
@@ -46,6 +48,7 @@ struct CompactArgs {
     uint64_t out_cap;
     ScanCtl scan;
     Stats* stats;
+    BlockStats* bstats;
 };
 
 struct ExportArgs {
@@ -109,6 +112,7 @@ hipError_t launch_export(int mb, bool f32, hipStream_t st, uint32_t grid, const 
 hipError_t launch_norm(int mb, bool f32, hipStream_t st, uint32_t grid, const NormArgs& a, double* norm2,
                        double* scale);
 hipError_t launch_index_build(hipStream_t st, uint32_t grid, const IndexArgs& a);
+hipError_t launch_stats_reduce(hipStream_t st, BlockStats* bs, uint32_t n_acc, uint32_t n_last, Stats* stats);
 hipError_t launch_load(bool f32, hipStream_t st, const LoadArgs& a, const EnumArgs& e, uint32_t enum_grid);
 
 }  // namespace sqe

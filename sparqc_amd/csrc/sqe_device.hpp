@@ -42,8 +42,39 @@ struct DOp {
     uint64_t cmB, cvB;  // controls on base bits: (base & cmB) == cvB
     uint32_t cmS, cvS;  // controls on slot bits: (s & cmS) == cvS
     double p[8];        // FP64 coefficients (host computed)
-    float pf[8];        // the same rounded to FP32
 };
+
+// Compact form of a DOp in a sweep program (40 bytes, read through the scalar
+// cache): hdr = kind | (tslot + 1) << 5 | target << 8 | coefficient-table
+// index << 14 (NO_COEF: only c0, c1 are used); ctl = cmS | cvS << 8.
+struct MOp {
+    uint32_t hdr;
+    uint32_t ctl;
+    uint64_t cmB, cvB;
+    double c0, c1;
+};
+constexpr uint32_t NO_COEF = 0x3ffffu;
+static_assert(sizeof(MOp) == 40, "MOp is ten dwords");
+
+__device__ __forceinline__ DOp unpack_op(const MOp& m, const double* __restrict__ coef) {
+    DOp d;
+    d.kind = m.hdr & 31u;
+    d.tslot = (int32_t)((m.hdr >> 5) & 7u) - 1;
+    d.tbase = d.tslot < 0 ? (1ull << ((m.hdr >> 8) & 63u)) : 0ull;
+    d.cmB = m.cmB;
+    d.cvB = m.cvB;
+    d.cmS = m.ctl & 0xffu;
+    d.cvS = (m.ctl >> 8) & 0xffu;
+    const uint32_t ci = m.hdr >> 14;
+    if (ci != NO_COEF) {
+        for (int k = 0; k < 8; ++k) d.p[k] = coef[ci * 8 + k];
+    } else {
+        d.p[0] = m.c0;
+        d.p[1] = m.c1;
+        for (int k = 2; k < 8; ++k) d.p[k] = 0.0;
+    }
+    return d;
+}
 
 struct Slot {           // hash index slot: 16 B, 8 slots = one 128 B line
     uint64_t key;
@@ -165,7 +196,10 @@ __device__ __forceinline__ void pair64(const double* p, cd v0, cd v1, cd& n0, cd
 }
 
 template <uint32_t K>
-__device__ __forceinline__ void pair32(const float* p, cf v0, cf v1, cf& n0, cf& n1) {
+__device__ __forceinline__ void pair32(const double* pd, cf v0, cf v1, cf& n0, cf& n1) {
+    float p[8];
+    constexpr int NP = (K == KU) ? 8 : (K == KDIAG || K == KPERM) ? 4 : 2;
+    for (int k = 0; k < 8; ++k) p[k] = k < NP ? (float)pd[k] : 0.0f;
     if constexpr (K == KX) { n0 = v1; n1 = v0; }
     else if constexpr (K == KY) { n0 = mkf(v1.y, -v1.x); n1 = mkf(-v0.y, v0.x); }
     else if constexpr (K == KZ) { n0 = v0; n1 = mkf(-v1.x, -v1.y); }
@@ -221,7 +255,7 @@ __device__ __forceinline__ void branch_pair(const DOp& op, double thr, cd v0, cd
 }
 template <uint32_t K>
 __device__ __forceinline__ void branch_pair(const DOp& op, double thr, cf v0, cf v1, cf& n0, cf& n1) {
-    pair32<K>(op.pf, v0, v1, n0, n1);
+    pair32<K>(op.p, v0, v1, n0, n1);
     const double m0 = fmax(fabs((double)n0.x), fabs((double)n0.y));
     const double m1 = fmax(fabs((double)n1.x), fabs((double)n1.y));
     bool k0 = m0 >= 4.0 * thr, k1 = m1 >= 4.0 * thr;
@@ -240,7 +274,7 @@ template <uint32_t K, class C>
 __device__ __forceinline__ void any_pair(const DOp& op, double thr, C v0, C v1, C& n0, C& n1) {
     if constexpr (is_branch(K)) branch_pair<K>(op, thr, v0, v1, n0, n1);
     else if constexpr (sizeof(C) == 16) pair64<K>(op.p, v0, v1, n0, n1);
-    else pair32<K>(op.pf, v0, v1, n0, n1);
+    else pair32<K>(op.p, v0, v1, n0, n1);
 }
 
 template <class C>

@@ -108,10 +108,10 @@ struct sqe_engine {
     uint64_t scan_cap = 0;
     unsigned int* ticket = nullptr;
     uint32_t epoch = 0;
-    DOp* dops = nullptr;
-    uint64_t dops_cap = 0;
-    DOp* hops = nullptr;                    // pinned host staging of programs
-    uint64_t hops_cap = 0;
+    char* dprog = nullptr;                  // sweep program: MOp[] then coefficient table
+    uint64_t dprog_cap = 0;                 // bytes
+    char* hprog = nullptr;                  // pinned host staging of the program
+    uint64_t hprog_cap = 0;
     hipEvent_t upload_done = nullptr;
     bool upload_pending = false;
     unsigned int* occ = nullptr;
@@ -387,11 +387,28 @@ DOp plan_op(const sqe_gate& g, const QMap& qm) {
     d.cvB = g.ctrl_val & ~qm.mask;
     d.cmS = pext_host(g.ctrl_mask, qm);
     d.cvS = pext_host(g.ctrl_val, qm);
-    for (int k = 0; k < 8; ++k) {
-        d.p[k] = g.p[k];
-        d.pf[k] = (float)g.p[k];
-    }
+    for (int k = 0; k < 8; ++k) d.p[k] = g.p[k];
     return d;
+}
+
+// Pack a planned op into the 40-byte sweep record; U / DIAG / PERM keep their
+// eight coefficients in the side table.
+MOp encode_op(const DOp& d, std::vector<double>& coef) {
+    MOp m{};
+    uint32_t target = 0;
+    if (d.tslot < 0 && d.tbase) target = (uint32_t)__builtin_ctzll(d.tbase);
+    uint32_t ci = NO_COEF;
+    if (d.kind == SQE_GATE_U || d.kind == SQE_GATE_DIAG || d.kind == SQE_GATE_PERM) {
+        ci = (uint32_t)(coef.size() / 8);
+        coef.insert(coef.end(), d.p, d.p + 8);
+    }
+    m.hdr = (d.kind & 31u) | ((uint32_t)(d.tslot + 1) & 7u) << 5 | target << 8 | ci << 14;
+    m.ctl = (d.cmS & 0xffu) | (d.cvS & 0xffu) << 8;
+    m.cmB = d.cmB;
+    m.cvB = d.cvB;
+    m.c0 = d.p[0];
+    m.c1 = d.p[1];
+    return m;
 }
 
 DOp renorm_op() {
@@ -576,10 +593,10 @@ sqe_status grow(sqe_engine* e, uint64_t need) {
     return SQE_OK;
 }
 
-sqe_status run_sweep(sqe_engine* e, const DOp* dops, uint64_t nops, int store) {
+sqe_status run_sweep(sqe_engine* e, const MOp* mops, const double* coef, uint64_t nops, int store) {
     SweepArgs a{};
     Store& o = e->s[e->cur];
-    a.base = o.base; a.amp = o.amp; a.nb = &e->meta[e->cur]; a.ops = dops; a.nops = (uint32_t)nops;
+    a.base = o.base; a.amp = o.amp; a.nb = &e->meta[e->cur]; a.ops = mops; a.coef = coef; a.nops = (uint32_t)nops;
     a.store = store; a.thr = e->thr; a.scale = &e->dscal[1]; a.stats = e->stats;
     const uint32_t g = std::max<uint32_t>(1, tiles_of(round64(e->nb_upper)));
     SQCK(prep_bstats(e, g));
@@ -709,39 +726,52 @@ sqe_status flush(sqe_engine* e) {
     }
     if (e->renorm_pending) open_run();
     close_run();
-    // upload the program
+    // upload the program: MOp records, then the coefficient table
+    std::vector<MOp> mops;
+    std::vector<double> coef;
+    mops.reserve(prog.size());
+    for (const auto& d : prog) mops.push_back(encode_op(d, coef));
+    const size_t mbytes = mops.size() * sizeof(MOp);
+    const size_t cbytes = coef.size() * sizeof(double);
+    const size_t coff = (mbytes + 255) & ~(size_t)255;
+    const size_t total = coff + cbytes;
+    const MOp* dm = nullptr;
+    const double* dc = nullptr;
     if (!prog.empty()) {
         if (e->upload_pending) {
             HIPCK(hipEventSynchronize(e->upload_done));
             e->upload_pending = false;
         }
-        if (prog.size() > e->hops_cap) {
-            if (e->hops) (void)hipHostFree(e->hops);
-            e->hops = nullptr;
-            e->hops_cap = 0;
-            const uint64_t cap = pow2_at_least(prog.size());
-            if (hipHostMalloc((void**)&e->hops, cap * sizeof(DOp), hipHostMallocDefault) != hipSuccess)
+        if (total > e->hprog_cap) {
+            if (e->hprog) (void)hipHostFree(e->hprog);
+            e->hprog = nullptr;
+            e->hprog_cap = 0;
+            const uint64_t cap = pow2_at_least(total);
+            if (hipHostMalloc((void**)&e->hprog, cap, hipHostMallocDefault) != hipSuccess)
                 return fail(SQE_ENOMEM, "pinned program buffer");
-            e->hops_cap = cap;
+            e->hprog_cap = cap;
         }
-        if (prog.size() > e->dops_cap) {
+        if (total > e->dprog_cap) {
             HIPCK(hipStreamSynchronize(e->st));
-            dfree(e, e->dops, e->dops_cap * sizeof(DOp));
-            e->dops = nullptr;
-            e->dops_cap = 0;
-            const uint64_t cap = pow2_at_least(prog.size());
-            SQCK(dalloc(e, (void**)&e->dops, cap * sizeof(DOp)));
-            e->dops_cap = cap;
+            dfree(e, e->dprog, e->dprog_cap);
+            e->dprog = nullptr;
+            e->dprog_cap = 0;
+            const uint64_t cap = pow2_at_least(total);
+            SQCK(dalloc(e, (void**)&e->dprog, cap));
+            e->dprog_cap = cap;
         }
-        std::memcpy(e->hops, prog.data(), prog.size() * sizeof(DOp));
-        HIPCK(hipMemcpyAsync(e->dops, e->hops, prog.size() * sizeof(DOp), hipMemcpyHostToDevice, e->st));
+        std::memcpy(e->hprog, mops.data(), mbytes);
+        if (cbytes) std::memcpy(e->hprog + coff, coef.data(), cbytes);
+        HIPCK(hipMemcpyAsync(e->dprog, e->hprog, total, hipMemcpyHostToDevice, e->st));
         HIPCK(hipEventRecord(e->upload_done, e->st));
         e->upload_pending = true;
+        dm = (const MOp*)e->dprog;
+        dc = (const double*)(e->dprog + coff);
     }
     for (const auto& s : steps) {
         switch (s.kind) {
         case ST_SWEEP: {
-            SQCK(run_sweep(e, e->dops + s.off, s.n, 1));
+            SQCK(run_sweep(e, dm + s.off, dc, s.n, 1));
             ++e->n_sweeps;
             for (uint64_t i = s.off; i < s.off + s.n; ++i) {
                 const DOp& d = prog[i];
@@ -777,7 +807,7 @@ sqe_status count_now(sqe_engine* e, Stats* h) {
         h->nnz = e->hkeys.size();
         return SQE_OK;
     }
-    SQCK(run_sweep(e, nullptr, 0, 0));
+    SQCK(run_sweep(e, nullptr, nullptr, 0, 0));
     SQCK(read_stats(e, h));
     collect_times(e);
     return SQE_OK;
@@ -796,12 +826,12 @@ void destroy_all(sqe_engine* e) {
     dfree(e, e->tab, e->tcap * sizeof(Slot));
     dfree(e, e->scan_status, e->scan_cap * 8);
     dfree(e, e->ticket, 64);
-    dfree(e, e->dops, e->dops_cap * sizeof(DOp));
+    dfree(e, e->dprog, e->dprog_cap);
     dfree(e, e->occ, e->occ_cap * 4);
     dfree(e, e->xkeys, e->xcap * 8);
     dfree(e, e->xre, e->xcap * 8);
     dfree(e, e->xim, e->xcap * 8);
-    if (e->hops) (void)hipHostFree(e->hops);
+    if (e->hprog) (void)hipHostFree(e->hprog);
     for (auto& t : e->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
     if (e->upload_done) (void)hipEventDestroy(e->upload_done);
@@ -922,7 +952,8 @@ sqe_status sqe_apply_gates(sqe_engine* e, const sqe_gate* g, uint64_t n) {
         if (x.ctrl_val & ~x.ctrl_mask) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": ctrl_val outside ctrl_mask");
     }
     e->pending.insert(e->pending.end(), g, g + n);
-    if (e->pending.size() >= (1u << 20)) return flush(e);
+    // bounded batches keep the coefficient-table index within its 18 bits
+    if (e->pending.size() >= (1u << 17)) return flush(e);
     return SQE_OK;
 }
 

@@ -92,8 +92,7 @@ __device__ __forceinline__ void base_op(const DOp& op, typename CT<R>::T (&a)[1 
     using C = typename CT<R>::T;
     constexpr int S = 1 << MB;
     const bool b = (base & op.tbase) != 0;
-    const bool f64 = sizeof(R) == 8;
-    auto coef = [&](int k) __attribute__((always_inline)) { return f64 ? (R)op.p[k] : (R)op.pf[k]; };
+    auto coef = [&](int k) __attribute__((always_inline)) { return (R)op.p[k]; };
     auto put = [&](C& dst, C v) __attribute__((always_inline)) {
         if constexpr (PALL) dst = v; else dst = csel(pred, v, dst);
     };
@@ -217,7 +216,10 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(SweepArgs A) {
     const uint64_t b = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     const uint64_t nb = *A.nb;
     const uint64_t nb_pad = (nb + 63) & ~63ull;
-    const bool live = b < nb_pad;                     // wavefront-uniform
+    // liveness is decided per wavefront (tiles are 64 buckets) from a scalar wave
+    // index, so the gate loop below is uniform control flow for the compiler
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const bool live = (uint64_t)blockIdx.x * BLOCK + (uint64_t)wave * 64 < nb_pad;
     C* __restrict__ amp = (C*)A.amp;
     C a[S];
     uint64_t base = EMPTY_KEY;
@@ -240,9 +242,31 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(SweepArgs A) {
     const uint64_t base0 = base;
     double pr = 1.0, pi = 0.0;        // FP32 form: pending per-bucket factor
     bool pending = false;             // wavefront-uniform: a factor is pending
-    if (live) {
-        for (uint32_t i = 0; i < A.nops; ++i) {
-            const DOp& op = A.ops[i];
+    if (live && A.nops) {
+        // The program is staged 64 ops at a time in the lanes of the wavefront
+        // (lane j holds the 10 dwords of op c+j, one coalesced load per chunk);
+        // each op is then pulled into scalar registers with v_readlane, so no
+        // per-gate memory round trip sits on the critical path.
+        const uint32_t lane = threadIdx.x & 63;
+        const uint32_t nops = A.nops;
+        const uint32_t* __restrict__ words = (const uint32_t*)A.ops;
+        uint32_t w[10];
+        for (uint32_t c = 0; c < nops; c += 64) {
+            const uint32_t mine = c + lane < nops ? c + lane : nops - 1;
+#pragma unroll
+            for (int k = 0; k < 10; ++k) w[k] = words[(size_t)mine * 10 + k];
+            const uint32_t cn = __builtin_amdgcn_readfirstlane(nops - c < 64 ? nops - c : 64);
+            for (uint32_t j = 0; j < cn; ++j) {
+            MOp m;
+            m.hdr = __builtin_amdgcn_readlane(w[0], j);
+            m.ctl = __builtin_amdgcn_readlane(w[1], j);
+            m.cmB = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[2], j) | (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[3], j) << 32;
+            m.cvB = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[4], j) | (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[5], j) << 32;
+            m.c0 = __builtin_bit_cast(double, (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[6], j) |
+                                                  (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[7], j) << 32);
+            m.c1 = __builtin_bit_cast(double, (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[8], j) |
+                                                  (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[9], j) << 32);
+            const DOp op = unpack_op(m, A.coef);
             if (op.kind == K_RENORM) {
                 const double f = *A.scale;
                 if constexpr (F32) {
@@ -285,10 +309,13 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(SweepArgs A) {
             } else {
                 base_op<MB, R, false>(op, a, base, pred, valid);
             }
+            }
         }
         if constexpr (F32) {
             if (pending) apply_phase<MB>(a, pr, pi);
         }
+    }
+    if (live) {
         if (A.store) {
             static_for<S>([&](auto sc) __attribute__((always_inline)) {
                 constexpr int s = decltype(sc)::value;

@@ -12,7 +12,8 @@ struct SweepArgs {
     uint64_t* base;
     void* amp;
     const unsigned long long* nb;
-    const DOp* ops;
+    const MOp* ops;
+    const double* coef;     // 8 coefficients per entry for U / DIAG / PERM gates
     uint32_t nops;
     int32_t store;
     double thr;

@@ -41,7 +41,22 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (scale factor for the "
+                         "FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A device-to-device copy kernel of known size: 5 x 1 GiB read + 1 GiB written."""
+    import torch
+    n = 1 << 27                      # 2^27 float64 = 1 GiB
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_launch": n * 8, "launches": 5}), flush=True)
 
 
 def cpu_baseline(w, seconds):
@@ -80,6 +95,9 @@ def traffic_from_profiles(prec, window):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
               "(multi-GPU sharding is a design note, not built); use --gpus 1", file=sys.stderr)

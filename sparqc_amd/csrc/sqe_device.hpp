@@ -44,37 +44,42 @@ struct DOp {
     double p[8];        // FP64 coefficients (host computed)
 };
 
-// Compact form of a DOp in a sweep program (40 bytes, read through the scalar
-// cache): hdr = kind | (tslot + 1) << 5 | target << 8 | coefficient-table
-// index << 14 (NO_COEF: only c0, c1 are used); ctl = cmS | cvS << 8.
+// Sweep programs: each gate is pre-classified on the host into one of a few
+// handlers whose coefficients already encode the gate kind, so the kernel
+// dispatches on a handful of cases instead of kind x target-bit x controls.
+//   H_BASE_DIAG  diagonal gate on a base qubit: factor f0 / f1 by the base bit
+//   H_BASE_X     X on a base qubit: flip the bucket's base key
+//   H_BASE_PERM  Y / PERM on a base qubit: factor by the base bit, then flip
+//   H_SLOT_DIAG  diagonal gate on bucket qubit J: factor f0 / f1 by slot bit J
+//   H_SLOT_X     X on bucket qubit J: swap slot pairs
+//   H_SLOT_PERM  Y / PERM on bucket qubit J: swap slot pairs with factors
+//   H_SLOT_H     H on bucket qubit J (prunes)
+//   H_SLOT_U2    RX / RY on bucket qubit J as [[a, u01], [u10, a]] with real a
+//                (prunes)
+//   H_SLOT_U     any 2x2 on bucket qubit J, 8 coefficients in a side table
+//                (prunes)
+//   H_RENORM     scale by the device-computed 1/sqrt(norm2)
+// Coefficients c[]: DIAG f0r, f0i, f1r, f1i (Z: 1,0,-1,0; PHASE: 1,0,c,s;
+// RZ: c,-s,c,s); PERM u10r, u10i, u01r, u01i (the factor for the element
+// leaving bit 0, then for the element leaving bit 1); H f; U2 a, u01r, u01i,
+// u10r, u10i (RX: c,0,-s,0,-s; RY: c,-s,0,s,0).  With these the arithmetic is
+// operation for operation the reference formula of each kind (products with
+// exact 0 or +-1 only change the sign of zero results).
+enum : uint32_t {
+    H_BASE_DIAG = 0, H_BASE_X, H_BASE_PERM, H_SLOT_DIAG, H_SLOT_X, H_SLOT_PERM, H_SLOT_H, H_SLOT_U2,
+    H_SLOT_U, H_RENORM
+};
+
+// One sweep op: 16 dwords, staged 64 at a time in the lanes of a wavefront.
+// hdr = handler | J << 4 | target << 7 | side-table index << 13; ctl = cmS | cvS << 8.
 struct MOp {
     uint32_t hdr;
     uint32_t ctl;
     uint64_t cmB, cvB;
-    double c0, c1;
+    double c[5];
 };
 constexpr uint32_t NO_COEF = 0x3ffffu;
-static_assert(sizeof(MOp) == 40, "MOp is ten dwords");
-
-__device__ __forceinline__ DOp unpack_op(const MOp& m, const double* __restrict__ coef) {
-    DOp d;
-    d.kind = m.hdr & 31u;
-    d.tslot = (int32_t)((m.hdr >> 5) & 7u) - 1;
-    d.tbase = d.tslot < 0 ? (1ull << ((m.hdr >> 8) & 63u)) : 0ull;
-    d.cmB = m.cmB;
-    d.cvB = m.cvB;
-    d.cmS = m.ctl & 0xffu;
-    d.cvS = (m.ctl >> 8) & 0xffu;
-    const uint32_t ci = m.hdr >> 14;
-    if (ci != NO_COEF) {
-        for (int k = 0; k < 8; ++k) d.p[k] = coef[ci * 8 + k];
-    } else {
-        d.p[0] = m.c0;
-        d.p[1] = m.c1;
-        for (int k = 2; k < 8; ++k) d.p[k] = 0.0;
-    }
-    return d;
-}
+static_assert(sizeof(MOp) == 64, "MOp is sixteen dwords");
 
 struct Slot {           // hash index slot: 16 B, 8 slots = one 128 B line
     uint64_t key;

@@ -391,23 +391,41 @@ DOp plan_op(const sqe_gate& g, const QMap& qm) {
     return d;
 }
 
-// Pack a planned op into the 40-byte sweep record; U / DIAG / PERM keep their
-// eight coefficients in the side table.
+// Pack a planned op into a 64-byte sweep record: pick the handler and turn the
+// gate's coefficients into the handler's (see the handler table in
+// sqe_device.hpp).  General U keeps its eight coefficients in the side table.
 MOp encode_op(const DOp& d, std::vector<double>& coef) {
     MOp m{};
-    uint32_t target = 0;
-    if (d.tslot < 0 && d.tbase) target = (uint32_t)__builtin_ctzll(d.tbase);
-    uint32_t ci = NO_COEF;
-    if (d.kind == SQE_GATE_U || d.kind == SQE_GATE_DIAG || d.kind == SQE_GATE_PERM) {
+    const bool slot = d.tslot >= 0;
+    const uint32_t j = slot ? (uint32_t)d.tslot : 0u;
+    const uint32_t target = (!slot && d.tbase) ? (uint32_t)__builtin_ctzll(d.tbase) : 0u;
+    uint32_t h = H_RENORM, ci = NO_COEF;
+    const double* p = d.p;
+    auto set = [&](double a, double b, double c, double e, double f) {
+        m.c[0] = a; m.c[1] = b; m.c[2] = c; m.c[3] = e; m.c[4] = f;
+    };
+    switch (d.kind) {
+    case SQE_GATE_Z: set(1, 0, -1, 0, 0); h = slot ? H_SLOT_DIAG : H_BASE_DIAG; break;
+    case SQE_GATE_PHASE: set(1, 0, p[0], p[1], 0); h = slot ? H_SLOT_DIAG : H_BASE_DIAG; break;
+    case SQE_GATE_RZ: set(p[0], -p[1], p[0], p[1], 0); h = slot ? H_SLOT_DIAG : H_BASE_DIAG; break;
+    case SQE_GATE_DIAG: set(p[0], p[1], p[2], p[3], 0); h = slot ? H_SLOT_DIAG : H_BASE_DIAG; break;
+    case SQE_GATE_X: h = slot ? H_SLOT_X : H_BASE_X; break;
+    case SQE_GATE_Y: set(0, 1, 0, -1, 0); h = slot ? H_SLOT_PERM : H_BASE_PERM; break;      // u10 = i, u01 = -i
+    case SQE_GATE_PERM: set(p[2], p[3], p[0], p[1], 0); h = slot ? H_SLOT_PERM : H_BASE_PERM; break;
+    case SQE_GATE_H: set(p[0], 0, 0, 0, 0); h = H_SLOT_H; break;
+    case SQE_GATE_RX: set(p[0], 0, -p[1], 0, -p[1]); h = H_SLOT_U2; break;
+    case SQE_GATE_RY: set(p[0], -p[1], 0, p[1], 0); h = H_SLOT_U2; break;
+    case SQE_GATE_U:
+        h = H_SLOT_U;
         ci = (uint32_t)(coef.size() / 8);
-        coef.insert(coef.end(), d.p, d.p + 8);
+        coef.insert(coef.end(), p, p + 8);
+        break;
+    default: h = H_RENORM; break;                                                           // K_RENORM
     }
-    m.hdr = (d.kind & 31u) | ((uint32_t)(d.tslot + 1) & 7u) << 5 | target << 8 | ci << 14;
+    m.hdr = h | j << 4 | target << 7 | ci << 13;
     m.ctl = (d.cmS & 0xffu) | (d.cvS & 0xffu) << 8;
     m.cmB = d.cmB;
     m.cvB = d.cvB;
-    m.c0 = d.p[0];
-    m.c1 = d.p[1];
     return m;
 }
 

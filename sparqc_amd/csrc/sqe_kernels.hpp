@@ -19,58 +19,6 @@ namespace sqe {
 // Fused local sweep: one thread owns one bucket, keeps its 2^MB amplitudes in
 // registers and applies a whole run of bucket-local gates before writing back.
 // ---------------------------------------------------------------------------
-// PALL: the gate has no controls on base qubits, so every bucket takes it and
-// no per-lane select is needed.
-template <int MB, class R, uint32_t K, int J, bool PALL>
-__device__ __forceinline__ void slot_gate(const DOp& op, typename CT<R>::T (&a)[1 << MB], bool pred, double thr) {
-    using C = typename CT<R>::T;
-    static_for<(1 << MB) / 2>([&](auto qc) __attribute__((always_inline)) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int s0 = ((q >> J) << (J + 1)) | (q & ((1 << J) - 1));
-        constexpr int s1 = s0 | (1 << J);
-        if (((uint32_t)s0 & op.cmS) == op.cvS) {
-            C n0, n1;
-            any_pair<K>(op, thr, a[s0], a[s1], n0, n1);
-            if constexpr (PALL) {
-                a[s0] = n0;
-                a[s1] = n1;
-            } else {
-                a[s0] = csel(pred, n0, a[s0]);
-                a[s1] = csel(pred, n1, a[s1]);
-            }
-        }
-    });
-}
-
-template <int MB, class R, uint32_t K, bool PALL>
-__device__ __forceinline__ void slot_dispatch_j(const DOp& op, typename CT<R>::T (&a)[1 << MB], bool pred, double thr) {
-    switch (op.tslot) {
-    case 0: if constexpr (MB > 0) slot_gate<MB, R, K, 0, PALL>(op, a, pred, thr); break;
-    case 1: if constexpr (MB > 1) slot_gate<MB, R, K, 1, PALL>(op, a, pred, thr); break;
-    case 2: if constexpr (MB > 2) slot_gate<MB, R, K, 2, PALL>(op, a, pred, thr); break;
-    case 3: if constexpr (MB > 3) slot_gate<MB, R, K, 3, PALL>(op, a, pred, thr); break;
-    case 4: if constexpr (MB > 4) slot_gate<MB, R, K, 4, PALL>(op, a, pred, thr); break;
-    default: break;
-    }
-}
-
-template <int MB, class R, bool PALL>
-__device__ __forceinline__ void slot_op(const DOp& op, typename CT<R>::T (&a)[1 << MB], bool pred, double thr) {
-    switch (op.kind) {
-    case KX: slot_dispatch_j<MB, R, KX, PALL>(op, a, pred, thr); break;
-    case KY: slot_dispatch_j<MB, R, KY, PALL>(op, a, pred, thr); break;
-    case KZ: slot_dispatch_j<MB, R, KZ, PALL>(op, a, pred, thr); break;
-    case KH: slot_dispatch_j<MB, R, KH, PALL>(op, a, pred, thr); break;
-    case KPHASE: slot_dispatch_j<MB, R, KPHASE, PALL>(op, a, pred, thr); break;
-    case KRZ: slot_dispatch_j<MB, R, KRZ, PALL>(op, a, pred, thr); break;
-    case KRX: slot_dispatch_j<MB, R, KRX, PALL>(op, a, pred, thr); break;
-    case KRY: slot_dispatch_j<MB, R, KRY, PALL>(op, a, pred, thr); break;
-    case KU: slot_dispatch_j<MB, R, KU, PALL>(op, a, pred, thr); break;
-    case KDIAG: slot_dispatch_j<MB, R, KDIAG, PALL>(op, a, pred, thr); break;
-    case KPERM: slot_dispatch_j<MB, R, KPERM, PALL>(op, a, pred, thr); break;
-    default: break;
-    }
-}
 
 // Complex rotation by (c, s): (c x - s y, c y + s x), the reference's phase
 // formula (emulator/device/device_data.cuh:221-224); FP32 with fused multiply-add.
@@ -81,82 +29,40 @@ __device__ __forceinline__ cf rot(float c, float s, cf v) {
     return mkf(__fmaf_rn(c, v.x, -(s * v.y)), __fmaf_rn(c, v.y, s * v.x));
 }
 
-// Gate whose target is a base (non-bucket) qubit: diagonal gates scale the
-// whole bucket by the factor of its base bit; X / Y / PERM move the whole
-// bucket to base ^ (1 << target).  The planner never sends a branching gate,
-// or a permutation with slot controls, down this path.  PALL: no base
-// controls (every live bucket takes the gate; padding lanes hold zeros).
-template <int MB, class R, bool PALL>
-__device__ __forceinline__ void base_op(const DOp& op, typename CT<R>::T (&a)[1 << MB], uint64_t& base, bool pred,
-                                        bool valid) {
-    using C = typename CT<R>::T;
-    constexpr int S = 1 << MB;
-    const bool b = (base & op.tbase) != 0;
-    auto coef = [&](int k) __attribute__((always_inline)) { return (R)op.p[k]; };
-    auto put = [&](C& dst, C v) __attribute__((always_inline)) {
-        if constexpr (PALL) dst = v; else dst = csel(pred, v, dst);
-    };
-    switch (op.kind) {
-    case KZ:
-        static_for<S>([&](auto sc) __attribute__((always_inline)) {
-            constexpr int s = decltype(sc)::value;
-            if (((uint32_t)s & op.cmS) == op.cvS) {
-                C v = a[s];
-                C w;
-                w.x = b ? -v.x : v.x;
-                w.y = b ? -v.y : v.y;
-                put(a[s], w);
-            }
-        });
-        break;
-    case KPHASE: {
-        // bit 1 rotates by (c, s); bit 0 keeps the value (selected per lane)
-        const R c = coef(0), sn = coef(1);
-        static_for<S>([&](auto sc) __attribute__((always_inline)) {
-            constexpr int s = decltype(sc)::value;
-            if (((uint32_t)s & op.cmS) == op.cvS) put(a[s], csel(b, rot(c, sn, a[s]), a[s]));
-        });
-    } break;
-    case KRZ: {
-        const R c = coef(0), s0 = coef(1);
-        const R sn = b ? s0 : -s0;      // bit 0: e^{-i theta/2}, bit 1: e^{+i theta/2}
-        static_for<S>([&](auto sc) __attribute__((always_inline)) {
-            constexpr int s = decltype(sc)::value;
-            if (((uint32_t)s & op.cmS) == op.cvS) put(a[s], rot(c, sn, a[s]));
-        });
-    } break;
-    case KDIAG: {
-        const R dr = b ? coef(2) : coef(0), di = b ? coef(3) : coef(1);
-        static_for<S>([&](auto sc) __attribute__((always_inline)) {
-            constexpr int s = decltype(sc)::value;
-            if (((uint32_t)s & op.cmS) == op.cvS) put(a[s], rot(dr, di, a[s]));
-        });
-    } break;
-    case KX:
-        if (valid && pred) base ^= op.tbase;
-        break;
-    case KY:
-        static_for<S>([&](auto sc) __attribute__((always_inline)) {
-            constexpr int s = decltype(sc)::value;
-            C v = a[s];
-            C w;
-            if (b) { w.x = v.y; w.y = -v.x; } else { w.x = -v.y; w.y = v.x; }
-            put(a[s], w);
-        });
-        if (valid && pred) base ^= op.tbase;
-        break;
-    case KPERM: {
-        // bit 0 moves to bit 1 with factor u10, bit 1 moves to bit 0 with factor u01
-        const R dr = b ? coef(0) : coef(2), di = b ? coef(1) : coef(3);
-        static_for<S>([&](auto sc) __attribute__((always_inline)) {
-            constexpr int s = decltype(sc)::value;
-            put(a[s], rot(dr, di, a[s]));
-        });
-        if (valid && pred) base ^= op.tbase;
-    } break;
-    default:
-        break;
-    }
+// [[a, u01], [u10, a]] with real a: n0 = a v0 + u01 v1, n1 = u10 v0 + a v1,
+// evaluated so that RX / RY coefficients reproduce their reference formulas
+// (emulator/device/device_data.cuh:266-319) operation for operation.
+__device__ __forceinline__ void u2(double a, double u01r, double u01i, double u10r, double u10i, cd v0, cd v1,
+                                   cd& n0, cd& n1) {
+    n0 = mkd(a * v0.x + (u01r * v1.x - u01i * v1.y), a * v0.y + (u01r * v1.y + u01i * v1.x));
+    n1 = mkd((u10r * v0.x - u10i * v0.y) + a * v1.x, (u10r * v0.y + u10i * v0.x) + a * v1.y);
+}
+__device__ __forceinline__ void u2(float a, float u01r, float u01i, float u10r, float u10i, cf v0, cf v1, cf& n0,
+                                   cf& n1) {
+    n0 = mkf(__fmaf_rn(a, v0.x, u01r * v1.x - u01i * v1.y), __fmaf_rn(a, v0.y, u01r * v1.y + u01i * v1.x));
+    n1 = mkf(__fmaf_rn(a, v1.x, u10r * v0.x - u10i * v0.y), __fmaf_rn(a, v1.y, u10r * v0.y + u10i * v0.x));
+}
+__device__ __forceinline__ void hpair(double f, cd v0, cd v1, cd& n0, cd& n1) {
+    n0 = mkd(f * (v0.x + v1.x), f * (v0.y + v1.y));
+    n1 = mkd(f * (v0.x - v1.x), f * (v0.y - v1.y));
+}
+__device__ __forceinline__ void hpair(float f, cf v0, cf v1, cf& n0, cf& n1) {
+    n0 = mkf(f * (v0.x + v1.x), f * (v0.y + v1.y));
+    n1 = mkf(f * (v0.x - v1.x), f * (v0.y - v1.y));
+}
+
+// Prune a branching output (componentwise |re|,|im| < thr,
+// emulator/include/data_t.h:50-60).  FP32: outputs with max(|re|,|im|) in
+// [thr/4, 4 thr) are decided on the FP64 recomputation `exact`.
+__device__ __forceinline__ cd prune(cd n, double thr) { return negl64(n, thr) ? mkd(0.0, 0.0) : n; }
+__device__ __forceinline__ bool near32(cf n, double thr) {
+    const double m = fmax(fabs((double)n.x), fabs((double)n.y));
+    return m < 4.0 * thr && m >= 0.25 * thr;
+}
+__device__ __forceinline__ cf prune32(cf n, double thr, bool keep_exact) {
+    const double m = fmax(fabs((double)n.x), fabs((double)n.y));
+    const bool keep = m >= 4.0 * thr ? true : (m < 0.25 * thr ? false : keep_exact);
+    return keep ? n : mkf(0.0f, 0.0f);
 }
 
 template <int MB, class R>
@@ -169,33 +75,164 @@ __device__ __forceinline__ uint32_t count_nz(const typename CT<R>::T (&a)[1 << M
     return c;
 }
 
-// FP32 form: a gate on a base qubit that acts on every slot of a bucket alike
-// (diagonal, Y, PERM without slot controls) only multiplies the bucket by one
-// complex factor.  The FP32 sweep keeps that factor per bucket in FP64 and
-// applies it to the slots lazily -- before a branching gate (whose prune
-// decision needs the actual values) and at the end of the sweep -- instead of
-// rescaling every slot per gate.  Returns false when `op` is not such a gate.
-__device__ __forceinline__ bool fold_phase(const DOp& op, uint64_t& base, bool pred, bool valid, double& pr,
-                                           double& pi) {
-    if (op.tslot >= 0 || op.cmS != 0) return false;
-    const bool b = (base & op.tbase) != 0;
-    double fr, fi;
-    switch (op.kind) {
-    case KZ: fr = b ? -1.0 : 1.0; fi = 0.0; break;
-    case KPHASE: fr = b ? op.p[0] : 1.0; fi = b ? op.p[1] : 0.0; break;
-    case KRZ: fr = op.p[0]; fi = b ? op.p[1] : -op.p[1]; break;
-    case KDIAG: fr = b ? op.p[2] : op.p[0]; fi = b ? op.p[3] : op.p[1]; break;
-    case KY: fr = 0.0; fi = b ? -1.0 : 1.0; break;          // bit 0 -> i v, bit 1 -> -i v
-    case KPERM: fr = b ? op.p[0] : op.p[2]; fi = b ? op.p[1] : op.p[3]; break;
-    default: return false;
+// Decoded sweep op (uniform across the wavefront).
+struct SOp {
+    uint32_t h, j, cmS, cvS, ci;
+    uint64_t tbase, cmB, cvB;
+    double c[5];
+};
+
+template <int S, int J, int Q>
+struct PairIdx {
+    static constexpr int s0 = ((Q >> J) << (J + 1)) | (Q & ((1 << J) - 1));
+    static constexpr int s1 = s0 | (1 << J);
+};
+
+// Slot handlers with compile-time target bit J.
+template <int MB, class R, int J>
+__device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&a)[1 << MB], bool pred, double thr,
+                                             const double* __restrict__ coef) {
+    using C = typename CT<R>::T;
+    constexpr int HALF = (1 << MB) / 2;
+    constexpr bool F32 = sizeof(R) == 4;
+    const bool pall = op.cmB == 0;               // every bucket takes the gate
+    switch (op.h) {
+    case H_SLOT_DIAG: {
+        // factor of slot bit J; a lane whose base controls fail takes 1
+        R f0r = (R)op.c[0], f0i = (R)op.c[1], f1r = (R)op.c[2], f1i = (R)op.c[3];
+        if (!pall) {
+            f0r = pred ? f0r : (R)1; f0i = pred ? f0i : (R)0;
+            f1r = pred ? f1r : (R)1; f1i = pred ? f1i : (R)0;
+        }
+        static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+            constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+            constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+            if (((uint32_t)s0 & op.cmS) == op.cvS) {
+                a[s0] = rot(f0r, f0i, a[s0]);
+                a[s1] = rot(f1r, f1i, a[s1]);
+            }
+        });
+    } break;
+    case H_SLOT_X:
+        static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+            constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+            constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+            if (((uint32_t)s0 & op.cmS) == op.cvS) {
+                const C x = a[s0], y = a[s1];
+                if (pall) { a[s0] = y; a[s1] = x; }
+                else { a[s0] = csel(pred, y, x); a[s1] = csel(pred, x, y); }
+            }
+        });
+        break;
+    case H_SLOT_PERM: {
+        const R u10r = (R)op.c[0], u10i = (R)op.c[1], u01r = (R)op.c[2], u01i = (R)op.c[3];
+        static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+            constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+            constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+            if (((uint32_t)s0 & op.cmS) == op.cvS) {
+                const C x = a[s0], y = a[s1];
+                const C n0 = rot(u01r, u01i, y), n1 = rot(u10r, u10i, x);
+                a[s0] = csel(pred || pall, n0, x);
+                a[s1] = csel(pred || pall, n1, y);
+            }
+        });
+    } break;
+    case H_SLOT_H: {
+        const R f = (R)op.c[0];
+        static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+            constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+            constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+            if (((uint32_t)s0 & op.cmS) == op.cvS) {
+                const C x = a[s0], y = a[s1];
+                C n0, n1;
+                hpair(f, x, y, n0, n1);
+                if constexpr (F32) {
+                    if (near32(n0, thr) || near32(n1, thr)) {
+                        cd e0, e1;
+                        hpair(op.c[0], mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
+                        n0 = prune32(n0, thr, !negl64(e0, thr));
+                        n1 = prune32(n1, thr, !negl64(e1, thr));
+                    } else {
+                        n0 = prune32(n0, thr, true);
+                        n1 = prune32(n1, thr, true);
+                    }
+                } else {
+                    n0 = prune(n0, thr);
+                    n1 = prune(n1, thr);
+                }
+                if (pall) { a[s0] = n0; a[s1] = n1; }
+                else { a[s0] = csel(pred, n0, x); a[s1] = csel(pred, n1, y); }
+            }
+        });
+    } break;
+    case H_SLOT_U2: {
+        const R ar = (R)op.c[0], u01r = (R)op.c[1], u01i = (R)op.c[2], u10r = (R)op.c[3], u10i = (R)op.c[4];
+        static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+            constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+            constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+            if (((uint32_t)s0 & op.cmS) == op.cvS) {
+                const C x = a[s0], y = a[s1];
+                C n0, n1;
+                u2(ar, u01r, u01i, u10r, u10i, x, y, n0, n1);
+                if constexpr (F32) {
+                    if (near32(n0, thr) || near32(n1, thr)) {
+                        cd e0, e1;
+                        u2(op.c[0], op.c[1], op.c[2], op.c[3], op.c[4], mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
+                        n0 = prune32(n0, thr, !negl64(e0, thr));
+                        n1 = prune32(n1, thr, !negl64(e1, thr));
+                    } else {
+                        n0 = prune32(n0, thr, true);
+                        n1 = prune32(n1, thr, true);
+                    }
+                } else {
+                    n0 = prune(n0, thr);
+                    n1 = prune(n1, thr);
+                }
+                if (pall) { a[s0] = n0; a[s1] = n1; }
+                else { a[s0] = csel(pred, n0, x); a[s1] = csel(pred, n1, y); }
+            }
+        });
+    } break;
+    case H_SLOT_U: {
+        DOp d{};
+        for (int k = 0; k < 8; ++k) d.p[k] = coef[op.ci * 8 + k];
+        static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+            constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+            constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+            if (((uint32_t)s0 & op.cmS) == op.cvS) {
+                const C x = a[s0], y = a[s1];
+                C n0, n1;
+                branch_pair<KU>(d, thr, x, y, n0, n1);
+                a[s0] = csel(pred || pall, n0, x);
+                a[s1] = csel(pred || pall, n1, y);
+            }
+        });
+    } break;
+    default:
+        break;
     }
-    if (pred) {
-        const double r = pr * fr - pi * fi, i = pr * fi + pi * fr;
-        pr = r;
-        pi = i;
-        if (valid && is_perm(op.kind)) base ^= op.tbase;
+}
+
+template <int MB, class R>
+__device__ __forceinline__ void slot_dispatch(const SOp& op, typename CT<R>::T (&a)[1 << MB], bool pred, double thr,
+                                              const double* __restrict__ coef) {
+    switch (op.j) {
+    case 0: if constexpr (MB > 0) slot_handler<MB, R, 0>(op, a, pred, thr, coef); break;
+    case 1: if constexpr (MB > 1) slot_handler<MB, R, 1>(op, a, pred, thr, coef); break;
+    case 2: if constexpr (MB > 2) slot_handler<MB, R, 2>(op, a, pred, thr, coef); break;
+    case 3: if constexpr (MB > 3) slot_handler<MB, R, 3>(op, a, pred, thr, coef); break;
+    case 4: if constexpr (MB > 4) slot_handler<MB, R, 4>(op, a, pred, thr, coef); break;
+    default: break;
     }
-    return true;
+}
+
+// Per-lane factor of a base-qubit DIAG / PERM op (identity where the base
+// controls fail).
+// (DIAG: f0 / f1 by the bit; PERM: u10 for the element leaving bit 0, u01 for
+// the one leaving bit 1 -- both stored as c[0..1] / c[2..3]).
+__device__ __forceinline__ void base_factor(const SOp& op, bool b, bool pred, double& fr, double& fi) {
+    fr = pred ? (b ? op.c[2] : op.c[0]) : 1.0;
+    fi = pred ? (b ? op.c[3] : op.c[1]) : 0.0;
 }
 
 template <int MB>
@@ -244,71 +281,89 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(SweepArgs A) {
     bool pending = false;             // wavefront-uniform: a factor is pending
     if (live && A.nops) {
         // The program is staged 64 ops at a time in the lanes of the wavefront
-        // (lane j holds the 10 dwords of op c+j, one coalesced load per chunk);
+        // (lane j holds the 16 dwords of op c+j, one coalesced load per chunk);
         // each op is then pulled into scalar registers with v_readlane, so no
         // per-gate memory round trip sits on the critical path.
         const uint32_t lane = threadIdx.x & 63;
         const uint32_t nops = A.nops;
         const uint32_t* __restrict__ words = (const uint32_t*)A.ops;
-        uint32_t w[10];
+        uint32_t w[16];
         for (uint32_t c = 0; c < nops; c += 64) {
             const uint32_t mine = c + lane < nops ? c + lane : nops - 1;
 #pragma unroll
-            for (int k = 0; k < 10; ++k) w[k] = words[(size_t)mine * 10 + k];
+            for (int k = 0; k < 16; ++k) w[k] = words[(size_t)mine * 16 + k];
             const uint32_t cn = __builtin_amdgcn_readfirstlane(nops - c < 64 ? nops - c : 64);
-            for (uint32_t j = 0; j < cn; ++j) {
-            MOp m;
-            m.hdr = __builtin_amdgcn_readlane(w[0], j);
-            m.ctl = __builtin_amdgcn_readlane(w[1], j);
-            m.cmB = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[2], j) | (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[3], j) << 32;
-            m.cvB = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[4], j) | (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[5], j) << 32;
-            m.c0 = __builtin_bit_cast(double, (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[6], j) |
-                                                  (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[7], j) << 32);
-            m.c1 = __builtin_bit_cast(double, (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[8], j) |
-                                                  (uint64_t)(uint32_t)__builtin_amdgcn_readlane(w[9], j) << 32);
-            const DOp op = unpack_op(m, A.coef);
-            if (op.kind == K_RENORM) {
-                const double f = *A.scale;
-                if constexpr (F32) {
-                    pr *= f;
-                    pi *= f;
-                    pending = true;
-                } else {
-                    static_for<S>([&](auto sc) __attribute__((always_inline)) {
-                        constexpr int s = decltype(sc)::value;
-                        a[s].x = (R)((double)a[s].x * f);
-                        a[s].y = (R)((double)a[s].y * f);
-                    });
-                }
-                continue;
-            }
-            upd += cnt;
-            const bool pred = valid && ((base & op.cmB) == op.cvB);
-            if constexpr (F32) {
-                if (fold_phase(op, base, op.cmB == 0 || pred, valid, pr, pi)) {
-                    pending = true;
+            for (uint32_t jj = 0; jj < cn; ++jj) {
+                auto rd = [&](int k) __attribute__((always_inline)) {
+                    return (uint32_t)__builtin_amdgcn_readlane(w[k], jj);
+                };
+                auto rd64 = [&](int k) __attribute__((always_inline)) {
+                    return (uint64_t)rd(k) | (uint64_t)rd(k + 1) << 32;
+                };
+                SOp op;
+                const uint32_t hdr = rd(0), ctl = rd(1);
+                op.h = hdr & 15u;
+                op.j = (hdr >> 4) & 7u;
+                op.tbase = 1ull << ((hdr >> 7) & 63u);
+                op.ci = hdr >> 13;
+                op.cmS = ctl & 0xffu;
+                op.cvS = (ctl >> 8) & 0xffu;
+                op.cmB = rd64(2);
+                op.cvB = rd64(4);
+#pragma unroll
+                for (int k = 0; k < 5; ++k) op.c[k] = __builtin_bit_cast(double, rd64(6 + 2 * k));
+                if (op.h == H_RENORM) {
+                    const double f = *A.scale;
+                    if constexpr (F32) {
+                        pr *= f;
+                        pi *= f;
+                        pending = true;
+                    } else {
+                        static_for<S>([&](auto sc) __attribute__((always_inline)) {
+                            constexpr int s = decltype(sc)::value;
+                            a[s].x = (R)((double)a[s].x * f);
+                            a[s].y = (R)((double)a[s].y * f);
+                        });
+                    }
                     continue;
                 }
-                if (pending && op.tslot >= 0 && is_branch(op.kind)) {
-                    apply_phase<MB>(a, pr, pi);
-                    pr = 1.0;
-                    pi = 0.0;
-                    pending = false;
+                upd += cnt;
+                const bool pred = valid && ((base & op.cmB) == op.cvB);
+                if (op.h <= H_BASE_PERM) {
+                    // gate on a base qubit
+                    if (op.h == H_BASE_X) {
+                        if (pred) base ^= op.tbase;
+                        continue;
+                    }
+                    const bool bit = (base & op.tbase) != 0;
+                    double fr, fi;
+                    base_factor(op, bit, pred, fr, fi);
+                    if (F32 && op.cmS == 0) {
+                        const double r = pr * fr - pi * fi, i = pr * fi + pi * fr;
+                        pr = r;
+                        pi = i;
+                        pending = true;
+                    } else {
+                        const R cr = (R)fr, ci = (R)fi;
+                        static_for<S>([&](auto sc) __attribute__((always_inline)) {
+                            constexpr int s = decltype(sc)::value;
+                            if (((uint32_t)s & op.cmS) == op.cvS) a[s] = rot(cr, ci, a[s]);
+                        });
+                    }
+                    if (op.h == H_BASE_PERM && pred) base ^= op.tbase;
+                    continue;
                 }
-            }
-            if (op.tslot >= 0) {
                 if constexpr (MB > 0) {
-                    // padding lanes hold zeros, which every gate maps to zeros, so a
-                    // gate without base controls runs unpredicated on all lanes
-                    if (op.cmB == 0) slot_op<MB, R, true>(op, a, true, A.thr);
-                    else slot_op<MB, R, false>(op, a, pred, A.thr);
-                    if (is_branch(op.kind)) cnt = valid ? count_nz<MB, R>(a) : 0u;
+                    const bool branch = op.h >= H_SLOT_H && op.h <= H_SLOT_U;
+                    if (F32 && pending && branch) {
+                        if constexpr (F32) apply_phase<MB>(a, pr, pi);
+                        pr = 1.0;
+                        pi = 0.0;
+                        pending = false;
+                    }
+                    slot_dispatch<MB, R>(op, a, pred, A.thr, A.coef);
+                    if (branch) cnt = valid ? count_nz<MB, R>(a) : 0u;
                 }
-            } else if (op.cmB == 0) {
-                base_op<MB, R, true>(op, a, base, true, valid);
-            } else {
-                base_op<MB, R, false>(op, a, base, pred, valid);
-            }
             }
         }
         if constexpr (F32) {

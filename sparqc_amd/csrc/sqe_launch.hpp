@@ -13,7 +13,7 @@ struct SweepArgs {
     void* amp;
     const unsigned long long* nb;
     const MOp* ops;
-    const double* coef;     // 8 coefficients per entry for U / DIAG / PERM gates
+    const double* coef;     // 8 coefficients per general-U gate (side table)
     uint32_t nops;
     int32_t store;
     double thr;

@@ -167,6 +167,81 @@ sqe_status sqe_snapshot_restore(sqe_engine* e);
 /* Last error message of the calling thread ("" when none). */
 const char* sqe_last_error(void);
 
+/* ------------------------------------------------------------------------
+ * Batch engine: many independent circuits resident on one GPU, advanced
+ * together by persistent launches.  No reference counterpart -- the
+ * reference runs one circuit per process (emulator/src/main.cu:87-148) and
+ * has no request queue; this is the batched / serving form of the same
+ * gate kernels.  Every gate of an admitted circuit must be bucket-local for
+ * the bucket qubits the planner picks (its branching targets, at most 4):
+ * sqe_batch_add rejects a circuit with any other gate (SQE_EINVAL), which
+ * then runs on the single-circuit engine.
+ * ------------------------------------------------------------------------ */
+typedef struct sqe_batch sqe_batch;
+
+typedef struct sqe_batch_circuit {
+    uint32_t done;              /* 1 when every gate has been applied                  */
+    int32_t bucket_bits;
+    uint64_t gates_total;
+    uint64_t gates_done;
+    uint64_t updates;           /* sum over applied gates of nonzeros entering them    */
+    uint64_t nnz;               /* nonzeros after the last step that touched it        */
+    uint64_t buckets;
+    uint64_t bytes;             /* pool bytes charged to this circuit                  */
+} sqe_batch_circuit;
+
+typedef struct sqe_batch_tuning {
+    uint32_t block;             /* workgroup size of the batched sweep: 64/128/256     */
+    uint32_t tiles_per_item;    /* 64-bucket tiles per queue item, 1..64               */
+    uint32_t waves;             /* resident wavefronts of the persistent launch        */
+    uint32_t pad;
+    double load_factor;         /* hash index load factor used by sqe_batch_add        */
+    double sweep_ms;            /* summed batched sweep time (timing = 1), or the
+                                   best trial time from sqe_batch_autotune             */
+    uint64_t launches;
+} sqe_batch_tuning;
+
+/* cfg->nq is ignored (each circuit has its own); prec, prune_thr, max_load,
+ * hbm_budget_bytes (admission budget), device, bucket_bits, renorm_every and
+ * timing apply to every circuit. */
+sqe_status sqe_batch_create(const sqe_config* cfg, sqe_batch** out);
+sqe_status sqe_batch_destroy(sqe_batch* b);
+
+/* Admit a circuit: nq qubits, n distinct keys with amplitudes, ng gates.
+ * *id receives its handle.  SQE_ETABLE_FULL when its pool bytes would exceed
+ * the budget (nothing is admitted; retry after sqe_batch_remove). */
+sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const double* re, const double* im,
+                         uint64_t n, const sqe_gate* gates, uint64_t ng, int64_t* id);
+
+/* Advance every unfinished circuit by up to max_gates program records
+ * (0: to its next FP64 renormalisation point or its end). */
+sqe_status sqe_batch_step(sqe_batch* b, uint32_t max_gates);
+
+/* Step until every admitted circuit is done. */
+sqe_status sqe_batch_run(sqe_batch* b);
+sqe_status sqe_batch_sync(sqe_batch* b);
+sqe_status sqe_batch_info(sqe_batch* b, int64_t id, sqe_batch_circuit* out);
+
+/* Same contract as sqe_export, for one circuit of the batch. */
+sqe_status sqe_batch_export(sqe_batch* b, int64_t id, uint64_t* keys, double* re, double* im, uint64_t cap,
+                            uint64_t* n);
+
+/* Release a circuit's pool space (finished or not). */
+sqe_status sqe_batch_remove(sqe_batch* b, int64_t id);
+
+/* Admitted bytes, budget, and circuits not yet done. */
+sqe_status sqe_batch_usage(sqe_batch* b, uint64_t* used, uint64_t* budget, uint64_t* active);
+
+sqe_status sqe_batch_get_tuning(sqe_batch* b, sqe_batch_tuning* t);
+sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t);
+
+/* Time each (workgroup size, tiles per item, resident waves) candidate on the
+ * admitted circuits (their state is restored after every trial) and keep the
+ * fastest; *best (optional) receives it. */
+sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best);
+
+const char* sqe_batch_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

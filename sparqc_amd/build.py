@@ -32,7 +32,8 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
          "-mllvm", "-simplifycfg-sink-common=false",
          "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
-SOURCES = ["sqe_kernels_f64.hip", "sqe_kernels_f32.hip", "sqe_kernels_common.hip", "sqe_engine.cpp"]
+SOURCES = ["sqe_kernels_f64.hip", "sqe_kernels_f32.hip", "sqe_kernels_common.hip", "sqe_engine.cpp",
+           "sqe_batch.cpp"]
 
 
 def _compile(src):

@@ -1,0 +1,812 @@
+// Batch engine: many independent circuits resident in one device pool and
+// advanced together by persistent launches (no reference counterpart; the
+// reference runs one circuit per process, emulator/src/main.cu:87-148).
+//
+// Each admitted circuit gets a region of the pool (bucket bases + tile-
+// interleaved amplitudes, the layout of sqe_device.hpp), its own bucket-qubit
+// set and its own pre-encoded sweep program.  sqe_batch_step advances every
+// active circuit by up to N gates with one k_sweep_batch launch per bucket
+// size; FP32 circuits stop at their renormalisation points, where a batched
+// FP64 norm runs before the next step.  Admission is by bytes of the pool
+// against the configured HBM budget, so a serving loop can admit arrivals as
+// finished circuits are removed (continuous batching at step granularity).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/sqe.h"
+#include "sqe_launch.hpp"
+#include "sqe_plan.hpp"
+
+using namespace sqe;
+
+namespace {
+
+thread_local std::string b_err;
+
+sqe_status bfail(sqe_status s, const std::string& m) {
+    b_err = m;
+    return s;
+}
+
+#define BHIP(x)                                                                          \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess) return bfail(SQE_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define BCK(x)                       \
+    do {                             \
+        sqe_status s_ = (x);         \
+        if (s_ != SQE_OK) return s_; \
+    } while (0)
+
+inline uint64_t p2(uint64_t x) {
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// First-fit allocator over a range of units (tiles or amplitude rows).
+struct RangeAlloc {
+    std::map<uint64_t, uint64_t> free_;   // start -> length
+    uint64_t cap = 0;
+    void grow(uint64_t new_cap) {
+        if (new_cap <= cap) return;
+        // merge with a trailing free range
+        if (!free_.empty()) {
+            auto it = std::prev(free_.end());
+            if (it->first + it->second == cap) {
+                it->second += new_cap - cap;
+                cap = new_cap;
+                return;
+            }
+        }
+        free_[cap] = new_cap - cap;
+        cap = new_cap;
+    }
+    bool take(uint64_t n, uint64_t* at) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                *at = it->first;
+                const uint64_t rest = it->second - n, st = it->first + n;
+                free_.erase(it);
+                if (rest) free_[st] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void give(uint64_t at, uint64_t n) {
+        auto it = free_.emplace(at, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    uint64_t used_extent() const {        // end of the last used unit
+        if (free_.empty()) return cap;
+        auto it = std::prev(free_.end());
+        return it->first + it->second == cap ? it->first : cap;
+    }
+};
+
+struct Circ {
+    bool live = false;
+    uint32_t nq = 0;
+    QMap qm{};
+    uint64_t tile0 = 0, tiles = 0;        // base pool tiles
+    uint64_t row0 = 0, rows = 0;          // amp pool rows (64 amplitudes)
+    uint64_t prog0 = 0, nops = 0;         // program pool ops
+    uint64_t coef0 = 0, ncoef = 0;        // coefficient pool doubles
+    uint64_t pos = 0;                     // next op
+    std::vector<uint32_t> renorm_at;      // op indices of RENORM ops
+    uint64_t gates = 0;
+    uint64_t bytes = 0;
+};
+
+struct Pool {
+    char* p = nullptr;
+    uint64_t bytes = 0;
+};
+
+}  // namespace
+
+struct sqe_batch {
+    sqe_config cfg;
+    bool f32 = false;
+    double thr = 1e-14;
+    size_t csize = 16;
+    hipStream_t st = nullptr;
+    std::vector<Circ> c;                   // indexed by slot (= id)
+    std::vector<uint32_t> free_slots;
+    RangeAlloc ta, ra, pa, ca;             // tiles, rows, program ops, coefficients
+    Pool base, amp, prog, coef;
+    uint64_t used = 0;                     // admitted bytes
+    // per-slot device arrays
+    uint64_t nslot = 0;
+    unsigned long long *upd = nullptr, *nnz = nullptr, *meta = nullptr;
+    double *norm2 = nullptr, *scale = nullptr;
+    // load path scratch
+    Slot* tab = nullptr;
+    uint64_t tcap = 0;
+    uint64_t *xkeys = nullptr;
+    double *xre = nullptr, *xim = nullptr;
+    uint64_t xcap = 0;
+    unsigned int* occ = nullptr;
+    uint64_t occ_cap = 0;
+    Stats* stats = nullptr;
+    unsigned long long* scan_status = nullptr;
+    uint64_t scan_cap = 0;
+    unsigned int* ticket = nullptr;
+    uint32_t epoch = 0;
+    unsigned long long* total = nullptr;
+    // launch staging
+    BTask* htask = nullptr;
+    BItem* hitem = nullptr;
+    uint64_t hcap_t = 0, hcap_i = 0;
+    BTask* dtask = nullptr;
+    BItem* ditem = nullptr;
+    uint64_t dcap_t = 0, dcap_i = 0;
+    unsigned int* queue = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // tuning and counters
+    uint32_t block = BLOCK, gran = 1, grid_waves = 0;
+    double sweep_ms = 0.0;
+    uint64_t launches = 0, steps = 0, bytes_dev = 0;
+};
+
+namespace {
+
+sqe_status dal(sqe_batch* b, void** p, size_t n) {
+    *p = nullptr;
+    if (!n) return SQE_OK;
+    hipError_t r = hipMalloc(p, n);
+    if (r != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return bfail(SQE_ENOMEM, "hipMalloc(" + std::to_string(n) + ") failed");
+    }
+    b->bytes_dev += n;
+    return SQE_OK;
+}
+void dfr(sqe_batch* b, void* p, size_t n) {
+    if (p) {
+        (void)hipFree(p);
+        b->bytes_dev -= n;
+    }
+}
+
+// Grow a pool so that `need` bytes fit, keeping its used contents.
+sqe_status pool_fit(sqe_batch* b, Pool& pl, uint64_t need, uint64_t keep) {
+    if (need <= pl.bytes) return SQE_OK;
+    const uint64_t nb = std::max<uint64_t>(p2(need), 1ull << 24);
+    char* np = nullptr;
+    BCK(dal(b, (void**)&np, nb));
+    if (pl.p && keep) BHIP(hipMemcpyAsync(np, pl.p, std::min(keep, pl.bytes), hipMemcpyDeviceToDevice, b->st));
+    BHIP(hipStreamSynchronize(b->st));
+    dfr(b, pl.p, pl.bytes);
+    pl.p = np;
+    pl.bytes = nb;
+    return SQE_OK;
+}
+
+sqe_status slots_fit(sqe_batch* b, uint64_t need) {
+    if (need <= b->nslot) return SQE_OK;
+    const uint64_t n = std::max<uint64_t>(p2(need), 256);
+    BHIP(hipStreamSynchronize(b->st));
+    auto regrow = [&](void** ptr, size_t elem) -> sqe_status {
+        void* np = nullptr;
+        BCK(dal(b, &np, n * elem));
+        BHIP(hipMemset(np, 0, n * elem));
+        if (*ptr) BHIP(hipMemcpy(np, *ptr, b->nslot * elem, hipMemcpyDeviceToDevice));
+        dfr(b, *ptr, b->nslot * elem);
+        *ptr = np;
+        return SQE_OK;
+    };
+    BCK(regrow((void**)&b->upd, 8));
+    BCK(regrow((void**)&b->nnz, 8));
+    BCK(regrow((void**)&b->meta, 8));
+    BCK(regrow((void**)&b->norm2, 8));
+    BCK(regrow((void**)&b->scale, 8));
+    b->nslot = n;
+    return SQE_OK;
+}
+
+template <class T>
+sqe_status buf_fit(sqe_batch* b, T** p, uint64_t* cap, uint64_t need) {
+    if (need <= *cap) return SQE_OK;
+    BHIP(hipStreamSynchronize(b->st));
+    dfr(b, *p, *cap * sizeof(T));
+    *p = nullptr;
+    const uint64_t n = std::max<uint64_t>(p2(need), 1024);
+    BCK(dal(b, (void**)p, n * sizeof(T)));
+    *cap = n;
+    return SQE_OK;
+}
+
+sqe_status next_scan(sqe_batch* b, uint32_t ntiles, ScanCtl* out) {
+    if (ntiles > b->scan_cap) {
+        BHIP(hipStreamSynchronize(b->st));
+        dfr(b, b->scan_status, b->scan_cap * 8);
+        b->scan_status = nullptr;
+        const uint64_t cap = std::max<uint64_t>(1024, p2(ntiles));
+        BCK(dal(b, (void**)&b->scan_status, cap * 8));
+        b->scan_cap = cap;
+        BHIP(launch_fill_u64(b->st, (uint64_t*)b->scan_status, cap, 0));
+        b->epoch = 0;
+    }
+    if (++b->epoch >= (1u << 22) - 1) {
+        BHIP(launch_fill_u64(b->st, (uint64_t*)b->scan_status, b->scan_cap, 0));
+        b->epoch = 1;
+    }
+    *out = ScanCtl{b->scan_status, b->ticket, &b->stats->err, b->epoch};
+    return SQE_OK;
+}
+
+sqe_status check_err(sqe_batch* b) {
+    unsigned long long err = 0;
+    BHIP(hipMemcpyAsync(&err, &b->stats->err, 8, hipMemcpyDeviceToHost, b->st));
+    BHIP(hipStreamSynchronize(b->st));
+    if (err) {
+        const unsigned long long z = 0;
+        BHIP(hipMemcpy(&b->stats->err, &z, 8, hipMemcpyHostToDevice));
+        return bfail((err & ERR_DUP) ? SQE_EINVAL : SQE_EHIP,
+                     (err & ERR_DUP) ? "duplicate key in load" : "device-side error in batch");
+    }
+    return SQE_OK;
+}
+
+// FP64 key / re / im staging arrays of at least `need` entries.
+sqe_status stage_fit(sqe_batch* b, uint64_t need) {
+    if (need <= b->xcap) return SQE_OK;
+    BHIP(hipStreamSynchronize(b->st));
+    dfr(b, b->xkeys, b->xcap * 8);
+    dfr(b, b->xre, b->xcap * 8);
+    dfr(b, b->xim, b->xcap * 8);
+    b->xkeys = nullptr;
+    b->xre = nullptr;
+    b->xim = nullptr;
+    b->xcap = 0;
+    const uint64_t nc = std::max<uint64_t>(1024, p2(need));
+    BCK(dal(b, (void**)&b->xkeys, nc * 8));
+    BCK(dal(b, (void**)&b->xre, nc * 8));
+    BCK(dal(b, (void**)&b->xim, nc * 8));
+    b->xcap = nc;
+    return SQE_OK;
+}
+
+char* amp_ptr(sqe_batch* b, const Circ& c) { return b->amp.p + c.row0 * 64 * b->csize; }
+uint64_t* base_ptr(sqe_batch* b, const Circ& c) { return (uint64_t*)b->base.p + c.tile0 * 64; }
+
+void release(sqe_batch* b, uint32_t id) {
+    Circ& c = b->c[id];
+    if (!c.live) return;
+    b->ta.give(c.tile0, c.tiles);
+    b->ra.give(c.row0, c.rows);
+    if (c.nops) b->pa.give(c.prog0, c.nops);
+    if (c.ncoef) b->ca.give(c.coef0, c.ncoef);
+    b->used -= c.bytes;
+    c = Circ{};
+    b->free_slots.push_back(id);
+}
+
+// Launch the sweeps for one step: every live circuit whose program is not
+// finished runs ops [pos, end).  Returns the circuits that stopped before a
+// RENORM op (they need a norm).
+sqe_status run_step(sqe_batch* b, uint64_t max_ops, std::vector<uint32_t>* need_norm) {
+    struct Seg { uint32_t id; uint64_t from, to; };
+    std::vector<Seg> segs;
+    for (uint32_t id = 0; id < b->c.size(); ++id) {
+        Circ& c = b->c[id];
+        if (!c.live || c.pos >= c.nops) continue;
+        uint64_t end = max_ops ? std::min<uint64_t>(c.nops, c.pos + max_ops) : c.nops;
+        for (uint32_t r : c.renorm_at)
+            if (r > c.pos && r < end) { end = r; break; }
+        segs.push_back(Seg{id, c.pos, end});
+    }
+    if (segs.empty()) return SQE_OK;
+    // group by bucket size; longest segment first within the item queue
+    std::sort(segs.begin(), segs.end(), [](const Seg& x, const Seg& y) { return x.to - x.from > y.to - y.from; });
+    for (int mb = 0; mb <= 5; ++mb) {
+        std::vector<BTask> tasks;
+        std::vector<BItem> items;
+        for (const Seg& s : segs) {
+            const Circ& c = b->c[s.id];
+            if (c.qm.mb != mb) continue;
+            BTask t{};
+            t.base = base_ptr(b, c);
+            t.amp = amp_ptr(b, c);
+            t.ops = (const MOp*)b->prog.p + c.prog0 + s.from;
+            t.coef = (const double*)b->coef.p + c.coef0;
+            t.scale = b->scale + s.id;
+            t.upd = b->upd + s.id;
+            t.nnz = b->nnz + s.id;
+            t.norm2 = b->norm2 + s.id;
+            t.nops = (uint32_t)(s.to - s.from);
+            t.ntiles = (uint32_t)c.tiles;
+            const uint32_t ti = (uint32_t)tasks.size();
+            tasks.push_back(t);
+            for (uint64_t tl = 0; tl < c.tiles; tl += b->gran) items.push_back(BItem{ti, (uint32_t)tl});
+        }
+        if (tasks.empty()) continue;
+        // items are already in longest-program-first order (segments were sorted)
+        if (tasks.size() > b->hcap_t) {
+            if (b->htask) (void)hipHostFree(b->htask);
+            b->hcap_t = p2(tasks.size());
+            BHIP(hipHostMalloc((void**)&b->htask, b->hcap_t * sizeof(BTask), hipHostMallocDefault));
+        }
+        if (items.size() > b->hcap_i) {
+            if (b->hitem) (void)hipHostFree(b->hitem);
+            b->hcap_i = p2(items.size());
+            BHIP(hipHostMalloc((void**)&b->hitem, b->hcap_i * sizeof(BItem), hipHostMallocDefault));
+        }
+        BCK(buf_fit(b, &b->dtask, &b->dcap_t, tasks.size()));
+        BCK(buf_fit(b, &b->ditem, &b->dcap_i, items.size()));
+        BHIP(hipStreamSynchronize(b->st));          // the pinned staging is reused
+        std::memcpy(b->htask, tasks.data(), tasks.size() * sizeof(BTask));
+        std::memcpy(b->hitem, items.data(), items.size() * sizeof(BItem));
+        BHIP(hipMemcpyAsync(b->dtask, b->htask, tasks.size() * sizeof(BTask), hipMemcpyHostToDevice, b->st));
+        BHIP(hipMemcpyAsync(b->ditem, b->hitem, items.size() * sizeof(BItem), hipMemcpyHostToDevice, b->st));
+        BHIP(launch_fill_u64(b->st, (uint64_t*)b->queue, 1, 0));
+        BHIP(launch_batch_zero(b->st, b->dtask, (uint32_t)tasks.size()));
+        BatchArgs a{b->dtask, b->ditem, (uint32_t)items.size(), b->gran, b->queue, b->thr};
+        const uint32_t waves = std::min<uint32_t>((uint32_t)items.size(), b->grid_waves);
+        const uint32_t wpb = b->block / 64;
+        const uint32_t grid = std::max<uint32_t>(1, (waves + wpb - 1) / wpb);
+        if (b->cfg.timing) BHIP(hipEventRecord(b->ev0, b->st));
+        BHIP(launch_sweep_batch(mb, b->f32, b->st, grid, b->block, a));
+        if (b->cfg.timing) {
+            BHIP(hipEventRecord(b->ev1, b->st));
+            BHIP(hipEventSynchronize(b->ev1));
+            float ms = 0.f;
+            BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            b->sweep_ms += ms;
+        }
+        ++b->launches;
+        // FP32 renormalisation points reached by this launch
+        for (const Seg& s : segs) {
+            const Circ& c = b->c[s.id];
+            if (c.qm.mb != mb) continue;
+            bool at_renorm = false;
+            for (uint32_t r : c.renorm_at) at_renorm |= (r == s.to);
+            if (at_renorm && need_norm) need_norm->push_back(s.id);
+        }
+    }
+    for (const Seg& s : segs) b->c[s.id].pos = s.to;
+    return SQE_OK;
+}
+
+sqe_status run_norms(sqe_batch* b, const std::vector<uint32_t>& ids) {
+    if (ids.empty()) return SQE_OK;
+    for (int mb = 0; mb <= 5; ++mb) {
+        std::vector<BTask> tasks;
+        std::vector<BItem> items;
+        for (uint32_t id : ids) {
+            const Circ& c = b->c[id];
+            if (c.qm.mb != mb) continue;
+            BTask t{};
+            t.amp = amp_ptr(b, c);
+            t.norm2 = b->norm2 + id;
+            t.ntiles = (uint32_t)c.tiles;
+            const uint32_t ti = (uint32_t)tasks.size();
+            tasks.push_back(t);
+            for (uint64_t tl = 0; tl < c.tiles; tl += 4) items.push_back(BItem{ti, (uint32_t)tl});
+        }
+        if (tasks.empty()) continue;
+        BCK(buf_fit(b, &b->dtask, &b->dcap_t, tasks.size()));
+        BCK(buf_fit(b, &b->ditem, &b->dcap_i, items.size()));
+        BHIP(hipStreamSynchronize(b->st));
+        BHIP(hipMemcpy(b->dtask, tasks.data(), tasks.size() * sizeof(BTask), hipMemcpyHostToDevice));
+        BHIP(hipMemcpy(b->ditem, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice));
+        for (uint32_t id : ids)
+            if (b->c[id].qm.mb == mb) BHIP(launch_fill_u64(b->st, (uint64_t*)(b->norm2 + id), 1, 0));
+        BatchArgs a{b->dtask, b->ditem, (uint32_t)items.size(), 4, b->queue, b->thr};
+        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(4096, ((uint32_t)items.size() + 3) / 4));
+        BHIP(launch_norm_batch(mb, b->f32, b->st, grid, a));
+    }
+    BHIP(launch_batch_scale(b->st, b->norm2, b->scale, (uint32_t)b->nslot));
+    return SQE_OK;
+}
+
+sqe_status step_all(sqe_batch* b, uint64_t max_gates) {
+    // a gate budget of N ops per circuit (RENORM ops ride along)
+    std::vector<uint32_t> nn;
+    BCK(run_step(b, max_gates, &nn));
+    BCK(run_norms(b, nn));
+    ++b->steps;
+    return SQE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+sqe_status sqe_batch_create(const sqe_config* cfg, sqe_batch** out) {
+    if (!cfg || !out) return bfail(SQE_EINVAL, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return bfail(SQE_EDEVICE, "no HIP device");
+    }
+    if (cfg->prec != SQE_F64 && cfg->prec != SQE_F32) return bfail(SQE_EINVAL, "bad precision");
+    if (cfg->device < 0 || cfg->device >= ndev) return bfail(SQE_EINVAL, "bad device ordinal");
+    BHIP(hipSetDevice(cfg->device));
+    sqe_batch* b = new (std::nothrow) sqe_batch();
+    if (!b) return bfail(SQE_ENOMEM, "host allocation");
+    b->cfg = *cfg;
+    b->f32 = cfg->prec == SQE_F32;
+    b->csize = b->f32 ? 8 : 16;
+    b->thr = cfg->prune_thr > 0 ? cfg->prune_thr : (b->f32 ? 1e-7 : 1e-14);
+    if (!b->cfg.hbm_budget_bytes) {
+        size_t fr = 0, tot = 0;
+        (void)hipMemGetInfo(&fr, &tot);
+        b->cfg.hbm_budget_bytes = (uint64_t)(0.8 * (double)fr);
+    }
+    hipDeviceProp_t prop{};
+    (void)hipGetDeviceProperties(&prop, cfg->device);
+    const uint32_t cus = prop.multiProcessorCount > 0 ? (uint32_t)prop.multiProcessorCount : 256;
+    b->grid_waves = cus * 4 * 3;                 // 3 wavefronts per SIMD of the sweep kernels
+    sqe_status r = SQE_OK;
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) r = bfail(SQE_EHIP, "stream");
+    if (r == SQE_OK) r = dal(b, (void**)&b->stats, sizeof(Stats));
+    if (r == SQE_OK) r = dal(b, (void**)&b->ticket, 64);
+    if (r == SQE_OK) r = dal(b, (void**)&b->queue, 64);
+    if (r == SQE_OK) r = dal(b, (void**)&b->total, 64);
+    if (r == SQE_OK && hipMemset(b->stats, 0, sizeof(Stats)) != hipSuccess) r = bfail(SQE_EHIP, "memset");
+    if (r == SQE_OK && hipMemset(b->ticket, 0, 64) != hipSuccess) r = bfail(SQE_EHIP, "memset");
+    if (r == SQE_OK && (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess))
+        r = bfail(SQE_EHIP, "event");
+    if (r == SQE_OK) r = slots_fit(b, 256);
+    if (r != SQE_OK) {
+        sqe_batch_destroy(b);
+        return r;
+    }
+    *out = b;
+    return SQE_OK;
+}
+
+sqe_status sqe_batch_destroy(sqe_batch* b) {
+    if (!b) return SQE_OK;
+    (void)hipSetDevice(b->cfg.device);
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    for (void* p : {(void*)b->base.p, (void*)b->amp.p, (void*)b->prog.p, (void*)b->coef.p, (void*)b->upd,
+                    (void*)b->nnz, (void*)b->meta, (void*)b->norm2, (void*)b->scale, (void*)b->tab,
+                    (void*)b->xkeys, (void*)b->xre, (void*)b->xim, (void*)b->occ, (void*)b->stats,
+                    (void*)b->scan_status, (void*)b->ticket, (void*)b->total, (void*)b->dtask, (void*)b->ditem,
+                    (void*)b->queue})
+        if (p) (void)hipFree(p);
+    if (b->htask) (void)hipHostFree(b->htask);
+    if (b->hitem) (void)hipHostFree(b->hitem);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->st) (void)hipStreamDestroy(b->st);
+    delete b;
+    return SQE_OK;
+}
+
+sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const double* re, const double* im,
+                         uint64_t n, const sqe_gate* gates, uint64_t ng, int64_t* id_out) {
+    if (!b || !id_out) return bfail(SQE_EINVAL, "null argument");
+    if (nq < 1 || nq > 63) return bfail(SQE_EINVAL, "nq must be in 1..63");
+    if ((n && (!keys || !re || !im)) || (ng && !gates)) return bfail(SQE_EINVAL, "null array");
+    BHIP(hipSetDevice(b->cfg.device));
+    const uint64_t lim = (1ull << nq) - 1;
+    for (uint64_t j = 0; j < ng; ++j) {
+        const sqe_gate& x = gates[j];
+        if (x.kind > SQE_GATE_PERM || x.target >= nq || (x.ctrl_mask & ~lim) || ((x.ctrl_mask >> x.target) & 1ull) ||
+            (x.ctrl_val & ~x.ctrl_mask))
+            return bfail(SQE_EINVAL, "gate " + std::to_string(j) + " is invalid");
+    }
+    const QMap qm = choose_map((int)nq, b->cfg.bucket_bits, gates, ng);
+    for (uint64_t j = 0; j < ng; ++j)
+        if (!gate_local(gates[j], qm))
+            return bfail(SQE_EINVAL, "gate " + std::to_string(j) +
+                                         " crosses buckets: run this circuit with the single-circuit engine");
+    std::vector<uint64_t> k;
+    std::vector<double> r, i;
+    k.reserve(n); r.reserve(n); i.reserve(n);
+    for (uint64_t j = 0; j < n; ++j) {
+        if (keys[j] & ~lim) return bfail(SQE_EINVAL, "key has bits beyond nq");
+        if (re[j] == 0.0 && im[j] == 0.0) continue;
+        k.push_back(keys[j]); r.push_back(re[j]); i.push_back(im[j]);
+    }
+    const uint64_t m = k.size();
+    // program: gates, with a RENORM op after every renorm_every-th gate
+    std::vector<MOp> prog;
+    std::vector<double> coef;
+    std::vector<uint32_t> renorm_at;
+    prog.reserve(ng + 8);
+    for (uint64_t j = 0; j < ng; ++j) {
+        prog.push_back(encode_op(plan_op(gates[j], qm), coef));
+        if (b->cfg.renorm_every && (j + 1) % b->cfg.renorm_every == 0) {
+            renorm_at.push_back((uint32_t)prog.size());
+            prog.push_back(encode_op(renorm_op(), coef));
+        }
+    }
+    // region sizes and admission
+    const uint64_t S = 1ull << qm.mb;
+    const uint64_t tiles = std::max<uint64_t>(1, (m + 63) / 64);
+    const uint64_t rows = tiles * S;
+    const uint64_t bytes = tiles * 64 * 8 + rows * 64 * b->csize + prog.size() * sizeof(MOp) + coef.size() * 8;
+    if (b->used + bytes > b->cfg.hbm_budget_bytes)
+        return bfail(SQE_ETABLE_FULL, "admission: circuit needs " + std::to_string(bytes) + " bytes, " +
+                                          std::to_string(b->cfg.hbm_budget_bytes - b->used) + " left in the budget");
+    uint32_t id;
+    if (!b->free_slots.empty()) { id = b->free_slots.back(); b->free_slots.pop_back(); }
+    else { id = (uint32_t)b->c.size(); b->c.emplace_back(); }
+    BCK(slots_fit(b, id + 1));
+    Circ c;
+    c.nq = nq;
+    c.qm = qm;
+    c.tiles = tiles;
+    c.rows = rows;
+    c.nops = prog.size();
+    c.ncoef = coef.size();
+    c.renorm_at = renorm_at;
+    c.gates = ng;
+    c.bytes = bytes;
+    auto place = [&](RangeAlloc& al, Pool& pl, uint64_t units, uint64_t unit_bytes, uint64_t* at) -> sqe_status {
+        if (!units) { *at = 0; return SQE_OK; }
+        if (!al.take(units, at)) {
+            const uint64_t want = std::max<uint64_t>(al.cap * 2, al.cap + units);
+            BCK(pool_fit(b, pl, want * unit_bytes, al.used_extent() * unit_bytes));
+            al.grow(pl.bytes / unit_bytes);
+            if (!al.take(units, at)) return bfail(SQE_ENOMEM, "pool allocation");
+        }
+        return SQE_OK;
+    };
+    sqe_status st = place(b->ta, b->base, tiles, 64 * 8, &c.tile0);
+    if (st == SQE_OK) st = place(b->ra, b->amp, rows, 64 * b->csize, &c.row0);
+    if (st == SQE_OK) st = place(b->pa, b->prog, c.nops, sizeof(MOp), &c.prog0);
+    if (st == SQE_OK) st = place(b->ca, b->coef, c.ncoef, 8, &c.coef0);
+    if (st != SQE_OK) {
+        b->free_slots.push_back(id);
+        return st;
+    }
+    c.live = true;
+    b->c[id] = c;
+    b->used += bytes;
+    Circ& cc = b->c[id];
+    // upload program and coefficients
+    if (cc.nops) BHIP(hipMemcpyAsync(b->prog.p + cc.prog0 * sizeof(MOp), prog.data(), cc.nops * sizeof(MOp),
+                                     hipMemcpyHostToDevice, b->st));
+    if (cc.ncoef) BHIP(hipMemcpyAsync(b->coef.p + cc.coef0 * 8, coef.data(), cc.ncoef * 8, hipMemcpyHostToDevice,
+                                      b->st));
+    // load the state into the region (insert bases, number buckets, scatter)
+    BCK(stage_fit(b, std::max<uint64_t>(m, 1)));
+    if (m) {
+        BHIP(hipMemcpyAsync(b->xkeys, k.data(), m * 8, hipMemcpyHostToDevice, b->st));
+        BHIP(hipMemcpyAsync(b->xre, r.data(), m * 8, hipMemcpyHostToDevice, b->st));
+        BHIP(hipMemcpyAsync(b->xim, i.data(), m * 8, hipMemcpyHostToDevice, b->st));
+    }
+    const double ml = b->cfg.max_load > 0 && b->cfg.max_load < 0.95 ? b->cfg.max_load : 0.5;
+    const uint64_t tneed = std::max<uint64_t>(1024, p2((uint64_t)std::ceil((double)m / ml) + 8));
+    if (tneed > b->tcap) {
+        BHIP(hipStreamSynchronize(b->st));
+        dfr(b, b->tab, b->tcap * sizeof(Slot));
+        b->tab = nullptr;
+        BCK(dal(b, (void**)&b->tab, tneed * sizeof(Slot)));
+        b->tcap = tneed;
+    }
+    BCK(buf_fit(b, &b->occ, &b->occ_cap, tiles * 64));
+    BHIP(launch_fill_slots(b->st, b->tab, b->tcap));
+    BHIP(launch_fill_u64(b->st, base_ptr(b, cc), tiles * 64, EMPTY_KEY));
+    BHIP(launch_fill_u64(b->st, (uint64_t*)amp_ptr(b, cc), rows * 64 * b->csize / 8, 0));
+    BHIP(launch_fill_u64(b->st, (uint64_t*)b->occ, (tiles * 64 + 1) / 2, 0));
+    BHIP(launch_fill_u64(b->st, (uint64_t*)(b->upd + id), 1, 0));
+    BHIP(launch_fill_u64(b->st, (uint64_t*)(b->nnz + id), 1, 0));
+    LoadArgs la{};
+    la.keys = b->xkeys; la.re = b->xre; la.im = b->xim; la.n = m; la.qm = qm;
+    la.tab = b->tab; la.tmask = b->tcap - 1; la.amp = amp_ptr(b, cc); la.occ = b->occ; la.stats = b->stats;
+    EnumArgs ea{};
+    ea.tab = b->tab; ea.tcap = b->tcap; ea.out_base = base_ptr(b, cc); ea.out_nb = b->meta + id;
+    ea.out_cap = tiles * 64; ea.stats = b->stats;
+    const uint32_t eg = (uint32_t)((b->tcap + BLOCK - 1) / BLOCK);
+    BCK(next_scan(b, eg, &ea.scan));
+    BHIP(launch_load(b->f32, b->st, la, ea, eg));
+    const sqe_status e = check_err(b);
+    if (e != SQE_OK) {
+        release(b, id);
+        return e;
+    }
+    *id_out = id;
+    return SQE_OK;
+}
+
+sqe_status sqe_batch_step(sqe_batch* b, uint32_t max_gates) {
+    if (!b) return bfail(SQE_EINVAL, "null batch");
+    BHIP(hipSetDevice(b->cfg.device));
+    return step_all(b, max_gates);
+}
+
+sqe_status sqe_batch_run(sqe_batch* b) {
+    if (!b) return bfail(SQE_EINVAL, "null batch");
+    BHIP(hipSetDevice(b->cfg.device));
+    for (;;) {
+        bool more = false;
+        for (const Circ& c : b->c) more |= c.live && c.pos < c.nops;
+        if (!more) break;
+        BCK(step_all(b, 0));
+    }
+    return SQE_OK;
+}
+
+sqe_status sqe_batch_sync(sqe_batch* b) {
+    if (!b) return bfail(SQE_EINVAL, "null batch");
+    BHIP(hipSetDevice(b->cfg.device));
+    BHIP(hipStreamSynchronize(b->st));
+    return check_err(b);
+}
+
+sqe_status sqe_batch_info(sqe_batch* b, int64_t id, sqe_batch_circuit* out) {
+    if (!b || !out || id < 0 || (uint64_t)id >= b->c.size() || !b->c[id].live)
+        return bfail(SQE_EINVAL, "unknown circuit");
+    BHIP(hipSetDevice(b->cfg.device));
+    BHIP(hipStreamSynchronize(b->st));
+    const Circ& c = b->c[id];
+    unsigned long long v[3] = {0, 0, 0};
+    BHIP(hipMemcpy(&v[0], b->upd + id, 8, hipMemcpyDeviceToHost));
+    BHIP(hipMemcpy(&v[1], b->nnz + id, 8, hipMemcpyDeviceToHost));
+    BHIP(hipMemcpy(&v[2], b->meta + id, 8, hipMemcpyDeviceToHost));
+    out->done = c.pos >= c.nops ? 1 : 0;
+    out->gates_total = c.gates;
+    uint64_t gd = 0;
+    for (uint64_t j = 0; j < c.pos; ++j) {
+        bool rn = false;
+        for (uint32_t r : c.renorm_at) rn |= (r == j);
+        gd += rn ? 0 : 1;
+    }
+    out->gates_done = gd;
+    out->updates = v[0];
+    out->nnz = v[1];
+    out->buckets = v[2];
+    out->bucket_bits = c.qm.mb;
+    out->bytes = c.bytes;
+    return SQE_OK;
+}
+
+sqe_status sqe_batch_export(sqe_batch* b, int64_t id, uint64_t* keys, double* re, double* im, uint64_t cap,
+                            uint64_t* n) {
+    if (!b || !n || id < 0 || (uint64_t)id >= b->c.size() || !b->c[id].live)
+        return bfail(SQE_EINVAL, "unknown circuit");
+    BHIP(hipSetDevice(b->cfg.device));
+    const Circ& c = b->c[id];
+    unsigned long long nb = 0;
+    BHIP(hipStreamSynchronize(b->st));
+    BHIP(hipMemcpy(&nb, b->meta + id, 8, hipMemcpyDeviceToHost));
+    const uint64_t bound = std::max<uint64_t>(nb << c.qm.mb, 1);
+    BCK(stage_fit(b, bound));
+    ExportArgs xa{};
+    xa.base = base_ptr(b, c); xa.amp = amp_ptr(b, c); xa.nb = b->meta + id; xa.qm = c.qm; xa.thr = b->thr;
+    xa.filter = 1; xa.keys = b->xkeys; xa.re = b->xre; xa.im = b->xim; xa.cap = b->xcap; xa.total = b->total;
+    const uint32_t g = std::max<uint32_t>(1, (uint32_t)((((nb + 63) & ~63ull) + BLOCK - 1) / BLOCK));
+    BCK(next_scan(b, g, &xa.scan));
+    BHIP(launch_export(c.qm.mb, b->f32, b->st, g, xa));
+    unsigned long long tot = 0;
+    BHIP(hipMemcpyAsync(&tot, b->total, 8, hipMemcpyDeviceToHost, b->st));
+    BHIP(hipStreamSynchronize(b->st));
+    *n = tot;
+    if (tot > cap) return bfail(SQE_EINVAL, "export capacity too small (" + std::to_string(tot) + " needed)");
+    if (tot) {
+        if (!keys || !re || !im) return bfail(SQE_EINVAL, "null array");
+        BHIP(hipMemcpy(keys, b->xkeys, tot * 8, hipMemcpyDeviceToHost));
+        BHIP(hipMemcpy(re, b->xre, tot * 8, hipMemcpyDeviceToHost));
+        BHIP(hipMemcpy(im, b->xim, tot * 8, hipMemcpyDeviceToHost));
+    }
+    return check_err(b);
+}
+
+sqe_status sqe_batch_remove(sqe_batch* b, int64_t id) {
+    if (!b || id < 0 || (uint64_t)id >= b->c.size() || !b->c[id].live) return bfail(SQE_EINVAL, "unknown circuit");
+    BHIP(hipSetDevice(b->cfg.device));
+    BHIP(hipStreamSynchronize(b->st));
+    release(b, (uint32_t)id);
+    return SQE_OK;
+}
+
+sqe_status sqe_batch_usage(sqe_batch* b, uint64_t* used, uint64_t* budget, uint64_t* active) {
+    if (!b) return bfail(SQE_EINVAL, "null batch");
+    if (used) *used = b->used;
+    if (budget) *budget = b->cfg.hbm_budget_bytes;
+    if (active) {
+        uint64_t a = 0;
+        for (const Circ& c : b->c) a += c.live && c.pos < c.nops ? 1 : 0;
+        *active = a;
+    }
+    return SQE_OK;
+}
+
+sqe_status sqe_batch_get_tuning(sqe_batch* b, sqe_batch_tuning* t) {
+    if (!b || !t) return bfail(SQE_EINVAL, "null argument");
+    t->block = b->block;
+    t->tiles_per_item = b->gran;
+    t->waves = b->grid_waves;
+    t->load_factor = b->cfg.max_load > 0 ? b->cfg.max_load : 0.5;
+    t->sweep_ms = b->sweep_ms;
+    t->launches = b->launches;
+    return SQE_OK;
+}
+
+sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t) {
+    if (!b || !t) return bfail(SQE_EINVAL, "null argument");
+    if (t->block != 64 && t->block != 128 && t->block != 256) return bfail(SQE_EINVAL, "block must be 64/128/256");
+    if (t->tiles_per_item < 1 || t->tiles_per_item > 64) return bfail(SQE_EINVAL, "tiles_per_item in 1..64");
+    b->block = t->block;
+    b->gran = t->tiles_per_item;
+    if (t->waves) b->grid_waves = t->waves;
+    if (t->load_factor > 0 && t->load_factor < 0.95) b->cfg.max_load = t->load_factor;
+    return SQE_OK;
+}
+
+// Time every (workgroup size, tiles per item, resident waves) candidate on the
+// batch as it stands -- state and positions are saved and restored around each
+// trial -- and keep the fastest.
+sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
+    if (!b) return bfail(SQE_EINVAL, "null batch");
+    BHIP(hipSetDevice(b->cfg.device));
+    BHIP(hipStreamSynchronize(b->st));
+    const uint64_t bext = b->ta.used_extent() * 64 * 8, aext = b->ra.used_extent() * 64 * b->csize;
+    char *sb = nullptr, *sa = nullptr;
+    BCK(dal(b, (void**)&sb, std::max<uint64_t>(bext, 8)));
+    BCK(dal(b, (void**)&sa, std::max<uint64_t>(aext, 8)));
+    if (bext) BHIP(hipMemcpy(sb, b->base.p, bext, hipMemcpyDeviceToDevice));
+    if (aext) BHIP(hipMemcpy(sa, b->amp.p, aext, hipMemcpyDeviceToDevice));
+    std::vector<uint64_t> pos;
+    for (const Circ& c : b->c) pos.push_back(c.pos);
+    const uint32_t cus = b->grid_waves / 12;
+    struct Cand { uint32_t block, gran, waves; };
+    std::vector<Cand> cands;
+    for (uint32_t bl : {64u, 128u, 256u})
+        for (uint32_t g : {1u, 2u, 4u})
+            for (uint32_t wv : {cus * 8, cus * 12}) cands.push_back(Cand{bl, g, wv});
+    double best = 1e300;
+    Cand bc{b->block, b->gran, b->grid_waves};
+    const uint32_t timing = b->cfg.timing;
+    b->cfg.timing = 0;
+    for (const Cand& cd : cands) {
+        b->block = cd.block;
+        b->gran = cd.gran;
+        b->grid_waves = cd.waves;
+        BHIP(hipEventRecord(b->ev0, b->st));
+        BCK(sqe_batch_run(b));
+        BHIP(hipEventRecord(b->ev1, b->st));
+        BHIP(hipEventSynchronize(b->ev1));
+        float ms = 0.f;
+        BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        if (ms < best) { best = ms; bc = cd; }
+        if (bext) BHIP(hipMemcpyAsync(b->base.p, sb, bext, hipMemcpyDeviceToDevice, b->st));
+        if (aext) BHIP(hipMemcpyAsync(b->amp.p, sa, aext, hipMemcpyDeviceToDevice, b->st));
+        BHIP(hipStreamSynchronize(b->st));
+        for (size_t j = 0; j < b->c.size(); ++j) b->c[j].pos = pos[j];
+    }
+    b->cfg.timing = timing;
+    dfr(b, sb, std::max<uint64_t>(bext, 8));
+    dfr(b, sa, std::max<uint64_t>(aext, 8));
+    BHIP(launch_fill_u64(b->st, (uint64_t*)b->upd, b->nslot, 0));
+    b->block = bc.block;
+    b->gran = bc.gran;
+    b->grid_waves = bc.waves;
+    if (best_out) {
+        BCK(sqe_batch_get_tuning(b, best_out));
+        best_out->sweep_ms = best;
+    }
+    return SQE_OK;
+}
+
+const char* sqe_batch_last_error(void) { return b_err.c_str(); }
+
+}  // extern "C"

@@ -10,6 +10,7 @@
 // Norm reduction, controlled gates, hashing of 64-bit keys and the FP32 form
 // have no reference counterpart.
 #pragma once
+#include <algorithm>
 #include "sqe_device.hpp"
 #include "sqe_launch.hpp"
 
@@ -88,124 +89,143 @@ struct PairIdx {
     static constexpr int s1 = s0 | (1 << J);
 };
 
-// Slot handlers with compile-time target bit J.
-template <int MB, class R, int J>
-__device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&a)[1 << MB], bool pred, double thr,
-                                             const double* __restrict__ coef) {
+// Slot handlers with compile-time target bit J, applied to the EPT buckets a
+// thread owns (a[e], predicate pred[e]).
+template <int MB, class R, int EPT, int J>
+__device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
+                                             const bool (&pred)[EPT], double thr, const double* __restrict__ coef) {
     using C = typename CT<R>::T;
     constexpr int HALF = (1 << MB) / 2;
     constexpr bool F32 = sizeof(R) == 4;
     const bool pall = op.cmB == 0;               // every bucket takes the gate
     switch (op.h) {
     case H_SLOT_DIAG: {
-        // factor of slot bit J; a lane whose base controls fail takes 1
-        R f0r = (R)op.c[0], f0i = (R)op.c[1], f1r = (R)op.c[2], f1i = (R)op.c[3];
-        if (!pall) {
-            f0r = pred ? f0r : (R)1; f0i = pred ? f0i : (R)0;
-            f1r = pred ? f1r : (R)1; f1i = pred ? f1i : (R)0;
-        }
-        static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
-            constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
-            constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
-            if (((uint32_t)s0 & op.cmS) == op.cvS) {
-                a[s0] = rot(f0r, f0i, a[s0]);
-                a[s1] = rot(f1r, f1i, a[s1]);
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+            constexpr int e = decltype(ec)::value;
+            // factor of slot bit J; a lane whose base controls fail takes 1
+            R f0r = (R)op.c[0], f0i = (R)op.c[1], f1r = (R)op.c[2], f1i = (R)op.c[3];
+            if (!pall) {
+                f0r = pred[e] ? f0r : (R)1; f0i = pred[e] ? f0i : (R)0;
+                f1r = pred[e] ? f1r : (R)1; f1i = pred[e] ? f1i : (R)0;
             }
+            static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+                constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+                constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+                if (((uint32_t)s0 & op.cmS) == op.cvS) {
+                    a[e][s0] = rot(f0r, f0i, a[e][s0]);
+                    a[e][s1] = rot(f1r, f1i, a[e][s1]);
+                }
+            });
         });
     } break;
     case H_SLOT_X:
-        static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
-            constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
-            constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
-            if (((uint32_t)s0 & op.cmS) == op.cvS) {
-                const C x = a[s0], y = a[s1];
-                if (pall) { a[s0] = y; a[s1] = x; }
-                else { a[s0] = csel(pred, y, x); a[s1] = csel(pred, x, y); }
-            }
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+            constexpr int e = decltype(ec)::value;
+            static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+                constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+                constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+                if (((uint32_t)s0 & op.cmS) == op.cvS) {
+                    const C x = a[e][s0], y = a[e][s1];
+                    if (pall) { a[e][s0] = y; a[e][s1] = x; }
+                    else { a[e][s0] = csel(pred[e], y, x); a[e][s1] = csel(pred[e], x, y); }
+                }
+            });
         });
         break;
     case H_SLOT_PERM: {
         const R u10r = (R)op.c[0], u10i = (R)op.c[1], u01r = (R)op.c[2], u01i = (R)op.c[3];
-        static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
-            constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
-            constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
-            if (((uint32_t)s0 & op.cmS) == op.cvS) {
-                const C x = a[s0], y = a[s1];
-                const C n0 = rot(u01r, u01i, y), n1 = rot(u10r, u10i, x);
-                a[s0] = csel(pred || pall, n0, x);
-                a[s1] = csel(pred || pall, n1, y);
-            }
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+            constexpr int e = decltype(ec)::value;
+            static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+                constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+                constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+                if (((uint32_t)s0 & op.cmS) == op.cvS) {
+                    const C x = a[e][s0], y = a[e][s1];
+                    const C n0 = rot(u01r, u01i, y), n1 = rot(u10r, u10i, x);
+                    a[e][s0] = csel(pred[e] || pall, n0, x);
+                    a[e][s1] = csel(pred[e] || pall, n1, y);
+                }
+            });
         });
     } break;
     case H_SLOT_H: {
         const R f = (R)op.c[0];
-        static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
-            constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
-            constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
-            if (((uint32_t)s0 & op.cmS) == op.cvS) {
-                const C x = a[s0], y = a[s1];
-                C n0, n1;
-                hpair(f, x, y, n0, n1);
-                if constexpr (F32) {
-                    if (near32(n0, thr) || near32(n1, thr)) {
-                        cd e0, e1;
-                        hpair(op.c[0], mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
-                        n0 = prune32(n0, thr, !negl64(e0, thr));
-                        n1 = prune32(n1, thr, !negl64(e1, thr));
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+            constexpr int e = decltype(ec)::value;
+            static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+                constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+                constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+                if (((uint32_t)s0 & op.cmS) == op.cvS) {
+                    const C x = a[e][s0], y = a[e][s1];
+                    C n0, n1;
+                    hpair(f, x, y, n0, n1);
+                    if constexpr (F32) {
+                        if (near32(n0, thr) || near32(n1, thr)) {
+                            cd e0, e1;
+                            hpair(op.c[0], mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
+                            n0 = prune32(n0, thr, !negl64(e0, thr));
+                            n1 = prune32(n1, thr, !negl64(e1, thr));
+                        } else {
+                            n0 = prune32(n0, thr, true);
+                            n1 = prune32(n1, thr, true);
+                        }
                     } else {
-                        n0 = prune32(n0, thr, true);
-                        n1 = prune32(n1, thr, true);
+                        n0 = prune(n0, thr);
+                        n1 = prune(n1, thr);
                     }
-                } else {
-                    n0 = prune(n0, thr);
-                    n1 = prune(n1, thr);
+                    if (pall) { a[e][s0] = n0; a[e][s1] = n1; }
+                    else { a[e][s0] = csel(pred[e], n0, x); a[e][s1] = csel(pred[e], n1, y); }
                 }
-                if (pall) { a[s0] = n0; a[s1] = n1; }
-                else { a[s0] = csel(pred, n0, x); a[s1] = csel(pred, n1, y); }
-            }
+            });
         });
     } break;
     case H_SLOT_U2: {
         const R ar = (R)op.c[0], u01r = (R)op.c[1], u01i = (R)op.c[2], u10r = (R)op.c[3], u10i = (R)op.c[4];
-        static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
-            constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
-            constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
-            if (((uint32_t)s0 & op.cmS) == op.cvS) {
-                const C x = a[s0], y = a[s1];
-                C n0, n1;
-                u2(ar, u01r, u01i, u10r, u10i, x, y, n0, n1);
-                if constexpr (F32) {
-                    if (near32(n0, thr) || near32(n1, thr)) {
-                        cd e0, e1;
-                        u2(op.c[0], op.c[1], op.c[2], op.c[3], op.c[4], mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
-                        n0 = prune32(n0, thr, !negl64(e0, thr));
-                        n1 = prune32(n1, thr, !negl64(e1, thr));
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+            constexpr int e = decltype(ec)::value;
+            static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+                constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+                constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+                if (((uint32_t)s0 & op.cmS) == op.cvS) {
+                    const C x = a[e][s0], y = a[e][s1];
+                    C n0, n1;
+                    u2(ar, u01r, u01i, u10r, u10i, x, y, n0, n1);
+                    if constexpr (F32) {
+                        if (near32(n0, thr) || near32(n1, thr)) {
+                            cd e0, e1;
+                            u2(op.c[0], op.c[1], op.c[2], op.c[3], op.c[4], mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
+                            n0 = prune32(n0, thr, !negl64(e0, thr));
+                            n1 = prune32(n1, thr, !negl64(e1, thr));
+                        } else {
+                            n0 = prune32(n0, thr, true);
+                            n1 = prune32(n1, thr, true);
+                        }
                     } else {
-                        n0 = prune32(n0, thr, true);
-                        n1 = prune32(n1, thr, true);
+                        n0 = prune(n0, thr);
+                        n1 = prune(n1, thr);
                     }
-                } else {
-                    n0 = prune(n0, thr);
-                    n1 = prune(n1, thr);
+                    if (pall) { a[e][s0] = n0; a[e][s1] = n1; }
+                    else { a[e][s0] = csel(pred[e], n0, x); a[e][s1] = csel(pred[e], n1, y); }
                 }
-                if (pall) { a[s0] = n0; a[s1] = n1; }
-                else { a[s0] = csel(pred, n0, x); a[s1] = csel(pred, n1, y); }
-            }
+            });
         });
     } break;
     case H_SLOT_U: {
         DOp d{};
         for (int k = 0; k < 8; ++k) d.p[k] = coef[op.ci * 8 + k];
-        static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
-            constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
-            constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
-            if (((uint32_t)s0 & op.cmS) == op.cvS) {
-                const C x = a[s0], y = a[s1];
-                C n0, n1;
-                branch_pair<KU>(d, thr, x, y, n0, n1);
-                a[s0] = csel(pred || pall, n0, x);
-                a[s1] = csel(pred || pall, n1, y);
-            }
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+            constexpr int e = decltype(ec)::value;
+            static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+                constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+                constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+                if (((uint32_t)s0 & op.cmS) == op.cvS) {
+                    const C x = a[e][s0], y = a[e][s1];
+                    C n0, n1;
+                    branch_pair<KU>(d, thr, x, y, n0, n1);
+                    a[e][s0] = csel(pred[e] || pall, n0, x);
+                    a[e][s1] = csel(pred[e] || pall, n1, y);
+                }
+            });
         });
     } break;
     default:
@@ -213,15 +233,15 @@ __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&
     }
 }
 
-template <int MB, class R>
-__device__ __forceinline__ void slot_dispatch(const SOp& op, typename CT<R>::T (&a)[1 << MB], bool pred, double thr,
-                                              const double* __restrict__ coef) {
+template <int MB, class R, int EPT>
+__device__ __forceinline__ void slot_dispatch(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
+                                              const bool (&pred)[EPT], double thr, const double* __restrict__ coef) {
     switch (op.j) {
-    case 0: if constexpr (MB > 0) slot_handler<MB, R, 0>(op, a, pred, thr, coef); break;
-    case 1: if constexpr (MB > 1) slot_handler<MB, R, 1>(op, a, pred, thr, coef); break;
-    case 2: if constexpr (MB > 2) slot_handler<MB, R, 2>(op, a, pred, thr, coef); break;
-    case 3: if constexpr (MB > 3) slot_handler<MB, R, 3>(op, a, pred, thr, coef); break;
-    case 4: if constexpr (MB > 4) slot_handler<MB, R, 4>(op, a, pred, thr, coef); break;
+    case 0: if constexpr (MB > 0) slot_handler<MB, R, EPT, 0>(op, a, pred, thr, coef); break;
+    case 1: if constexpr (MB > 1) slot_handler<MB, R, EPT, 1>(op, a, pred, thr, coef); break;
+    case 2: if constexpr (MB > 2) slot_handler<MB, R, EPT, 2>(op, a, pred, thr, coef); break;
+    case 3: if constexpr (MB > 3) slot_handler<MB, R, EPT, 3>(op, a, pred, thr, coef); break;
+    case 4: if constexpr (MB > 4) slot_handler<MB, R, EPT, 4>(op, a, pred, thr, coef); break;
     default: break;
     }
 }
@@ -245,48 +265,62 @@ __device__ __forceinline__ void apply_phase(Cplx<float> (&a)[1 << MB], double pr
     });
 }
 
-template <int MB, class R>
-__global__ __launch_bounds__(BLOCK) void k_sweep(SweepArgs A) {
+// Per-thread counters of one sweep group.
+struct SweepCount {
+    unsigned long long upd, nnz_in, nnz_out, empty;
+};
+
+// Apply one program to EPT tiles (64 buckets each) with one wavefront: every op
+// is decoded once and applied to all of the thread's buckets.  tile[e] < 0
+// marks a missing tile (the wavefront computes on zeros and stores nothing).
+// Tiles live at basep / ampv in the tile-interleaved layout of sqe_device.hpp.
+template <int MB, class R, int EPT>
+__device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* ampv, const int64_t (&tile)[EPT],
+                                            const MOp* __restrict__ ops, uint32_t nops,
+                                            const double* __restrict__ coef, double thr,
+                                            const double* __restrict__ scale, int store, SweepCount& out) {
     using C = typename CT<R>::T;
     constexpr int S = 1 << MB;
     constexpr bool F32 = sizeof(R) == 4;
-    const uint64_t b = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    const uint64_t nb = *A.nb;
-    const uint64_t nb_pad = (nb + 63) & ~63ull;
-    // liveness is decided per wavefront (tiles are 64 buckets) from a scalar wave
-    // index, so the gate loop below is uniform control flow for the compiler
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const bool live = (uint64_t)blockIdx.x * BLOCK + (uint64_t)wave * 64 < nb_pad;
-    C* __restrict__ amp = (C*)A.amp;
-    C a[S];
-    uint64_t base = EMPTY_KEY;
-    if (live) {
-        base = A.base[b];
-        static_for<S>([&](auto sc) __attribute__((always_inline)) {
-            constexpr int s = decltype(sc)::value;
-            a[s] = amp[amp_index(b, s, S)];
-        });
-    } else {
-        static_for<S>([&](auto sc) __attribute__((always_inline)) {
-            constexpr int s = decltype(sc)::value;
-            a[s].x = 0; a[s].y = 0;
-        });
-    }
-    const bool valid = live && base != EMPTY_KEY;
-    uint32_t cnt = valid ? count_nz<MB, R>(a) : 0u;
-    const uint32_t cnt_in = cnt;
-    unsigned long long upd = 0;
-    const uint64_t base0 = base;
-    double pr = 1.0, pi = 0.0;        // FP32 form: pending per-bucket factor
+    const uint32_t lane = threadIdx.x & 63;
+    C* __restrict__ amp = (C*)ampv;
+    C a[EPT][S];
+    uint64_t base[EPT], base0[EPT];
+    bool valid[EPT];
+    uint32_t cnt[EPT];
+    double pr[EPT], pi[EPT];          // FP32 form: pending per-bucket factor
     bool pending = false;             // wavefront-uniform: a factor is pending
-    if (live && A.nops) {
+    out.upd = 0;
+    out.nnz_in = 0;
+    static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+        constexpr int e = decltype(ec)::value;
+        base[e] = EMPTY_KEY;
+        if (tile[e] >= 0) {
+            const uint64_t b = (uint64_t)tile[e] * TILE + lane;
+            base[e] = basep[b];
+            static_for<S>([&](auto sc) __attribute__((always_inline)) {
+                constexpr int s = decltype(sc)::value;
+                a[e][s] = amp[amp_index(b, s, S)];
+            });
+        } else {
+            static_for<S>([&](auto sc) __attribute__((always_inline)) {
+                constexpr int s = decltype(sc)::value;
+                a[e][s].x = 0; a[e][s].y = 0;
+            });
+        }
+        base0[e] = base[e];
+        valid[e] = base[e] != EMPTY_KEY;
+        cnt[e] = valid[e] ? count_nz<MB, R>(a[e]) : 0u;
+        out.nnz_in += cnt[e];
+        pr[e] = 1.0;
+        pi[e] = 0.0;
+    });
+    if (nops) {
         // The program is staged 64 ops at a time in the lanes of the wavefront
         // (lane j holds the 16 dwords of op c+j, one coalesced load per chunk);
         // each op is then pulled into scalar registers with v_readlane, so no
         // per-gate memory round trip sits on the critical path.
-        const uint32_t lane = threadIdx.x & 63;
-        const uint32_t nops = A.nops;
-        const uint32_t* __restrict__ words = (const uint32_t*)A.ops;
+        const uint32_t* __restrict__ words = (const uint32_t*)ops;
         uint32_t w[16];
         for (uint32_t c = 0; c < nops; c += 64) {
             const uint32_t mine = c + lane < nops ? c + lane : nops - 1;
@@ -313,74 +347,183 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(SweepArgs A) {
 #pragma unroll
                 for (int k = 0; k < 5; ++k) op.c[k] = __builtin_bit_cast(double, rd64(6 + 2 * k));
                 if (op.h == H_RENORM) {
-                    const double f = *A.scale;
-                    if constexpr (F32) {
-                        pr *= f;
-                        pi *= f;
-                        pending = true;
-                    } else {
-                        static_for<S>([&](auto sc) __attribute__((always_inline)) {
-                            constexpr int s = decltype(sc)::value;
-                            a[s].x = (R)((double)a[s].x * f);
-                            a[s].y = (R)((double)a[s].y * f);
-                        });
-                    }
+                    const double f = *scale;
+                    static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                        constexpr int e = decltype(ec)::value;
+                        if constexpr (F32) {
+                            pr[e] *= f;
+                            pi[e] *= f;
+                        } else {
+                            static_for<S>([&](auto sc) __attribute__((always_inline)) {
+                                constexpr int s = decltype(sc)::value;
+                                a[e][s].x = (R)((double)a[e][s].x * f);
+                                a[e][s].y = (R)((double)a[e][s].y * f);
+                            });
+                        }
+                    });
+                    if (F32) pending = true;
                     continue;
                 }
-                upd += cnt;
-                const bool pred = valid && ((base & op.cmB) == op.cvB);
+                bool pred[EPT];
+                static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                    constexpr int e = decltype(ec)::value;
+                    out.upd += cnt[e];
+                    pred[e] = valid[e] && ((base[e] & op.cmB) == op.cvB);
+                });
                 if (op.h <= H_BASE_PERM) {
                     // gate on a base qubit
-                    if (op.h == H_BASE_X) {
-                        if (pred) base ^= op.tbase;
-                        continue;
-                    }
-                    const bool bit = (base & op.tbase) != 0;
-                    double fr, fi;
-                    base_factor(op, bit, pred, fr, fi);
-                    if (F32 && op.cmS == 0) {
-                        const double r = pr * fr - pi * fi, i = pr * fi + pi * fr;
-                        pr = r;
-                        pi = i;
-                        pending = true;
-                    } else {
-                        const R cr = (R)fr, ci = (R)fi;
-                        static_for<S>([&](auto sc) __attribute__((always_inline)) {
-                            constexpr int s = decltype(sc)::value;
-                            if (((uint32_t)s & op.cmS) == op.cvS) a[s] = rot(cr, ci, a[s]);
-                        });
-                    }
-                    if (op.h == H_BASE_PERM && pred) base ^= op.tbase;
+                    const bool fold = F32 && op.cmS == 0;
+                    static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                        constexpr int e = decltype(ec)::value;
+                        if (op.h != H_BASE_X) {
+                            const bool bit = (base[e] & op.tbase) != 0;
+                            double fr, fi;
+                            base_factor(op, bit, pred[e], fr, fi);
+                            if (fold) {
+                                const double r = pr[e] * fr - pi[e] * fi, i = pr[e] * fi + pi[e] * fr;
+                                pr[e] = r;
+                                pi[e] = i;
+                            } else {
+                                const R cr = (R)fr, ci = (R)fi;
+                                static_for<S>([&](auto sc) __attribute__((always_inline)) {
+                                    constexpr int s = decltype(sc)::value;
+                                    if (((uint32_t)s & op.cmS) == op.cvS) a[e][s] = rot(cr, ci, a[e][s]);
+                                });
+                            }
+                        }
+                        if (op.h != H_BASE_DIAG && pred[e]) base[e] ^= op.tbase;
+                    });
+                    if (fold && op.h != H_BASE_X) pending = true;
                     continue;
                 }
                 if constexpr (MB > 0) {
                     const bool branch = op.h >= H_SLOT_H && op.h <= H_SLOT_U;
                     if (F32 && pending && branch) {
-                        if constexpr (F32) apply_phase<MB>(a, pr, pi);
-                        pr = 1.0;
-                        pi = 0.0;
+                        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                            constexpr int e = decltype(ec)::value;
+                            if constexpr (F32) apply_phase<MB>(a[e], pr[e], pi[e]);
+                            pr[e] = 1.0;
+                            pi[e] = 0.0;
+                        });
                         pending = false;
                     }
-                    slot_dispatch<MB, R>(op, a, pred, A.thr, A.coef);
-                    if (branch) cnt = valid ? count_nz<MB, R>(a) : 0u;
+                    slot_dispatch<MB, R, EPT>(op, a, pred, thr, coef);
+                    if (branch) {
+                        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                            constexpr int e = decltype(ec)::value;
+                            cnt[e] = valid[e] ? count_nz<MB, R>(a[e]) : 0u;
+                        });
+                    }
                 }
             }
         }
-        if constexpr (F32) {
-            if (pending) apply_phase<MB>(a, pr, pi);
-        }
     }
-    if (live) {
-        if (A.store) {
+    out.nnz_out = 0;
+    out.empty = 0;
+    static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+        constexpr int e = decltype(ec)::value;
+        if constexpr (F32) {
+            if (pending) apply_phase<MB>(a[e], pr[e], pi[e]);
+        }
+        if (tile[e] >= 0 && store) {
+            const uint64_t b = (uint64_t)tile[e] * TILE + lane;
             static_for<S>([&](auto sc) __attribute__((always_inline)) {
                 constexpr int s = decltype(sc)::value;
-                amp[amp_index(b, s, S)] = a[s];
+                amp[amp_index(b, s, S)] = a[e][s];
             });
-            if (valid && base != base0) A.base[b] = base;
+            if (valid[e] && base[e] != base0[e]) basep[b] = base[e];
+        }
+        out.nnz_out += cnt[e];
+        out.empty += (valid[e] && cnt[e] == 0) ? 1ull : 0ull;
+    });
+}
+
+// One program over the engine's store: wavefront g of the grid takes tiles
+// EPT*g .. EPT*g + EPT-1.
+template <int MB, class R, int EPT>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MB >= 5 ? 1 : 3))) void k_sweep(SweepArgs A) {
+    const uint64_t nb = *A.nb;
+    const int64_t ntiles = (int64_t)((nb + 63) >> 6);
+    // tile ownership is decided per wavefront from a scalar wave index, so the
+    // gate loop is uniform control flow for the compiler
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t g = (int64_t)blockIdx.x * (BLOCK / 64) + wave;
+    int64_t tile[EPT];
+    bool any = false;
+    static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+        constexpr int e = decltype(ec)::value;
+        const int64_t t = g * EPT + e;
+        tile[e] = t < ntiles ? t : -1;
+        any |= t < ntiles;
+    });
+    SweepCount cnt{0, 0, 0, 0};
+    if (any) sweep_group<MB, R, EPT>(A.base, A.amp, tile, A.ops, A.nops, A.coef, A.thr, A.scale, A.store, cnt);
+    block_stats(A.bstats, cnt.upd, cnt.nnz_out, cnt.empty, A.store ? cnt.nnz_in + cnt.nnz_out : 0ull);
+}
+
+// ---------------------------------------------------------------------------
+// Batched sweep: many independent circuits, each with its own bucket region
+// and program, advanced by one persistent launch.  Wavefronts take work items
+// (a circuit and a run of `gran` of its tiles) from a queue that the host has
+// ordered longest program first, so uneven depths and sizes balance out.
+// ---------------------------------------------------------------------------
+template <int MB, class R>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MB >= 5 ? 1 : 3))) void k_sweep_batch(
+    BatchArgs A) {
+    const uint32_t lane = threadIdx.x & 63;
+    for (;;) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(A.queue, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= A.nitems) break;
+        const BItem item = A.items[it];
+        const BTask t = A.tasks[item.task];
+        unsigned long long upd = 0, nnz = 0;
+        for (uint32_t g = 0; g < A.gran; ++g) {
+            const uint32_t tile = item.tile + g;
+            if (tile >= t.ntiles) break;
+            const int64_t tl[1] = {(int64_t)tile};
+            SweepCount c{0, 0, 0, 0};
+            sweep_group<MB, R, 1>(t.base, t.amp, tl, t.ops, t.nops, t.coef, A.thr, t.scale, 1, c);
+            upd += c.upd;
+            nnz += c.nnz_out;
+        }
+        for (int d = 32; d >= 1; d >>= 1) {
+            upd += __shfl_xor(upd, d);
+            nnz += __shfl_xor(nnz, d);
+        }
+        if (lane == 0) {
+            if (upd) atomicAdd(t.upd, upd);
+            atomicAdd(t.nnz, nnz);
         }
     }
-    block_stats(A.bstats, upd, cnt, (valid && cnt == 0) ? 1ull : 0ull,
-                A.store ? (unsigned long long)cnt_in + cnt : 0ull);
+}
+
+// FP64 sum of |a|^2 per circuit over the work items' tiles (for FP32 renorm).
+template <int MB, class R>
+__global__ __launch_bounds__(BLOCK) void k_norm_batch(BatchArgs A) {
+    using C = typename CT<R>::T;
+    constexpr int S = 1 << MB;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (uint32_t it = blockIdx.x * (BLOCK / 64) + wave; it < A.nitems; it += gridDim.x * (BLOCK / 64)) {
+        const BItem item = A.items[it];
+        const BTask t = A.tasks[item.task];
+        double acc = 0.0;
+        for (uint32_t g = 0; g < A.gran; ++g) {
+            const uint32_t tile = item.tile + g;
+            if (tile >= t.ntiles) break;
+            const uint64_t b = (uint64_t)tile * TILE + lane;
+            const C* __restrict__ amp = (const C*)t.amp;
+            static_for<S>([&](auto sc) __attribute__((always_inline)) {
+                constexpr int s = decltype(sc)::value;
+                const C v = amp[amp_index(b, s, S)];
+                acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+            });
+        }
+        for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
+        if (lane == 0) atomicAdd(t.norm2, acc);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -650,9 +793,17 @@ __global__ __launch_bounds__(BLOCK) void k_load_scatter(LoadArgs A) {
 
 template <class R>
 struct Launch {
-    static hipError_t sweep(int mb, hipStream_t st, uint32_t grid, const SweepArgs& a) {
-        SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_sweep<MB_, R>), dim3(grid), dim3(BLOCK), 0, st, a));
+    // Buckets (tiles) per thread for the single-program sweep: one keeps the
+    // kernels within 3-4 wavefronts per SIMD.
+    static constexpr int EPT = 1;
+    static hipError_t sweep(int mb, hipStream_t st, uint64_t ntiles, const SweepArgs& a) {
+        const uint32_t grid = sweep_grid(mb, ntiles);
+        SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_sweep<MB_, R, EPT>), dim3(grid), dim3(BLOCK), 0, st, a));
         return hipGetLastError();
+    }
+    static uint32_t sweep_grid(int, uint64_t ntiles) {
+        const uint64_t waves = (ntiles + EPT - 1) / EPT;
+        return (uint32_t)std::max<uint64_t>(1, (waves + BLOCK / 64 - 1) / (BLOCK / 64));
     }
     template <int MB>
     static void pair_kind(uint32_t kind, hipStream_t st, uint32_t grid, const PairArgs& a) {
@@ -681,6 +832,14 @@ struct Launch {
     }
     static hipError_t norm_partial(int mb, hipStream_t st, uint32_t grid, const NormArgs& a) {
         SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_norm_partial<MB_, R>), dim3(grid), dim3(BLOCK), 0, st, a));
+        return hipGetLastError();
+    }
+    static hipError_t sweep_batch(int mb, hipStream_t st, uint32_t grid, uint32_t block, const BatchArgs& a) {
+        SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_sweep_batch<MB_, R>), dim3(grid), dim3(block), 0, st, a));
+        return hipGetLastError();
+    }
+    static hipError_t norm_batch(int mb, hipStream_t st, uint32_t grid, const BatchArgs& a) {
+        SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_norm_batch<MB_, R>), dim3(grid), dim3(BLOCK), 0, st, a));
         return hipGetLastError();
     }
     static hipError_t scatter(hipStream_t st, uint32_t grid, const LoadArgs& a) {

@@ -147,9 +147,12 @@ hipError_t launch_fill_slots(hipStream_t st, Slot* p, size_t n) {
     return hipGetLastError();
 }
 
-hipError_t launch_sweep(int mb, bool f32, hipStream_t st, uint32_t grid, const SweepArgs& a) {
-    if (!grid) return hipSuccess;
-    return f32 ? Launch<float>::sweep(mb, st, grid, a) : Launch<double>::sweep(mb, st, grid, a);
+hipError_t launch_sweep(int mb, bool f32, hipStream_t st, uint64_t ntiles, const SweepArgs& a) {
+    return f32 ? Launch<float>::sweep(mb, st, ntiles, a) : Launch<double>::sweep(mb, st, ntiles, a);
+}
+
+uint32_t sweep_grid(int mb, bool f32, uint64_t ntiles) {
+    return f32 ? Launch<float>::sweep_grid(mb, ntiles) : Launch<double>::sweep_grid(mb, ntiles);
 }
 
 hipError_t launch_pair(int mb, bool f32, uint32_t kind, hipStream_t st, uint32_t grid, const PairArgs& a) {
@@ -173,6 +176,36 @@ hipError_t launch_norm(int mb, bool f32, hipStream_t st, uint32_t grid, const No
     hipError_t r = f32 ? Launch<float>::norm_partial(mb, st, grid, a) : Launch<double>::norm_partial(mb, st, grid, a);
     if (r != hipSuccess) return r;
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(BLOCK), 0, st, (const double*)a.partials, grid, norm2, scale);
+    return hipGetLastError();
+}
+
+__global__ void k_batch_scale(const double* __restrict__ norm2, double* __restrict__ scale, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) scale[i] = norm2[i] > 0.0 ? 1.0 / sqrt(norm2[i]) : 1.0;
+}
+
+__global__ void k_batch_zero(const BTask* __restrict__ t, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) *t[i].nnz = 0;
+}
+
+hipError_t launch_batch_zero(hipStream_t st, const BTask* tasks, uint32_t n) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_zero, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, tasks, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_batch(int mb, bool f32, hipStream_t st, uint32_t grid, uint32_t block, const BatchArgs& a) {
+    return f32 ? Launch<float>::sweep_batch(mb, st, grid, block, a) : Launch<double>::sweep_batch(mb, st, grid, block, a);
+}
+
+hipError_t launch_norm_batch(int mb, bool f32, hipStream_t st, uint32_t grid, const BatchArgs& a) {
+    return f32 ? Launch<float>::norm_batch(mb, st, grid, a) : Launch<double>::norm_batch(mb, st, grid, a);
+}
+
+hipError_t launch_batch_scale(hipStream_t st, const double* norm2, double* scale, uint32_t n) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_scale, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, norm2, scale, n);
     return hipGetLastError();
 }
 

@@ -22,6 +22,34 @@ struct SweepArgs {
     BlockStats* bstats;
 };
 
+// One circuit's share of a batched launch.
+struct BTask {
+    uint64_t* base;              // bucket base keys of the circuit's region
+    void* amp;                   // its amplitudes (tile-interleaved)
+    const MOp* ops;              // program segment of this launch
+    const double* coef;          // side table of general-U coefficients
+    const double* scale;         // renormalisation factor slot
+    unsigned long long* upd;     // per-circuit counters (atomically accumulated)
+    unsigned long long* nnz;
+    double* norm2;               // per-circuit FP64 norm accumulator
+    uint32_t nops;
+    uint32_t ntiles;
+};
+
+struct BItem {                   // a circuit and the first of `gran` consecutive tiles
+    uint32_t task;
+    uint32_t tile;
+};
+
+struct BatchArgs {
+    const BTask* tasks;
+    const BItem* items;
+    uint32_t nitems;
+    uint32_t gran;
+    unsigned int* queue;         // work-item counter (zeroed before the launch)
+    double thr;
+};
+
 struct PairArgs {
     const uint64_t* in_base;
     const void* in_amp;
@@ -106,13 +134,18 @@ struct EnumArgs {
 
 hipError_t launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v);
 hipError_t launch_fill_slots(hipStream_t st, Slot* p, size_t n);
-hipError_t launch_sweep(int mb, bool f32, hipStream_t st, uint32_t grid, const SweepArgs& a);
+hipError_t launch_sweep(int mb, bool f32, hipStream_t st, uint64_t ntiles, const SweepArgs& a);
+uint32_t sweep_grid(int mb, bool f32, uint64_t ntiles);   // workgroups launch_sweep uses
 hipError_t launch_pair(int mb, bool f32, uint32_t kind, hipStream_t st, uint32_t grid, const PairArgs& a);
 hipError_t launch_compact(int mb, bool f32, hipStream_t st, uint32_t grid, const CompactArgs& a);
 hipError_t launch_export(int mb, bool f32, hipStream_t st, uint32_t grid, const ExportArgs& a);
 hipError_t launch_norm(int mb, bool f32, hipStream_t st, uint32_t grid, const NormArgs& a, double* norm2,
                        double* scale);
 hipError_t launch_index_build(hipStream_t st, uint32_t grid, const IndexArgs& a);
+hipError_t launch_sweep_batch(int mb, bool f32, hipStream_t st, uint32_t grid, uint32_t block, const BatchArgs& a);
+hipError_t launch_norm_batch(int mb, bool f32, hipStream_t st, uint32_t grid, const BatchArgs& a);
+hipError_t launch_batch_zero(hipStream_t st, const BTask* tasks, uint32_t n);
+hipError_t launch_batch_scale(hipStream_t st, const double* norm2, double* scale, uint32_t n);
 hipError_t launch_stats_reduce(hipStream_t st, BlockStats* bs, uint32_t n_acc, uint32_t n_last, Stats* stats);
 hipError_t launch_load(bool f32, hipStream_t st, const LoadArgs& a, const EnumArgs& e, uint32_t enum_grid);
 

@@ -22,7 +22,10 @@ F64, F32 = 0, 1
 EXPORTS = ["sqe_config_default", "sqe_create", "sqe_destroy", "sqe_set_basis", "sqe_load",
            "sqe_apply", "sqe_apply_gates", "sqe_nnz", "sqe_norm2", "sqe_export", "sqe_sync",
            "sqe_get_stats", "sqe_prepare", "sqe_snapshot_save", "sqe_snapshot_restore",
-           "sqe_last_error"]
+           "sqe_last_error",
+           "sqe_batch_create", "sqe_batch_destroy", "sqe_batch_add", "sqe_batch_step", "sqe_batch_run",
+           "sqe_batch_sync", "sqe_batch_info", "sqe_batch_export", "sqe_batch_remove", "sqe_batch_usage",
+           "sqe_batch_get_tuning", "sqe_batch_set_tuning", "sqe_batch_autotune", "sqe_batch_last_error"]
 
 
 class SqeError(RuntimeError):
@@ -48,8 +51,22 @@ class Stats(ctypes.Structure):
                 ("moved", ctypes.c_uint64), ("sweep_ms", ctypes.c_double), ("pair_ms", ctypes.c_double), ("other_ms", ctypes.c_double)]
 
 
+class BatchCircuit(ctypes.Structure):
+    _fields_ = [("done", ctypes.c_uint32), ("bucket_bits", ctypes.c_int32), ("gates_total", ctypes.c_uint64),
+                ("gates_done", ctypes.c_uint64), ("updates", ctypes.c_uint64), ("nnz", ctypes.c_uint64),
+                ("buckets", ctypes.c_uint64), ("bytes", ctypes.c_uint64)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_uint32), ("tiles_per_item", ctypes.c_uint32), ("waves", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32), ("load_factor", ctypes.c_double), ("sweep_ms", ctypes.c_double),
+                ("launches", ctypes.c_uint64)]
+
+
 assert ctypes.sizeof(Config) == 80
 assert ctypes.sizeof(Stats) == 128
+assert ctypes.sizeof(BatchCircuit) == 56
+assert ctypes.sizeof(BatchTuning) == 40
 
 _lib = None
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -97,6 +114,24 @@ def lib():
         L.sqe_snapshot_restore.restype = st
         L.sqe_last_error.argtypes = []
         L.sqe_last_error.restype = ctypes.c_char_p
+        L.sqe_batch_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
+        L.sqe_batch_add.argtypes = [vp, ctypes.c_uint32, _u64p, _f64p, _f64p, ctypes.c_uint64, vp,
+                                    ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64)]
+        L.sqe_batch_step.argtypes = [vp, ctypes.c_uint32]
+        L.sqe_batch_info.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(BatchCircuit)]
+        L.sqe_batch_export.argtypes = [vp, ctypes.c_int64, _u64p, _f64p, _f64p, ctypes.c_uint64, _u64p]
+        L.sqe_batch_remove.argtypes = [vp, ctypes.c_int64]
+        L.sqe_batch_usage.argtypes = [vp, _u64p, _u64p, _u64p]
+        L.sqe_batch_get_tuning.argtypes = [vp, ctypes.POINTER(BatchTuning)]
+        L.sqe_batch_set_tuning.argtypes = [vp, ctypes.POINTER(BatchTuning)]
+        L.sqe_batch_autotune.argtypes = [vp, ctypes.POINTER(BatchTuning)]
+        for f in ("sqe_batch_destroy", "sqe_batch_run", "sqe_batch_sync"):
+            getattr(L, f).argtypes = [vp]
+        for f in EXPORTS:
+            if f.startswith("sqe_batch_") and f != "sqe_batch_last_error":
+                getattr(L, f).restype = st
+        L.sqe_batch_last_error.argtypes = []
+        L.sqe_batch_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -104,6 +139,11 @@ def lib():
 def _check(rc):
     if rc != SQE_OK:
         raise SqeError(rc, lib().sqe_last_error().decode())
+
+
+def _bcheck(rc):
+    if rc != SQE_OK:
+        raise SqeError(rc, lib().sqe_batch_last_error().decode())
 
 
 def _ptr(a, t):
@@ -222,6 +262,130 @@ class Engine:
 
     def snapshot_restore(self):
         _check(lib().sqe_snapshot_restore(self.h))
+
+
+class Batch:
+    """Many independent circuits on one GPU, advanced together (sqe_batch_*)."""
+
+    def __init__(self, prec="f64", prune_thr=0.0, bucket_bits=-1, renorm_every=0, timing=False,
+                 device=0, hbm_budget_bytes=0, max_load=0.5):
+        L = lib()
+        cfg = Config()
+        L.sqe_config_default(ctypes.byref(cfg), 1)
+        cfg.prec = F32 if prec in ("f32", F32) else F64
+        cfg.prune_thr = prune_thr
+        cfg.bucket_bits = bucket_bits
+        cfg.renorm_every = renorm_every
+        cfg.timing = 1 if timing else 0
+        cfg.device = device
+        cfg.hbm_budget_bytes = hbm_budget_bytes
+        cfg.max_load = max_load
+        self.prec = "f32" if cfg.prec == F32 else "f64"
+        h = ctypes.c_void_p()
+        _bcheck(L.sqe_batch_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sqe_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, nq, keys, re, im, gates):
+        """Admit one circuit; returns its id (raises SqeError SQE_ETABLE_FULL when over budget)."""
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        r = np.ascontiguousarray(re, dtype=np.float64)
+        i = np.ascontiguousarray(im, dtype=np.float64)
+        if not (k.size == r.size == i.size):
+            raise ValueError("keys/re/im sizes differ")
+        g = G.gate_array(gates)
+        cid = ctypes.c_int64()
+        _bcheck(lib().sqe_batch_add(self.h, int(nq), _ptr(k, _u64p), _ptr(r, _f64p), _ptr(i, _f64p), k.size,
+                                    g.ctypes.data, g.size, ctypes.byref(cid)))
+        return cid.value
+
+    def try_add(self, nq, keys, re, im, gates):
+        """Like add, but returns None when the circuit does not fit the budget now."""
+        try:
+            return self.add(nq, keys, re, im, gates)
+        except SqeError as ex:
+            if ex.status == SQE_ETABLE_FULL:
+                return None
+            raise
+
+    def step(self, max_gates=0):
+        _bcheck(lib().sqe_batch_step(self.h, int(max_gates)))
+
+    def run(self):
+        _bcheck(lib().sqe_batch_run(self.h))
+
+    def sync(self):
+        _bcheck(lib().sqe_batch_sync(self.h))
+
+    def info(self, cid):
+        c = BatchCircuit()
+        _bcheck(lib().sqe_batch_info(self.h, int(cid), ctypes.byref(c)))
+        return {f: getattr(c, f) for f, _ in BatchCircuit._fields_}
+
+    def export(self, cid, sort=True):
+        n = ctypes.c_uint64()
+        rc = lib().sqe_batch_export(self.h, int(cid), None, None, None, 0, ctypes.byref(n))
+        if rc not in (SQE_OK, SQE_EINVAL):
+            _bcheck(rc)
+        cap = n.value
+        k = np.zeros(cap, dtype=np.uint64)
+        r = np.zeros(cap)
+        i = np.zeros(cap)
+        _bcheck(lib().sqe_batch_export(self.h, int(cid), _ptr(k, _u64p), _ptr(r, _f64p), _ptr(i, _f64p), cap,
+                                       ctypes.byref(n)))
+        m = n.value
+        k, r, i = k[:m], r[:m], i[:m]
+        if sort:
+            o = np.argsort(k, kind="stable")
+            k, r, i = k[o], r[o], i[o]
+        return k, r, i
+
+    def remove(self, cid):
+        _bcheck(lib().sqe_batch_remove(self.h, int(cid)))
+
+    def usage(self):
+        u, b, a = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _bcheck(lib().sqe_batch_usage(self.h, ctypes.byref(u), ctypes.byref(b), ctypes.byref(a)))
+        return {"used": u.value, "budget": b.value, "active": a.value}
+
+    def tuning(self):
+        t = BatchTuning()
+        _bcheck(lib().sqe_batch_get_tuning(self.h, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in BatchTuning._fields_ if f != "pad"}
+
+    def set_tuning(self, block=None, tiles_per_item=None, waves=None, load_factor=None):
+        t = BatchTuning()
+        _bcheck(lib().sqe_batch_get_tuning(self.h, ctypes.byref(t)))
+        if block is not None:
+            t.block = block
+        if tiles_per_item is not None:
+            t.tiles_per_item = tiles_per_item
+        if waves is not None:
+            t.waves = waves
+        if load_factor is not None:
+            t.load_factor = load_factor
+        _bcheck(lib().sqe_batch_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self):
+        t = BatchTuning()
+        _bcheck(lib().sqe_batch_autotune(self.h, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in BatchTuning._fields_ if f != "pad"}
 
 
 def run(w, prec="f64", **kw):

@@ -1,0 +1,173 @@
+"""GPU parity of the batch engine (sqe_batch_* through the C ABI).
+
+Every circuit run inside a batch must give what the oracle gives for it alone:
+bit-exact keys and FP64 amplitudes within 1e-12 (same gate arithmetic as the
+single-circuit engine), the same update count, and -- for FP32 with FP64
+renormalisation -- agreement with the FP32 oracle within 1e-6.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from sparqc_amd import engine as E
+from sparqc_amd import gates as G
+from sparqc_amd import gen
+from tests.helpers import assert_same_state
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def small_batch(seed, n, nq=(14, 22), log2_nnz=(4.0, 11.0), depth=(40, 160)):
+    return gen.batch_circuits(seed, n_circuits=n, nq_range=nq, log2_nnz=log2_nnz, depth_range=depth)
+
+
+def oracle_of(O, c, f32=False, renorm_every=0):
+    s = O.Sparse(c["nq"], f32=f32, thr=1e-7 if f32 else 1e-14, renorm_every=renorm_every)
+    s.load(c["keys"], c["re"], c["im"])
+    s.apply(c["gates"])
+    return s.export(), s.updates()
+
+
+def test_batch_matches_oracle_f64(oracle):
+    cs = small_batch(5, 24)
+    say("batch f64: %d circuits" % len(cs))
+    with E.Batch() as b:
+        ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        b.run()
+        b.sync()
+        for cid, c in zip(ids, cs):
+            info = b.info(cid)
+            assert info["done"] == 1 and info["gates_done"] == len(c["gates"])
+            want, upd = oracle_of(oracle, c)
+            assert_same_state(b.export(cid), want, atol=1e-12, label="circuit %d" % cid)
+            assert info["updates"] == upd, "circuit %d: updates %d vs %d" % (cid, info["updates"], upd)
+            assert info["nnz"] == want[0].size
+
+
+def test_batch_steps_equal_full_run(oracle):
+    """Advancing in steps of a few gates gives the same states as one run."""
+    cs = small_batch(6, 10)
+    with E.Batch() as b1, E.Batch() as b2:
+        i1 = [b1.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        i2 = [b2.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        b1.run()
+        steps = 0
+        while b2.usage()["active"]:
+            b2.step(7)
+            steps += 1
+        say("stepped batch: %d steps" % steps)
+        assert steps >= max(len(c["gates"]) for c in cs) // 7
+        for a, c in zip(zip(i1, i2), cs):
+            assert_same_state(b2.export(a[1]), b1.export(a[0]), atol=0.0, label="circuit %d" % a[0])
+            assert b1.info(a[0])["updates"] == b2.info(a[1])["updates"]
+
+
+def test_batch_f32_renorm(oracle):
+    cs = small_batch(7, 12, depth=(150, 320))
+    say("batch f32 renorm: %d circuits" % len(cs))
+    with E.Batch(prec="f32", renorm_every=100) as b:
+        ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        b.run()
+        for cid, c in zip(ids, cs):
+            want, upd = oracle_of(oracle, c, f32=True, renorm_every=100)
+            got = b.export(cid)
+            # keys equal except entries within a few ulps of the FP32 prune threshold
+            gk, wk = got[0], want[0]
+            diff = np.setxor1d(gk, wk)
+            assert diff.size <= max(2, gk.size // 1000), "circuit %d: %d keys differ" % (cid, diff.size)
+            common, ig, iw = np.intersect1d(gk, wk, return_indices=True)
+            d = max(np.max(np.abs(got[1][ig] - want[1][iw])), np.max(np.abs(got[2][ig] - want[2][iw])))
+            assert d <= 1e-6, "circuit %d: amplitude difference %.3e" % (cid, d)
+            n2 = float(np.sum(got[1] ** 2 + got[2] ** 2))
+            assert abs(n2 - 1.0) < 1e-5
+
+
+def test_batch_f32_matches_solo_engine():
+    """FP32 batch and the single-circuit FP32 engine agree (both renormalise in FP64)."""
+    cs = small_batch(8, 6, depth=(120, 220))
+    with E.Batch(prec="f32", renorm_every=100) as b:
+        ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        b.run()
+        for cid, c in zip(ids, cs):
+            with E.Engine(c["nq"], prec="f32", renorm_every=100) as e:
+                e.load(c["keys"], c["re"], c["im"])
+                e.apply(c["gates"])
+                solo = e.export()
+            got = b.export(cid)
+            common, ig, iw = np.intersect1d(got[0], solo[0], return_indices=True)
+            assert common.size >= solo[0].size - 2
+            d = max(np.max(np.abs(got[1][ig] - solo[1][iw])), np.max(np.abs(got[2][ig] - solo[2][iw])))
+            assert d <= 1e-6
+
+
+def test_admission_budget_and_reuse(oracle):
+    """Over-budget circuits are refused; removing finished ones frees room."""
+    cs = small_batch(9, 8, log2_nnz=(9.0, 10.0))
+    with E.Batch(hbm_budget_bytes=400_000) as b:
+        admitted, refused = [], []
+        for c in cs:
+            cid = b.try_add(c["nq"], c["keys"], c["re"], c["im"], c["gates"])
+            (admitted if cid is not None else refused).append((cid, c))
+        u = b.usage()
+        say("admitted", len(admitted), "refused", len(refused), u)
+        assert admitted and refused, "budget should split this batch"
+        assert u["used"] <= u["budget"]
+        b.run()
+        for cid, c in admitted:
+            want, _ = oracle_of(oracle, c)
+            assert_same_state(b.export(cid), want, atol=1e-12)
+            b.remove(cid)
+        assert b.usage()["used"] == 0
+        again = []
+        for _, c in refused:
+            cid = b.try_add(c["nq"], c["keys"], c["re"], c["im"], c["gates"])
+            if cid is not None:
+                again.append((cid, c))
+        assert again
+        b.run()
+        for cid, c in again:
+            want, _ = oracle_of(oracle, c)
+            assert_same_state(b.export(cid), want, atol=1e-12, label="readmitted %d" % cid)
+
+
+def test_autotune_keeps_results(oracle):
+    cs = small_batch(10, 16)
+    with E.Batch() as b:
+        ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        t = b.autotune()
+        say("autotune best", t)
+        assert t["block"] in (64, 128, 256) and t["tiles_per_item"] >= 1
+        b.run()
+        for cid, c in zip(ids, cs):
+            want, upd = oracle_of(oracle, c)
+            assert_same_state(b.export(cid), want, atol=1e-12)
+            assert b.info(cid)["updates"] == upd
+
+
+@pytest.mark.parametrize("block,gran", [(64, 1), (128, 4), (256, 2)])
+def test_tuning_grid(oracle, block, gran):
+    cs = small_batch(11, 6)
+    with E.Batch() as b:
+        b.set_tuning(block=block, tiles_per_item=gran)
+        ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        b.run()
+        for cid, c in zip(ids, cs):
+            want, _ = oracle_of(oracle, c)
+            assert_same_state(b.export(cid), want, atol=1e-12)
+
+
+def test_batch_rejects_cross_bucket_circuit():
+    """Branching on more qubits than a bucket holds is refused with EINVAL."""
+    nq = 12
+    gl = G.gate_array([G.make_gate(G.H, q) for q in range(6)])
+    with E.Batch() as b:
+        with pytest.raises(E.SqeError) as ex:
+            b.add(nq, np.zeros(1, dtype=np.uint64), np.ones(1), np.zeros(1), gl)
+        assert ex.value.status == E.SQE_EINVAL
+        assert b.usage()["used"] == 0

@@ -192,9 +192,7 @@ typedef struct sqe_batch_circuit {
 
 typedef struct sqe_batch_tuning {
     uint32_t block;             /* workgroup size of the batched sweep: 64/128/256     */
-    uint32_t tiles_per_item;    /* 64-bucket tiles per queue item, 1..64               */
-    uint32_t waves;             /* resident wavefronts of the persistent launch        */
-    uint32_t pad;
+    uint32_t tiles_per_item;    /* 64-bucket tiles per wavefront work item, 1..64      */
     double load_factor;         /* hash index load factor used by sqe_batch_add        */
     double sweep_ms;            /* summed batched sweep time (timing = 1), or the
                                    best trial time from sqe_batch_autotune             */
@@ -235,7 +233,7 @@ sqe_status sqe_batch_usage(sqe_batch* b, uint64_t* used, uint64_t* budget, uint6
 sqe_status sqe_batch_get_tuning(sqe_batch* b, sqe_batch_tuning* t);
 sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t);
 
-/* Time each (workgroup size, tiles per item, resident waves) candidate on the
+/* Time each (workgroup size, tiles per item) candidate on the
  * admitted circuits (their state is restored after every trial) and keep the
  * fastest; *best (optional) receives it. */
 sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best);

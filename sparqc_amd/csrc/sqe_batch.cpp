@@ -157,13 +157,16 @@ struct sqe_batch {
     BTask* htask = nullptr;
     BItem* hitem = nullptr;
     uint64_t hcap_t = 0, hcap_i = 0;
+    double* partials = nullptr;
+    uint64_t pcap = 0;
     BTask* dtask = nullptr;
     BItem* ditem = nullptr;
     uint64_t dcap_t = 0, dcap_i = 0;
-    unsigned int* queue = nullptr;
+    unsigned long long* counts = nullptr;     // per-item sweep counts
+    uint64_t ccap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // tuning and counters
-    uint32_t block = BLOCK, gran = 1, grid_waves = 0;
+    uint32_t block = BLOCK, gran = 1;
     double sweep_ms = 0.0;
     uint64_t launches = 0, steps = 0, bytes_dev = 0;
 };
@@ -337,9 +340,11 @@ sqe_status run_step(sqe_batch* b, uint64_t max_ops, std::vector<uint32_t>* need_
             t.norm2 = b->norm2 + s.id;
             t.nops = (uint32_t)(s.to - s.from);
             t.ntiles = (uint32_t)c.tiles;
+            t.item0 = (uint32_t)items.size();
             const uint32_t ti = (uint32_t)tasks.size();
-            tasks.push_back(t);
             for (uint64_t tl = 0; tl < c.tiles; tl += b->gran) items.push_back(BItem{ti, (uint32_t)tl});
+            t.nitem = (uint32_t)items.size() - t.item0;
+            tasks.push_back(t);
         }
         if (tasks.empty()) continue;
         // items are already in longest-program-first order (segments were sorted)
@@ -355,19 +360,15 @@ sqe_status run_step(sqe_batch* b, uint64_t max_ops, std::vector<uint32_t>* need_
         }
         BCK(buf_fit(b, &b->dtask, &b->dcap_t, tasks.size()));
         BCK(buf_fit(b, &b->ditem, &b->dcap_i, items.size()));
+        BCK(buf_fit(b, &b->counts, &b->ccap, 2 * items.size()));
         BHIP(hipStreamSynchronize(b->st));          // the pinned staging is reused
         std::memcpy(b->htask, tasks.data(), tasks.size() * sizeof(BTask));
         std::memcpy(b->hitem, items.data(), items.size() * sizeof(BItem));
         BHIP(hipMemcpyAsync(b->dtask, b->htask, tasks.size() * sizeof(BTask), hipMemcpyHostToDevice, b->st));
         BHIP(hipMemcpyAsync(b->ditem, b->hitem, items.size() * sizeof(BItem), hipMemcpyHostToDevice, b->st));
-        BHIP(launch_fill_u64(b->st, (uint64_t*)b->queue, 1, 0));
-        BHIP(launch_batch_zero(b->st, b->dtask, (uint32_t)tasks.size()));
-        BatchArgs a{b->dtask, b->ditem, (uint32_t)items.size(), b->gran, b->queue, b->thr};
-        const uint32_t waves = std::min<uint32_t>((uint32_t)items.size(), b->grid_waves);
-        const uint32_t wpb = b->block / 64;
-        const uint32_t grid = std::max<uint32_t>(1, (waves + wpb - 1) / wpb);
+        BatchArgs a{b->dtask, b->ditem, (uint32_t)items.size(), b->gran, b->thr, nullptr, b->counts};
         if (b->cfg.timing) BHIP(hipEventRecord(b->ev0, b->st));
-        BHIP(launch_sweep_batch(mb, b->f32, b->st, grid, b->block, a));
+        BHIP(launch_sweep_batch(mb, b->f32, b->st, b->block, a));
         if (b->cfg.timing) {
             BHIP(hipEventRecord(b->ev1, b->st));
             BHIP(hipEventSynchronize(b->ev1));
@@ -375,6 +376,7 @@ sqe_status run_step(sqe_batch* b, uint64_t max_ops, std::vector<uint32_t>* need_
             BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
             b->sweep_ms += ms;
         }
+        BHIP(launch_batch_fold(b->st, b->dtask, (uint32_t)tasks.size(), b->counts));
         ++b->launches;
         // FP32 renormalisation points reached by this launch
         for (const Seg& s : segs) {
@@ -391,6 +393,7 @@ sqe_status run_step(sqe_batch* b, uint64_t max_ops, std::vector<uint32_t>* need_
 
 sqe_status run_norms(sqe_batch* b, const std::vector<uint32_t>& ids) {
     if (ids.empty()) return SQE_OK;
+    constexpr uint32_t NG = 4;                       // tiles per norm work item
     for (int mb = 0; mb <= 5; ++mb) {
         std::vector<BTask> tasks;
         std::vector<BItem> items;
@@ -400,24 +403,25 @@ sqe_status run_norms(sqe_batch* b, const std::vector<uint32_t>& ids) {
             BTask t{};
             t.amp = amp_ptr(b, c);
             t.norm2 = b->norm2 + id;
+            t.scale = b->scale + id;
             t.ntiles = (uint32_t)c.tiles;
+            t.item0 = (uint32_t)items.size();
             const uint32_t ti = (uint32_t)tasks.size();
+            for (uint64_t tl = 0; tl < c.tiles; tl += NG) items.push_back(BItem{ti, (uint32_t)tl});
+            t.nitem = (uint32_t)items.size() - t.item0;
             tasks.push_back(t);
-            for (uint64_t tl = 0; tl < c.tiles; tl += 4) items.push_back(BItem{ti, (uint32_t)tl});
         }
         if (tasks.empty()) continue;
         BCK(buf_fit(b, &b->dtask, &b->dcap_t, tasks.size()));
         BCK(buf_fit(b, &b->ditem, &b->dcap_i, items.size()));
+        BCK(buf_fit(b, &b->partials, &b->pcap, items.size()));
         BHIP(hipStreamSynchronize(b->st));
         BHIP(hipMemcpy(b->dtask, tasks.data(), tasks.size() * sizeof(BTask), hipMemcpyHostToDevice));
         BHIP(hipMemcpy(b->ditem, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice));
-        for (uint32_t id : ids)
-            if (b->c[id].qm.mb == mb) BHIP(launch_fill_u64(b->st, (uint64_t*)(b->norm2 + id), 1, 0));
-        BatchArgs a{b->dtask, b->ditem, (uint32_t)items.size(), 4, b->queue, b->thr};
+        BatchArgs a{b->dtask, b->ditem, (uint32_t)items.size(), NG, b->thr, b->partials, nullptr};
         const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(4096, ((uint32_t)items.size() + 3) / 4));
-        BHIP(launch_norm_batch(mb, b->f32, b->st, grid, a));
+        BHIP(launch_norm_batch(mb, b->f32, b->st, grid, (uint32_t)tasks.size(), a));
     }
-    BHIP(launch_batch_scale(b->st, b->norm2, b->scale, (uint32_t)b->nslot));
     return SQE_OK;
 }
 
@@ -456,15 +460,10 @@ sqe_status sqe_batch_create(const sqe_config* cfg, sqe_batch** out) {
         (void)hipMemGetInfo(&fr, &tot);
         b->cfg.hbm_budget_bytes = (uint64_t)(0.8 * (double)fr);
     }
-    hipDeviceProp_t prop{};
-    (void)hipGetDeviceProperties(&prop, cfg->device);
-    const uint32_t cus = prop.multiProcessorCount > 0 ? (uint32_t)prop.multiProcessorCount : 256;
-    b->grid_waves = cus * 4 * 3;                 // 3 wavefronts per SIMD of the sweep kernels
     sqe_status r = SQE_OK;
     if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) r = bfail(SQE_EHIP, "stream");
     if (r == SQE_OK) r = dal(b, (void**)&b->stats, sizeof(Stats));
     if (r == SQE_OK) r = dal(b, (void**)&b->ticket, 64);
-    if (r == SQE_OK) r = dal(b, (void**)&b->queue, 64);
     if (r == SQE_OK) r = dal(b, (void**)&b->total, 64);
     if (r == SQE_OK && hipMemset(b->stats, 0, sizeof(Stats)) != hipSuccess) r = bfail(SQE_EHIP, "memset");
     if (r == SQE_OK && hipMemset(b->ticket, 0, 64) != hipSuccess) r = bfail(SQE_EHIP, "memset");
@@ -486,8 +485,8 @@ sqe_status sqe_batch_destroy(sqe_batch* b) {
     for (void* p : {(void*)b->base.p, (void*)b->amp.p, (void*)b->prog.p, (void*)b->coef.p, (void*)b->upd,
                     (void*)b->nnz, (void*)b->meta, (void*)b->norm2, (void*)b->scale, (void*)b->tab,
                     (void*)b->xkeys, (void*)b->xre, (void*)b->xim, (void*)b->occ, (void*)b->stats,
-                    (void*)b->scan_status, (void*)b->ticket, (void*)b->total, (void*)b->dtask, (void*)b->ditem,
-                    (void*)b->queue})
+                    (void*)b->scan_status, (void*)b->ticket, (void*)b->total, (void*)b->dtask, (void*)b->ditem, (void*)b->partials,
+                    (void*)b->counts})
         if (p) (void)hipFree(p);
     if (b->htask) (void)hipHostFree(b->htask);
     if (b->hitem) (void)hipHostFree(b->hitem);
@@ -569,11 +568,15 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         }
         return SQE_OK;
     };
+    int placed = 0;
     sqe_status st = place(b->ta, b->base, tiles, 64 * 8, &c.tile0);
-    if (st == SQE_OK) st = place(b->ra, b->amp, rows, 64 * b->csize, &c.row0);
-    if (st == SQE_OK) st = place(b->pa, b->prog, c.nops, sizeof(MOp), &c.prog0);
-    if (st == SQE_OK) st = place(b->ca, b->coef, c.ncoef, 8, &c.coef0);
+    if (st == SQE_OK && ++placed) st = place(b->ra, b->amp, rows, 64 * b->csize, &c.row0);
+    if (st == SQE_OK && ++placed) st = place(b->pa, b->prog, c.nops, sizeof(MOp), &c.prog0);
+    if (st == SQE_OK && ++placed) st = place(b->ca, b->coef, c.ncoef, 8, &c.coef0);
     if (st != SQE_OK) {
+        if (placed > 0) b->ta.give(c.tile0, c.tiles);
+        if (placed > 1) b->ra.give(c.row0, c.rows);
+        if (placed > 2 && c.nops) b->pa.give(c.prog0, c.nops);
         b->free_slots.push_back(id);
         return st;
     }
@@ -734,7 +737,6 @@ sqe_status sqe_batch_get_tuning(sqe_batch* b, sqe_batch_tuning* t) {
     if (!b || !t) return bfail(SQE_EINVAL, "null argument");
     t->block = b->block;
     t->tiles_per_item = b->gran;
-    t->waves = b->grid_waves;
     t->load_factor = b->cfg.max_load > 0 ? b->cfg.max_load : 0.5;
     t->sweep_ms = b->sweep_ms;
     t->launches = b->launches;
@@ -747,12 +749,11 @@ sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t) {
     if (t->tiles_per_item < 1 || t->tiles_per_item > 64) return bfail(SQE_EINVAL, "tiles_per_item in 1..64");
     b->block = t->block;
     b->gran = t->tiles_per_item;
-    if (t->waves) b->grid_waves = t->waves;
     if (t->load_factor > 0 && t->load_factor < 0.95) b->cfg.max_load = t->load_factor;
     return SQE_OK;
 }
 
-// Time every (workgroup size, tiles per item, resident waves) candidate on the
+// Time every (workgroup size, tiles per item) candidate on the
 // batch as it stands -- state and positions are saved and restored around each
 // trial -- and keep the fastest.
 sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
@@ -767,20 +768,17 @@ sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
     if (aext) BHIP(hipMemcpy(sa, b->amp.p, aext, hipMemcpyDeviceToDevice));
     std::vector<uint64_t> pos;
     for (const Circ& c : b->c) pos.push_back(c.pos);
-    const uint32_t cus = b->grid_waves / 12;
-    struct Cand { uint32_t block, gran, waves; };
+    struct Cand { uint32_t block, gran; };
     std::vector<Cand> cands;
     for (uint32_t bl : {64u, 128u, 256u})
-        for (uint32_t g : {1u, 2u, 4u})
-            for (uint32_t wv : {cus * 8, cus * 12}) cands.push_back(Cand{bl, g, wv});
+        for (uint32_t g : {1u, 2u, 4u, 8u}) cands.push_back(Cand{bl, g});
     double best = 1e300;
-    Cand bc{b->block, b->gran, b->grid_waves};
+    Cand bc{b->block, b->gran};
     const uint32_t timing = b->cfg.timing;
     b->cfg.timing = 0;
     for (const Cand& cd : cands) {
         b->block = cd.block;
         b->gran = cd.gran;
-        b->grid_waves = cd.waves;
         BHIP(hipEventRecord(b->ev0, b->st));
         BCK(sqe_batch_run(b));
         BHIP(hipEventRecord(b->ev1, b->st));
@@ -799,7 +797,6 @@ sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
     BHIP(launch_fill_u64(b->st, (uint64_t*)b->upd, b->nslot, 0));
     b->block = bc.block;
     b->gran = bc.gran;
-    b->grid_waves = bc.waves;
     if (best_out) {
         BCK(sqe_batch_get_tuning(b, best_out));
         best_out->sweep_ms = best;

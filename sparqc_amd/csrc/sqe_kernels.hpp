@@ -463,43 +463,43 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MB >= 5 ?
 
 // ---------------------------------------------------------------------------
 // Batched sweep: many independent circuits, each with its own bucket region
-// and program, advanced by one persistent launch.  Wavefronts take work items
-// (a circuit and a run of `gran` of its tiles) from a queue that the host has
-// ordered longest program first, so uneven depths and sizes balance out.
+// and program, advanced by one launch.  One wavefront per work item (a circuit
+// and a run of `gran` of its tiles); the host orders the items longest program
+// first, so the in-order dispatch of workgroups balances uneven depths and
+// sizes.  Per-item counts go to counts[2*item .. 2*item+1] (updates, nonzeros)
+// and are folded per circuit by k_batch_fold in a fixed order.
 // ---------------------------------------------------------------------------
 template <int MB, class R>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MB >= 5 ? 1 : 3))) void k_sweep_batch(
     BatchArgs A) {
     const uint32_t lane = threadIdx.x & 63;
-    for (;;) {
-        uint32_t it = 0;
-        if (lane == 0) it = atomicAdd(A.queue, 1u);
-        it = __builtin_amdgcn_readfirstlane(it);
-        if (it >= A.nitems) break;
-        const BItem item = A.items[it];
-        const BTask t = A.tasks[item.task];
-        unsigned long long upd = 0, nnz = 0;
-        for (uint32_t g = 0; g < A.gran; ++g) {
-            const uint32_t tile = item.tile + g;
-            if (tile >= t.ntiles) break;
-            const int64_t tl[1] = {(int64_t)tile};
-            SweepCount c{0, 0, 0, 0};
-            sweep_group<MB, R, 1>(t.base, t.amp, tl, t.ops, t.nops, t.coef, A.thr, t.scale, 1, c);
-            upd += c.upd;
-            nnz += c.nnz_out;
-        }
-        for (int d = 32; d >= 1; d >>= 1) {
-            upd += __shfl_xor(upd, d);
-            nnz += __shfl_xor(nnz, d);
-        }
-        if (lane == 0) {
-            if (upd) atomicAdd(t.upd, upd);
-            atomicAdd(t.nnz, nnz);
-        }
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t it = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (it >= A.nitems) return;
+    const BItem item = A.items[it];
+    const BTask t = A.tasks[item.task];
+    unsigned long long upd = 0, nnz = 0;
+    for (uint32_t g = 0; g < A.gran; ++g) {
+        const uint32_t tile = item.tile + g;
+        if (tile >= t.ntiles) break;
+        const int64_t tl[1] = {(int64_t)tile};
+        SweepCount c{0, 0, 0, 0};
+        sweep_group<MB, R, 1>(t.base, t.amp, tl, t.ops, t.nops, t.coef, A.thr, t.scale, 1, c);
+        upd += c.upd;
+        nnz += c.nnz_out;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        upd += __shfl_xor(upd, d);
+        nnz += __shfl_xor(nnz, d);
+    }
+    if (lane == 0) {
+        A.counts[2 * (uint64_t)it] = upd;
+        A.counts[2 * (uint64_t)it + 1] = nnz;
     }
 }
 
-// FP64 sum of |a|^2 per circuit over the work items' tiles (for FP32 renorm).
+// FP64 sum of |a|^2 of each work item's tiles -> partials[item] (FP32 renorm);
+// k_norm_batch_final folds them per circuit.
 template <int MB, class R>
 __global__ __launch_bounds__(BLOCK) void k_norm_batch(BatchArgs A) {
     using C = typename CT<R>::T;
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(BLOCK) void k_norm_batch(BatchArgs A) {
             });
         }
         for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
-        if (lane == 0) atomicAdd(t.norm2, acc);
+        if (lane == 0) A.partials[it] = acc;
     }
 }
 
@@ -838,7 +838,7 @@ struct Launch {
         SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_sweep_batch<MB_, R>), dim3(grid), dim3(block), 0, st, a));
         return hipGetLastError();
     }
-    static hipError_t norm_batch(int mb, hipStream_t st, uint32_t grid, const BatchArgs& a) {
+    static hipError_t norm_batch_partial(int mb, hipStream_t st, uint32_t grid, const BatchArgs& a) {
         SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_norm_batch<MB_, R>), dim3(grid), dim3(BLOCK), 0, st, a));
         return hipGetLastError();
     }

@@ -179,33 +179,65 @@ hipError_t launch_norm(int mb, bool f32, hipStream_t st, uint32_t grid, const No
     return hipGetLastError();
 }
 
-__global__ void k_batch_scale(const double* __restrict__ norm2, double* __restrict__ scale, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) scale[i] = norm2[i] > 0.0 ? 1.0 / sqrt(norm2[i]) : 1.0;
+// One wavefront per circuit: fold its items' partial sums in a fixed order.
+__global__ __launch_bounds__(BLOCK) void k_norm_batch_final(const BTask* __restrict__ tasks, uint32_t ntasks,
+                                                            const double* __restrict__ partials) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t ti = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    if (ti >= ntasks) return;
+    const BTask t = tasks[ti];
+    double acc = 0.0;
+    for (uint32_t i = lane; i < t.nitem; i += 64) acc += partials[t.item0 + i];
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
+    if (lane == 0) {
+        *t.norm2 = acc;
+        *t.scale = acc > 0.0 ? 1.0 / sqrt(acc) : 1.0;
+    }
 }
 
-__global__ void k_batch_zero(const BTask* __restrict__ t, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) *t[i].nnz = 0;
+// One wavefront per circuit: fold its items' counts in a fixed order (each
+// circuit appears in one task of a launch, so the plain update is race-free).
+__global__ __launch_bounds__(BLOCK) void k_batch_fold(const BTask* __restrict__ tasks, uint32_t ntasks,
+                                                      const unsigned long long* __restrict__ counts) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t ti = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    if (ti >= ntasks) return;
+    const BTask t = tasks[ti];
+    unsigned long long u = 0, z = 0;
+    for (uint32_t i = lane; i < t.nitem; i += 64) {
+        u += counts[2 * (uint64_t)(t.item0 + i)];
+        z += counts[2 * (uint64_t)(t.item0 + i) + 1];
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        u += __shfl_xor(u, d);
+        z += __shfl_xor(z, d);
+    }
+    if (lane == 0) {
+        *t.upd += u;
+        *t.nnz = z;
+    }
 }
 
-hipError_t launch_batch_zero(hipStream_t st, const BTask* tasks, uint32_t n) {
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_batch_zero, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, tasks, n);
+hipError_t launch_batch_fold(hipStream_t st, const BTask* tasks, uint32_t ntasks, const unsigned long long* counts) {
+    if (!ntasks) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_fold, dim3((ntasks + BLOCK / 64 - 1) / (BLOCK / 64)), dim3(BLOCK), 0, st, tasks,
+                       ntasks, counts);
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_batch(int mb, bool f32, hipStream_t st, uint32_t grid, uint32_t block, const BatchArgs& a) {
+hipError_t launch_sweep_batch(int mb, bool f32, hipStream_t st, uint32_t block, const BatchArgs& a) {
+    if (!a.nitems) return hipSuccess;
+    if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+    const uint32_t grid = (a.nitems + block / 64 - 1) / (block / 64);
     return f32 ? Launch<float>::sweep_batch(mb, st, grid, block, a) : Launch<double>::sweep_batch(mb, st, grid, block, a);
 }
 
-hipError_t launch_norm_batch(int mb, bool f32, hipStream_t st, uint32_t grid, const BatchArgs& a) {
-    return f32 ? Launch<float>::norm_batch(mb, st, grid, a) : Launch<double>::norm_batch(mb, st, grid, a);
-}
-
-hipError_t launch_batch_scale(hipStream_t st, const double* norm2, double* scale, uint32_t n) {
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_batch_scale, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, norm2, scale, n);
+hipError_t launch_norm_batch(int mb, bool f32, hipStream_t st, uint32_t grid, uint32_t ntasks, const BatchArgs& a) {
+    const hipError_t r = f32 ? Launch<float>::norm_batch_partial(mb, st, grid, a)
+                             : Launch<double>::norm_batch_partial(mb, st, grid, a);
+    if (r != hipSuccess || !ntasks) return r;
+    hipLaunchKernelGGL(k_norm_batch_final, dim3((ntasks + BLOCK / 64 - 1) / (BLOCK / 64)), dim3(BLOCK), 0, st,
+                       a.tasks, ntasks, (const double*)a.partials);
     return hipGetLastError();
 }
 

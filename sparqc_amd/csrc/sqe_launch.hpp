@@ -28,12 +28,14 @@ struct BTask {
     void* amp;                   // its amplitudes (tile-interleaved)
     const MOp* ops;              // program segment of this launch
     const double* coef;          // side table of general-U coefficients
-    const double* scale;         // renormalisation factor slot
-    unsigned long long* upd;     // per-circuit counters (atomically accumulated)
+    double* scale;               // renormalisation factor slot
+    unsigned long long* upd;     // per-circuit counters (folded after each launch)
     unsigned long long* nnz;
-    double* norm2;               // per-circuit FP64 norm accumulator
+    double* norm2;               // per-circuit FP64 norm
     uint32_t nops;
     uint32_t ntiles;
+    uint32_t item0;              // first work item of this circuit (items are grouped by circuit)
+    uint32_t nitem;
 };
 
 struct BItem {                   // a circuit and the first of `gran` consecutive tiles
@@ -46,8 +48,9 @@ struct BatchArgs {
     const BItem* items;
     uint32_t nitems;
     uint32_t gran;
-    unsigned int* queue;         // work-item counter (zeroed before the launch)
     double thr;
+    double* partials;            // per-item FP64 partial sums (norm)
+    unsigned long long* counts;  // per-item (updates, nonzeros) of a sweep
 };
 
 struct PairArgs {
@@ -142,10 +145,13 @@ hipError_t launch_export(int mb, bool f32, hipStream_t st, uint32_t grid, const 
 hipError_t launch_norm(int mb, bool f32, hipStream_t st, uint32_t grid, const NormArgs& a, double* norm2,
                        double* scale);
 hipError_t launch_index_build(hipStream_t st, uint32_t grid, const IndexArgs& a);
-hipError_t launch_sweep_batch(int mb, bool f32, hipStream_t st, uint32_t grid, uint32_t block, const BatchArgs& a);
-hipError_t launch_norm_batch(int mb, bool f32, hipStream_t st, uint32_t grid, const BatchArgs& a);
-hipError_t launch_batch_zero(hipStream_t st, const BTask* tasks, uint32_t n);
-hipError_t launch_batch_scale(hipStream_t st, const double* norm2, double* scale, uint32_t n);
+// One wavefront per work item: grid = ceil(nitems / (block / 64)).
+hipError_t launch_sweep_batch(int mb, bool f32, hipStream_t st, uint32_t block, const BatchArgs& a);
+// Per-circuit FP64 norms of the tasks' regions -> norm2 and scale = 1/sqrt(norm2),
+// summed in a fixed order (independent of the other circuits in the launch).
+hipError_t launch_norm_batch(int mb, bool f32, hipStream_t st, uint32_t grid, uint32_t ntasks, const BatchArgs& a);
+// Fold the sweep's per-item counts per circuit: *upd += updates, *nnz = nonzeros.
+hipError_t launch_batch_fold(hipStream_t st, const BTask* tasks, uint32_t ntasks, const unsigned long long* counts);
 hipError_t launch_stats_reduce(hipStream_t st, BlockStats* bs, uint32_t n_acc, uint32_t n_last, Stats* stats);
 hipError_t launch_load(bool f32, hipStream_t st, const LoadArgs& a, const EnumArgs& e, uint32_t enum_grid);
 

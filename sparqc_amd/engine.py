@@ -58,15 +58,14 @@ class BatchCircuit(ctypes.Structure):
 
 
 class BatchTuning(ctypes.Structure):
-    _fields_ = [("block", ctypes.c_uint32), ("tiles_per_item", ctypes.c_uint32), ("waves", ctypes.c_uint32),
-                ("pad", ctypes.c_uint32), ("load_factor", ctypes.c_double), ("sweep_ms", ctypes.c_double),
-                ("launches", ctypes.c_uint64)]
+    _fields_ = [("block", ctypes.c_uint32), ("tiles_per_item", ctypes.c_uint32),
+                ("load_factor", ctypes.c_double), ("sweep_ms", ctypes.c_double), ("launches", ctypes.c_uint64)]
 
 
 assert ctypes.sizeof(Config) == 80
 assert ctypes.sizeof(Stats) == 128
 assert ctypes.sizeof(BatchCircuit) == 56
-assert ctypes.sizeof(BatchTuning) == 40
+assert ctypes.sizeof(BatchTuning) == 32
 
 _lib = None
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -367,17 +366,15 @@ class Batch:
     def tuning(self):
         t = BatchTuning()
         _bcheck(lib().sqe_batch_get_tuning(self.h, ctypes.byref(t)))
-        return {f: getattr(t, f) for f, _ in BatchTuning._fields_ if f != "pad"}
+        return {f: getattr(t, f) for f, _ in BatchTuning._fields_}
 
-    def set_tuning(self, block=None, tiles_per_item=None, waves=None, load_factor=None):
+    def set_tuning(self, block=None, tiles_per_item=None, load_factor=None):
         t = BatchTuning()
         _bcheck(lib().sqe_batch_get_tuning(self.h, ctypes.byref(t)))
         if block is not None:
             t.block = block
         if tiles_per_item is not None:
             t.tiles_per_item = tiles_per_item
-        if waves is not None:
-            t.waves = waves
         if load_factor is not None:
             t.load_factor = load_factor
         _bcheck(lib().sqe_batch_set_tuning(self.h, ctypes.byref(t)))
@@ -385,7 +382,7 @@ class Batch:
     def autotune(self):
         t = BatchTuning()
         _bcheck(lib().sqe_batch_autotune(self.h, ctypes.byref(t)))
-        return {f: getattr(t, f) for f, _ in BatchTuning._fields_ if f != "pad"}
+        return {f: getattr(t, f) for f, _ in BatchTuning._fields_}
 
 
 def run(w, prec="f64", **kw):

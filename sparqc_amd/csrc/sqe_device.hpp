@@ -48,8 +48,11 @@ struct DOp {
 // handlers whose coefficients already encode the gate kind, so the kernel
 // dispatches on a handful of cases instead of kind x target-bit x controls.
 //   H_BASE_DIAG  diagonal gate on a base qubit: factor f0 / f1 by the base bit
-//   H_BASE_X     X on a base qubit: flip the bucket's base key
+//   H_BASE_X     X on a base qubit: flip the bucket's base key (factors 1, 1)
 //   H_BASE_PERM  Y / PERM on a base qubit: factor by the base bit, then flip
+//                (without bucket controls these three only multiply the
+//                bucket's pending FP64 factor, which is applied to the slots
+//                before the next branching gate and before the store)
 //   H_SLOT_DIAG  diagonal gate on bucket qubit J: factor f0 / f1 by slot bit J
 //   H_SLOT_X     X on bucket qubit J: swap slot pairs
 //   H_SLOT_PERM  Y / PERM on bucket qubit J: swap slot pairs with factors
@@ -62,9 +65,10 @@ struct DOp {
 // Coefficients c[]: DIAG f0r, f0i, f1r, f1i (Z: 1,0,-1,0; PHASE: 1,0,c,s;
 // RZ: c,-s,c,s); PERM u10r, u10i, u01r, u01i (the factor for the element
 // leaving bit 0, then for the element leaving bit 1); H f; U2 a, u01r, u01i,
-// u10r, u10i (RX: c,0,-s,0,-s; RY: c,-s,0,s,0).  With these the arithmetic is
-// operation for operation the reference formula of each kind (products with
-// exact 0 or +-1 only change the sign of zero results).
+// u10r, u10i (RX: c,0,-s,0,-s; RY: c,-s,0,s,0).  With these the arithmetic of
+// a slot gate is operation for operation the reference formula of its kind
+// (products with exact 0 or +-1 only change the sign of zero results); base
+// gates differ from it only by the folding of consecutive factors (a few ulps).
 enum : uint32_t {
     H_BASE_DIAG = 0, H_BASE_X, H_BASE_PERM, H_SLOT_DIAG, H_SLOT_X, H_SLOT_PERM, H_SLOT_H, H_SLOT_U2,
     H_SLOT_U, H_RENORM

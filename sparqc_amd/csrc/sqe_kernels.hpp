@@ -255,6 +255,9 @@ __device__ __forceinline__ void base_factor(const SOp& op, bool b, bool pred, do
     fi = pred ? (b ? op.c[3] : op.c[1]) : 0.0;
 }
 
+// Multiply every slot by the pending per-bucket factor (pr, pi), computed in
+// FP64: the reference's phase formula (emulator/device/device_data.cuh:221-224)
+// with the folded factor.
 template <int MB>
 __device__ __forceinline__ void apply_phase(Cplx<float> (&a)[1 << MB], double pr, double pi) {
     static_for<(1 << MB)>([&](auto sc) __attribute__((always_inline)) {
@@ -264,16 +267,35 @@ __device__ __forceinline__ void apply_phase(Cplx<float> (&a)[1 << MB], double pr
         a[s].y = (float)(pr * y + pi * x);
     });
 }
+template <int MB>
+__device__ __forceinline__ void apply_phase(Cplx<double> (&a)[1 << MB], double pr, double pi) {
+    static_for<(1 << MB)>([&](auto sc) __attribute__((always_inline)) {
+        constexpr int s = decltype(sc)::value;
+        a[s] = rot(pr, pi, a[s]);
+    });
+}
 
 // Per-thread counters of one sweep group.
 struct SweepCount {
     unsigned long long upd, nnz_in, nnz_out, empty;
 };
 
+// Ops that touch no amplitude register: gates on a base qubit without bucket
+// controls (X flips the base key; diagonal and Y/PERM factors are folded into
+// the bucket's pending FP64 factor).
+__device__ __forceinline__ bool base_class(uint32_t h, uint32_t cmS) { return h <= H_BASE_PERM && cmS == 0; }
+
 // Apply one program to EPT tiles (64 buckets each) with one wavefront: every op
 // is decoded once and applied to all of the thread's buckets.  tile[e] < 0
 // marks a missing tile (the wavefront computes on zeros and stores nothing).
 // Tiles live at basep / ampv in the tile-interleaved layout of sqe_device.hpp.
+//
+// The program runs as alternating runs of base-class ops, which only update
+// the base keys and the pending factors (the amplitude registers stay
+// untouched, so the loop carries them without copies), and slot ops, which
+// work on the amplitude registers.  The pending factor is a scalar per bucket,
+// so it commutes with every bucket-local gate; it is multiplied in before a
+// branching op (whose prune test needs true values) and before the store.
 template <int MB, class R, int EPT>
 __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* ampv, const int64_t (&tile)[EPT],
                                             const MOp* __restrict__ ops, uint32_t nops,
@@ -281,14 +303,13 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                                             const double* __restrict__ scale, int store, SweepCount& out) {
     using C = typename CT<R>::T;
     constexpr int S = 1 << MB;
-    constexpr bool F32 = sizeof(R) == 4;
     const uint32_t lane = threadIdx.x & 63;
     C* __restrict__ amp = (C*)ampv;
     C a[EPT][S];
     uint64_t base[EPT], base0[EPT];
     bool valid[EPT];
     uint32_t cnt[EPT];
-    double pr[EPT], pi[EPT];          // FP32 form: pending per-bucket factor
+    double pr[EPT], pi[EPT];          // pending per-bucket factor
     bool pending = false;             // wavefront-uniform: a factor is pending
     out.upd = 0;
     out.nnz_in = 0;
@@ -315,105 +336,125 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
         pr[e] = 1.0;
         pi[e] = 0.0;
     });
-    if (nops) {
-        // The program is staged 64 ops at a time in the lanes of the wavefront
-        // (lane j holds the 16 dwords of op c+j, one coalesced load per chunk);
-        // each op is then pulled into scalar registers with v_readlane, so no
-        // per-gate memory round trip sits on the critical path.
-        const uint32_t* __restrict__ words = (const uint32_t*)ops;
-        uint32_t w[16];
-        for (uint32_t c = 0; c < nops; c += 64) {
-            const uint32_t mine = c + lane < nops ? c + lane : nops - 1;
+    // The program is staged 64 ops at a time in the lanes of the wavefront
+    // (lane j holds the 16 dwords of op c0+j, one coalesced load per chunk);
+    // each op is then pulled into scalar registers with v_readlane, so no
+    // per-gate memory round trip sits on the critical path.
+    const uint32_t* __restrict__ words = (const uint32_t*)ops;
+    uint32_t w[16];
+    uint32_t c0 = 0;
+    auto stage = [&](uint32_t from) __attribute__((always_inline)) {
+        c0 = from;
+        const uint32_t mine = from + lane < nops ? from + lane : nops - 1;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) w[k] = words[(size_t)mine * 16 + k];
-            const uint32_t cn = __builtin_amdgcn_readfirstlane(nops - c < 64 ? nops - c : 64);
-            for (uint32_t jj = 0; jj < cn; ++jj) {
-                auto rd = [&](int k) __attribute__((always_inline)) {
-                    return (uint32_t)__builtin_amdgcn_readlane(w[k], jj);
-                };
-                auto rd64 = [&](int k) __attribute__((always_inline)) {
-                    return (uint64_t)rd(k) | (uint64_t)rd(k + 1) << 32;
-                };
-                SOp op;
-                const uint32_t hdr = rd(0), ctl = rd(1);
-                op.h = hdr & 15u;
-                op.j = (hdr >> 4) & 7u;
-                op.tbase = 1ull << ((hdr >> 7) & 63u);
-                op.ci = hdr >> 13;
-                op.cmS = ctl & 0xffu;
-                op.cvS = (ctl >> 8) & 0xffu;
-                op.cmB = rd64(2);
-                op.cvB = rd64(4);
-#pragma unroll
-                for (int k = 0; k < 5; ++k) op.c[k] = __builtin_bit_cast(double, rd64(6 + 2 * k));
-                if (op.h == H_RENORM) {
-                    const double f = *scale;
-                    static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
-                        constexpr int e = decltype(ec)::value;
-                        if constexpr (F32) {
-                            pr[e] *= f;
-                            pi[e] *= f;
-                        } else {
-                            static_for<S>([&](auto sc) __attribute__((always_inline)) {
-                                constexpr int s = decltype(sc)::value;
-                                a[e][s].x = (R)((double)a[e][s].x * f);
-                                a[e][s].y = (R)((double)a[e][s].y * f);
-                            });
-                        }
-                    });
-                    if (F32) pending = true;
-                    continue;
-                }
-                bool pred[EPT];
+        for (int k = 0; k < 16; ++k) w[k] = words[(size_t)mine * 16 + k];
+    };
+    if (nops) stage(0);
+    auto rd = [&](int k, uint32_t jj) __attribute__((always_inline)) {
+        return (uint32_t)__builtin_amdgcn_readlane(w[k], jj);
+    };
+    auto rd64 = [&](int k, uint32_t jj) __attribute__((always_inline)) {
+        return (uint64_t)rd(k, jj) | (uint64_t)rd(k + 1, jj) << 32;
+    };
+    uint32_t i = 0;
+    while (i < nops) {
+        // ---- run of base ops: base keys and pending factors only ----
+        // one branch-free form for X / diagonal / Y-PERM: factor c0 for a
+        // bucket whose target bit is 0, c1 for bit 1 (X: 1 and 1), then the
+        // base key flips for X / Y / PERM; lanes whose base controls fail take
+        // factor 1 and keep their key.
+        for (; i < nops; ++i) {
+            if (i - c0 >= 64) stage(i);
+            const uint32_t jj = i - c0;
+            const uint32_t hdr = rd(0, jj), ctl = rd(1, jj);
+            const uint32_t h = hdr & 15u;
+            if (!base_class(h, ctl & 0xffu)) break;
+            const uint64_t tbase = 1ull << ((hdr >> 7) & 63u);
+            const uint64_t flip = (h == H_BASE_X || h == H_BASE_PERM) ? tbase : 0ull;
+            const uint64_t cmB = rd64(2, jj), cvB = rd64(4, jj);
+            const double c0r = __builtin_bit_cast(double, rd64(6, jj));
+            const double c0i = __builtin_bit_cast(double, rd64(8, jj));
+            const double c1r = __builtin_bit_cast(double, rd64(10, jj));
+            const double c1i = __builtin_bit_cast(double, rd64(12, jj));
+            static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                constexpr int e = decltype(ec)::value;
+                out.upd += cnt[e];
+                const bool pred = valid[e] && ((base[e] & cmB) == cvB);
+                const bool bit = (base[e] & tbase) != 0;
+                const double fr = pred ? (bit ? c1r : c0r) : 1.0;
+                const double fi = pred ? (bit ? c1i : c0i) : 0.0;
+                const double r = pr[e] * fr - pi[e] * fi, m = pr[e] * fi + pi[e] * fr;
+                pr[e] = r;
+                pi[e] = m;
+                base[e] ^= pred ? flip : 0ull;
+            });
+            pending = true;
+        }
+        // ---- run of slot ops: amplitude registers ----
+        for (; i < nops; ++i) {
+            if (i - c0 >= 64) stage(i);
+            const uint32_t jj = i - c0;
+            SOp op;
+            const uint32_t hdr = rd(0, jj), ctl = rd(1, jj);
+            op.h = hdr & 15u;
+            op.cmS = ctl & 0xffu;
+            if (base_class(op.h, op.cmS)) break;
+            if (op.h == H_RENORM) {
+                const double f = *scale;
                 static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                     constexpr int e = decltype(ec)::value;
-                    out.upd += cnt[e];
-                    pred[e] = valid[e] && ((base[e] & op.cmB) == op.cvB);
+                    pr[e] *= f;
+                    pi[e] *= f;
                 });
-                if (op.h <= H_BASE_PERM) {
-                    // gate on a base qubit
-                    const bool fold = F32 && op.cmS == 0;
+                pending = true;
+                continue;
+            }
+            op.cvS = (ctl >> 8) & 0xffu;
+            op.j = (hdr >> 4) & 7u;
+            op.tbase = 1ull << ((hdr >> 7) & 63u);
+            op.ci = hdr >> 13;
+            op.cmB = rd64(2, jj);
+            op.cvB = rd64(4, jj);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) op.c[k] = __builtin_bit_cast(double, rd64(6 + 2 * k, jj));
+            bool pred[EPT];
+            static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                constexpr int e = decltype(ec)::value;
+                out.upd += cnt[e];
+                pred[e] = valid[e] && ((base[e] & op.cmB) == op.cvB);
+            });
+            if (op.h <= H_BASE_PERM) {
+                // diagonal / Y / PERM on a base qubit, restricted by bucket controls
+                static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                    constexpr int e = decltype(ec)::value;
+                    const bool bit = (base[e] & op.tbase) != 0;
+                    double fr, fi;
+                    base_factor(op, bit, pred[e], fr, fi);
+                    const R cr = (R)fr, ci = (R)fi;
+                    static_for<S>([&](auto sc) __attribute__((always_inline)) {
+                        constexpr int s = decltype(sc)::value;
+                        if (((uint32_t)s & op.cmS) == op.cvS) a[e][s] = rot(cr, ci, a[e][s]);
+                    });
+                });
+                continue;
+            }
+            if constexpr (MB > 0) {
+                const bool branch = op.h >= H_SLOT_H && op.h <= H_SLOT_U;
+                if (pending && branch) {
                     static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                         constexpr int e = decltype(ec)::value;
-                        if (op.h != H_BASE_X) {
-                            const bool bit = (base[e] & op.tbase) != 0;
-                            double fr, fi;
-                            base_factor(op, bit, pred[e], fr, fi);
-                            if (fold) {
-                                const double r = pr[e] * fr - pi[e] * fi, i = pr[e] * fi + pi[e] * fr;
-                                pr[e] = r;
-                                pi[e] = i;
-                            } else {
-                                const R cr = (R)fr, ci = (R)fi;
-                                static_for<S>([&](auto sc) __attribute__((always_inline)) {
-                                    constexpr int s = decltype(sc)::value;
-                                    if (((uint32_t)s & op.cmS) == op.cvS) a[e][s] = rot(cr, ci, a[e][s]);
-                                });
-                            }
-                        }
-                        if (op.h != H_BASE_DIAG && pred[e]) base[e] ^= op.tbase;
+                        apply_phase<MB>(a[e], pr[e], pi[e]);
+                        pr[e] = 1.0;
+                        pi[e] = 0.0;
                     });
-                    if (fold && op.h != H_BASE_X) pending = true;
-                    continue;
+                    pending = false;
                 }
-                if constexpr (MB > 0) {
-                    const bool branch = op.h >= H_SLOT_H && op.h <= H_SLOT_U;
-                    if (F32 && pending && branch) {
-                        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
-                            constexpr int e = decltype(ec)::value;
-                            if constexpr (F32) apply_phase<MB>(a[e], pr[e], pi[e]);
-                            pr[e] = 1.0;
-                            pi[e] = 0.0;
-                        });
-                        pending = false;
-                    }
-                    slot_dispatch<MB, R, EPT>(op, a, pred, thr, coef);
-                    if (branch) {
-                        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
-                            constexpr int e = decltype(ec)::value;
-                            cnt[e] = valid[e] ? count_nz<MB, R>(a[e]) : 0u;
-                        });
-                    }
+                slot_dispatch<MB, R, EPT>(op, a, pred, thr, coef);
+                if (branch) {
+                    static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                        constexpr int e = decltype(ec)::value;
+                        cnt[e] = valid[e] ? count_nz<MB, R>(a[e]) : 0u;
+                    });
                 }
             }
         }
@@ -422,9 +463,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
     out.empty = 0;
     static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
         constexpr int e = decltype(ec)::value;
-        if constexpr (F32) {
-            if (pending) apply_phase<MB>(a[e], pr[e], pi[e]);
-        }
+        if (pending) apply_phase<MB>(a[e], pr[e], pi[e]);
         if (tile[e] >= 0 && store) {
             const uint64_t b = (uint64_t)tile[e] * TILE + lane;
             static_for<S>([&](auto sc) __attribute__((always_inline)) {
@@ -438,10 +477,26 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
     });
 }
 
+// Minimum wavefronts per SIMD the sweep kernels are compiled for (caps the
+// VGPRs at 512 / value): the amplitude registers of a bucket take 4 * 2^MB
+// (FP64) or 2 * 2^MB (FP32) VGPRs.  FP64 with 16 slots needs ~240 VGPRs to
+// keep the slot handlers free of spills, hence 2 there.
+#ifndef SQE_OCC_F64_MB4
+#define SQE_OCC_F64_MB4 2
+#endif
+#ifndef SQE_OCC_F32_MB4
+#define SQE_OCC_F32_MB4 3
+#endif
+template <int MB, class R>
+struct SweepOcc {
+    static constexpr int value =
+        MB >= 5 ? 1 : (MB == 4 ? (sizeof(R) == 8 ? SQE_OCC_F64_MB4 : SQE_OCC_F32_MB4) : 3);
+};
+
 // One program over the engine's store: wavefront g of the grid takes tiles
 // EPT*g .. EPT*g + EPT-1.
 template <int MB, class R, int EPT>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MB >= 5 ? 1 : 3))) void k_sweep(SweepArgs A) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<MB, R>::value))) void k_sweep(SweepArgs A) {
     const uint64_t nb = *A.nb;
     const int64_t ntiles = (int64_t)((nb + 63) >> 6);
     // tile ownership is decided per wavefront from a scalar wave index, so the
@@ -470,7 +525,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MB >= 5 ?
 // and are folded per circuit by k_batch_fold in a fixed order.
 // ---------------------------------------------------------------------------
 template <int MB, class R>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MB >= 5 ? 1 : 3))) void k_sweep_batch(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<MB, R>::value))) void k_sweep_batch(
     BatchArgs A) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

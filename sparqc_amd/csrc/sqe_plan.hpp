@@ -104,7 +104,7 @@ inline MOp encode_op(const DOp& d, std::vector<double>& coef) {
     case SQE_GATE_PHASE: set(1, 0, p[0], p[1], 0); h = slot ? H_SLOT_DIAG : H_BASE_DIAG; break;
     case SQE_GATE_RZ: set(p[0], -p[1], p[0], p[1], 0); h = slot ? H_SLOT_DIAG : H_BASE_DIAG; break;
     case SQE_GATE_DIAG: set(p[0], p[1], p[2], p[3], 0); h = slot ? H_SLOT_DIAG : H_BASE_DIAG; break;
-    case SQE_GATE_X: h = slot ? H_SLOT_X : H_BASE_X; break;
+    case SQE_GATE_X: set(1, 0, 1, 0, 0); h = slot ? H_SLOT_X : H_BASE_X; break;           // base: factor 1
     case SQE_GATE_Y: set(0, 1, 0, -1, 0); h = slot ? H_SLOT_PERM : H_BASE_PERM; break;      // u10 = i, u01 = -i
     case SQE_GATE_PERM: set(p[2], p[3], p[0], p[1], 0); h = slot ? H_SLOT_PERM : H_BASE_PERM; break;
     case SQE_GATE_H: set(p[0], 0, 0, 0, 0); h = H_SLOT_H; break;

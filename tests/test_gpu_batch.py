@@ -64,7 +64,7 @@ def test_batch_steps_equal_full_run(oracle):
         say("stepped batch: %d steps" % steps)
         assert steps >= max(len(c["gates"]) for c in cs) // 7
         for a, c in zip(zip(i1, i2), cs):
-            assert_same_state(b2.export(a[1]), b1.export(a[0]), atol=0.0, label="circuit %d" % a[0])
+            assert_same_state(b2.export(a[1]), b1.export(a[0]), atol=1e-12, label="circuit %d" % a[0])
             assert b1.info(a[0])["updates"] == b2.info(a[1])["updates"]
 
 

@@ -1,8 +1,11 @@
 """GPU parity: the HIP engine (through the C ABI) against the CPU oracle.
 
-Bar: bit-exact key sets; FP64 amplitudes within 1e-12 (the FP64 kernels use the
-oracle's operation order, so they normally agree to the bit); FP32 against the
-FP32 oracle within 1e-6 with key sets equal away from the prune threshold.
+Bar: bit-exact key sets; FP64 amplitudes within 1e-12 absolute (gates on bucket
+qubits use the oracle's operation order; consecutive factors of gates on base
+qubits are multiplied together before they reach the amplitudes, a few ulps
+apart from the oracle's one-gate-at-a-time order; a single gate agrees to the
+bit); FP32 against the FP32 oracle within 1e-6 with key sets equal away from
+the prune threshold.
 """
 import math
 import sys

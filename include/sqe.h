@@ -74,13 +74,15 @@ typedef struct sqe_config {
     uint32_t prec;              /* sqe_prec                                            */
     double prune_thr;           /* 0: 1e-14 (F64), 1e-7 (F32)                          */
     double max_load;            /* hash index load factor, 0: 0.5                      */
-    uint64_t hbm_budget_bytes;  /* 0: 90% of the device's free memory                  */
+    uint64_t hbm_budget_bytes;  /* 0: free device memory less the reserve below        */
     int32_t device;             /* HIP device ordinal                                  */
     int32_t bucket_bits;        /* -1: planner chooses; else 0..4 (F64) / 0..5 (F32)   */
     uint32_t renorm_every;      /* renormalise in FP64 every N gates, 0: never         */
     uint32_t fuse_window;       /* max gates per fused sweep, 0: unlimited             */
     uint32_t timing;            /* 1: time every kernel launch with HIP events         */
-    uint32_t reserved[7];
+    uint32_t mem_reserve_ppm;   /* with hbm_budget_bytes = 0: fraction of free device
+                                   memory kept unused, parts per million (0: 10%)     */
+    uint32_t reserved[6];
 } sqe_config;
 
 typedef struct sqe_stats {
@@ -227,6 +229,10 @@ sqe_status sqe_batch_export(sqe_batch* b, int64_t id, uint64_t* keys, double* re
 /* Release a circuit's pool space (finished or not). */
 sqe_status sqe_batch_remove(sqe_batch* b, int64_t id);
 
+/* Ids of admitted circuits whose every gate has been enqueued (host state, no
+ * wait); *n receives the count, SQE_EINVAL when it exceeds cap. */
+sqe_status sqe_batch_finished(sqe_batch* b, int64_t* ids, uint64_t cap, uint64_t* n);
+
 /* Admitted bytes, budget, and circuits not yet done. */
 sqe_status sqe_batch_usage(sqe_batch* b, uint64_t* used, uint64_t* budget, uint64_t* active);
 
@@ -237,6 +243,11 @@ sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t);
  * admitted circuits (their state is restored after every trial) and keep the
  * fastest; *best (optional) receives it. */
 sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best);
+
+/* Keep a device-side copy of every admitted circuit's state, position and
+ * counters / return to it (invalid after sqe_batch_add or sqe_batch_remove). */
+sqe_status sqe_batch_snapshot_save(sqe_batch* b);
+sqe_status sqe_batch_snapshot_restore(sqe_batch* b);
 
 const char* sqe_batch_last_error(void);
 

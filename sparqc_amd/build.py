@@ -60,6 +60,25 @@ def build(verbose=False):
     return LIB
 
 
+CHECK_SRC = os.path.join(ROOT, "tests", "cpp", "circuit_check.cpp")
+CHECK_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "circuit_check")
+
+
+def build_circuit_check(verbose=False):
+    """Host-only C++ driver of include/sparqc_circuit.hpp linked to libsqe.so
+    (g++; it finds the library through an rpath relative to itself)."""
+    os.makedirs(os.path.dirname(CHECK_BIN), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), CHECK_SRC,
+           "-L", OUT_DIR, "-lsqe", "-Wl,-rpath,$ORIGIN/../../../sparqc_amd/_build", "-o", CHECK_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed for circuit_check:\n%s" % r.stderr[-4000:])
+    if verbose:
+        print("built", CHECK_BIN)
+    return CHECK_BIN
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_circuit_check(verbose=True)
     sys.exit(0)

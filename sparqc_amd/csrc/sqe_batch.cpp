@@ -165,6 +165,12 @@ struct sqe_batch {
     unsigned long long* counts = nullptr;     // per-item sweep counts
     uint64_t ccap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // device-side snapshot of every admitted circuit (pools, positions, counters)
+    char *snap_base = nullptr, *snap_amp = nullptr;
+    unsigned long long* snap_cnt = nullptr;
+    uint64_t snap_bcap = 0, snap_acap = 0, snap_ccap = 0, snap_bext = 0, snap_aext = 0;
+    std::vector<uint64_t> snap_pos;
+    uint64_t generation = 0, snap_generation = ~0ull;
     // tuning and counters
     uint32_t block = BLOCK, gran = 1;
     double sweep_ms = 0.0;
@@ -304,6 +310,7 @@ void release(sqe_batch* b, uint32_t id) {
     b->used -= c.bytes;
     c = Circ{};
     b->free_slots.push_back(id);
+    ++b->generation;
 }
 
 // Launch the sweeps for one step: every live circuit whose program is not
@@ -425,6 +432,45 @@ sqe_status run_norms(sqe_batch* b, const std::vector<uint32_t>& ids) {
     return SQE_OK;
 }
 
+sqe_status snap_save(sqe_batch* b) {
+    BHIP(hipStreamSynchronize(b->st));
+    const uint64_t bext = b->ta.used_extent() * 64 * 8, aext = b->ra.used_extent() * 64 * b->csize;
+    auto fit = [&](char** p, uint64_t* cap, uint64_t need) -> sqe_status {
+        if (need <= *cap) return SQE_OK;
+        dfr(b, *p, *cap);
+        *p = nullptr;
+        *cap = 0;
+        BCK(dal(b, (void**)p, need));
+        *cap = need;
+        return SQE_OK;
+    };
+    BCK(fit(&b->snap_base, &b->snap_bcap, std::max<uint64_t>(bext, 8)));
+    BCK(fit(&b->snap_amp, &b->snap_acap, std::max<uint64_t>(aext, 8)));
+    BCK(fit((char**)&b->snap_cnt, &b->snap_ccap, 2 * b->nslot * 8));
+    if (bext) BHIP(hipMemcpyAsync(b->snap_base, b->base.p, bext, hipMemcpyDeviceToDevice, b->st));
+    if (aext) BHIP(hipMemcpyAsync(b->snap_amp, b->amp.p, aext, hipMemcpyDeviceToDevice, b->st));
+    BHIP(hipMemcpyAsync(b->snap_cnt, b->upd, b->nslot * 8, hipMemcpyDeviceToDevice, b->st));
+    BHIP(hipMemcpyAsync(b->snap_cnt + b->nslot, b->nnz, b->nslot * 8, hipMemcpyDeviceToDevice, b->st));
+    BHIP(hipStreamSynchronize(b->st));
+    b->snap_bext = bext;
+    b->snap_aext = aext;
+    b->snap_pos.clear();
+    for (const Circ& c : b->c) b->snap_pos.push_back(c.pos);
+    b->snap_generation = b->generation;
+    return SQE_OK;
+}
+
+sqe_status snap_restore(sqe_batch* b) {
+    if (b->snap_generation != b->generation)
+        return bfail(SQE_EINVAL, "no snapshot of the current set of circuits (add / remove since the last save)");
+    if (b->snap_bext) BHIP(hipMemcpyAsync(b->base.p, b->snap_base, b->snap_bext, hipMemcpyDeviceToDevice, b->st));
+    if (b->snap_aext) BHIP(hipMemcpyAsync(b->amp.p, b->snap_amp, b->snap_aext, hipMemcpyDeviceToDevice, b->st));
+    BHIP(hipMemcpyAsync(b->upd, b->snap_cnt, b->nslot * 8, hipMemcpyDeviceToDevice, b->st));
+    BHIP(hipMemcpyAsync(b->nnz, b->snap_cnt + b->nslot, b->nslot * 8, hipMemcpyDeviceToDevice, b->st));
+    for (size_t j = 0; j < b->c.size() && j < b->snap_pos.size(); ++j) b->c[j].pos = b->snap_pos[j];
+    return SQE_OK;
+}
+
 sqe_status step_all(sqe_batch* b, uint64_t max_gates) {
     // a gate budget of N ops per circuit (RENORM ops ride along)
     std::vector<uint32_t> nn;
@@ -458,7 +504,8 @@ sqe_status sqe_batch_create(const sqe_config* cfg, sqe_batch** out) {
     if (!b->cfg.hbm_budget_bytes) {
         size_t fr = 0, tot = 0;
         (void)hipMemGetInfo(&fr, &tot);
-        b->cfg.hbm_budget_bytes = (uint64_t)(0.8 * (double)fr);
+        const double keep = b->cfg.mem_reserve_ppm ? std::min(1.0, b->cfg.mem_reserve_ppm * 1e-6) : 0.1;
+        b->cfg.hbm_budget_bytes = (uint64_t)((1.0 - keep) * (double)fr);
     }
     sqe_status r = SQE_OK;
     if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) r = bfail(SQE_EHIP, "stream");
@@ -486,7 +533,7 @@ sqe_status sqe_batch_destroy(sqe_batch* b) {
                     (void*)b->nnz, (void*)b->meta, (void*)b->norm2, (void*)b->scale, (void*)b->tab,
                     (void*)b->xkeys, (void*)b->xre, (void*)b->xim, (void*)b->occ, (void*)b->stats,
                     (void*)b->scan_status, (void*)b->ticket, (void*)b->total, (void*)b->dtask, (void*)b->ditem, (void*)b->partials,
-                    (void*)b->counts})
+                    (void*)b->counts, (void*)b->snap_base, (void*)b->snap_amp, (void*)b->snap_cnt})
         if (p) (void)hipFree(p);
     if (b->htask) (void)hipHostFree(b->htask);
     if (b->hitem) (void)hipHostFree(b->hitem);
@@ -583,6 +630,7 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
     c.live = true;
     b->c[id] = c;
     b->used += bytes;
+    ++b->generation;
     Circ& cc = b->c[id];
     // upload program and coefficients
     if (cc.nops) BHIP(hipMemcpyAsync(b->prog.p + cc.prog0 * sizeof(MOp), prog.data(), cc.nops * sizeof(MOp),
@@ -721,6 +769,19 @@ sqe_status sqe_batch_remove(sqe_batch* b, int64_t id) {
     return SQE_OK;
 }
 
+sqe_status sqe_batch_finished(sqe_batch* b, int64_t* ids, uint64_t cap, uint64_t* n) {
+    if (!b || !n) return bfail(SQE_EINVAL, "null argument");
+    uint64_t k = 0;
+    for (uint32_t id = 0; id < b->c.size(); ++id) {
+        const Circ& c = b->c[id];
+        if (!c.live || c.pos < c.nops) continue;
+        if (ids && k < cap) ids[k] = id;
+        ++k;
+    }
+    *n = k;
+    return k > cap ? bfail(SQE_EINVAL, "id capacity too small") : SQE_OK;
+}
+
 sqe_status sqe_batch_usage(sqe_batch* b, uint64_t* used, uint64_t* budget, uint64_t* active) {
     if (!b) return bfail(SQE_EINVAL, "null batch");
     if (used) *used = b->used;
@@ -759,15 +820,7 @@ sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t) {
 sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
     if (!b) return bfail(SQE_EINVAL, "null batch");
     BHIP(hipSetDevice(b->cfg.device));
-    BHIP(hipStreamSynchronize(b->st));
-    const uint64_t bext = b->ta.used_extent() * 64 * 8, aext = b->ra.used_extent() * 64 * b->csize;
-    char *sb = nullptr, *sa = nullptr;
-    BCK(dal(b, (void**)&sb, std::max<uint64_t>(bext, 8)));
-    BCK(dal(b, (void**)&sa, std::max<uint64_t>(aext, 8)));
-    if (bext) BHIP(hipMemcpy(sb, b->base.p, bext, hipMemcpyDeviceToDevice));
-    if (aext) BHIP(hipMemcpy(sa, b->amp.p, aext, hipMemcpyDeviceToDevice));
-    std::vector<uint64_t> pos;
-    for (const Circ& c : b->c) pos.push_back(c.pos);
+    BCK(snap_save(b));
     struct Cand { uint32_t block, gran; };
     std::vector<Cand> cands;
     for (uint32_t bl : {64u, 128u, 256u})
@@ -786,15 +839,10 @@ sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
         float ms = 0.f;
         BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
         if (ms < best) { best = ms; bc = cd; }
-        if (bext) BHIP(hipMemcpyAsync(b->base.p, sb, bext, hipMemcpyDeviceToDevice, b->st));
-        if (aext) BHIP(hipMemcpyAsync(b->amp.p, sa, aext, hipMemcpyDeviceToDevice, b->st));
+        BCK(snap_restore(b));
         BHIP(hipStreamSynchronize(b->st));
-        for (size_t j = 0; j < b->c.size(); ++j) b->c[j].pos = pos[j];
     }
     b->cfg.timing = timing;
-    dfr(b, sb, std::max<uint64_t>(bext, 8));
-    dfr(b, sa, std::max<uint64_t>(aext, 8));
-    BHIP(launch_fill_u64(b->st, (uint64_t*)b->upd, b->nslot, 0));
     b->block = bc.block;
     b->gran = bc.gran;
     if (best_out) {
@@ -802,6 +850,18 @@ sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
         best_out->sweep_ms = best;
     }
     return SQE_OK;
+}
+
+sqe_status sqe_batch_snapshot_save(sqe_batch* b) {
+    if (!b) return bfail(SQE_EINVAL, "null batch");
+    BHIP(hipSetDevice(b->cfg.device));
+    return snap_save(b);
+}
+
+sqe_status sqe_batch_snapshot_restore(sqe_batch* b) {
+    if (!b) return bfail(SQE_EINVAL, "null batch");
+    BHIP(hipSetDevice(b->cfg.device));
+    return snap_restore(b);
 }
 
 const char* sqe_batch_last_error(void) { return b_err.c_str(); }

@@ -774,7 +774,8 @@ sqe_status sqe_create(const sqe_config* cfg, sqe_engine** out) {
     if (e->cfg.hbm_budget_bytes == 0) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete e; return fail(SQE_EHIP, "hipMemGetInfo"); }
-        e->cfg.hbm_budget_bytes = (uint64_t)(0.9 * (double)fr);
+        const double keep = e->cfg.mem_reserve_ppm ? std::min(1.0, e->cfg.mem_reserve_ppm * 1e-6) : 0.1;
+        e->cfg.hbm_budget_bytes = (uint64_t)((1.0 - keep) * (double)fr);
     }
     sqe_status r = SQE_OK;
     if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) r = fail(SQE_EHIP, "stream");

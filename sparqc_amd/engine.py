@@ -24,8 +24,10 @@ EXPORTS = ["sqe_config_default", "sqe_create", "sqe_destroy", "sqe_set_basis", "
            "sqe_get_stats", "sqe_prepare", "sqe_snapshot_save", "sqe_snapshot_restore",
            "sqe_last_error",
            "sqe_batch_create", "sqe_batch_destroy", "sqe_batch_add", "sqe_batch_step", "sqe_batch_run",
-           "sqe_batch_sync", "sqe_batch_info", "sqe_batch_export", "sqe_batch_remove", "sqe_batch_usage",
-           "sqe_batch_get_tuning", "sqe_batch_set_tuning", "sqe_batch_autotune", "sqe_batch_last_error"]
+           "sqe_batch_sync", "sqe_batch_info", "sqe_batch_export", "sqe_batch_remove", "sqe_batch_finished",
+           "sqe_batch_usage",
+           "sqe_batch_get_tuning", "sqe_batch_set_tuning", "sqe_batch_autotune", "sqe_batch_snapshot_save",
+           "sqe_batch_snapshot_restore", "sqe_batch_last_error"]
 
 
 class SqeError(RuntimeError):
@@ -39,7 +41,8 @@ class Config(ctypes.Structure):
                 ("max_load", ctypes.c_double), ("hbm_budget_bytes", ctypes.c_uint64),
                 ("device", ctypes.c_int32), ("bucket_bits", ctypes.c_int32),
                 ("renorm_every", ctypes.c_uint32), ("fuse_window", ctypes.c_uint32),
-                ("timing", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 7)]
+                ("timing", ctypes.c_uint32), ("mem_reserve_ppm", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 6)]
 
 
 class Stats(ctypes.Structure):
@@ -121,10 +124,12 @@ def lib():
         L.sqe_batch_export.argtypes = [vp, ctypes.c_int64, _u64p, _f64p, _f64p, ctypes.c_uint64, _u64p]
         L.sqe_batch_remove.argtypes = [vp, ctypes.c_int64]
         L.sqe_batch_usage.argtypes = [vp, _u64p, _u64p, _u64p]
+        L.sqe_batch_finished.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.c_uint64, _u64p]
         L.sqe_batch_get_tuning.argtypes = [vp, ctypes.POINTER(BatchTuning)]
         L.sqe_batch_set_tuning.argtypes = [vp, ctypes.POINTER(BatchTuning)]
         L.sqe_batch_autotune.argtypes = [vp, ctypes.POINTER(BatchTuning)]
-        for f in ("sqe_batch_destroy", "sqe_batch_run", "sqe_batch_sync"):
+        for f in ("sqe_batch_destroy", "sqe_batch_run", "sqe_batch_sync", "sqe_batch_snapshot_save",
+                  "sqe_batch_snapshot_restore"):
             getattr(L, f).argtypes = [vp]
         for f in EXPORTS:
             if f.startswith("sqe_batch_") and f != "sqe_batch_last_error":
@@ -358,6 +363,19 @@ class Batch:
     def remove(self, cid):
         _bcheck(lib().sqe_batch_remove(self.h, int(cid)))
 
+    def finished(self):
+        """Ids of admitted circuits that have all their gates enqueued (no wait)."""
+        n = ctypes.c_uint64()
+        cap = 256
+        while True:
+            ids = (ctypes.c_int64 * cap)()
+            rc = lib().sqe_batch_finished(self.h, ids, cap, ctypes.byref(n))
+            if rc == SQE_OK:
+                return list(ids[:n.value])
+            if n.value <= cap:
+                _bcheck(rc)
+            cap = int(n.value)
+
     def usage(self):
         u, b, a = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         _bcheck(lib().sqe_batch_usage(self.h, ctypes.byref(u), ctypes.byref(b), ctypes.byref(a)))
@@ -378,6 +396,12 @@ class Batch:
         if load_factor is not None:
             t.load_factor = load_factor
         _bcheck(lib().sqe_batch_set_tuning(self.h, ctypes.byref(t)))
+
+    def snapshot_save(self):
+        _bcheck(lib().sqe_batch_snapshot_save(self.h))
+
+    def snapshot_restore(self):
+        _bcheck(lib().sqe_batch_snapshot_restore(self.h))
 
     def autotune(self):
         t = BatchTuning()

@@ -1,0 +1,127 @@
+"""Serving loop over the batch engine: open arrivals, HBM-budget admission,
+continuous batching at gate-step granularity (no reference counterpart; the
+reference runs one circuit per process, emulator/src/main.cu:87-148).
+
+Requests arrive at given times (seconds from the start).  The loop
+  1. admits every request that has arrived, in arrival order, while its
+     pool bytes fit the engine's HBM budget (a request that does not fit waits
+     at the head of the queue until finished circuits free room);
+  2. advances every admitted, unfinished circuit by `step_gates` gates with
+     one batched launch per bucket size (circuits that arrived while others
+     were running join at the next step, so their work overlaps on the GPU);
+  3. exports each circuit that finished, records arrival -> export latency and
+     releases its pool space.
+It sleeps only when nothing is admitted and the next request is in the future.
+
+Each circuit advances by the same gate step from its own first gate whatever
+else is in the batch, so its result does not depend on the mix: the solo and
+mixed runs of a circuit agree to the bit (checked by tests/test_gpu_serve.py).
+"""
+import time
+
+import numpy as np
+
+from . import engine as E
+
+
+def percentile(values, q):
+    if not values:
+        return None
+    return float(np.percentile(np.asarray(values, dtype=np.float64), q))
+
+
+class ServeLoop:
+    def __init__(self, prec="f64", hbm_budget_bytes=200 * 10**9, step_gates=64, renorm_every=None,
+                 timing=False, device=0, batch=None):
+        """batch: an existing engine.Batch (or an object with its methods) to
+        serve on instead of a new one."""
+        if renorm_every is None:
+            renorm_every = 100 if prec in ("f32", E.F32) else 0
+        if batch is None:
+            batch = E.Batch(prec=prec, renorm_every=renorm_every, hbm_budget_bytes=hbm_budget_bytes,
+                            timing=timing, device=device)
+        self.batch = batch
+        self.step_gates = int(step_gates)
+        self.prec = getattr(batch, "prec", prec)
+
+    def close(self):
+        self.batch.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def run(self, requests, arrivals, keep_results=False, clock=time.perf_counter, sleep=time.sleep):
+        """Serve `requests` (circuit dicts: nq, keys, re, im, gates) arriving at
+        `arrivals` (seconds, nondecreasing).  Returns a report dict; with
+        keep_results the exported states are in report["results"][i]."""
+        n = len(requests)
+        arrivals = np.asarray(arrivals, dtype=np.float64)
+        assert arrivals.size == n and np.all(np.diff(arrivals) >= 0)
+        b = self.batch
+        nxt = 0                                 # next request not yet admitted
+        live = {}                               # circuit id -> request index
+        latency = [None] * n
+        admit_t = [None] * n
+        results = [None] * n if keep_results else None
+        updates = 0
+        steps = 0
+        peak_live = 0
+        peak_bytes = 0
+        t0 = clock()
+        while nxt < n or live:
+            now = clock() - t0
+            # 1. admission (FIFO, by HBM budget)
+            while nxt < n and arrivals[nxt] <= now:
+                r = requests[nxt]
+                cid = b.try_add(r["nq"], r["keys"], r["re"], r["im"], r["gates"])
+                if cid is None:
+                    if not live:
+                        raise E.SqeError(E.SQE_ETABLE_FULL,
+                                         "request %d does not fit the HBM budget on an empty device" % nxt)
+                    break
+                live[cid] = nxt
+                admit_t[nxt] = clock() - t0
+                nxt += 1
+            peak_live = max(peak_live, len(live))
+            if not live:
+                wait = arrivals[nxt] - (clock() - t0)
+                if wait > 0:
+                    sleep(min(wait, 0.05))
+                continue
+            u = b.usage()
+            peak_bytes = max(peak_bytes, u["used"])
+            # 2. one step of every admitted circuit
+            b.step(self.step_gates)
+            steps += 1
+            # 3. completions
+            for cid in b.finished():
+                i = live.pop(cid)
+                out = b.export(cid, sort=keep_results)
+                info = b.info(cid)
+                updates += info["updates"]
+                latency[i] = clock() - t0 - arrivals[i]
+                if keep_results:
+                    results[i] = dict(state=out, updates=info["updates"], nnz=info["nnz"])
+                b.remove(cid)
+        wall = clock() - t0
+        lat = [x for x in latency if x is not None]
+        rep = dict(requests=n, wall_s=wall, steps=steps, step_gates=self.step_gates, prec=self.prec,
+                   updates=updates, updates_per_s=updates / wall if wall > 0 else 0.0,
+                   p50_latency_s=percentile(lat, 50), p99_latency_s=percentile(lat, 99),
+                   mean_latency_s=float(np.mean(lat)) if lat else None,
+                   peak_live=peak_live, peak_pool_bytes=peak_bytes, latency_s=latency,
+                   queue_wait_s=[(a - arrivals[i]) if a is not None else None for i, a in enumerate(admit_t)])
+        if keep_results:
+            rep["results"] = results
+        return rep
+
+
+def run_solo(request, prec="f64", step_gates=64, renorm_every=None):
+    """One request alone through the same batch engine and gate step."""
+    with ServeLoop(prec=prec, step_gates=step_gates, renorm_every=renorm_every,
+                   hbm_budget_bytes=0) as s:
+        rep = s.run([request], [0.0], keep_results=True)
+    return rep["results"][0]

@@ -171,3 +171,22 @@ def test_batch_rejects_cross_bucket_circuit():
             b.add(nq, np.zeros(1, dtype=np.uint64), np.ones(1), np.zeros(1), gl)
         assert ex.value.status == E.SQE_EINVAL
         assert b.usage()["used"] == 0
+
+
+def test_batch_snapshot_restore_repeats(oracle):
+    cs = small_batch(12, 8)
+    with E.Batch() as b:
+        ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        b.snapshot_save()
+        b.run()
+        first = [b.export(i) for i in ids]
+        upd = [b.info(i)["updates"] for i in ids]
+        b.snapshot_restore()
+        assert b.usage()["active"] == len(ids)
+        b.run()
+        for k, i in enumerate(ids):
+            assert_same_state(b.export(i), first[k], atol=0.0, label="circuit %d" % i)
+            assert b.info(i)["updates"] == upd[k]
+        b.remove(ids[0])
+        with pytest.raises(E.SqeError):
+            b.snapshot_restore()

@@ -369,7 +369,9 @@ __device__ __forceinline__ bool coop_insert(Slot* __restrict__ tab, uint64_t mas
 }
 
 // ---------------------------------------------------------------------------
-// Single-pass device-wide exclusive scan (decoupled look-back).  Tiles are
+// Single-pass device-wide exclusive scan (decoupled look-back, Merrill and
+// Garland, "Single-pass Parallel Prefix Scan with Decoupled Look-back",
+// NVIDIA TR 2016-002), with the look-back done by a whole wavefront.  Tiles are
 // handed out in start order by a ticket counter that the last tile resets, so
 // a tile only ever waits on tiles that are already running.  Status words
 // carry flag (2 bits) | epoch (22 bits) | value (40 bits); a word from an older
@@ -413,36 +415,56 @@ __device__ __forceinline__ unsigned long long block_scan(uint32_t count, ScanCtl
         if (i < w) wave_off += s_wave[i];
         agg += s_wave[i];
     }
-    if (threadIdx.x == 0) {
+    if (w == 0) {
+        // Wavefront 0 looks back 64 predecessors at a time (lane k reads tile
+        // hi - k): it folds in every aggregate up to the nearest inclusive
+        // prefix, or all 64 when none is there yet, and waits only while a
+        // nearer predecessor has not published its aggregate.  Every
+        // predecessor holds an earlier ticket and is running, so the wait
+        // ends; the bound only turns a broken invariant into a reported error.
         unsigned long long prefix = 0;
         if (tile == 0) {
-            __hip_atomic_store(&sc.status[0], st_pack(ST_PRE, sc.epoch, agg), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&sc.status[0], st_pack(ST_PRE, sc.epoch, agg), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            __hip_atomic_store(&sc.status[tile], st_pack(ST_AGG, sc.epoch, agg), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            int j = (int)tile - 1;
-            // every predecessor holds an earlier ticket and is running, so the wait
-            // ends; the bound only turns a broken invariant into a reported error
+            if (lane == 0)
+                __hip_atomic_store(&sc.status[tile], st_pack(ST_AGG, sc.epoch, agg), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            int hi = (int)tile - 1;
             for (unsigned long long spins = 0;; ++spins) {
-                if (spins > (1ull << 24)) { atomicOr(sc.err, ERR_SCAN); break; }
-                const unsigned long long s = __hip_atomic_load(&sc.status[j], __ATOMIC_ACQUIRE,
-                                                               __HIP_MEMORY_SCOPE_AGENT);
-                const uint32_t ep = (uint32_t)((s >> 40) & 0x3fffff);
-                const unsigned long long fl = s & (3ull << 62);
-                if (ep != (sc.epoch & 0x3fffff) || fl == 0) continue;   // predecessor not ready yet
-                prefix += s & VAL_MASK;
-                if (fl == ST_PRE) break;
-                --j;
+                if (spins > (1ull << 24)) {
+                    if (lane == 0) atomicOr(sc.err, ERR_SCAN);
+                    break;
+                }
+                const int j = hi - lane;
+                unsigned long long st = st_pack(ST_PRE, sc.epoch, 0);     // before tile 0: empty prefix
+                if (j >= 0) st = __hip_atomic_load(&sc.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t ep = (uint32_t)((st >> 40) & 0x3fffff);
+                const unsigned long long fl = st & (3ull << 62);
+                const bool ready = ep == (sc.epoch & 0x3fffff) && fl != 0;
+                const unsigned long long okm = __ballot(ready);
+                const unsigned long long prem = __ballot(ready && fl == ST_PRE);
+                const int first = prem ? __ffsll((long long)prem) - 1 : 63;
+                const unsigned long long need = (2ull << first) - 1;       // lanes 0..first
+                if ((okm & need) != need) continue;                         // a nearer one is not ready
+                unsigned long long v = lane <= first ? (st & VAL_MASK) : 0ull;
+                for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+                prefix += v;
+                if (prem) break;
+                hi -= 64;
             }
-            __hip_atomic_store(&sc.status[tile], st_pack(ST_PRE, sc.epoch, prefix + agg), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&sc.status[tile], st_pack(ST_PRE, sc.epoch, prefix + agg), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_prefix = prefix;
-        s_total = prefix + agg;
-        if (tile == ntiles - 1) {
-            *gtotal = prefix + agg;
-            __hip_atomic_store(sc.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) {
+            s_prefix = prefix;
+            s_total = prefix + agg;
+            if (tile == ntiles - 1) {
+                *gtotal = prefix + agg;
+                __hip_atomic_store(sc.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
     }
     __syncthreads();

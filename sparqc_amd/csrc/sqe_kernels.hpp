@@ -516,6 +516,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
     block_stats(A.bstats, cnt.upd, cnt.nnz_out, cnt.empty, A.store ? cnt.nnz_in + cnt.nnz_out : 0ull);
 }
 
+// Count-only pass (no program, no store): nonzeros and empty buckets.
+template <int MB, class R>
+__global__ __launch_bounds__(BLOCK) void k_count(SweepArgs A) {
+    const uint64_t nb = *A.nb;
+    const int64_t ntiles = (int64_t)((nb + 63) >> 6);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t t = (int64_t)blockIdx.x * (BLOCK / 64) + wave;
+    const int64_t tile[1] = {t < ntiles ? t : -1};
+    SweepCount cnt{0, 0, 0, 0};
+    if (t < ntiles) sweep_group<MB, R, 1>(A.base, A.amp, tile, nullptr, 0, nullptr, A.thr, nullptr, 0, cnt);
+    block_stats(A.bstats, 0ull, cnt.nnz_out, cnt.empty, 0ull);
+}
+
 // ---------------------------------------------------------------------------
 // Batched sweep: many independent circuits, each with its own bucket region
 // and program, advanced by one launch.  One wavefront per work item (a circuit
@@ -853,7 +866,11 @@ struct Launch {
     static constexpr int EPT = 1;
     static hipError_t sweep(int mb, hipStream_t st, uint64_t ntiles, const SweepArgs& a) {
         const uint32_t grid = sweep_grid(mb, ntiles);
-        SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_sweep<MB_, R, EPT>), dim3(grid), dim3(BLOCK), 0, st, a));
+        if (a.nops == 0 && !a.store) {
+            SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_count<MB_, R>), dim3(grid), dim3(BLOCK), 0, st, a));
+        } else {
+            SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_sweep<MB_, R, EPT>), dim3(grid), dim3(BLOCK), 0, st, a));
+        }
         return hipGetLastError();
     }
     static uint32_t sweep_grid(int, uint64_t ntiles) {

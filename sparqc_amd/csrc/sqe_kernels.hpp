@@ -343,11 +343,13 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
     const uint32_t* __restrict__ words = (const uint32_t*)ops;
     uint32_t w[16];
     uint32_t c0 = 0;
+    uint64_t bmask = 0;               // bit k: op c0 + k is a base-class op
     auto stage = [&](uint32_t from) __attribute__((always_inline)) {
         c0 = from;
         const uint32_t mine = from + lane < nops ? from + lane : nops - 1;
 #pragma unroll
         for (int k = 0; k < 16; ++k) w[k] = words[(size_t)mine * 16 + k];
+        bmask = __ballot(from + lane < nops && base_class(w[0] & 15u, w[1] & 0xffu));
     };
     if (nops) stage(0);
     auto rd = [&](int k, uint32_t jj) __attribute__((always_inline)) {
@@ -363,12 +365,9 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
         // bucket whose target bit is 0, c1 for bit 1 (X: 1 and 1), then the
         // base key flips for X / Y / PERM; lanes whose base controls fail take
         // factor 1 and keep their key.
-        for (; i < nops; ++i) {
-            if (i - c0 >= 64) stage(i);
-            const uint32_t jj = i - c0;
-            const uint32_t hdr = rd(0, jj), ctl = rd(1, jj);
+        auto base_op = [&](uint32_t jj) __attribute__((always_inline)) {
+            const uint32_t hdr = rd(0, jj);
             const uint32_t h = hdr & 15u;
-            if (!base_class(h, ctl & 0xffu)) break;
             const uint64_t tbase = 1ull << ((hdr >> 7) & 63u);
             const uint64_t flip = (h == H_BASE_X || h == H_BASE_PERM) ? tbase : 0ull;
             const uint64_t cmB = rd64(2, jj), cvB = rd64(4, jj);
@@ -388,6 +387,20 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                 pi[e] = m;
                 base[e] ^= pred ? flip : 0ull;
             });
+        };
+        while (i < nops) {
+            if (i - c0 >= 64) stage(i);
+            const uint32_t jj = i - c0;
+            const uint64_t rest = bmask >> jj;          // base-class ops from i on, in this chunk
+            const uint32_t n = rest == ~0ull ? 64u : (uint32_t)__builtin_ctzll(~rest);
+            if (n == 0) break;
+            uint32_t k = 0;
+            for (; k + 2 <= n; k += 2) {               // two ops per trip: independent work to overlap
+                base_op(jj + k);
+                base_op(jj + k + 1);
+            }
+            if (k < n) base_op(jj + k);
+            i += n;
             pending = true;
         }
         // ---- run of slot ops: amplitude registers ----

@@ -90,142 +90,100 @@ struct PairIdx {
 };
 
 // Slot handlers with compile-time target bit J, applied to the EPT buckets a
-// thread owns (a[e], predicate pred[e]).
-template <int MB, class R, int EPT, int J>
+// thread owns (a[e], predicate pred[e]).  MASK = false is the common case of a
+// gate without controls: every pair of every bucket is updated, straight-line.
+// MASK = true covers bucket controls (a scalar condition per slot pair) and
+// base controls (the lane predicate): the pair's new values are computed and
+// selected, still without branches, so no pair leaves a merge of old and new
+// register copies behind.
+template <int MB, class R, int EPT, int J, bool MASK>
 __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
                                              const bool (&pred)[EPT], double thr, const double* __restrict__ coef) {
     using C = typename CT<R>::T;
     constexpr int HALF = (1 << MB) / 2;
     constexpr bool F32 = sizeof(R) == 4;
     const bool pall = op.cmB == 0;               // every bucket takes the gate
-    switch (op.h) {
-    case H_SLOT_DIAG: {
+    // apply f(x, y, n0, n1) to every pair (s0 = slot with bit J clear, s1 = set)
+    auto pairs = [&](auto&& f) __attribute__((always_inline)) {
         static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
             constexpr int e = decltype(ec)::value;
-            // factor of slot bit J; a lane whose base controls fail takes 1
-            R f0r = (R)op.c[0], f0i = (R)op.c[1], f1r = (R)op.c[2], f1i = (R)op.c[3];
-            if (!pall) {
-                f0r = pred[e] ? f0r : (R)1; f0i = pred[e] ? f0i : (R)0;
-                f1r = pred[e] ? f1r : (R)1; f1i = pred[e] ? f1i : (R)0;
-            }
             static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
                 constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
                 constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
-                if (((uint32_t)s0 & op.cmS) == op.cvS) {
-                    a[e][s0] = rot(f0r, f0i, a[e][s0]);
-                    a[e][s1] = rot(f1r, f1i, a[e][s1]);
+                const C x = a[e][s0], y = a[e][s1];
+                C n0, n1;
+                f(x, y, n0, n1);
+                if constexpr (MASK) {
+                    const bool m = (((uint32_t)s0 & op.cmS) == op.cvS) && (pall || pred[e]);
+                    a[e][s0] = csel(m, n0, x);
+                    a[e][s1] = csel(m, n1, y);
+                } else {
+                    a[e][s0] = n0;
+                    a[e][s1] = n1;
                 }
             });
         });
+    };
+    auto prune_pair = [&](C x, C y, C& n0, C& n1, auto&& exact) __attribute__((always_inline)) {
+        if constexpr (F32) {
+            if (near32(n0, thr) || near32(n1, thr)) {
+                cd e0, e1;
+                exact(mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
+                n0 = prune32(n0, thr, !negl64(e0, thr));
+                n1 = prune32(n1, thr, !negl64(e1, thr));
+            } else {
+                n0 = prune32(n0, thr, true);
+                n1 = prune32(n1, thr, true);
+            }
+        } else {
+            n0 = prune(n0, thr);
+            n1 = prune(n1, thr);
+        }
+    };
+    switch (op.h) {
+    case H_SLOT_DIAG: {
+        const R f0r = (R)op.c[0], f0i = (R)op.c[1], f1r = (R)op.c[2], f1i = (R)op.c[3];
+        pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+            n0 = rot(f0r, f0i, x);
+            n1 = rot(f1r, f1i, y);
+        });
     } break;
     case H_SLOT_X:
-        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
-            constexpr int e = decltype(ec)::value;
-            static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
-                constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
-                constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
-                if (((uint32_t)s0 & op.cmS) == op.cvS) {
-                    const C x = a[e][s0], y = a[e][s1];
-                    if (pall) { a[e][s0] = y; a[e][s1] = x; }
-                    else { a[e][s0] = csel(pred[e], y, x); a[e][s1] = csel(pred[e], x, y); }
-                }
-            });
+        pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+            n0 = y;
+            n1 = x;
         });
         break;
     case H_SLOT_PERM: {
         const R u10r = (R)op.c[0], u10i = (R)op.c[1], u01r = (R)op.c[2], u01i = (R)op.c[3];
-        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
-            constexpr int e = decltype(ec)::value;
-            static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
-                constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
-                constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
-                if (((uint32_t)s0 & op.cmS) == op.cvS) {
-                    const C x = a[e][s0], y = a[e][s1];
-                    const C n0 = rot(u01r, u01i, y), n1 = rot(u10r, u10i, x);
-                    a[e][s0] = csel(pred[e] || pall, n0, x);
-                    a[e][s1] = csel(pred[e] || pall, n1, y);
-                }
-            });
+        pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+            n0 = rot(u01r, u01i, y);
+            n1 = rot(u10r, u10i, x);
         });
     } break;
     case H_SLOT_H: {
         const R f = (R)op.c[0];
-        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
-            constexpr int e = decltype(ec)::value;
-            static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
-                constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
-                constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
-                if (((uint32_t)s0 & op.cmS) == op.cvS) {
-                    const C x = a[e][s0], y = a[e][s1];
-                    C n0, n1;
-                    hpair(f, x, y, n0, n1);
-                    if constexpr (F32) {
-                        if (near32(n0, thr) || near32(n1, thr)) {
-                            cd e0, e1;
-                            hpair(op.c[0], mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
-                            n0 = prune32(n0, thr, !negl64(e0, thr));
-                            n1 = prune32(n1, thr, !negl64(e1, thr));
-                        } else {
-                            n0 = prune32(n0, thr, true);
-                            n1 = prune32(n1, thr, true);
-                        }
-                    } else {
-                        n0 = prune(n0, thr);
-                        n1 = prune(n1, thr);
-                    }
-                    if (pall) { a[e][s0] = n0; a[e][s1] = n1; }
-                    else { a[e][s0] = csel(pred[e], n0, x); a[e][s1] = csel(pred[e], n1, y); }
-                }
+        pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+            hpair(f, x, y, n0, n1);
+            prune_pair(x, y, n0, n1, [&](cd ex, cd ey, cd& e0, cd& e1) __attribute__((always_inline)) {
+                hpair(op.c[0], ex, ey, e0, e1);
             });
         });
     } break;
     case H_SLOT_U2: {
         const R ar = (R)op.c[0], u01r = (R)op.c[1], u01i = (R)op.c[2], u10r = (R)op.c[3], u10i = (R)op.c[4];
-        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
-            constexpr int e = decltype(ec)::value;
-            static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
-                constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
-                constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
-                if (((uint32_t)s0 & op.cmS) == op.cvS) {
-                    const C x = a[e][s0], y = a[e][s1];
-                    C n0, n1;
-                    u2(ar, u01r, u01i, u10r, u10i, x, y, n0, n1);
-                    if constexpr (F32) {
-                        if (near32(n0, thr) || near32(n1, thr)) {
-                            cd e0, e1;
-                            u2(op.c[0], op.c[1], op.c[2], op.c[3], op.c[4], mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
-                            n0 = prune32(n0, thr, !negl64(e0, thr));
-                            n1 = prune32(n1, thr, !negl64(e1, thr));
-                        } else {
-                            n0 = prune32(n0, thr, true);
-                            n1 = prune32(n1, thr, true);
-                        }
-                    } else {
-                        n0 = prune(n0, thr);
-                        n1 = prune(n1, thr);
-                    }
-                    if (pall) { a[e][s0] = n0; a[e][s1] = n1; }
-                    else { a[e][s0] = csel(pred[e], n0, x); a[e][s1] = csel(pred[e], n1, y); }
-                }
+        pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+            u2(ar, u01r, u01i, u10r, u10i, x, y, n0, n1);
+            prune_pair(x, y, n0, n1, [&](cd ex, cd ey, cd& e0, cd& e1) __attribute__((always_inline)) {
+                u2(op.c[0], op.c[1], op.c[2], op.c[3], op.c[4], ex, ey, e0, e1);
             });
         });
     } break;
     case H_SLOT_U: {
         DOp d{};
         for (int k = 0; k < 8; ++k) d.p[k] = coef[op.ci * 8 + k];
-        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
-            constexpr int e = decltype(ec)::value;
-            static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
-                constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
-                constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
-                if (((uint32_t)s0 & op.cmS) == op.cvS) {
-                    const C x = a[e][s0], y = a[e][s1];
-                    C n0, n1;
-                    branch_pair<KU>(d, thr, x, y, n0, n1);
-                    a[e][s0] = csel(pred[e] || pall, n0, x);
-                    a[e][s1] = csel(pred[e] || pall, n1, y);
-                }
-            });
+        pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+            branch_pair<KU>(d, thr, x, y, n0, n1);
         });
     } break;
     default:
@@ -233,17 +191,24 @@ __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&
     }
 }
 
+template <int MB, class R, int EPT, bool MASK>
+__device__ __forceinline__ void slot_dispatch_j(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
+                                                const bool (&pred)[EPT], double thr, const double* __restrict__ coef) {
+    switch (op.j) {
+    case 0: if constexpr (MB > 0) slot_handler<MB, R, EPT, 0, MASK>(op, a, pred, thr, coef); break;
+    case 1: if constexpr (MB > 1) slot_handler<MB, R, EPT, 1, MASK>(op, a, pred, thr, coef); break;
+    case 2: if constexpr (MB > 2) slot_handler<MB, R, EPT, 2, MASK>(op, a, pred, thr, coef); break;
+    case 3: if constexpr (MB > 3) slot_handler<MB, R, EPT, 3, MASK>(op, a, pred, thr, coef); break;
+    case 4: if constexpr (MB > 4) slot_handler<MB, R, EPT, 4, MASK>(op, a, pred, thr, coef); break;
+    default: break;
+    }
+}
+
 template <int MB, class R, int EPT>
 __device__ __forceinline__ void slot_dispatch(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
                                               const bool (&pred)[EPT], double thr, const double* __restrict__ coef) {
-    switch (op.j) {
-    case 0: if constexpr (MB > 0) slot_handler<MB, R, EPT, 0>(op, a, pred, thr, coef); break;
-    case 1: if constexpr (MB > 1) slot_handler<MB, R, EPT, 1>(op, a, pred, thr, coef); break;
-    case 2: if constexpr (MB > 2) slot_handler<MB, R, EPT, 2>(op, a, pred, thr, coef); break;
-    case 3: if constexpr (MB > 3) slot_handler<MB, R, EPT, 3>(op, a, pred, thr, coef); break;
-    case 4: if constexpr (MB > 4) slot_handler<MB, R, EPT, 4>(op, a, pred, thr, coef); break;
-    default: break;
-    }
+    if (op.cmS == 0 && op.cmB == 0) slot_dispatch_j<MB, R, EPT, false>(op, a, pred, thr, coef);
+    else slot_dispatch_j<MB, R, EPT, true>(op, a, pred, thr, coef);
 }
 
 // Per-lane factor of a base-qubit DIAG / PERM op (identity where the base
@@ -395,11 +360,13 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             const uint32_t n = rest == ~0ull ? 64u : (uint32_t)__builtin_ctzll(~rest);
             if (n == 0) break;
             uint32_t k = 0;
-            for (; k + 2 <= n; k += 2) {               // two ops per trip: independent work to overlap
+            for (; k + 4 <= n; k += 4) {               // four ops per trip: independent work to overlap
                 base_op(jj + k);
                 base_op(jj + k + 1);
+                base_op(jj + k + 2);
+                base_op(jj + k + 3);
             }
-            if (k < n) base_op(jj + k);
+            for (; k < n; ++k) base_op(jj + k);
             i += n;
             pending = true;
         }

@@ -301,27 +301,44 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
         pr[e] = 1.0;
         pi[e] = 0.0;
     });
-    // The program is staged 64 ops at a time in the lanes of the wavefront
-    // (lane j holds the 16 dwords of op c0+j, one coalesced load per chunk);
-    // each op is then pulled into scalar registers with v_readlane, so no
-    // per-gate memory round trip sits on the critical path.
-    const uint32_t* __restrict__ words = (const uint32_t*)ops;
-    uint32_t w[16];
+    // The program is staged 64 ops at a time through LDS (4 KB per wavefront,
+    // one coalesced 1 KB load per 16 dwords): a gate on a base qubit then reads
+    // its record with four broadcast LDS loads straight into registers that
+    // the VALU uses as operands, and a bucket-qubit gate moves its fields to
+    // scalar registers with v_readfirstlane.  The nonzero count that every gate
+    // adds to the update counter changes only at branching gates, so gates are
+    // counted in a scalar and multiplied in when the count changes.
+    __shared__ uint4 s_prog[BLOCK / 64][64 * 4];
+    uint4* __restrict__ sp = s_prog[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
+    const uint4* __restrict__ recs = (const uint4*)ops;
     uint32_t c0 = 0;
     uint64_t bmask = 0;               // bit k: op c0 + k is a base-class op
     auto stage = [&](uint32_t from) __attribute__((always_inline)) {
         c0 = from;
         const uint32_t mine = from + lane < nops ? from + lane : nops - 1;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) w[k] = words[(size_t)mine * 16 + k];
-        bmask = __ballot(from + lane < nops && base_class(w[0] & 15u, w[1] & 0xffu));
+        const uint4 q0 = recs[(size_t)mine * 4], q1 = recs[(size_t)mine * 4 + 1];
+        const uint4 q2 = recs[(size_t)mine * 4 + 2], q3 = recs[(size_t)mine * 4 + 3];
+        bmask = __ballot(from + lane < nops && base_class(q0.x & 15u, q0.y & 0xffu));
+        sp[lane * 4] = q0;
+        sp[lane * 4 + 1] = q1;
+        sp[lane * 4 + 2] = q2;
+        sp[lane * 4 + 3] = q3;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     };
     if (nops) stage(0);
-    auto rd = [&](int k, uint32_t jj) __attribute__((always_inline)) {
-        return (uint32_t)__builtin_amdgcn_readlane(w[k], jj);
+    uint32_t pend = 0;                // gates since cnt[] last changed
+    auto fold_upd = [&]() __attribute__((always_inline)) {
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+            constexpr int e = decltype(ec)::value;
+            out.upd += (unsigned long long)cnt[e] * pend;
+        });
+        pend = 0;
     };
-    auto rd64 = [&](int k, uint32_t jj) __attribute__((always_inline)) {
-        return (uint64_t)rd(k, jj) | (uint64_t)rd(k + 1, jj) << 32;
+    auto u64of = [](uint32_t lo, uint32_t hi) __attribute__((always_inline)) {
+        return (uint64_t)lo | (uint64_t)hi << 32;
+    };
+    auto f64of = [](uint32_t lo, uint32_t hi) __attribute__((always_inline)) {
+        return __builtin_bit_cast(double, (uint64_t)lo | (uint64_t)hi << 32);
     };
     uint32_t i = 0;
     while (i < nops) {
@@ -331,22 +348,28 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
         // base key flips for X / Y / PERM; lanes whose base controls fail take
         // factor 1 and keep their key.
         auto base_op = [&](uint32_t jj) __attribute__((always_inline)) {
-            const uint32_t hdr = rd(0, jj);
+            const uint4 q0 = sp[jj * 4], q1 = sp[jj * 4 + 1], q2 = sp[jj * 4 + 2], q3 = sp[jj * 4 + 3];
+            const uint32_t hdr = __builtin_amdgcn_readfirstlane(q0.x);
             const uint32_t h = hdr & 15u;
             const uint64_t tbase = 1ull << ((hdr >> 7) & 63u);
             const uint64_t flip = (h == H_BASE_X || h == H_BASE_PERM) ? tbase : 0ull;
-            const uint64_t cmB = rd64(2, jj), cvB = rd64(4, jj);
-            const double c0r = __builtin_bit_cast(double, rd64(6, jj));
-            const double c0i = __builtin_bit_cast(double, rd64(8, jj));
-            const double c1r = __builtin_bit_cast(double, rd64(10, jj));
-            const double c1i = __builtin_bit_cast(double, rd64(12, jj));
+            const bool ctrl = __builtin_amdgcn_readfirstlane(q0.z | q0.w) != 0;
+            const uint64_t cmB = u64of(q0.z, q0.w), cvB = u64of(q1.x, q1.y);
+            const double c0r = f64of(q1.z, q1.w), c0i = f64of(q2.x, q2.y);
+            const double c1r = f64of(q2.z, q2.w), c1i = f64of(q3.x, q3.y);
             static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                 constexpr int e = decltype(ec)::value;
-                out.upd += cnt[e];
-                const bool pred = valid[e] && ((base[e] & cmB) == cvB);
                 const bool bit = (base[e] & tbase) != 0;
-                const double fr = pred ? (bit ? c1r : c0r) : 1.0;
-                const double fi = pred ? (bit ? c1i : c0i) : 0.0;
+                double fr = bit ? c1r : c0r;
+                double fi = bit ? c1i : c0i;
+                bool pred = valid[e];
+                if (ctrl) {
+                    pred = pred && ((base[e] & cmB) == cvB);
+                    fr = pred ? fr : 1.0;
+                    fi = pred ? fi : 0.0;
+                }
+                // a padding bucket (valid false) holds zeros: its factor is
+                // irrelevant, only its key must not change
                 const double r = pr[e] * fr - pi[e] * fi, m = pr[e] * fi + pi[e] * fr;
                 pr[e] = r;
                 pi[e] = m;
@@ -368,17 +391,20 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             }
             for (; k < n; ++k) base_op(jj + k);
             i += n;
+            pend += n;
             pending = true;
         }
         // ---- run of slot ops: amplitude registers ----
         for (; i < nops; ++i) {
             if (i - c0 >= 64) stage(i);
             const uint32_t jj = i - c0;
+            if ((bmask >> jj) & 1ull) break;
             SOp op;
-            const uint32_t hdr = rd(0, jj), ctl = rd(1, jj);
+            const uint4 q0 = sp[jj * 4], q1 = sp[jj * 4 + 1], q2 = sp[jj * 4 + 2], q3 = sp[jj * 4 + 3];
+            auto sg = [](uint32_t v) __attribute__((always_inline)) { return __builtin_amdgcn_readfirstlane(v); };
+            const uint32_t hdr = sg(q0.x), ctl = sg(q0.y);
             op.h = hdr & 15u;
             op.cmS = ctl & 0xffu;
-            if (base_class(op.h, op.cmS)) break;
             if (op.h == H_RENORM) {
                 const double f = *scale;
                 static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
@@ -389,18 +415,21 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                 pending = true;
                 continue;
             }
+            ++pend;
             op.cvS = (ctl >> 8) & 0xffu;
             op.j = (hdr >> 4) & 7u;
             op.tbase = 1ull << ((hdr >> 7) & 63u);
             op.ci = hdr >> 13;
-            op.cmB = rd64(2, jj);
-            op.cvB = rd64(4, jj);
-#pragma unroll
-            for (int k = 0; k < 5; ++k) op.c[k] = __builtin_bit_cast(double, rd64(6 + 2 * k, jj));
+            op.cmB = u64of(sg(q0.z), sg(q0.w));
+            op.cvB = u64of(sg(q1.x), sg(q1.y));
+            op.c[0] = f64of(sg(q1.z), sg(q1.w));
+            op.c[1] = f64of(sg(q2.x), sg(q2.y));
+            op.c[2] = f64of(sg(q2.z), sg(q2.w));
+            op.c[3] = f64of(sg(q3.x), sg(q3.y));
+            op.c[4] = f64of(sg(q3.z), sg(q3.w));
             bool pred[EPT];
             static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                 constexpr int e = decltype(ec)::value;
-                out.upd += cnt[e];
                 pred[e] = valid[e] && ((base[e] & op.cmB) == op.cvB);
             });
             if (op.h <= H_BASE_PERM) {
@@ -431,6 +460,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                 }
                 slot_dispatch<MB, R, EPT>(op, a, pred, thr, coef);
                 if (branch) {
+                    fold_upd();
                     static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                         constexpr int e = decltype(ec)::value;
                         cnt[e] = valid[e] ? count_nz<MB, R>(a[e]) : 0u;
@@ -439,6 +469,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             }
         }
     }
+    fold_upd();
     out.nnz_out = 0;
     out.empty = 0;
     static_for<EPT>([&](auto ec) __attribute__((always_inline)) {

@@ -11,6 +11,7 @@
 // have no reference counterpart.
 #pragma once
 #include <algorithm>
+#include <type_traits>
 #include "sqe_device.hpp"
 #include "sqe_launch.hpp"
 
@@ -313,12 +314,14 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
     const uint4* __restrict__ recs = (const uint4*)ops;
     uint32_t c0 = 0;
     uint64_t bmask = 0;               // bit k: op c0 + k is a base-class op
+    uint64_t cmask = 0;               // bit k: op c0 + k has base controls
     auto stage = [&](uint32_t from) __attribute__((always_inline)) {
         c0 = from;
         const uint32_t mine = from + lane < nops ? from + lane : nops - 1;
         const uint4 q0 = recs[(size_t)mine * 4], q1 = recs[(size_t)mine * 4 + 1];
         const uint4 q2 = recs[(size_t)mine * 4 + 2], q3 = recs[(size_t)mine * 4 + 3];
         bmask = __ballot(from + lane < nops && base_class(q0.x & 15u, q0.y & 0xffu));
+        cmask = __ballot(from + lane < nops && (q0.z | q0.w) != 0);
         sp[lane * 4] = q0;
         sp[lane * 4 + 1] = q1;
         sp[lane * 4 + 2] = q2;
@@ -347,13 +350,16 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
         // bucket whose target bit is 0, c1 for bit 1 (X: 1 and 1), then the
         // base key flips for X / Y / PERM; lanes whose base controls fail take
         // factor 1 and keep their key.
-        auto base_op = [&](uint32_t jj) __attribute__((always_inline)) {
+        // CTRL: the run holds gates with base controls (one more compare per
+        // gate); both forms are straight-line, so a trip of four gates is one
+        // block and its LDS reads can all be in flight together.
+        auto base_op = [&](uint32_t jj, auto ctrl_c) __attribute__((always_inline)) {
+            constexpr bool CTRL = decltype(ctrl_c)::value;
             const uint4 q0 = sp[jj * 4], q1 = sp[jj * 4 + 1], q2 = sp[jj * 4 + 2], q3 = sp[jj * 4 + 3];
             const uint32_t hdr = __builtin_amdgcn_readfirstlane(q0.x);
             const uint32_t h = hdr & 15u;
             const uint64_t tbase = 1ull << ((hdr >> 7) & 63u);
             const uint64_t flip = (h == H_BASE_X || h == H_BASE_PERM) ? tbase : 0ull;
-            const bool ctrl = __builtin_amdgcn_readfirstlane(q0.z | q0.w) != 0;
             const uint64_t cmB = u64of(q0.z, q0.w), cvB = u64of(q1.x, q1.y);
             const double c0r = f64of(q1.z, q1.w), c0i = f64of(q2.x, q2.y);
             const double c1r = f64of(q2.z, q2.w), c1i = f64of(q3.x, q3.y);
@@ -363,7 +369,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                 double fr = bit ? c1r : c0r;
                 double fi = bit ? c1i : c0i;
                 bool pred = valid[e];
-                if (ctrl) {
+                if constexpr (CTRL) {
                     pred = pred && ((base[e] & cmB) == cvB);
                     fr = pred ? fr : 1.0;
                     fi = pred ? fi : 0.0;
@@ -376,20 +382,25 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                 base[e] ^= pred ? flip : 0ull;
             });
         };
+        auto base_run = [&](uint32_t jj, uint32_t n, auto ctrl_c) __attribute__((always_inline)) {
+            uint32_t k = 0;
+            for (; k + 4 <= n; k += 4) {               // four ops per trip: independent work to overlap
+                base_op(jj + k, ctrl_c);
+                base_op(jj + k + 1, ctrl_c);
+                base_op(jj + k + 2, ctrl_c);
+                base_op(jj + k + 3, ctrl_c);
+            }
+            for (; k < n; ++k) base_op(jj + k, ctrl_c);
+        };
         while (i < nops) {
             if (i - c0 >= 64) stage(i);
             const uint32_t jj = i - c0;
             const uint64_t rest = bmask >> jj;          // base-class ops from i on, in this chunk
             const uint32_t n = rest == ~0ull ? 64u : (uint32_t)__builtin_ctzll(~rest);
             if (n == 0) break;
-            uint32_t k = 0;
-            for (; k + 4 <= n; k += 4) {               // four ops per trip: independent work to overlap
-                base_op(jj + k);
-                base_op(jj + k + 1);
-                base_op(jj + k + 2);
-                base_op(jj + k + 3);
-            }
-            for (; k < n; ++k) base_op(jj + k);
+            const uint64_t run = n == 64 ? ~0ull : ((1ull << n) - 1);
+            if ((cmask >> jj) & run) base_run(jj, n, std::true_type{});
+            else base_run(jj, n, std::false_type{});
             i += n;
             pend += n;
             pending = true;

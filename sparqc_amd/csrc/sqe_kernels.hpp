@@ -90,6 +90,14 @@ struct PairIdx {
     static constexpr int s1 = s0 | (1 << J);
 };
 
+// One amplitude times the pending FP64 factor (the reference's phase formula,
+// emulator/device/device_data.cuh:221-224).
+__device__ __forceinline__ cd phase1(cd v, double pr, double pi) { return rot(pr, pi, v); }
+__device__ __forceinline__ cf phase1(cf v, double pr, double pi) {
+    const double x = v.x, y = v.y;
+    return mkf((float)(pr * x - pi * y), (float)(pr * y + pi * x));
+}
+
 // Slot handlers with compile-time target bit J, applied to the EPT buckets a
 // thread owns (a[e], predicate pred[e]).  MASK = false is the common case of a
 // gate without controls: every pair of every bucket is updated, straight-line.
@@ -99,19 +107,27 @@ struct PairIdx {
 // register copies behind.
 template <int MB, class R, int EPT, int J, bool MASK>
 __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
-                                             const bool (&pred)[EPT], double thr, const double* __restrict__ coef) {
+                                             const bool (&pred)[EPT], const double (&pr)[EPT],
+                                             const double (&pi)[EPT], double thr, const double* __restrict__ coef) {
     using C = typename CT<R>::T;
     constexpr int HALF = (1 << MB) / 2;
     constexpr bool F32 = sizeof(R) == 4;
     const bool pall = op.cmB == 0;               // every bucket takes the gate
-    // apply f(x, y, n0, n1) to every pair (s0 = slot with bit J clear, s1 = set)
-    auto pairs = [&](auto&& f) __attribute__((always_inline)) {
+    // apply f(x, y, n0, n1) to every pair (s0 = slot with bit J clear, s1 = set);
+    // with FLUSH (branching gates) the bucket's pending factor is multiplied
+    // into both inputs first, so every slot leaves with its true value
+    auto pairs = [&](auto&& f, auto flush_c) __attribute__((always_inline)) {
+        constexpr bool FLUSH = decltype(flush_c)::value;
         static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
             constexpr int e = decltype(ec)::value;
             static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
                 constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
                 constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
-                const C x = a[e][s0], y = a[e][s1];
+                C x = a[e][s0], y = a[e][s1];
+                if constexpr (FLUSH) {
+                    x = phase1(x, pr[e], pi[e]);
+                    y = phase1(y, pr[e], pi[e]);
+                }
                 C n0, n1;
                 f(x, y, n0, n1);
                 if constexpr (MASK) {
@@ -147,20 +163,20 @@ __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&
         pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
             n0 = rot(f0r, f0i, x);
             n1 = rot(f1r, f1i, y);
-        });
+        }, std::false_type{});
     } break;
     case H_SLOT_X:
         pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
             n0 = y;
             n1 = x;
-        });
+        }, std::false_type{});
         break;
     case H_SLOT_PERM: {
         const R u10r = (R)op.c[0], u10i = (R)op.c[1], u01r = (R)op.c[2], u01i = (R)op.c[3];
         pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
             n0 = rot(u01r, u01i, y);
             n1 = rot(u10r, u10i, x);
-        });
+        }, std::false_type{});
     } break;
     case H_SLOT_H: {
         const R f = (R)op.c[0];
@@ -169,7 +185,7 @@ __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&
             prune_pair(x, y, n0, n1, [&](cd ex, cd ey, cd& e0, cd& e1) __attribute__((always_inline)) {
                 hpair(op.c[0], ex, ey, e0, e1);
             });
-        });
+        }, std::true_type{});
     } break;
     case H_SLOT_U2: {
         const R ar = (R)op.c[0], u01r = (R)op.c[1], u01i = (R)op.c[2], u10r = (R)op.c[3], u10i = (R)op.c[4];
@@ -178,14 +194,14 @@ __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&
             prune_pair(x, y, n0, n1, [&](cd ex, cd ey, cd& e0, cd& e1) __attribute__((always_inline)) {
                 u2(op.c[0], op.c[1], op.c[2], op.c[3], op.c[4], ex, ey, e0, e1);
             });
-        });
+        }, std::true_type{});
     } break;
     case H_SLOT_U: {
         DOp d{};
         for (int k = 0; k < 8; ++k) d.p[k] = coef[op.ci * 8 + k];
         pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
             branch_pair<KU>(d, thr, x, y, n0, n1);
-        });
+        }, std::true_type{});
     } break;
     default:
         break;
@@ -194,22 +210,24 @@ __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&
 
 template <int MB, class R, int EPT, bool MASK>
 __device__ __forceinline__ void slot_dispatch_j(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
-                                                const bool (&pred)[EPT], double thr, const double* __restrict__ coef) {
+                                                const bool (&pred)[EPT], const double (&pr)[EPT],
+                                                const double (&pi)[EPT], double thr, const double* __restrict__ coef) {
     switch (op.j) {
-    case 0: if constexpr (MB > 0) slot_handler<MB, R, EPT, 0, MASK>(op, a, pred, thr, coef); break;
-    case 1: if constexpr (MB > 1) slot_handler<MB, R, EPT, 1, MASK>(op, a, pred, thr, coef); break;
-    case 2: if constexpr (MB > 2) slot_handler<MB, R, EPT, 2, MASK>(op, a, pred, thr, coef); break;
-    case 3: if constexpr (MB > 3) slot_handler<MB, R, EPT, 3, MASK>(op, a, pred, thr, coef); break;
-    case 4: if constexpr (MB > 4) slot_handler<MB, R, EPT, 4, MASK>(op, a, pred, thr, coef); break;
+    case 0: if constexpr (MB > 0) slot_handler<MB, R, EPT, 0, MASK>(op, a, pred, pr, pi, thr, coef); break;
+    case 1: if constexpr (MB > 1) slot_handler<MB, R, EPT, 1, MASK>(op, a, pred, pr, pi, thr, coef); break;
+    case 2: if constexpr (MB > 2) slot_handler<MB, R, EPT, 2, MASK>(op, a, pred, pr, pi, thr, coef); break;
+    case 3: if constexpr (MB > 3) slot_handler<MB, R, EPT, 3, MASK>(op, a, pred, pr, pi, thr, coef); break;
+    case 4: if constexpr (MB > 4) slot_handler<MB, R, EPT, 4, MASK>(op, a, pred, pr, pi, thr, coef); break;
     default: break;
     }
 }
 
 template <int MB, class R, int EPT>
 __device__ __forceinline__ void slot_dispatch(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
-                                              const bool (&pred)[EPT], double thr, const double* __restrict__ coef) {
-    if (op.cmS == 0 && op.cmB == 0) slot_dispatch_j<MB, R, EPT, false>(op, a, pred, thr, coef);
-    else slot_dispatch_j<MB, R, EPT, true>(op, a, pred, thr, coef);
+                                              const bool (&pred)[EPT], const double (&pr)[EPT],
+                                              const double (&pi)[EPT], double thr, const double* __restrict__ coef) {
+    if (op.cmS == 0 && op.cmB == 0) slot_dispatch_j<MB, R, EPT, false>(op, a, pred, pr, pi, thr, coef);
+    else slot_dispatch_j<MB, R, EPT, true>(op, a, pred, pr, pi, thr, coef);
 }
 
 // Per-lane factor of a base-qubit DIAG / PERM op (identity where the base
@@ -276,7 +294,6 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
     bool valid[EPT];
     uint32_t cnt[EPT];
     double pr[EPT], pi[EPT];          // pending per-bucket factor
-    bool pending = false;             // wavefront-uniform: a factor is pending
     out.upd = 0;
     out.nnz_in = 0;
     static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
@@ -403,7 +420,6 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             else base_run(jj, n, std::false_type{});
             i += n;
             pend += n;
-            pending = true;
         }
         // ---- run of slot ops: amplitude registers ----
         for (; i < nops; ++i) {
@@ -423,7 +439,6 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                     pr[e] *= f;
                     pi[e] *= f;
                 });
-                pending = true;
                 continue;
             }
             ++pend;
@@ -453,27 +468,22 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                     const R cr = (R)fr, ci = (R)fi;
                     static_for<S>([&](auto sc) __attribute__((always_inline)) {
                         constexpr int s = decltype(sc)::value;
-                        if (((uint32_t)s & op.cmS) == op.cvS) a[e][s] = rot(cr, ci, a[e][s]);
+                        const bool on = ((uint32_t)s & op.cmS) == op.cvS;
+                        a[e][s] = csel(on, rot(cr, ci, a[e][s]), a[e][s]);
                     });
                 });
                 continue;
             }
             if constexpr (MB > 0) {
                 const bool branch = op.h >= H_SLOT_H && op.h <= H_SLOT_U;
-                if (pending && branch) {
-                    static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
-                        constexpr int e = decltype(ec)::value;
-                        apply_phase<MB>(a[e], pr[e], pi[e]);
-                        pr[e] = 1.0;
-                        pi[e] = 0.0;
-                    });
-                    pending = false;
-                }
-                slot_dispatch<MB, R, EPT>(op, a, pred, thr, coef);
+                slot_dispatch<MB, R, EPT>(op, a, pred, pr, pi, thr, coef);
                 if (branch) {
+                    // the branching handlers multiplied the pending factor in
                     fold_upd();
                     static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                         constexpr int e = decltype(ec)::value;
+                        pr[e] = 1.0;
+                        pi[e] = 0.0;
                         cnt[e] = valid[e] ? count_nz<MB, R>(a[e]) : 0u;
                     });
                 }
@@ -485,7 +495,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
     out.empty = 0;
     static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
         constexpr int e = decltype(ec)::value;
-        if (pending) apply_phase<MB>(a[e], pr[e], pi[e]);
+        apply_phase<MB>(a[e], pr[e], pi[e]);
         if (tile[e] >= 0 && store) {
             const uint64_t b = (uint64_t)tile[e] * TILE + lane;
             static_for<S>([&](auto sc) __attribute__((always_inline)) {
@@ -501,13 +511,13 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
 
 // Minimum wavefronts per SIMD the sweep kernels are compiled for (caps the
 // VGPRs at 512 / value): the amplitude registers of a bucket take 4 * 2^MB
-// (FP64) or 2 * 2^MB (FP32) VGPRs.  FP64 with 16 slots needs ~240 VGPRs to
-// keep the slot handlers free of spills, hence 2 there.
+// (FP64) or 2 * 2^MB (FP32) VGPRs; with 16 slots FP64 fits 3 waves (about
+// 160 VGPRs) and FP32 runs 4 (a few spilled dwords, measured faster than 3).
 #ifndef SQE_OCC_F64_MB4
-#define SQE_OCC_F64_MB4 2
+#define SQE_OCC_F64_MB4 3
 #endif
 #ifndef SQE_OCC_F32_MB4
-#define SQE_OCC_F32_MB4 3
+#define SQE_OCC_F32_MB4 4
 #endif
 template <int MB, class R>
 struct SweepOcc {

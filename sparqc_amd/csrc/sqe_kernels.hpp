@@ -57,13 +57,16 @@ __device__ __forceinline__ void hpair(float f, cf v0, cf v1, cf& n0, cf& n1) {
 // emulator/include/data_t.h:50-60).  FP32: outputs with max(|re|,|im|) in
 // [thr/4, 4 thr) are decided on the FP64 recomputation `exact`.
 __device__ __forceinline__ cd prune(cd n, double thr) { return negl64(n, thr) ? mkd(0.0, 0.0) : n; }
+// The band edges are compared in FP32: an output within an ulp of an edge is
+// far from thr, where the FP32 and FP64 decisions agree, so the edge rounding
+// cannot change a decision.
 __device__ __forceinline__ bool near32(cf n, double thr) {
-    const double m = fmax(fabs((double)n.x), fabs((double)n.y));
-    return m < 4.0 * thr && m >= 0.25 * thr;
+    const float m = fmaxf(fabsf(n.x), fabsf(n.y));
+    return m < (float)(4.0 * thr) && m >= (float)(0.25 * thr);
 }
 __device__ __forceinline__ cf prune32(cf n, double thr, bool keep_exact) {
-    const double m = fmax(fabs((double)n.x), fabs((double)n.y));
-    const bool keep = m >= 4.0 * thr ? true : (m < 0.25 * thr ? false : keep_exact);
+    const float m = fmaxf(fabsf(n.x), fabsf(n.y));
+    const bool keep = m >= (float)(4.0 * thr) ? true : (m < (float)(0.25 * thr) ? false : keep_exact);
     return keep ? n : mkf(0.0f, 0.0f);
 }
 
@@ -90,14 +93,6 @@ struct PairIdx {
     static constexpr int s1 = s0 | (1 << J);
 };
 
-// One amplitude times the pending FP64 factor (the reference's phase formula,
-// emulator/device/device_data.cuh:221-224).
-__device__ __forceinline__ cd phase1(cd v, double pr, double pi) { return rot(pr, pi, v); }
-__device__ __forceinline__ cf phase1(cf v, double pr, double pi) {
-    const double x = v.x, y = v.y;
-    return mkf((float)(pr * x - pi * y), (float)(pr * y + pi * x));
-}
-
 // Slot handlers with compile-time target bit J, applied to the EPT buckets a
 // thread owns (a[e], predicate pred[e]).  MASK = false is the common case of a
 // gate without controls: every pair of every bucket is updated, straight-line.
@@ -115,7 +110,8 @@ __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&
     const bool pall = op.cmB == 0;               // every bucket takes the gate
     // apply f(x, y, n0, n1) to every pair (s0 = slot with bit J clear, s1 = set);
     // with FLUSH (branching gates) the bucket's pending factor is multiplied
-    // into both inputs first, so every slot leaves with its true value
+    // into both inputs first (in the amplitude precision), so every slot
+    // leaves with its true value
     auto pairs = [&](auto&& f, auto flush_c) __attribute__((always_inline)) {
         constexpr bool FLUSH = decltype(flush_c)::value;
         static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
@@ -125,8 +121,9 @@ __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&
                 constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
                 C x = a[e][s0], y = a[e][s1];
                 if constexpr (FLUSH) {
-                    x = phase1(x, pr[e], pi[e]);
-                    y = phase1(y, pr[e], pi[e]);
+                    const R fr = (R)pr[e], fi = (R)pi[e];
+                    x = rot(fr, fi, x);
+                    y = rot(fr, fi, y);
                 }
                 C n0, n1;
                 f(x, y, n0, n1);

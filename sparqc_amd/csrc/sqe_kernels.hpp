@@ -329,13 +329,25 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
     uint32_t c0 = 0;
     uint64_t bmask = 0;               // bit k: op c0 + k is a base-class op
     uint64_t cmask = 0;               // bit k: op c0 + k has base controls
+    uint64_t rmask = 0;               // bit k: op c0 + k is a renormalisation
     auto stage = [&](uint32_t from) __attribute__((always_inline)) {
         c0 = from;
         const uint32_t mine = from + lane < nops ? from + lane : nops - 1;
-        const uint4 q0 = recs[(size_t)mine * 4], q1 = recs[(size_t)mine * 4 + 1];
-        const uint4 q2 = recs[(size_t)mine * 4 + 2], q3 = recs[(size_t)mine * 4 + 3];
-        bmask = __ballot(from + lane < nops && base_class(q0.x & 15u, q0.y & 0xffu));
+        uint4 q0 = recs[(size_t)mine * 4], q1 = recs[(size_t)mine * 4 + 1];
+        uint4 q2 = recs[(size_t)mine * 4 + 2], q3 = recs[(size_t)mine * 4 + 3];
+        const bool ren = (q0.x & 15u) == H_RENORM;
+        if (ren) {
+            // a renormalisation is the diagonal factor (f, f) with the device
+            // computed f = 1/sqrt(norm2): give it that record in LDS
+            const uint64_t fb = __builtin_bit_cast(uint64_t, *scale);
+            const uint32_t lo = (uint32_t)fb, hi = (uint32_t)(fb >> 32);
+            q0.z = 0; q0.w = 0; q1.x = 0; q1.y = 0;
+            q1.z = lo; q1.w = hi; q2.x = 0; q2.y = 0;
+            q2.z = lo; q2.w = hi; q3.x = 0; q3.y = 0;
+        }
+        bmask = __ballot(from + lane < nops && (ren || base_class(q0.x & 15u, q0.y & 0xffu)));
         cmask = __ballot(from + lane < nops && (q0.z | q0.w) != 0);
+        rmask = __ballot(from + lane < nops && ren);
         sp[lane * 4] = q0;
         sp[lane * 4 + 1] = q1;
         sp[lane * 4 + 2] = q2;
@@ -406,83 +418,73 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             }
             for (; k < n; ++k) base_op(jj + k, ctrl_c);
         };
-        while (i < nops) {
-            if (i - c0 >= 64) stage(i);
-            const uint32_t jj = i - c0;
-            const uint64_t rest = bmask >> jj;          // base-class ops from i on, in this chunk
-            const uint32_t n = rest == ~0ull ? 64u : (uint32_t)__builtin_ctzll(~rest);
-            if (n == 0) break;
+        // each trip of the loop is either a run of base ops (within the
+        // staged chunk) or one slot op, so the amplitude registers pass
+        // through one merge per trip
+        if (i - c0 >= 64) stage(i);
+        const uint32_t jj = i - c0;
+        const uint64_t rest = bmask >> jj;              // base-class ops from i on, in this chunk
+        const uint32_t n = rest == ~0ull ? 64u : (uint32_t)__builtin_ctzll(~rest);
+        if (n != 0) {
             const uint64_t run = n == 64 ? ~0ull : ((1ull << n) - 1);
             if ((cmask >> jj) & run) base_run(jj, n, std::true_type{});
             else base_run(jj, n, std::false_type{});
             i += n;
-            pend += n;
-        }
-        // ---- run of slot ops: amplitude registers ----
-        for (; i < nops; ++i) {
-            if (i - c0 >= 64) stage(i);
-            const uint32_t jj = i - c0;
-            if ((bmask >> jj) & 1ull) break;
+            pend += n - (uint32_t)__builtin_popcountll((rmask >> jj) & run);   // renormalisations are not gates
+        } else {
+            // ---- one slot op: amplitude registers ----
+            ++i;
             SOp op;
             const uint4 q0 = sp[jj * 4], q1 = sp[jj * 4 + 1], q2 = sp[jj * 4 + 2], q3 = sp[jj * 4 + 3];
             auto sg = [](uint32_t v) __attribute__((always_inline)) { return __builtin_amdgcn_readfirstlane(v); };
             const uint32_t hdr = sg(q0.x), ctl = sg(q0.y);
             op.h = hdr & 15u;
             op.cmS = ctl & 0xffu;
-            if (op.h == H_RENORM) {
-                const double f = *scale;
+            {
+                ++pend;
+                op.cvS = (ctl >> 8) & 0xffu;
+                op.j = (hdr >> 4) & 7u;
+                op.tbase = 1ull << ((hdr >> 7) & 63u);
+                op.ci = hdr >> 13;
+                op.cmB = u64of(sg(q0.z), sg(q0.w));
+                op.cvB = u64of(sg(q1.x), sg(q1.y));
+                op.c[0] = f64of(sg(q1.z), sg(q1.w));
+                op.c[1] = f64of(sg(q2.x), sg(q2.y));
+                op.c[2] = f64of(sg(q2.z), sg(q2.w));
+                op.c[3] = f64of(sg(q3.x), sg(q3.y));
+                op.c[4] = f64of(sg(q3.z), sg(q3.w));
+                bool pred[EPT];
                 static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                     constexpr int e = decltype(ec)::value;
-                    pr[e] *= f;
-                    pi[e] *= f;
+                    pred[e] = valid[e] && ((base[e] & op.cmB) == op.cvB);
                 });
-                continue;
-            }
-            ++pend;
-            op.cvS = (ctl >> 8) & 0xffu;
-            op.j = (hdr >> 4) & 7u;
-            op.tbase = 1ull << ((hdr >> 7) & 63u);
-            op.ci = hdr >> 13;
-            op.cmB = u64of(sg(q0.z), sg(q0.w));
-            op.cvB = u64of(sg(q1.x), sg(q1.y));
-            op.c[0] = f64of(sg(q1.z), sg(q1.w));
-            op.c[1] = f64of(sg(q2.x), sg(q2.y));
-            op.c[2] = f64of(sg(q2.z), sg(q2.w));
-            op.c[3] = f64of(sg(q3.x), sg(q3.y));
-            op.c[4] = f64of(sg(q3.z), sg(q3.w));
-            bool pred[EPT];
-            static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
-                constexpr int e = decltype(ec)::value;
-                pred[e] = valid[e] && ((base[e] & op.cmB) == op.cvB);
-            });
-            if (op.h <= H_BASE_PERM) {
-                // diagonal / Y / PERM on a base qubit, restricted by bucket controls
-                static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
-                    constexpr int e = decltype(ec)::value;
-                    const bool bit = (base[e] & op.tbase) != 0;
-                    double fr, fi;
-                    base_factor(op, bit, pred[e], fr, fi);
-                    const R cr = (R)fr, ci = (R)fi;
-                    static_for<S>([&](auto sc) __attribute__((always_inline)) {
-                        constexpr int s = decltype(sc)::value;
-                        const bool on = ((uint32_t)s & op.cmS) == op.cvS;
-                        a[e][s] = csel(on, rot(cr, ci, a[e][s]), a[e][s]);
-                    });
-                });
-                continue;
-            }
-            if constexpr (MB > 0) {
-                const bool branch = op.h >= H_SLOT_H && op.h <= H_SLOT_U;
-                slot_dispatch<MB, R, EPT>(op, a, pred, pr, pi, thr, coef);
-                if (branch) {
-                    // the branching handlers multiplied the pending factor in
-                    fold_upd();
+                if (op.h <= H_BASE_PERM) {
+                    // diagonal / Y / PERM on a base qubit, restricted by bucket controls
                     static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                         constexpr int e = decltype(ec)::value;
-                        pr[e] = 1.0;
-                        pi[e] = 0.0;
-                        cnt[e] = valid[e] ? count_nz<MB, R>(a[e]) : 0u;
+                        const bool bit = (base[e] & op.tbase) != 0;
+                        double fr, fi;
+                        base_factor(op, bit, pred[e], fr, fi);
+                        const R cr = (R)fr, ci = (R)fi;
+                        static_for<S>([&](auto sc) __attribute__((always_inline)) {
+                            constexpr int s = decltype(sc)::value;
+                            const bool on = ((uint32_t)s & op.cmS) == op.cvS;
+                            a[e][s] = csel(on, rot(cr, ci, a[e][s]), a[e][s]);
+                        });
                     });
+                } else if constexpr (MB > 0) {
+                    const bool branch = op.h >= H_SLOT_H && op.h <= H_SLOT_U;
+                    slot_dispatch<MB, R, EPT>(op, a, pred, pr, pi, thr, coef);
+                    if (branch) {
+                        // the branching handlers multiplied the pending factor in
+                        fold_upd();
+                        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                            constexpr int e = decltype(ec)::value;
+                            pr[e] = 1.0;
+                            pi[e] = 0.0;
+                            cnt[e] = valid[e] ? count_nz<MB, R>(a[e]) : 0u;
+                        });
+                    }
                 }
             }
         }

@@ -62,6 +62,9 @@ struct DOp {
 //   H_SLOT_U     any 2x2 on bucket qubit J, 8 coefficients in a side table
 //                (prunes)
 //   H_RENORM     scale by the device-computed 1/sqrt(norm2)
+//   H_SLOT_BLOCK dense 16x16 complex block of fused bucket-qubit gates (FP32,
+//                MFMA; side table: MFMA operand table, then the FP64 matrix;
+//                c[0] = number of gates) (prunes)
 // Coefficients c[]: DIAG f0r, f0i, f1r, f1i (Z: 1,0,-1,0; PHASE: 1,0,c,s;
 // RZ: c,-s,c,s); PERM u10r, u10i, u01r, u01i (the factor for the element
 // leaving bit 0, then for the element leaving bit 1); H f; U2 a, u01r, u01i,
@@ -71,7 +74,7 @@ struct DOp {
 // gates differ from it only by the folding of consecutive factors (a few ulps).
 enum : uint32_t {
     H_BASE_DIAG = 0, H_BASE_X, H_BASE_PERM, H_SLOT_DIAG, H_SLOT_X, H_SLOT_PERM, H_SLOT_H, H_SLOT_U2,
-    H_SLOT_U, H_RENORM
+    H_SLOT_U, H_RENORM, H_SLOT_BLOCK
 };
 
 // One sweep op: 16 dwords, staged 64 at a time in the lanes of a wavefront.

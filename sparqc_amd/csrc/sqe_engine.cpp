@@ -489,8 +489,10 @@ sqe_status grow(sqe_engine* e, uint64_t need) {
     return SQE_OK;
 }
 
-sqe_status run_sweep(sqe_engine* e, const MOp* mops, const double* coef, uint64_t nops, int store) {
+sqe_status run_sweep(sqe_engine* e, const MOp* mops, const double* coef, uint64_t nops, int store,
+                     bool blocks = false) {
     SweepArgs a{};
+    a.blocks = blocks ? 1 : 0;
     Store& o = e->s[e->cur];
     a.base = o.base; a.amp = o.amp; a.nb = &e->meta[e->cur]; a.ops = mops; a.coef = coef; a.nops = (uint32_t)nops;
     a.store = store; a.thr = e->thr; a.scale = &e->dscal[1]; a.stats = e->stats;
@@ -586,9 +588,15 @@ sqe_status flush(sqe_engine* e) {
     // plan into steps over one uploaded program
     std::vector<DOp> prog;
     std::vector<Step> steps;
+    std::vector<std::array<cx, 256>> blocks;
+    BlockBuilder bb;
+    bb.on = e->cfg.fuse_blocks && e->f32 && e->qm.mb == 4;
+    bb.qm = e->qm;
+    bb.blocks = &blocks;
     const uint64_t W = e->cfg.fuse_window ? e->cfg.fuse_window : ~0ull;
     uint64_t run_start = 0;
     auto close_run = [&]() {
+        bb.flush(prog);
         uint64_t n = prog.size() - run_start;
         uint64_t off = run_start;
         while (n) {
@@ -607,8 +615,17 @@ sqe_status flush(sqe_engine* e) {
     };
     for (const auto& g : gl) {
         if (gate_local(g, e->qm)) {
-            if (prog.size() == run_start) open_run();
-            prog.push_back(plan_op(g, e->qm));
+            if (prog.size() == run_start && bb.count == 0) open_run();
+            if (!bb.on) {
+                prog.push_back(plan_op(g, e->qm));
+            } else if (BlockBuilder::plain_slot(g, e->qm)) {
+                bb.add(g);
+            } else if (BlockBuilder::commutes(g, e->qm)) {
+                prog.push_back(plan_op(g, e->qm));          // ahead of the open window
+            } else {
+                bb.flush(prog);
+                prog.push_back(plan_op(g, e->qm));
+            }
         } else {
             if (e->renorm_pending) open_run();
             close_run();
@@ -627,7 +644,8 @@ sqe_status flush(sqe_engine* e) {
     std::vector<MOp> mops;
     std::vector<double> coef;
     mops.reserve(prog.size());
-    for (const auto& d : prog) mops.push_back(encode_op(d, coef));
+    for (const auto& d : prog)
+        mops.push_back(d.kind == K_BLOCK ? encode_block(d, blocks[(size_t)d.p[0]], coef) : encode_op(d, coef));
     const size_t mbytes = mops.size() * sizeof(MOp);
     const size_t cbytes = coef.size() * sizeof(double);
     const size_t coff = (mbytes + 255) & ~(size_t)255;
@@ -668,11 +686,14 @@ sqe_status flush(sqe_engine* e) {
     for (const auto& s : steps) {
         switch (s.kind) {
         case ST_SWEEP: {
-            SQCK(run_sweep(e, dm + s.off, dc, s.n, 1));
+            bool has_block = false;
+            for (uint64_t i = s.off; i < s.off + s.n; ++i) has_block |= prog[i].kind == K_BLOCK;
+            SQCK(run_sweep(e, dm + s.off, dc, s.n, 1, has_block));
             ++e->n_sweeps;
             for (uint64_t i = s.off; i < s.off + s.n; ++i) {
                 const DOp& d = prog[i];
-                if (d.kind != K_RENORM) ++e->n_sweep_ops;
+                if (d.kind == K_BLOCK) e->n_sweep_ops += (uint64_t)d.p[1];
+                else if (d.kind != K_RENORM) ++e->n_sweep_ops;
                 if (d.tbase && kind_is_perm(d.kind)) e->tab_valid = false;
             }
         } break;

@@ -256,6 +256,69 @@ __device__ __forceinline__ void apply_phase(Cplx<double> (&a)[1 << MB], double p
     });
 }
 
+// Fused dense block on a 16-slot FP32 bucket (one bucket per lane, 64 per
+// wavefront): Y = M X with M the 16x16 complex matrix of the block, computed
+// as the real product R X' (R = [[Mr, -Mi], [Mi, Mr]] 32x32, X' = 16 real
+// parts over 16 imaginary parts, 32 x 64 buckets) by two v_mfma_f32_32x32x2f32
+// chains (buckets 0-31 and 32-63) over 16 steps of two rows.  The lane-per-bucket
+// registers become B operands with one v_permlane32_swap per row pair, and
+// the outputs go back the same way.  Outputs are pruned componentwise; those
+// within 4x of the threshold are decided on an FP64 recomputation from the
+// same inputs.  Needs all 64 lanes active (wavefront-uniform control flow).
+typedef float v16f __attribute__((ext_vector_type(16)));
+__device__ __forceinline__ void block_apply(Cplx<float> (&a)[16], double pr, double pi, const double* __restrict__ side,
+                                            double thr) {
+    const uint32_t lane = threadIdx.x & 63;
+    const float* __restrict__ tab = (const float*)side;          // 16 steps x 64 lanes
+    const double* __restrict__ m64 = side + 512;                  // 16x16 complex, FP64
+    float x[32];
+    {
+        const float fr = (float)pr, fi = (float)pi;
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const Cplx<float> v = rot(fr, fi, a[s]);
+            x[s] = v.x;
+            x[16 + s] = v.y;
+        }
+    }
+    float av[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) av[t] = tab[t * 64 + lane];
+    v16f acc0 = {}, acc1 = {};
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[2 * t]), __float_as_uint(x[2 * t + 1]),
+                                                         false, false);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], __uint_as_float(sw[0]), acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], __uint_as_float(sw[1]), acc1, 0, 0, 0);
+    }
+    float y[32];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc0[r]), __float_as_uint(acc1[r]), false,
+                                                         false);
+        const int i0 = 8 * (r / 4) + (r % 4);
+        y[i0] = __uint_as_float(sw[0]);
+        y[i0 + 4] = __uint_as_float(sw[1]);
+    }
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        Cplx<float> n = mkf(y[s], y[16 + s]);
+        bool keep = true;
+        if (near32(n, thr)) {
+            double er = 0.0, ei = 0.0;
+            for (int k = 0; k < 16; ++k) {
+                const double mr = m64[(s * 16 + k) * 2], mi = m64[(s * 16 + k) * 2 + 1];
+                const double xr = x[k], xi = x[16 + k];
+                er += mr * xr - mi * xi;
+                ei += mr * xi + mi * xr;
+            }
+            keep = !negl64(mkd(er, ei), thr);
+        }
+        a[s] = prune32(n, thr, keep);
+    }
+}
+
 // Per-thread counters of one sweep group.
 struct SweepCount {
     unsigned long long upd, nnz_in, nnz_out, empty;
@@ -277,7 +340,7 @@ __device__ __forceinline__ bool base_class(uint32_t h, uint32_t cmS) { return h 
 // work on the amplitude registers.  The pending factor is a scalar per bucket,
 // so it commutes with every bucket-local gate; it is multiplied in before a
 // branching op (whose prune test needs true values) and before the store.
-template <int MB, class R, int EPT>
+template <int MB, class R, int EPT, bool BLOCKS = false>
 __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* ampv, const int64_t (&tile)[EPT],
                                             const MOp* __restrict__ ops, uint32_t nops,
                                             const double* __restrict__ coef, double thr,
@@ -473,8 +536,17 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                         });
                     });
                 } else if constexpr (MB > 0) {
-                    const bool branch = op.h >= H_SLOT_H && op.h <= H_SLOT_U;
-                    slot_dispatch<MB, R, EPT>(op, a, pred, pr, pi, thr, coef);
+                    const bool branch = op.h >= H_SLOT_H && op.h <= H_SLOT_BLOCK && op.h != H_RENORM;
+                    if constexpr (BLOCKS && MB == 4 && sizeof(R) == 4 && EPT == 1) {
+                        if (op.h == H_SLOT_BLOCK) {
+                            block_apply(a[0], pr[0], pi[0], coef + (size_t)op.ci * 8, thr);
+                            pend += (uint32_t)op.c[0] - 1;       // the block's gates (lower bound of their nonzeros)
+                        } else {
+                            slot_dispatch<MB, R, EPT>(op, a, pred, pr, pi, thr, coef);
+                        }
+                    } else {
+                        slot_dispatch<MB, R, EPT>(op, a, pred, pr, pi, thr, coef);
+                    }
                     if (branch) {
                         // the branching handlers multiplied the pending factor in
                         fold_upd();
@@ -526,7 +598,7 @@ struct SweepOcc {
 
 // One program over the engine's store: wavefront g of the grid takes tiles
 // EPT*g .. EPT*g + EPT-1.
-template <int MB, class R, int EPT>
+template <int MB, class R, int EPT, bool BLOCKS = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<MB, R>::value))) void k_sweep(SweepArgs A) {
     const uint64_t nb = *A.nb;
     const int64_t ntiles = (int64_t)((nb + 63) >> 6);
@@ -543,7 +615,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
         any |= t < ntiles;
     });
     SweepCount cnt{0, 0, 0, 0};
-    if (any) sweep_group<MB, R, EPT>(A.base, A.amp, tile, A.ops, A.nops, A.coef, A.thr, A.scale, A.store, cnt);
+    if (any) sweep_group<MB, R, EPT, BLOCKS>(A.base, A.amp, tile, A.ops, A.nops, A.coef, A.thr, A.scale, A.store, cnt);
     block_stats(A.bstats, cnt.upd, cnt.nnz_out, cnt.empty, A.store ? cnt.nnz_in + cnt.nnz_out : 0ull);
 }
 
@@ -899,6 +971,12 @@ struct Launch {
         const uint32_t grid = sweep_grid(mb, ntiles);
         if (a.nops == 0 && !a.store) {
             SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_count<MB_, R>), dim3(grid), dim3(BLOCK), 0, st, a));
+        } else if (mb == 4 && a.blocks) {
+            if constexpr (sizeof(R) == 4) {
+                hipLaunchKernelGGL((k_sweep<4, R, EPT, true>), dim3(grid), dim3(BLOCK), 0, st, a);
+            } else {
+                return hipErrorInvalidValue;
+            }
         } else {
             SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_sweep<MB_, R, EPT>), dim3(grid), dim3(BLOCK), 0, st, a));
         }

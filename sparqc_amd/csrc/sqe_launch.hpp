@@ -13,9 +13,10 @@ struct SweepArgs {
     void* amp;
     const unsigned long long* nb;
     const MOp* ops;
-    const double* coef;     // 8 coefficients per general-U gate (side table)
+    const double* coef;     // side table: 8 coefficients per general-U gate, fused blocks
     uint32_t nops;
     int32_t store;
+    int32_t blocks;         // the program holds fused blocks (FP32, 16 slots)
     double thr;
     const double* scale;
     Stats* stats;

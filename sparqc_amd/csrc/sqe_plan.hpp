@@ -3,6 +3,9 @@
 // and encoding of bucket-local gates into sweep-program records.
 #pragma once
 #include <algorithm>
+#include <array>
+#include <cstring>
+#include <complex>
 #include <cstdint>
 #include <vector>
 
@@ -122,6 +125,116 @@ inline MOp encode_op(const DOp& d, std::vector<double>& coef) {
     m.cmB = d.cmB;
     m.cvB = d.cvB;
     return m;
+}
+
+// ---------------------------------------------------------------------------
+// Fused blocks (FP32, 16-slot buckets): a window of gates on bucket qubits
+// without controls is multiplied into one dense 16x16 complex matrix that the
+// sweep applies with MFMA.  Gates on base qubits without bucket controls
+// commute with such gates (they only scale or relabel whole buckets) and are
+// emitted ahead of the window; any other gate closes it.  A window of one gate
+// stays a gate.  Prune decisions are made once, on the block's outputs.
+// ---------------------------------------------------------------------------
+using cx = std::complex<double>;
+
+// 2x2 matrix of a gate record, row-major (the conventions of sqe.h).
+inline void gate_matrix(const sqe_gate& g, cx m[4]) {
+    const double* p = g.p;
+    const cx i1(0.0, 1.0);
+    switch (g.kind) {
+    case SQE_GATE_X: m[0] = 0; m[1] = 1; m[2] = 1; m[3] = 0; break;
+    case SQE_GATE_Y: m[0] = 0; m[1] = -i1; m[2] = i1; m[3] = 0; break;
+    case SQE_GATE_Z: m[0] = 1; m[1] = 0; m[2] = 0; m[3] = -1; break;
+    case SQE_GATE_H: m[0] = p[0]; m[1] = p[0]; m[2] = p[0]; m[3] = -p[0]; break;
+    case SQE_GATE_PHASE: m[0] = 1; m[1] = 0; m[2] = 0; m[3] = cx(p[0], p[1]); break;
+    case SQE_GATE_RZ: m[0] = cx(p[0], -p[1]); m[1] = 0; m[2] = 0; m[3] = cx(p[0], p[1]); break;
+    case SQE_GATE_RX: m[0] = p[0]; m[1] = cx(0, -p[1]); m[2] = cx(0, -p[1]); m[3] = p[0]; break;
+    case SQE_GATE_RY: m[0] = p[0]; m[1] = -p[1]; m[2] = p[1]; m[3] = p[0]; break;
+    case SQE_GATE_U: for (int k = 0; k < 4; ++k) m[k] = cx(p[2 * k], p[2 * k + 1]); break;
+    case SQE_GATE_DIAG: m[0] = cx(p[0], p[1]); m[1] = 0; m[2] = 0; m[3] = cx(p[2], p[3]); break;
+    case SQE_GATE_PERM: m[0] = 0; m[1] = cx(p[0], p[1]); m[2] = cx(p[2], p[3]); m[3] = 0; break;
+    default: m[0] = 1; m[1] = 0; m[2] = 0; m[3] = 1; break;
+    }
+}
+
+constexpr uint32_t K_BLOCK = 17;                  // engine-internal op: fused dense block
+
+struct BlockBuilder {
+    bool on = false;                              // fusion enabled for this program
+    QMap qm{};
+    std::array<cx, 256> m{};                      // 16x16, row-major, out x in
+    uint32_t count = 0;
+    DOp first{};
+    std::vector<std::array<cx, 256>>* blocks = nullptr;
+
+    static bool plain_slot(const sqe_gate& g, const QMap& qm) {
+        return ((qm.mask >> g.target) & 1ull) && g.ctrl_mask == 0;
+    }
+    static bool commutes(const sqe_gate& g, const QMap& qm) {   // base-class gate
+        return !((qm.mask >> g.target) & 1ull) && (g.ctrl_mask & qm.mask) == 0;
+    }
+    void add(const sqe_gate& g) {
+        if (count == 0) {
+            for (int r = 0; r < 16; ++r)
+                for (int c = 0; c < 16; ++c) m[r * 16 + c] = r == c ? 1.0 : 0.0;
+            first = plan_op(g, qm);
+        }
+        int j = 0;
+        for (int k = 0; k < qm.mb; ++k)
+            if (qm.q[k] == g.target) j = k;
+        cx u[4];
+        gate_matrix(g, u);
+        std::array<cx, 256> out{};
+        for (int r = 0; r < 16; ++r)
+            for (int c = 0; c < 16; ++c) {
+                // (G M)[r][c] = sum over s of G[r][s] M[s][c]; G mixes r with r ^ 2^j
+                const int r0 = r & ~(1 << j), r1 = r | (1 << j), br = (r >> j) & 1;
+                out[r * 16 + c] = u[br * 2] * m[r0 * 16 + c] + u[br * 2 + 1] * m[r1 * 16 + c];
+            }
+        m = out;
+        ++count;
+    }
+    void flush(std::vector<DOp>& prog) {
+        if (count == 1) prog.push_back(first);
+        if (count >= 2) {
+            DOp d{};
+            d.kind = K_BLOCK;
+            d.tslot = -1;
+            d.p[0] = (double)blocks->size();
+            d.p[1] = (double)count;
+            blocks->push_back(m);
+            prog.push_back(d);
+        }
+        count = 0;
+    }
+};
+
+// Record of a block: the side table gets the 32x32 real form R = [[Mr, -Mi],
+// [Mi, Mr]] as the A operands of 16 v_mfma_f32_32x32x2f32 steps (step t, lane
+// l: R[l % 32][2t + l / 32], FP32), then M itself in FP64 (row-major re, im)
+// for the FP64 recomputation of near-threshold outputs.
+inline MOp encode_block(const DOp& d, const std::array<cx, 256>& m, std::vector<double>& coef) {
+    MOp r{};
+    const uint32_t ci = (uint32_t)(coef.size() / 8);
+    std::vector<float> tab(1024);
+    auto R = [&](int i, int k) -> double {
+        const cx v = m[(i & 15) * 16 + (k & 15)];
+        if (i < 16) return k < 16 ? v.real() : -v.imag();
+        return k < 16 ? v.imag() : v.real();
+    };
+    for (int t = 0; t < 16; ++t)
+        for (int l = 0; l < 64; ++l) tab[t * 64 + l] = (float)R(l % 32, 2 * t + l / 32);
+    const size_t at = coef.size();
+    coef.resize(at + 512);
+    std::memcpy(&coef[at], tab.data(), 1024 * sizeof(float));
+    for (int k = 0; k < 256; ++k) {
+        coef.push_back(m[k].real());
+        coef.push_back(m[k].imag());
+    }
+    r.hdr = H_SLOT_BLOCK | ci << 13;
+    r.ctl = 0;
+    r.c[0] = d.p[1];                               // gates in the block
+    return r;
 }
 
 inline DOp renorm_op() {

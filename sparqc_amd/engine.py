@@ -42,7 +42,7 @@ class Config(ctypes.Structure):
                 ("device", ctypes.c_int32), ("bucket_bits", ctypes.c_int32),
                 ("renorm_every", ctypes.c_uint32), ("fuse_window", ctypes.c_uint32),
                 ("timing", ctypes.c_uint32), ("mem_reserve_ppm", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 6)]
+                ("fuse_blocks", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 5)]
 
 
 class Stats(ctypes.Structure):
@@ -158,10 +158,12 @@ class Engine:
     """One sparse statevector on one GPU (owns its device memory and stream)."""
 
     def __init__(self, nq, prec="f64", prune_thr=0.0, bucket_bits=-1, renorm_every=0,
-                 fuse_window=0, timing=False, device=0, hbm_budget_bytes=0, max_load=0.5):
+                 fuse_window=0, timing=False, device=0, hbm_budget_bytes=0, max_load=0.5,
+                 fuse_blocks=False):
         L = lib()
         cfg = Config()
         L.sqe_config_default(ctypes.byref(cfg), nq)
+        cfg.fuse_blocks = 1 if fuse_blocks else 0
         cfg.prec = F32 if prec in ("f32", F32) else F64
         cfg.prune_thr = prune_thr
         cfg.bucket_bits = bucket_bits

@@ -1,0 +1,112 @@
+"""FP32 fused-block variant (windows of bucket-qubit gates applied as dense
+16x16 blocks with MFMA) against the FP32 oracle and the FP64 engine.
+
+A block prunes once, on its outputs, where gate-by-gate execution prunes after
+every branching gate; both prune at 1e-7 componentwise with FP64 decisions near
+the threshold, so key sets agree away from the threshold and amplitudes within
+1e-5 (FP32 accumulation over the block).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from sparqc_amd import engine as E
+from sparqc_amd import gates as G
+from sparqc_amd import gen
+from tests.helpers import random_state
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dense_block_circuit(seed, nq=16, layers=30):
+    """Layers of H / RY / RZ / PHASE / X on 4 qubits, each layer followed by
+    diagonal, X and controlled gates elsewhere (long fusable windows)."""
+    rng = gen.SplitMix64(seed)
+    bq = [1, 4, 7, 10]
+    others = [q for q in range(nq) if q not in bq]
+    gl = []
+    for _ in range(layers):
+        for q in bq:
+            k = [G.H, G.RY, G.RZ, G.PHASE, G.X, G.RX][rng.randint(6)]
+            gl.append(G.make_gate(k, q, theta=rng.uniform() * 6.28))
+            if rng.uniform() < 0.5:
+                t = others[rng.randint(len(others))]
+                gl.append(G.make_gate(G.RZ, t, theta=rng.uniform() * 6.28))
+        c, t = others[rng.randint(len(others))], others[rng.randint(len(others))]
+        if c != t:
+            gl.append(G.make_gate(G.X, t, controls=(c,)))
+        gl.append(G.make_gate(G.PHASE, others[rng.randint(len(others))], theta=rng.uniform() * 6.28))
+    return G.gate_array(gl)
+
+
+def compare(got, want, atol, max_key_diff):
+    gk, gr, gi = got
+    wk, wr, wi = want
+    diff = np.setxor1d(gk, wk)
+    assert diff.size <= max_key_diff, "%d keys differ" % diff.size
+    common, ia, ib = np.intersect1d(gk, wk, return_indices=True)
+    d = max(np.max(np.abs(gr[ia] - wr[ib])), np.max(np.abs(gi[ia] - wi[ib])))
+    assert d <= atol, "amplitude difference %.3e" % d
+    return d
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_blocks_match_fp32_oracle_and_fp64(oracle, seed):
+    nq = 16
+    keys, re, im = random_state(300 + seed, nq, 300)
+    gl = dense_block_circuit(seed, nq)
+    say("fused blocks seed", seed, "gates", len(gl))
+    with E.Engine(nq, prec="f32", fuse_blocks=True, renorm_every=100, bucket_bits=4) as e:
+        e.load(keys, re, im)
+        e.apply(gl)
+        got = e.export()
+        st = e.stats()
+    o32 = oracle.Sparse(nq, f32=True, thr=1e-7, renorm_every=100)
+    o32.load(keys, re, im)
+    o32.apply(gl)
+    d32 = compare(got, o32.export(), 1e-5, max(2, got[0].size // 1000))
+    o64 = oracle.Sparse(nq)
+    o64.load(keys, re, im)
+    o64.apply(gl)
+    d64 = compare(got, o64.export(), 1e-5, max(2, got[0].size // 1000))
+    say("max diff vs fp32 oracle %.2e, vs fp64 %.2e, sweeps %d" % (d32, d64, st["sweeps"]))
+    n2 = float(np.sum(got[1] ** 2 + got[2] ** 2))
+    assert abs(n2 - 1.0) < 1e-5
+
+
+def test_blocks_on_large_mix_close_to_fp64(oracle):
+    """The 48-qubit mix (2^12 keys): windows are short, results as the FP32 form."""
+    w = gen.circuit_large(5, nq=48, n_init=1 << 12, n_gates=600)
+    with E.Engine(48, prec="f32", fuse_blocks=True, renorm_every=100) as e:
+        e.load(w["keys"], w["re"], w["im"])
+        e.apply(w["gates"])
+        got = e.export()
+    o64 = oracle.Sparse(48)
+    o64.load(w["keys"], w["re"], w["im"])
+    o64.apply(w["gates"])
+    wk, wr, wi = o64.export()
+    common, ia, ib = np.intersect1d(got[0], wk, return_indices=True)
+    ov = np.sum((wr[ib] - 1j * wi[ib]) * (got[1][ia] + 1j * got[2][ia]))
+    infid = 1.0 - abs(ov) ** 2 / (np.sum(wr ** 2 + wi ** 2) * np.sum(got[1] ** 2 + got[2] ** 2))
+    say("fused-block infidelity %.3e" % infid)
+    assert infid <= 1e-6
+
+
+def test_blocks_ignored_for_fp64():
+    """The flag only applies to FP32 with 16-slot buckets; FP64 runs gate by gate."""
+    nq = 12
+    keys, re, im = random_state(5, nq, 64)
+    gl = dense_block_circuit(3, nq=12, layers=5)
+    with E.Engine(nq, prec="f64", fuse_blocks=True, bucket_bits=4) as e, E.Engine(nq, prec="f64", bucket_bits=4) as f:
+        e.load(keys, re, im)
+        f.load(keys, re, im)
+        e.apply(gl)
+        f.apply(gl)
+        a, b = e.export(), f.export()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])

@@ -44,6 +44,12 @@ def parse():
     ap.add_argument("--n-init-log2", type=int, default=20)
     ap.add_argument("--gates", type=int, default=2000)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--fused-blocks", type=int, nargs="?", const=6, default=0,
+                    help="FP32 fused-block variant: windows of at least N (default 6) bucket-qubit "
+                         "gates as dense 16x16 MFMA blocks")
+    ap.add_argument("--circuit", default="mix", choices=["mix", "layers"],
+                    help="mix: the 60/35/5 diagonal/permutation/branching circuit; layers: layered "
+                         "single-qubit gates on the 4 branching qubits (long fusable windows)")
     ap.add_argument("--tail-gates", type=int, default=200,
                     help="also run the last N gates one gate per launch (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -242,9 +248,15 @@ def main():
     from sparqc_amd import engine as E
     from sparqc_amd import gen
 
-    w = gen.circuit_large(a.seed, nq=48, n_init=1 << a.n_init_log2, n_gates=a.gates)
+    if a.circuit == "layers":
+        w = gen.circuit_layers(a.seed, nq=48, n_init=1 << a.n_init_log2, n_gates=a.gates)
+    else:
+        w = gen.circuit_large(a.seed, nq=48, n_init=1 << a.n_init_log2, n_gates=a.gates)
+    if a.fused_blocks and a.prec != "f32":
+        print("--fused-blocks needs --prec f32", file=sys.stderr)
+        sys.exit(2)
     eng = E.Engine(48, prec=a.prec, timing=True, fuse_window=a.fuse_window, bucket_bits=a.bucket_bits,
-                   renorm_every=100 if a.prec == "f32" else 0)
+                   renorm_every=100 if a.prec == "f32" else 0, fuse_blocks=a.fused_blocks)
     eng.load(w["keys"], w["re"], w["im"])
     eng.prepare(w["gates"])
     eng.snapshot_save()
@@ -265,6 +277,13 @@ def main():
     nnz1 = s1["nnz"]                      # counted by the last sweep (no extra launch)
     dt = t1 - t0
     upd = s1["updates"] - s0["updates"]
+    if a.fused_blocks:
+        # a block does not see the nonzeros between its gates: take the update
+        # count of the gate-by-gate FP32 run of the same circuit
+        with E.Engine(48, prec="f32", renorm_every=100) as ref:
+            ref.load(w["keys"], w["re"], w["im"])
+            ref.apply(w["gates"])
+            upd = ref.stats()["updates"] * a.steps
     amp_b = 16 if a.prec == "f64" else 8
     sweeps = s1["sweeps"] - s0["sweeps"]
     sweep_s = (s1["sweep_ms"] - s0["sweep_ms"]) / 1e3
@@ -288,11 +307,15 @@ def main():
         "vs_baseline": None,
         "dtype": a.prec,
         "data": "synthetic (seeded splitmix64 circuit and state)",
-        "config": {"workload": "sparse statevector, 48 qubits, 2^%d initial keys, %d-gate mix "
-                               "60%% diag / 35%% perm / 5%% branch on 4 qubits, prune 1e-14"
-                               % (a.n_init_log2, a.gates),
+        "config": {"workload": ("sparse statevector, 48 qubits, 2^%d initial keys, %d-gate mix "
+                                "60%% diag / 35%% perm / 5%% branch on 4 qubits, prune 1e-14"
+                                % (a.n_init_log2, a.gates)) if a.circuit == "mix" else
+                               ("sparse statevector, 48 qubits, 2^%d initial keys, %d gates in layers of "
+                                "single-qubit gates on 4 qubits with diagonal / CX gates between"
+                                % (a.n_init_log2, a.gates)),
+                   "circuit": a.circuit,
                    "nq": 48, "n_init": 1 << a.n_init_log2, "gates": a.gates, "seed": a.seed,
-                   "prec": a.prec, "fuse_window": a.fuse_window},
+                   "prec": a.prec, "fuse_window": a.fuse_window, "fused_blocks": a.fused_blocks},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": "k_sweep (fused local sweep)",

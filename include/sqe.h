@@ -82,9 +82,10 @@ typedef struct sqe_config {
     uint32_t timing;            /* 1: time every kernel launch with HIP events         */
     uint32_t mem_reserve_ppm;   /* with hbm_budget_bytes = 0: fraction of free device
                                    memory kept unused, parts per million (0: 10%)     */
-    uint32_t fuse_blocks;       /* 1: FP32 with 16-slot buckets fuses windows of gates
-                                   on bucket qubits into dense 16x16 blocks applied
-                                   with MFMA (prunes once per block)                  */
+    uint32_t fuse_blocks;       /* FP32 with 16-slot buckets: fuse windows of at least
+                                   6 (1) or N (N >= 2) gates on bucket qubits into
+                                   dense 16x16 blocks applied with MFMA (prunes once
+                                   per block); 0: off                                 */
     uint32_t reserved[5];
 } sqe_config;
 

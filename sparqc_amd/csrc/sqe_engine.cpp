@@ -591,6 +591,7 @@ sqe_status flush(sqe_engine* e) {
     std::vector<std::array<cx, 256>> blocks;
     BlockBuilder bb;
     bb.on = e->cfg.fuse_blocks && e->f32 && e->qm.mb == 4;
+    if (e->cfg.fuse_blocks >= 2) bb.min_gates = e->cfg.fuse_blocks;
     bb.qm = e->qm;
     bb.blocks = &blocks;
     const uint64_t W = e->cfg.fuse_window ? e->cfg.fuse_window : ~0ull;

@@ -132,8 +132,10 @@ inline MOp encode_op(const DOp& d, std::vector<double>& coef) {
 // without controls is multiplied into one dense 16x16 complex matrix that the
 // sweep applies with MFMA.  Gates on base qubits without bucket controls
 // commute with such gates (they only scale or relabel whole buckets) and are
-// emitted ahead of the window; any other gate closes it.  A window of one gate
-// stays a gate.  Prune decisions are made once, on the block's outputs.
+// emitted ahead of the window; any other gate closes it.  A window
+// of fewer than min_gates gates runs gate by gate: a dense block costs 2048
+// MFMA FMAs per bucket whatever it holds, so it pays only for long windows.
+// Prune decisions are made once, on the block's outputs.
 // ---------------------------------------------------------------------------
 using cx = std::complex<double>;
 
@@ -161,10 +163,11 @@ constexpr uint32_t K_BLOCK = 17;                  // engine-internal op: fused d
 
 struct BlockBuilder {
     bool on = false;                              // fusion enabled for this program
+    uint32_t min_gates = 6;                       // smaller windows run gate by gate
     QMap qm{};
     std::array<cx, 256> m{};                      // 16x16, row-major, out x in
     uint32_t count = 0;
-    DOp first{};
+    std::vector<DOp> window;                      // the window's gates, in order
     std::vector<std::array<cx, 256>>* blocks = nullptr;
 
     static bool plain_slot(const sqe_gate& g, const QMap& qm) {
@@ -177,8 +180,9 @@ struct BlockBuilder {
         if (count == 0) {
             for (int r = 0; r < 16; ++r)
                 for (int c = 0; c < 16; ++c) m[r * 16 + c] = r == c ? 1.0 : 0.0;
-            first = plan_op(g, qm);
+            window.clear();
         }
+        window.push_back(plan_op(g, qm));
         int j = 0;
         for (int k = 0; k < qm.mb; ++k)
             if (qm.q[k] == g.target) j = k;
@@ -195,8 +199,8 @@ struct BlockBuilder {
         ++count;
     }
     void flush(std::vector<DOp>& prog) {
-        if (count == 1) prog.push_back(first);
-        if (count >= 2) {
+        if (count > 0 && count < min_gates) prog.insert(prog.end(), window.begin(), window.end());
+        if (count >= min_gates) {
             DOp d{};
             d.kind = K_BLOCK;
             d.tslot = -1;

@@ -163,7 +163,7 @@ class Engine:
         L = lib()
         cfg = Config()
         L.sqe_config_default(ctypes.byref(cfg), nq)
-        cfg.fuse_blocks = 1 if fuse_blocks else 0
+        cfg.fuse_blocks = int(fuse_blocks) if fuse_blocks is not True else 1
         cfg.prec = F32 if prec in ("f32", F32) else F64
         cfg.prune_thr = prune_thr
         cfg.bucket_bits = bucket_bits

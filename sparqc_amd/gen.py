@@ -15,6 +15,9 @@ Workloads:
   below n_init * 2^4.
 * ``batch_circuits`` -- independent circuits of 16..40 qubits, 2^8..2^18 initial
   keys, depth 50..500 and 3 branching qubits each.
+* ``circuit_layers`` -- the large state under layers of single-qubit gates on the
+  4 branching qubits separated by diagonal / CX gates elsewhere (long windows of
+  bucket-qubit gates, the shape the fused dense blocks are for).
 
 The reference's own driver applies H to every qubit of |0...0>
 (emulator/src/main.cu:142-143); ``circuit_all_h`` reproduces that scenario.
@@ -216,6 +219,30 @@ def batch_circuits(seed, n_circuits=256, nq_range=(16, 40), log2_nnz=(8.0, 18.0)
         gl = mixed_gates(sub, nq, depth, branch)
         out.append(dict(nq=nq, keys=keys, re=re, im=im, gates=gl, branch=branch, thr=1e-14))
     return out
+
+
+def circuit_layers(seed, nq=48, n_init=1 << 20, n_gates=2000, n_branch=4):
+    """The large state with a layered circuit: each layer applies a random
+    single-qubit gate (H, RY, RX, RZ, PHASE, X) to every one of the n_branch
+    branching qubits, then diagonal and CX gates on other qubits (one RZ or
+    PHASE per layer qubit and one CX per layer, controls off the branching
+    qubits).  Gates on the branching qubits come in long windows that commute
+    with everything between them."""
+    rng = SplitMix64(seed)
+    branch = rng.sample(nq, n_branch)
+    keys, re, im = random_state(rng, n_init, nq)
+    others = [q for q in range(nq) if q not in branch]
+    out = []
+    while len(out) < n_gates:
+        for q in branch:
+            k = (G.H, G.RY, G.RX, G.RZ, G.PHASE, G.X)[rng.randint(6)]
+            out.append(G.make_gate(k, q, theta=rng.uniform() * 2 * math.pi))
+            t = others[rng.randint(len(others))]
+            out.append(G.make_gate((G.RZ, G.PHASE)[rng.randint(2)], t, theta=rng.uniform() * 2 * math.pi))
+        c, t = _two_distinct(rng, nq, exclude_ctrl=branch)
+        if t not in branch:
+            out.append(G.make_gate(G.X, t, controls=(c,)))
+    return dict(nq=nq, keys=keys, re=re, im=im, gates=G.gate_array(out[:n_gates]), branch=branch, thr=1e-14)
 
 
 def circuit_all_h(nq):

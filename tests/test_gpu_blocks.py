@@ -62,7 +62,7 @@ def test_blocks_match_fp32_oracle_and_fp64(oracle, seed):
     keys, re, im = random_state(300 + seed, nq, 300)
     gl = dense_block_circuit(seed, nq)
     say("fused blocks seed", seed, "gates", len(gl))
-    with E.Engine(nq, prec="f32", fuse_blocks=True, renorm_every=100, bucket_bits=4) as e:
+    with E.Engine(nq, prec="f32", fuse_blocks=2, renorm_every=100, bucket_bits=4) as e:
         e.load(keys, re, im)
         e.apply(gl)
         got = e.export()
@@ -83,7 +83,7 @@ def test_blocks_match_fp32_oracle_and_fp64(oracle, seed):
 def test_blocks_on_large_mix_close_to_fp64(oracle):
     """The 48-qubit mix (2^12 keys): windows are short, results as the FP32 form."""
     w = gen.circuit_large(5, nq=48, n_init=1 << 12, n_gates=600)
-    with E.Engine(48, prec="f32", fuse_blocks=True, renorm_every=100) as e:
+    with E.Engine(48, prec="f32", fuse_blocks=2, renorm_every=100) as e:
         e.load(w["keys"], w["re"], w["im"])
         e.apply(w["gates"])
         got = e.export()

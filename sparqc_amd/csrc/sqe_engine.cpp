@@ -489,8 +489,21 @@ sqe_status grow(sqe_engine* e, uint64_t need) {
     return SQE_OK;
 }
 
+sqe_status ensure_partials(sqe_engine* e, uint32_t g) {
+    if (g <= e->partials_cap) return SQE_OK;
+    HIPCK(hipStreamSynchronize(e->st));
+    dfree(e, e->partials, e->partials_cap * 8);
+    e->partials = nullptr;
+    e->partials_cap = 0;
+    SQCK(dalloc(e, (void**)&e->partials, (uint64_t)g * 8));
+    e->partials_cap = g;
+    return SQE_OK;
+}
+
+// A sweep with norm_grid != nullptr also writes per-workgroup norm partials
+// (into e->partials) and returns their count.
 sqe_status run_sweep(sqe_engine* e, const MOp* mops, const double* coef, uint64_t nops, int store,
-                     bool blocks = false) {
+                     bool blocks = false, uint32_t* norm_grid = nullptr) {
     SweepArgs a{};
     a.blocks = blocks ? 1 : 0;
     Store& o = e->s[e->cur];
@@ -500,6 +513,11 @@ sqe_status run_sweep(sqe_engine* e, const MOp* mops, const double* coef, uint64_
     const uint32_t g = sweep_grid(e->qm.mb, e->f32, ntiles);
     SQCK(prep_bstats(e, g));
     a.bstats = e->bstats;
+    if (norm_grid) {
+        SQCK(ensure_partials(e, g));
+        a.norm_out = e->partials;
+        *norm_grid = g;
+    }
     {
         TimeScope ts(e, 0);
         HIPCK(launch_sweep(e->qm.mb, e->f32, e->st, ntiles, a));
@@ -536,14 +554,7 @@ sqe_status run_pair(sqe_engine* e, const DOp& op) {
 
 sqe_status run_norm(sqe_engine* e) {
     const uint32_t g = std::max<uint32_t>(1, tiles_of(round64(e->nb_upper)));
-    if (g > e->partials_cap) {
-        HIPCK(hipStreamSynchronize(e->st));
-        dfree(e, e->partials, e->partials_cap * 8);
-        e->partials = nullptr;
-        e->partials_cap = 0;
-        SQCK(dalloc(e, (void**)&e->partials, (uint64_t)g * 8));
-        e->partials_cap = g;
-    }
+    SQCK(ensure_partials(e, g));
     NormArgs a{};
     a.amp = e->s[e->cur].amp; a.nb = &e->meta[e->cur]; a.partials = e->partials;
     TimeScope ts(e, 2);
@@ -684,12 +695,16 @@ sqe_status flush(sqe_engine* e) {
         dm = (const MOp*)e->dprog;
         dc = (const double*)(e->dprog + coff);
     }
-    for (const auto& s : steps) {
+    uint32_t fused_norm = 0;              // partials of the last sweep, when it computed them
+    for (size_t si = 0; si < steps.size(); ++si) {
+        const Step& s = steps[si];
         switch (s.kind) {
         case ST_SWEEP: {
             bool has_block = false;
             for (uint64_t i = s.off; i < s.off + s.n; ++i) has_block |= prog[i].kind == K_BLOCK;
-            SQCK(run_sweep(e, dm + s.off, dc, s.n, 1, has_block));
+            // a renormalisation right after this sweep takes the norm from its epilogue
+            const bool norm_next = si + 1 < steps.size() && steps[si + 1].kind == ST_NORM;
+            SQCK(run_sweep(e, dm + s.off, dc, s.n, 1, has_block, norm_next ? &fused_norm : nullptr));
             ++e->n_sweeps;
             for (uint64_t i = s.off; i < s.off + s.n; ++i) {
                 const DOp& d = prog[i];
@@ -698,8 +713,16 @@ sqe_status flush(sqe_engine* e) {
                 if (d.tbase && kind_is_perm(d.kind)) e->tab_valid = false;
             }
         } break;
-        case ST_PAIR: SQCK(run_pair(e, s.op)); break;
-        case ST_NORM: SQCK(run_norm(e)); break;
+        case ST_PAIR: SQCK(run_pair(e, s.op)); fused_norm = 0; break;
+        case ST_NORM:
+            if (fused_norm) {
+                TimeScope ts(e, 2);
+                HIPCK(launch_norm_final(e->st, e->partials, fused_norm, &e->dscal[0], &e->dscal[1]));
+            } else {
+                SQCK(run_norm(e));
+            }
+            fused_norm = 0;
+            break;
         }
     }
     return reduce_stats(e);

@@ -322,6 +322,7 @@ __device__ __forceinline__ void block_apply(Cplx<float> (&a)[16], double pr, dou
 // Per-thread counters of one sweep group.
 struct SweepCount {
     unsigned long long upd, nnz_in, nnz_out, empty;
+    double norm;                      // FP64 sum of |a|^2 of the stored amplitudes
 };
 
 // Ops that touch no amplitude register: gates on a base qubit without bucket
@@ -564,9 +565,14 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
     fold_upd();
     out.nnz_out = 0;
     out.empty = 0;
+    out.norm = 0.0;
     static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
         constexpr int e = decltype(ec)::value;
         apply_phase<MB>(a[e], pr[e], pi[e]);
+        static_for<S>([&](auto sc) __attribute__((always_inline)) {
+            constexpr int s = decltype(sc)::value;
+            out.norm += (double)a[e][s].x * (double)a[e][s].x + (double)a[e][s].y * (double)a[e][s].y;
+        });
         if (tile[e] >= 0 && store) {
             const uint64_t b = (uint64_t)tile[e] * TILE + lane;
             static_for<S>([&](auto sc) __attribute__((always_inline)) {
@@ -614,9 +620,23 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
         tile[e] = t < ntiles ? t : -1;
         any |= t < ntiles;
     });
-    SweepCount cnt{0, 0, 0, 0};
+    SweepCount cnt{0, 0, 0, 0, 0.0};
     if (any) sweep_group<MB, R, EPT, BLOCKS>(A.base, A.amp, tile, A.ops, A.nops, A.coef, A.thr, A.scale, A.store, cnt);
     block_stats(A.bstats, cnt.upd, cnt.nnz_out, cnt.empty, A.store ? cnt.nnz_in + cnt.nnz_out : 0ull);
+    if (A.norm_out) {
+        // the FP64 norm of the stored state, one partial per workgroup (for a
+        // renormalisation right after this launch)
+        __shared__ double s_norm[BLOCK / 64];
+        double v = cnt.norm;
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+        if ((threadIdx.x & 63) == 0) s_norm[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < BLOCK / 64; ++w) t += s_norm[w];
+            A.norm_out[blockIdx.x] = t;
+        }
+    }
 }
 
 // Count-only pass (no program, no store): nonzeros and empty buckets.
@@ -627,7 +647,7 @@ __global__ __launch_bounds__(BLOCK) void k_count(SweepArgs A) {
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = (int64_t)blockIdx.x * (BLOCK / 64) + wave;
     const int64_t tile[1] = {t < ntiles ? t : -1};
-    SweepCount cnt{0, 0, 0, 0};
+    SweepCount cnt{0, 0, 0, 0, 0.0};
     if (t < ntiles) sweep_group<MB, R, 1>(A.base, A.amp, tile, nullptr, 0, nullptr, A.thr, nullptr, 0, cnt);
     block_stats(A.bstats, 0ull, cnt.nnz_out, cnt.empty, 0ull);
 }

@@ -179,6 +179,11 @@ hipError_t launch_norm(int mb, bool f32, hipStream_t st, uint32_t grid, const No
     return hipGetLastError();
 }
 
+hipError_t launch_norm_final(hipStream_t st, const double* partials, uint32_t n, double* norm2, double* scale) {
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(BLOCK), 0, st, partials, n, norm2, scale);
+    return hipGetLastError();
+}
+
 // One wavefront per circuit: fold its items' partial sums in a fixed order.
 __global__ __launch_bounds__(BLOCK) void k_norm_batch_final(const BTask* __restrict__ tasks, uint32_t ntasks,
                                                             const double* __restrict__ partials) {

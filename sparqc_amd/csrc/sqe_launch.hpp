@@ -17,6 +17,7 @@ struct SweepArgs {
     uint32_t nops;
     int32_t store;
     int32_t blocks;         // the program holds fused blocks (FP32, 16 slots)
+    double* norm_out;       // non-null: per-workgroup FP64 partial norms of the stored state
     double thr;
     const double* scale;
     Stats* stats;
@@ -146,6 +147,8 @@ hipError_t launch_export(int mb, bool f32, hipStream_t st, uint32_t grid, const 
 hipError_t launch_norm(int mb, bool f32, hipStream_t st, uint32_t grid, const NormArgs& a, double* norm2,
                        double* scale);
 hipError_t launch_index_build(hipStream_t st, uint32_t grid, const IndexArgs& a);
+// norm2 and scale from n per-workgroup partials (of a sweep with norm_out)
+hipError_t launch_norm_final(hipStream_t st, const double* partials, uint32_t n, double* norm2, double* scale);
 // One wavefront per work item: grid = ceil(nitems / (block / 64)).
 hipError_t launch_sweep_batch(int mb, bool f32, hipStream_t st, uint32_t block, const BatchArgs& a);
 // Per-circuit FP64 norms of the tasks' regions -> norm2 and scale = 1/sqrt(norm2),

@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E bytes/s
+VALU_PEAK = 1024 * 2.4e9 / 4   # wave64 VALU instructions/s: 1024 SIMDs x 2.4 GHz, 4 cycles each
 KEY_B = 8
 # bytes per update of an unfused per-gate engine on this gate mix (0.6 DIAG at
 # K+2A, 0.35 PERM at 2(K+A), 0.05 BRANCH at (1+2)(K+A) per entering nonzero)
@@ -168,6 +169,28 @@ def traffic_from_profiles(prec, window):
         return d.get("%s_w%d" % (prec, window))
     except (OSError, ValueError):
         return None
+
+
+def valu_from_profiles(a, sweep_s_per_step):
+    """VALU issue of the fused sweep: wave instructions per step counted by
+    rocprofv3 (SQ_INSTS_VALU, committed summary) over the live kernel time,
+    against the issue peak.  Only for the default workload the counters
+    were taken on."""
+    if (a.circuit != "mix" or a.n_init_log2 != 20 or a.gates != 2000 or a.seed != 1 or a.fuse_window
+            or a.fused_blocks or a.bucket_bits != -1):
+        return None
+    p = os.path.join(ROOT, "profiles", "sweep_traffic.json")
+    try:
+        with open(p) as f:
+            v = json.load(f).get("valu_%s" % a.prec)
+    except (OSError, ValueError):
+        return None
+    if not v or sweep_s_per_step <= 0:
+        return None
+    ach = v["valu_insts"] / sweep_s_per_step
+    return {"bound": "valu", "achieved": ach, "peak": VALU_PEAK, "unit": "wave64 VALU instructions/s",
+            "frac": ach / VALU_PEAK, "valu_per_wave_gate": v["valu_per_wave_gate"],
+            "insts_per_step": v["valu_insts"]}
 
 
 def batch_mode(a):
@@ -322,8 +345,9 @@ def main():
                      "algorithmic_bytes_per_launch": per_launch_bytes,
                      "avg_launch_ms": per_launch_s * 1e3, "launches": sweeps,
                      "note": "the fused sweep keeps each bucket in registers for the whole program, so it "
-                             "is instruction-issue bound, not HBM bound; 'unfused_tail' runs the same "
-                             "kernel one gate per launch, where HBM is the bound",
+                             "is instruction-issue bound, not HBM bound ('issue'); 'unfused_tail' runs the "
+                             "same kernel one gate per launch, where HBM is the bound",
+                     "issue": valu_from_profiles(a, sweep_s / a.steps),
                      "per_gate_model": {"bytes_per_update": PER_UPDATE_BYTES[a.prec],
                                         "equivalent_GBps": upd / dt * PER_UPDATE_BYTES[a.prec] / 1e9,
                                         "equivalent_frac": upd / dt * PER_UPDATE_BYTES[a.prec] / HBM_PEAK},

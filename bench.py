@@ -59,6 +59,8 @@ def parse():
                     help="large: the 48-qubit 2^20-key circuit (default, the headline metric); "
                          "batch: 256 independent circuits in one batch; serve: Poisson arrivals")
     ap.add_argument("--circuits", type=int, default=256, help="batch mode: circuits")
+    ap.add_argument("--eager", action="store_true",
+                    help="batch mode: launch every step directly instead of replaying a captured hipGraph")
     ap.add_argument("--requests", type=int, default=600, help="serve mode: requests per rate")
     ap.add_argument("--rates", default="200,1000,5000", help="serve mode: arrival rates (1/s)")
     ap.add_argument("--step-gates", type=int, default=64, help="serve mode: gates per batched step")
@@ -80,27 +82,83 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes_per_launch": n * 8, "launches": 5}), flush=True)
 
 
+def stream_copy():
+    """Device-to-device copy bandwidth (hipMemcpy of 1 GiB, 5 times, timed with
+    HIP events): the practical HBM ceiling beside the 8 TB/s datasheet peak."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        return None
+    n = 1 << 30
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        return None
+    try:
+        hip.hipMemset(a, 1, ctypes.c_size_t(n))
+        hip.hipEventCreate(ctypes.byref(e0))
+        hip.hipEventCreate(ctypes.byref(e1))
+        hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)                 # warm-up
+        hip.hipEventRecord(e0, None)
+        for _ in range(5):
+            hip.hipMemcpyAsync(b, a, ctypes.c_size_t(n), 3, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    if ms.value <= 0:
+        return None
+    return {"bytes_per_copy": 2 * n, "copies": 5, "GBps": 5 * 2 * n / (ms.value * 1e-3) / 1e9,
+            "frac_of_peak": 5 * 2 * n / (ms.value * 1e-3) / HBM_PEAK}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(w, seconds):
-    """Oracle (FP64 sparse map, OpenMP) on the first gates of the same workload."""
+    """Oracle (FP64 sparse map, OpenMP) on the first gates of the same workload,
+    on up to 16 host cores, then on one core over a shorter sample."""
     from oracle import oracle as O
     try:
-        cores = min(16, len(os.sched_getaffinity(0)))
+        avail = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = min(16, os.cpu_count() or 1)
-    O.set_threads(cores)
-    s = O.Sparse(w["nq"])
-    s.load(w["keys"], w["re"], w["im"])
+        avail = os.cpu_count() or 1
+    cores = min(16, avail)
     gl = w["gates"]
-    t0 = time.perf_counter()
-    done = 0
-    while done < len(gl) and time.perf_counter() - t0 < seconds:
-        s.apply(gl[done:done + 25])
-        done = min(len(gl), done + 25)
-    dt = time.perf_counter() - t0
-    return {"value": s.updates() / dt, "unit": "updates/s", "cores": cores, "kind": "port",
+
+    def timed(threads, limit):
+        O.set_threads(threads)
+        s = O.Sparse(w["nq"])
+        s.load(w["keys"], w["re"], w["im"])
+        t0 = time.perf_counter()
+        done = 0
+        while done < len(gl) and time.perf_counter() - t0 < limit:
+            s.apply(gl[done:done + 25])
+            done = min(len(gl), done + 25)
+        return s.updates() / (time.perf_counter() - t0), done, time.perf_counter() - t0
+
+    v, done, dt = timed(cores, seconds)
+    v1, done1, dt1 = timed(1, seconds / 3)
+    return {"value": v, "unit": "updates/s", "cores": cores, "kind": "port",
             "sample": "CPU oracle (FP64 sparse map) on the first %d of %d gates of the same "
                       "workload from the same 2^%d-key state, %.1f s" % (
-                          done, len(gl), int(round(__import__("math").log2(w["keys"].size))), dt)}
+                          done, len(gl), int(round(__import__("math").log2(w["keys"].size))), dt),
+            "one_core": {"value": v1, "gates": done1, "seconds": dt1},
+            "cpu_model": cpu_model(), "cores_available": avail}
 
 
 def unfused_tail(a, w, eng):
@@ -198,7 +256,7 @@ def batch_mode(a):
     from sparqc_amd import engine as E
     from sparqc_amd import gen
     cs = gen.batch_circuits(a.seed, n_circuits=a.circuits)
-    with E.Batch(prec=a.prec, renorm_every=100 if a.prec == "f32" else 0, timing=False) as b:
+    with E.Batch(prec=a.prec, renorm_every=100 if a.prec == "f32" else 0, timing=False, eager=a.eager) as b:
         ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
         tune = b.autotune()
         b.snapshot_save()
@@ -221,7 +279,8 @@ def batch_mode(a):
            "data": "synthetic (seeded splitmix64 circuits and states)",
            "config": {"workload": "%d circuits, 16-40 qubits, 2^8-2^18 initial keys, depth 50-500, "
                                   "60/35/5 diag/perm/branch mix on 3 branching qubits each" % a.circuits,
-                      "circuits": a.circuits, "seed": a.seed, "prec": a.prec},
+                      "circuits": a.circuits, "seed": a.seed, "prec": a.prec,
+                      "launch": "eager steps" if a.eager else "captured hipGraph replay"},
            "detail": {"updates_per_step": upd, "nnz_final": nnz, "initial_keys": int(sum(c["keys"].size for c in cs)),
                       "gates": int(sum(len(c["gates"]) for c in cs)), "pool_bytes": used,
                       "autotune": tune, "circuits_per_s": a.circuits / (dt / a.steps)}}
@@ -360,6 +419,7 @@ def main():
     }
     if a.prec == "f32":
         out["detail"]["precision_vs_fp64"] = precision_report(w, eng)
+    out["detail"]["stream_copy"] = stream_copy()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
     else:

@@ -86,7 +86,10 @@ typedef struct sqe_config {
                                    6 (1) or N (N >= 2) gates on bucket qubits into
                                    dense 16x16 blocks applied with MFMA (prunes once
                                    per block); 0: off                                 */
-    uint32_t reserved[5];
+    uint32_t batch_eager;       /* batch: 1 launches each step of sqe_batch_run
+                                   directly; 0 replays the run as one captured
+                                   hipGraph (planned once per set of circuits)    */
+    uint32_t reserved[4];
 } sqe_config;
 
 typedef struct sqe_stats {

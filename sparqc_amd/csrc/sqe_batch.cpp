@@ -171,6 +171,20 @@ struct sqe_batch {
     uint64_t snap_bcap = 0, snap_acap = 0, snap_ccap = 0, snap_bext = 0, snap_aext = 0;
     std::vector<uint64_t> snap_pos;
     uint64_t generation = 0, snap_generation = ~0ull;
+    uint64_t layout = 0;                   // bumped whenever a device pointer a launch uses may change
+    // captured gate sequence of a whole run (sqe_batch_run): valid for the
+    // layout, tuning and start positions it was planned from
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    uint64_t g_layout = ~0ull;
+    uint32_t g_block = 0, g_gran = 0;
+    std::vector<uint64_t> g_pos0, g_pos1;  // positions before / after the run
+    uint64_t g_steps = 0, g_launches = 0;
+    BTask* gtask = nullptr;
+    BItem* gitem = nullptr;
+    unsigned long long* gcounts = nullptr;
+    double* gpartials = nullptr;
+    uint64_t gcap_t = 0, gcap_i = 0, gcap_c = 0, gcap_p = 0;
     // tuning and counters
     uint32_t block = BLOCK, gran = 1;
     double sweep_ms = 0.0;
@@ -209,6 +223,7 @@ sqe_status pool_fit(sqe_batch* b, Pool& pl, uint64_t need, uint64_t keep) {
     dfr(b, pl.p, pl.bytes);
     pl.p = np;
     pl.bytes = nb;
+    ++b->layout;
     return SQE_OK;
 }
 
@@ -231,6 +246,7 @@ sqe_status slots_fit(sqe_batch* b, uint64_t need) {
     BCK(regrow((void**)&b->norm2, 8));
     BCK(regrow((void**)&b->scale, 8));
     b->nslot = n;
+    ++b->layout;
     return SQE_OK;
 }
 
@@ -311,28 +327,43 @@ void release(sqe_batch* b, uint32_t id) {
     c = Circ{};
     b->free_slots.push_back(id);
     ++b->generation;
+    ++b->layout;
 }
 
-// Launch the sweeps for one step: every live circuit whose program is not
-// finished runs ops [pos, end).  Returns the circuits that stopped before a
-// RENORM op (they need a norm).
-sqe_status run_step(sqe_batch* b, uint64_t max_ops, std::vector<uint32_t>* need_norm) {
+// One batched launch: a sweep (then the fold of its per-item counters) or an
+// FP64 norm, over a task list (one task per circuit) and its work items.
+struct Group {
+    bool norm = false;
+    int mb = 0;
+    std::vector<BTask> tasks;
+    std::vector<BItem> items;
+    size_t task_off = 0, item_off = 0;     // offsets in the packed upload
+};
+
+constexpr uint32_t NORM_TILES = 4;         // tiles per norm work item
+
+// Plan one step from positions pos[] (advanced in place): every live circuit
+// with ops left runs to its next RENORM op (at most max_ops ops when
+// max_ops > 0), one sweep group per bucket size with the longest programs
+// first; then one norm group per bucket size for the circuits that stopped
+// before a RENORM op.  Returns false when no circuit has ops left.
+bool plan_step(sqe_batch* b, uint64_t max_ops, std::vector<uint64_t>& pos, std::vector<Group>& out) {
     struct Seg { uint32_t id; uint64_t from, to; };
     std::vector<Seg> segs;
     for (uint32_t id = 0; id < b->c.size(); ++id) {
-        Circ& c = b->c[id];
-        if (!c.live || c.pos >= c.nops) continue;
-        uint64_t end = max_ops ? std::min<uint64_t>(c.nops, c.pos + max_ops) : c.nops;
+        const Circ& c = b->c[id];
+        if (!c.live || pos[id] >= c.nops) continue;
+        uint64_t end = max_ops ? std::min<uint64_t>(c.nops, pos[id] + max_ops) : c.nops;
         for (uint32_t r : c.renorm_at)
-            if (r > c.pos && r < end) { end = r; break; }
-        segs.push_back(Seg{id, c.pos, end});
+            if (r > pos[id] && r < end) { end = r; break; }
+        segs.push_back(Seg{id, pos[id], end});
     }
-    if (segs.empty()) return SQE_OK;
-    // group by bucket size; longest segment first within the item queue
+    if (segs.empty()) return false;
     std::sort(segs.begin(), segs.end(), [](const Seg& x, const Seg& y) { return x.to - x.from > y.to - y.from; });
+    std::vector<uint32_t> need_norm;
     for (int mb = 0; mb <= 5; ++mb) {
-        std::vector<BTask> tasks;
-        std::vector<BItem> items;
+        Group g;
+        g.mb = mb;
         for (const Seg& s : segs) {
             const Circ& c = b->c[s.id];
             if (c.qm.mb != mb) continue;
@@ -347,64 +378,23 @@ sqe_status run_step(sqe_batch* b, uint64_t max_ops, std::vector<uint32_t>* need_
             t.norm2 = b->norm2 + s.id;
             t.nops = (uint32_t)(s.to - s.from);
             t.ntiles = (uint32_t)c.tiles;
-            t.item0 = (uint32_t)items.size();
-            const uint32_t ti = (uint32_t)tasks.size();
-            for (uint64_t tl = 0; tl < c.tiles; tl += b->gran) items.push_back(BItem{ti, (uint32_t)tl});
-            t.nitem = (uint32_t)items.size() - t.item0;
-            tasks.push_back(t);
-        }
-        if (tasks.empty()) continue;
-        // items are already in longest-program-first order (segments were sorted)
-        if (tasks.size() > b->hcap_t) {
-            if (b->htask) (void)hipHostFree(b->htask);
-            b->hcap_t = p2(tasks.size());
-            BHIP(hipHostMalloc((void**)&b->htask, b->hcap_t * sizeof(BTask), hipHostMallocDefault));
-        }
-        if (items.size() > b->hcap_i) {
-            if (b->hitem) (void)hipHostFree(b->hitem);
-            b->hcap_i = p2(items.size());
-            BHIP(hipHostMalloc((void**)&b->hitem, b->hcap_i * sizeof(BItem), hipHostMallocDefault));
-        }
-        BCK(buf_fit(b, &b->dtask, &b->dcap_t, tasks.size()));
-        BCK(buf_fit(b, &b->ditem, &b->dcap_i, items.size()));
-        BCK(buf_fit(b, &b->counts, &b->ccap, 2 * items.size()));
-        BHIP(hipStreamSynchronize(b->st));          // the pinned staging is reused
-        std::memcpy(b->htask, tasks.data(), tasks.size() * sizeof(BTask));
-        std::memcpy(b->hitem, items.data(), items.size() * sizeof(BItem));
-        BHIP(hipMemcpyAsync(b->dtask, b->htask, tasks.size() * sizeof(BTask), hipMemcpyHostToDevice, b->st));
-        BHIP(hipMemcpyAsync(b->ditem, b->hitem, items.size() * sizeof(BItem), hipMemcpyHostToDevice, b->st));
-        BatchArgs a{b->dtask, b->ditem, (uint32_t)items.size(), b->gran, b->thr, nullptr, b->counts};
-        if (b->cfg.timing) BHIP(hipEventRecord(b->ev0, b->st));
-        BHIP(launch_sweep_batch(mb, b->f32, b->st, b->block, a));
-        if (b->cfg.timing) {
-            BHIP(hipEventRecord(b->ev1, b->st));
-            BHIP(hipEventSynchronize(b->ev1));
-            float ms = 0.f;
-            BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-            b->sweep_ms += ms;
-        }
-        BHIP(launch_batch_fold(b->st, b->dtask, (uint32_t)tasks.size(), b->counts));
-        ++b->launches;
-        // FP32 renormalisation points reached by this launch
-        for (const Seg& s : segs) {
-            const Circ& c = b->c[s.id];
-            if (c.qm.mb != mb) continue;
+            t.item0 = (uint32_t)g.items.size();
+            const uint32_t ti = (uint32_t)g.tasks.size();
+            for (uint64_t tl = 0; tl < c.tiles; tl += b->gran) g.items.push_back(BItem{ti, (uint32_t)tl});
+            t.nitem = (uint32_t)g.items.size() - t.item0;
+            g.tasks.push_back(t);
             bool at_renorm = false;
             for (uint32_t r : c.renorm_at) at_renorm |= (r == s.to);
-            if (at_renorm && need_norm) need_norm->push_back(s.id);
+            if (at_renorm) need_norm.push_back(s.id);
         }
+        if (!g.tasks.empty()) out.push_back(std::move(g));
     }
-    for (const Seg& s : segs) b->c[s.id].pos = s.to;
-    return SQE_OK;
-}
-
-sqe_status run_norms(sqe_batch* b, const std::vector<uint32_t>& ids) {
-    if (ids.empty()) return SQE_OK;
-    constexpr uint32_t NG = 4;                       // tiles per norm work item
+    for (const Seg& s : segs) pos[s.id] = s.to;
     for (int mb = 0; mb <= 5; ++mb) {
-        std::vector<BTask> tasks;
-        std::vector<BItem> items;
-        for (uint32_t id : ids) {
+        Group g;
+        g.norm = true;
+        g.mb = mb;
+        for (uint32_t id : need_norm) {
             const Circ& c = b->c[id];
             if (c.qm.mb != mb) continue;
             BTask t{};
@@ -412,23 +402,179 @@ sqe_status run_norms(sqe_batch* b, const std::vector<uint32_t>& ids) {
             t.norm2 = b->norm2 + id;
             t.scale = b->scale + id;
             t.ntiles = (uint32_t)c.tiles;
-            t.item0 = (uint32_t)items.size();
-            const uint32_t ti = (uint32_t)tasks.size();
-            for (uint64_t tl = 0; tl < c.tiles; tl += NG) items.push_back(BItem{ti, (uint32_t)tl});
-            t.nitem = (uint32_t)items.size() - t.item0;
-            tasks.push_back(t);
+            t.item0 = (uint32_t)g.items.size();
+            const uint32_t ti = (uint32_t)g.tasks.size();
+            for (uint64_t tl = 0; tl < c.tiles; tl += NORM_TILES) g.items.push_back(BItem{ti, (uint32_t)tl});
+            t.nitem = (uint32_t)g.items.size() - t.item0;
+            g.tasks.push_back(t);
         }
-        if (tasks.empty()) continue;
-        BCK(buf_fit(b, &b->dtask, &b->dcap_t, tasks.size()));
-        BCK(buf_fit(b, &b->ditem, &b->dcap_i, items.size()));
-        BCK(buf_fit(b, &b->partials, &b->pcap, items.size()));
-        BHIP(hipStreamSynchronize(b->st));
-        BHIP(hipMemcpy(b->dtask, tasks.data(), tasks.size() * sizeof(BTask), hipMemcpyHostToDevice));
-        BHIP(hipMemcpy(b->ditem, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice));
-        BatchArgs a{b->dtask, b->ditem, (uint32_t)items.size(), NG, b->thr, b->partials, nullptr};
-        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(4096, ((uint32_t)items.size() + 3) / 4));
-        BHIP(launch_norm_batch(mb, b->f32, b->st, grid, (uint32_t)tasks.size(), a));
+        if (!g.tasks.empty()) out.push_back(std::move(g));
     }
+    return true;
+}
+
+// Pack the groups' tasks and items back to back (recording each group's
+// offsets) and return the largest sweep and norm item counts.
+void pack(std::vector<Group>& gs, std::vector<BTask>& T, std::vector<BItem>& I, uint64_t* max_sweep,
+          uint64_t* max_norm) {
+    *max_sweep = 0;
+    *max_norm = 0;
+    for (Group& g : gs) {
+        g.task_off = T.size();
+        g.item_off = I.size();
+        T.insert(T.end(), g.tasks.begin(), g.tasks.end());
+        I.insert(I.end(), g.items.begin(), g.items.end());
+        uint64_t& m = g.norm ? *max_norm : *max_sweep;
+        m = std::max<uint64_t>(m, g.items.size());
+    }
+}
+
+// Enqueue one group on the stream (no host synchronisation unless `timed`).
+sqe_status launch_group(sqe_batch* b, const Group& g, const BTask* dt, const BItem* di,
+                        unsigned long long* counts, double* partials, bool timed) {
+    const BTask* t = dt + g.task_off;
+    const BItem* it = di + g.item_off;
+    if (!g.norm) {
+        BatchArgs a{t, it, (uint32_t)g.items.size(), b->gran, b->thr, nullptr, counts};
+        if (timed) BHIP(hipEventRecord(b->ev0, b->st));
+        BHIP(launch_sweep_batch(g.mb, b->f32, b->st, b->block, a));
+        if (timed) {
+            BHIP(hipEventRecord(b->ev1, b->st));
+            BHIP(hipEventSynchronize(b->ev1));
+            float ms = 0.f;
+            BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            b->sweep_ms += ms;
+        }
+        BHIP(launch_batch_fold(b->st, t, (uint32_t)g.tasks.size(), counts));
+    } else {
+        BatchArgs a{t, it, (uint32_t)g.items.size(), NORM_TILES, b->thr, partials, nullptr};
+        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(4096, ((uint32_t)g.items.size() + 3) / 4));
+        BHIP(launch_norm_batch(g.mb, b->f32, b->st, grid, (uint32_t)g.tasks.size(), a));
+    }
+    return SQE_OK;
+}
+
+// One step launched directly: its groups are uploaded together (one host
+// synchronisation per step, for the reused pinned staging), then enqueued.
+sqe_status step_all(sqe_batch* b, uint64_t max_gates) {
+    std::vector<uint64_t> pos(b->c.size());
+    for (size_t i = 0; i < b->c.size(); ++i) pos[i] = b->c[i].pos;
+    std::vector<Group> gs;
+    if (!plan_step(b, max_gates, pos, gs)) return SQE_OK;
+    std::vector<BTask> T;
+    std::vector<BItem> I;
+    uint64_t ms = 0, mn = 0;
+    pack(gs, T, I, &ms, &mn);
+    if (T.size() > b->hcap_t) {
+        BHIP(hipStreamSynchronize(b->st));
+        if (b->htask) (void)hipHostFree(b->htask);
+        b->htask = nullptr;
+        b->hcap_t = p2(T.size());
+        BHIP(hipHostMalloc((void**)&b->htask, b->hcap_t * sizeof(BTask), hipHostMallocDefault));
+    }
+    if (I.size() > b->hcap_i) {
+        BHIP(hipStreamSynchronize(b->st));
+        if (b->hitem) (void)hipHostFree(b->hitem);
+        b->hitem = nullptr;
+        b->hcap_i = p2(I.size());
+        BHIP(hipHostMalloc((void**)&b->hitem, b->hcap_i * sizeof(BItem), hipHostMallocDefault));
+    }
+    BCK(buf_fit(b, &b->dtask, &b->dcap_t, T.size()));
+    BCK(buf_fit(b, &b->ditem, &b->dcap_i, I.size()));
+    BCK(buf_fit(b, &b->counts, &b->ccap, std::max<uint64_t>(1, 2 * ms)));
+    BCK(buf_fit(b, &b->partials, &b->pcap, std::max<uint64_t>(1, mn)));
+    BHIP(hipStreamSynchronize(b->st));              // the pinned staging is reused
+    std::memcpy(b->htask, T.data(), T.size() * sizeof(BTask));
+    std::memcpy(b->hitem, I.data(), I.size() * sizeof(BItem));
+    BHIP(hipMemcpyAsync(b->dtask, b->htask, T.size() * sizeof(BTask), hipMemcpyHostToDevice, b->st));
+    BHIP(hipMemcpyAsync(b->ditem, b->hitem, I.size() * sizeof(BItem), hipMemcpyHostToDevice, b->st));
+    for (const Group& g : gs) {
+        BCK(launch_group(b, g, b->dtask, b->ditem, b->counts, b->partials, b->cfg.timing != 0));
+        if (!g.norm) ++b->launches;
+    }
+    for (size_t i = 0; i < b->c.size(); ++i) b->c[i].pos = pos[i];
+    ++b->steps;
+    return SQE_OK;
+}
+
+void drop_graph(sqe_batch* b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->g_layout = ~0ull;
+}
+
+// The rest of every circuit's program as one captured graph: all steps are
+// planned on the host (the schedule depends only on the programs and the
+// start positions), their task and item lists uploaded once, and the kernel
+// sequence (sweep, fold and norm per step and bucket size) captured from the
+// stream.  A run with the same layout, tuning and start positions (a run
+// after a snapshot restore) replays the instantiated graph.
+sqe_status run_graph(sqe_batch* b) {
+    std::vector<uint64_t> pos0(b->c.size());
+    for (size_t i = 0; i < b->c.size(); ++i) pos0[i] = b->c[i].pos;
+    const bool hit = b->gexec && b->g_layout == b->layout && b->g_block == b->block && b->g_gran == b->gran &&
+                     b->g_pos0 == pos0;
+    if (!hit) {
+        drop_graph(b);
+        std::vector<Group> gs;
+        std::vector<uint64_t> pos = pos0;
+        uint64_t steps = 0, launches = 0;
+        while (plan_step(b, 0, pos, gs)) ++steps;
+        if (gs.empty()) return SQE_OK;
+        for (const Group& g : gs) launches += g.norm ? 0 : 1;
+        std::vector<BTask> T;
+        std::vector<BItem> I;
+        uint64_t ms = 0, mn = 0;
+        pack(gs, T, I, &ms, &mn);
+        BCK(buf_fit(b, &b->gtask, &b->gcap_t, T.size()));
+        BCK(buf_fit(b, &b->gitem, &b->gcap_i, I.size()));
+        BCK(buf_fit(b, &b->gcounts, &b->gcap_c, std::max<uint64_t>(1, 2 * ms)));
+        BCK(buf_fit(b, &b->gpartials, &b->gcap_p, std::max<uint64_t>(1, mn)));
+        BHIP(hipStreamSynchronize(b->st));
+        BHIP(hipMemcpy(b->gtask, T.data(), T.size() * sizeof(BTask), hipMemcpyHostToDevice));
+        BHIP(hipMemcpy(b->gitem, I.data(), I.size() * sizeof(BItem), hipMemcpyHostToDevice));
+        BHIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        sqe_status st = SQE_OK;
+        for (const Group& g : gs) {
+            st = launch_group(b, g, b->gtask, b->gitem, b->gcounts, b->gpartials, false);
+            if (st != SQE_OK) break;
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(b->st, &graph);
+        if (st != SQE_OK || ec != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            return st != SQE_OK ? st : bfail(SQE_EHIP, std::string("graph capture: ") + hipGetErrorString(ec));
+        }
+        b->graph = graph;
+        if (hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            drop_graph(b);
+            return bfail(SQE_EHIP, "graph instantiate");
+        }
+        b->g_layout = b->layout;
+        b->g_block = b->block;
+        b->g_gran = b->gran;
+        b->g_pos0 = pos0;
+        b->g_pos1 = pos;
+        b->g_steps = steps;
+        b->g_launches = launches;
+    }
+    const bool timed = b->cfg.timing != 0;
+    if (timed) BHIP(hipEventRecord(b->ev0, b->st));
+    BHIP(hipGraphLaunch(b->gexec, b->st));
+    if (timed) {
+        BHIP(hipEventRecord(b->ev1, b->st));
+        BHIP(hipEventSynchronize(b->ev1));
+        float ms = 0.f;
+        BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->sweep_ms += ms;
+    }
+    for (size_t i = 0; i < b->c.size(); ++i) b->c[i].pos = b->g_pos1[i];
+    b->steps += b->g_steps;
+    b->launches += b->g_launches;
     return SQE_OK;
 }
 
@@ -468,15 +614,6 @@ sqe_status snap_restore(sqe_batch* b) {
     BHIP(hipMemcpyAsync(b->upd, b->snap_cnt, b->nslot * 8, hipMemcpyDeviceToDevice, b->st));
     BHIP(hipMemcpyAsync(b->nnz, b->snap_cnt + b->nslot, b->nslot * 8, hipMemcpyDeviceToDevice, b->st));
     for (size_t j = 0; j < b->c.size() && j < b->snap_pos.size(); ++j) b->c[j].pos = b->snap_pos[j];
-    return SQE_OK;
-}
-
-sqe_status step_all(sqe_batch* b, uint64_t max_gates) {
-    // a gate budget of N ops per circuit (RENORM ops ride along)
-    std::vector<uint32_t> nn;
-    BCK(run_step(b, max_gates, &nn));
-    BCK(run_norms(b, nn));
-    ++b->steps;
     return SQE_OK;
 }
 
@@ -533,8 +670,10 @@ sqe_status sqe_batch_destroy(sqe_batch* b) {
                     (void*)b->nnz, (void*)b->meta, (void*)b->norm2, (void*)b->scale, (void*)b->tab,
                     (void*)b->xkeys, (void*)b->xre, (void*)b->xim, (void*)b->occ, (void*)b->stats,
                     (void*)b->scan_status, (void*)b->ticket, (void*)b->total, (void*)b->dtask, (void*)b->ditem, (void*)b->partials,
-                    (void*)b->counts, (void*)b->snap_base, (void*)b->snap_amp, (void*)b->snap_cnt})
+                    (void*)b->counts, (void*)b->snap_base, (void*)b->snap_amp, (void*)b->snap_cnt, (void*)b->gtask,
+                    (void*)b->gitem, (void*)b->gcounts, (void*)b->gpartials})
         if (p) (void)hipFree(p);
+    drop_graph(b);
     if (b->htask) (void)hipHostFree(b->htask);
     if (b->hitem) (void)hipHostFree(b->hitem);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -631,6 +770,7 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
     b->c[id] = c;
     b->used += bytes;
     ++b->generation;
+    ++b->layout;
     Circ& cc = b->c[id];
     // upload program and coefficients
     if (cc.nops) BHIP(hipMemcpyAsync(b->prog.p + cc.prog0 * sizeof(MOp), prog.data(), cc.nops * sizeof(MOp),
@@ -687,6 +827,7 @@ sqe_status sqe_batch_step(sqe_batch* b, uint32_t max_gates) {
 sqe_status sqe_batch_run(sqe_batch* b) {
     if (!b) return bfail(SQE_EINVAL, "null batch");
     BHIP(hipSetDevice(b->cfg.device));
+    if (!b->cfg.batch_eager) return run_graph(b);
     for (;;) {
         bool more = false;
         for (const Circ& c : b->c) more |= c.live && c.pos < c.nops;
@@ -816,7 +957,8 @@ sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t) {
 
 // Time every (workgroup size, tiles per item) candidate on the
 // batch as it stands -- state and positions are saved and restored around each
-// trial -- and keep the fastest.
+// trial -- and keep the fastest.  With captured runs, each candidate is timed
+// on a replay of its graph.
 sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
     if (!b) return bfail(SQE_EINVAL, "null batch");
     BHIP(hipSetDevice(b->cfg.device));
@@ -832,6 +974,12 @@ sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
     for (const Cand& cd : cands) {
         b->block = cd.block;
         b->gran = cd.gran;
+        if (!b->cfg.batch_eager) {
+            // capture and instantiate this candidate's graph outside the timing
+            BCK(sqe_batch_run(b));
+            BCK(snap_restore(b));
+            BHIP(hipStreamSynchronize(b->st));
+        }
         BHIP(hipEventRecord(b->ev0, b->st));
         BCK(sqe_batch_run(b));
         BHIP(hipEventRecord(b->ev1, b->st));

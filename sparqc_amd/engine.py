@@ -42,7 +42,8 @@ class Config(ctypes.Structure):
                 ("device", ctypes.c_int32), ("bucket_bits", ctypes.c_int32),
                 ("renorm_every", ctypes.c_uint32), ("fuse_window", ctypes.c_uint32),
                 ("timing", ctypes.c_uint32), ("mem_reserve_ppm", ctypes.c_uint32),
-                ("fuse_blocks", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 5)]
+                ("fuse_blocks", ctypes.c_uint32), ("batch_eager", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 4)]
 
 
 class Stats(ctypes.Structure):
@@ -274,10 +275,13 @@ class Batch:
     """Many independent circuits on one GPU, advanced together (sqe_batch_*)."""
 
     def __init__(self, prec="f64", prune_thr=0.0, bucket_bits=-1, renorm_every=0, timing=False,
-                 device=0, hbm_budget_bytes=0, max_load=0.5):
+                 device=0, hbm_budget_bytes=0, max_load=0.5, eager=False):
+        """eager: run() launches every step directly instead of replaying one
+        captured hipGraph of the whole run."""
         L = lib()
         cfg = Config()
         L.sqe_config_default(ctypes.byref(cfg), 1)
+        cfg.batch_eager = 1 if eager else 0
         cfg.prec = F32 if prec in ("f32", F32) else F64
         cfg.prune_thr = prune_thr
         cfg.bucket_bits = bucket_bits

@@ -190,3 +190,48 @@ def test_batch_snapshot_restore_repeats(oracle):
         b.remove(ids[0])
         with pytest.raises(E.SqeError):
             b.snapshot_restore()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_graph_replays_equal_eager(oracle, prec):
+    """run() replays one captured hipGraph of the whole gate sequence; every
+    replay (after a snapshot restore, with eager work and synchronisation in
+    between) must equal the directly launched run to the bit."""
+    cs = small_batch(31, 10, depth=(120, 260))
+    ren = 100 if prec == "f32" else 0
+    with E.Batch(prec=prec, renorm_every=ren, eager=True) as be, E.Batch(prec=prec, renorm_every=ren) as bg:
+        ie = [be.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        ig = [bg.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        be.snapshot_save()
+        be.run()
+        want = [be.export(i) for i in ie]
+        upd = [be.info(i)["updates"] for i in ie]
+        bg.snapshot_save()
+        for rep in range(4):
+            say("graph replay", prec, rep)
+            if rep:
+                bg.snapshot_restore()
+                bg.export(ig[rep % len(ig)])          # eager work between replays
+                bg.sync()
+            bg.run()
+            bg.sync()
+            for k, i in enumerate(ig):
+                assert_same_state(bg.export(i), want[k], atol=0.0, label="replay %d circuit %d" % (rep, i))
+                assert bg.info(i)["updates"] == upd[k]
+                assert bg.info(i)["done"] == 1
+        # a run from mid-program positions, after the set changed, is planned
+        # anew: the same sequence on both batches gives the same bits
+        c = cs[0]
+        got = []
+        for b, ids in ((be, ie), (bg, ig)):
+            b.snapshot_restore()
+            b.step(17)
+            b.remove(ids[0])
+            nid = b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"])
+            b.run()
+            got.append([b.export(i) for i in ids[1:]] + [b.export(nid)])
+        for k, (x, y) in enumerate(zip(*got)):
+            assert_same_state(y, x, atol=0.0, label="mid-run circuit %d" % k)
+        if prec == "f64":
+            o, u = oracle_of(oracle, c)
+            assert_same_state(got[1][-1], o, atol=1e-12, label="oracle")

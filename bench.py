@@ -186,9 +186,13 @@ def unfused_tail(a, w, eng):
     k = s1["sweeps"] - s0["sweeps"]
     ms = (s1["sweep_ms"] - s0["sweep_ms"]) / max(k, 1)
     mv = (s1["moved"] - s0["moved"]) / max(k, 1)
-    alg = mv * (KEY_B + amp_b)
+    model = mv * (KEY_B + amp_b)
+    # what one gate-per-launch sweep must move in this layout: every bucket's
+    # base key and its 2^MB amplitude slots, read once and written once
+    alg = 2 * s1["buckets"] * (KEY_B + (1 << s1["bucket_bits"]) * amp_b)
     traffic = traffic_from_profiles(a.prec, 1)
     return {"gates": n, "launches": k, "avg_launch_ms": ms, "algorithmic_bytes_per_launch": alg,
+            "model_bytes_per_launch": model,
             "achieved_GBps": alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0,
             "frac": alg / (ms * 1e-3) / HBM_PEAK if ms > 0 else 0.0,
             "traffic": traffic,

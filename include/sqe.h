@@ -258,6 +258,14 @@ sqe_status sqe_batch_snapshot_restore(sqe_batch* b);
 
 const char* sqe_batch_last_error(void);
 
+/* Page-locked host memory for export (and load) arrays: device-to-host copies
+ * into it run at the full host-link rate instead of through the runtime's
+ * staging of pageable memory.  No counterpart in the reference (its readout
+ * copies into a pageable vector, emulator/device/table.cuh:47-75).  Errors are
+ * reported through sqe_batch_last_error. */
+sqe_status sqe_host_alloc(uint64_t bytes, void** out);
+sqe_status sqe_host_free(void* p);
+
 #ifdef __cplusplus
 }
 #endif

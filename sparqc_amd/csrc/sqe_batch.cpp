@@ -877,29 +877,33 @@ sqe_status sqe_batch_export(sqe_batch* b, int64_t id, uint64_t* keys, double* re
         return bfail(SQE_EINVAL, "unknown circuit");
     BHIP(hipSetDevice(b->cfg.device));
     const Circ& c = b->c[id];
-    unsigned long long nb = 0;
-    BHIP(hipStreamSynchronize(b->st));
-    BHIP(hipMemcpy(&nb, b->meta + id, 8, hipMemcpyDeviceToHost));
-    const uint64_t bound = std::max<uint64_t>(nb << c.qm.mb, 1);
-    BCK(stage_fit(b, bound));
+    // staging for every amplitude slot of the circuit's region, so the bucket
+    // count need not be read back first; the total and the error word come
+    // back together
+    const uint64_t nbk = c.tiles * 64;
+    BCK(stage_fit(b, std::max<uint64_t>(nbk << c.qm.mb, 1)));
     ExportArgs xa{};
     xa.base = base_ptr(b, c); xa.amp = amp_ptr(b, c); xa.nb = b->meta + id; xa.qm = c.qm; xa.thr = b->thr;
     xa.filter = 1; xa.keys = b->xkeys; xa.re = b->xre; xa.im = b->xim; xa.cap = b->xcap; xa.total = b->total;
-    const uint32_t g = std::max<uint32_t>(1, (uint32_t)((((nb + 63) & ~63ull) + BLOCK - 1) / BLOCK));
+    const uint32_t g = std::max<uint32_t>(1, (uint32_t)((nbk + BLOCK - 1) / BLOCK));
     BCK(next_scan(b, g, &xa.scan));
     BHIP(launch_export(c.qm.mb, b->f32, b->st, g, xa));
-    unsigned long long tot = 0;
-    BHIP(hipMemcpyAsync(&tot, b->total, 8, hipMemcpyDeviceToHost, b->st));
+    unsigned long long hv[2] = {0, 0};
+    BHIP(hipMemcpyAsync(&hv[0], b->total, 8, hipMemcpyDeviceToHost, b->st));
+    BHIP(hipMemcpyAsync(&hv[1], &b->stats->err, 8, hipMemcpyDeviceToHost, b->st));
     BHIP(hipStreamSynchronize(b->st));
+    if (hv[1]) BCK(check_err(b));
+    const uint64_t tot = hv[0];
     *n = tot;
     if (tot > cap) return bfail(SQE_EINVAL, "export capacity too small (" + std::to_string(tot) + " needed)");
     if (tot) {
         if (!keys || !re || !im) return bfail(SQE_EINVAL, "null array");
-        BHIP(hipMemcpy(keys, b->xkeys, tot * 8, hipMemcpyDeviceToHost));
-        BHIP(hipMemcpy(re, b->xre, tot * 8, hipMemcpyDeviceToHost));
-        BHIP(hipMemcpy(im, b->xim, tot * 8, hipMemcpyDeviceToHost));
+        BHIP(hipMemcpyAsync(keys, b->xkeys, tot * 8, hipMemcpyDeviceToHost, b->st));
+        BHIP(hipMemcpyAsync(re, b->xre, tot * 8, hipMemcpyDeviceToHost, b->st));
+        BHIP(hipMemcpyAsync(im, b->xim, tot * 8, hipMemcpyDeviceToHost, b->st));
+        BHIP(hipStreamSynchronize(b->st));
     }
-    return check_err(b);
+    return SQE_OK;
 }
 
 sqe_status sqe_batch_remove(sqe_batch* b, int64_t id) {
@@ -1013,5 +1017,25 @@ sqe_status sqe_batch_snapshot_restore(sqe_batch* b) {
 }
 
 const char* sqe_batch_last_error(void) { return b_err.c_str(); }
+
+sqe_status sqe_host_alloc(uint64_t bytes, void** out) {
+    if (!out) return bfail(SQE_EINVAL, "null argument");
+    *out = nullptr;
+    if (!bytes) return SQE_OK;
+    if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return bfail(SQE_ENOMEM, "hipHostMalloc(" + std::to_string(bytes) + ") failed");
+    }
+    return SQE_OK;
+}
+
+sqe_status sqe_host_free(void* p) {
+    if (p && hipHostFree(p) != hipSuccess) {
+        (void)hipGetLastError();
+        return bfail(SQE_EHIP, "hipHostFree failed");
+    }
+    return SQE_OK;
+}
 
 }  // extern "C"

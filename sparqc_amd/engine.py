@@ -27,7 +27,7 @@ EXPORTS = ["sqe_config_default", "sqe_create", "sqe_destroy", "sqe_set_basis", "
            "sqe_batch_sync", "sqe_batch_info", "sqe_batch_export", "sqe_batch_remove", "sqe_batch_finished",
            "sqe_batch_usage",
            "sqe_batch_get_tuning", "sqe_batch_set_tuning", "sqe_batch_autotune", "sqe_batch_snapshot_save",
-           "sqe_batch_snapshot_restore", "sqe_batch_last_error"]
+           "sqe_batch_snapshot_restore", "sqe_batch_last_error", "sqe_host_alloc", "sqe_host_free"]
 
 
 class SqeError(RuntimeError):
@@ -137,6 +137,10 @@ def lib():
                 getattr(L, f).restype = st
         L.sqe_batch_last_error.argtypes = []
         L.sqe_batch_last_error.restype = ctypes.c_char_p
+        L.sqe_host_alloc.argtypes = [ctypes.c_uint64, ctypes.POINTER(vp)]
+        L.sqe_host_alloc.restype = st
+        L.sqe_host_free.argtypes = [vp]
+        L.sqe_host_free.restype = st
         _lib = L
     return _lib
 
@@ -149,6 +153,46 @@ def _check(rc):
 def _bcheck(rc):
     if rc != SQE_OK:
         raise SqeError(rc, lib().sqe_batch_last_error().decode())
+
+
+class PinnedArrays:
+    """Page-locked host arrays (sqe_host_alloc) for keys / re / im of up to
+    `cap` entries, reused across exports; grow() replaces them when a larger
+    export comes."""
+
+    def __init__(self, cap=0):
+        self._p = None
+        self.cap = 0
+        self.keys = self.re = self.im = None
+        self.grow(cap)
+
+    def grow(self, cap):
+        cap = int(cap)
+        if cap <= self.cap and self._p is not None:
+            return
+        self.free()
+        cap = max(cap, 1024)
+        p = ctypes.c_void_p()
+        _bcheck(lib().sqe_host_alloc(3 * 8 * cap, ctypes.byref(p)))
+        self._p = p
+        self.cap = cap
+        buf = (ctypes.c_uint8 * (3 * 8 * cap)).from_address(p.value)
+        self.keys = np.frombuffer(buf, dtype=np.uint64, count=cap, offset=0)
+        self.re = np.frombuffer(buf, dtype=np.float64, count=cap, offset=8 * cap)
+        self.im = np.frombuffer(buf, dtype=np.float64, count=cap, offset=16 * cap)
+
+    def free(self):
+        if self._p is not None:
+            self.keys = self.re = self.im = None
+            lib().sqe_host_free(self._p)
+            self._p = None
+            self.cap = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def _ptr(a, t):
@@ -348,17 +392,30 @@ class Batch:
         _bcheck(lib().sqe_batch_info(self.h, int(cid), ctypes.byref(c)))
         return {f: getattr(c, f) for f, _ in BatchCircuit._fields_}
 
-    def export(self, cid, sort=True):
+    def export(self, cid, sort=True, out=None):
+        """(keys, re, im) of circuit cid.  out: a PinnedArrays to export into
+        (grown when too small); the result is then views of its arrays, valid
+        until its next use."""
         n = ctypes.c_uint64()
-        rc = lib().sqe_batch_export(self.h, int(cid), None, None, None, 0, ctypes.byref(n))
-        if rc not in (SQE_OK, SQE_EINVAL):
+        if out is not None:
+            rc = lib().sqe_batch_export(self.h, int(cid), _ptr(out.keys, _u64p), _ptr(out.re, _f64p),
+                                        _ptr(out.im, _f64p), out.cap, ctypes.byref(n))
+            if rc == SQE_EINVAL and n.value > out.cap:
+                out.grow(n.value)
+                rc = lib().sqe_batch_export(self.h, int(cid), _ptr(out.keys, _u64p), _ptr(out.re, _f64p),
+                                            _ptr(out.im, _f64p), out.cap, ctypes.byref(n))
             _bcheck(rc)
-        cap = n.value
-        k = np.zeros(cap, dtype=np.uint64)
-        r = np.zeros(cap)
-        i = np.zeros(cap)
-        _bcheck(lib().sqe_batch_export(self.h, int(cid), _ptr(k, _u64p), _ptr(r, _f64p), _ptr(i, _f64p), cap,
-                                       ctypes.byref(n)))
+            k, r, i = out.keys, out.re, out.im
+        else:
+            rc = lib().sqe_batch_export(self.h, int(cid), None, None, None, 0, ctypes.byref(n))
+            if rc not in (SQE_OK, SQE_EINVAL):
+                _bcheck(rc)
+            cap = n.value
+            k = np.zeros(cap, dtype=np.uint64)
+            r = np.zeros(cap)
+            i = np.zeros(cap)
+            _bcheck(lib().sqe_batch_export(self.h, int(cid), _ptr(k, _u64p), _ptr(r, _f64p), _ptr(i, _f64p), cap,
+                                           ctypes.byref(n)))
         m = n.value
         k, r, i = k[:m], r[:m], i[:m]
         if sort:

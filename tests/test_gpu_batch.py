@@ -235,3 +235,19 @@ def test_graph_replays_equal_eager(oracle, prec):
         if prec == "f64":
             o, u = oracle_of(oracle, c)
             assert_same_state(got[1][-1], o, atol=1e-12, label="oracle")
+
+
+def test_export_into_pinned_arrays():
+    """Export into reused page-locked arrays (grown on demand) equals the
+    plain export."""
+    cs = small_batch(41, 6)
+    with E.Batch() as b:
+        ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        b.run()
+        pin = E.PinnedArrays(16)
+        for i in ids:
+            want = b.export(i)
+            got = b.export(i, out=pin)
+            assert_same_state(tuple(np.array(x) for x in got), want, atol=0.0, label="circuit %d" % i)
+        assert pin.cap >= max(b.info(i)["nnz"] for i in ids)
+        pin.free()

@@ -458,18 +458,19 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                 const bool bit = (base[e] & tbase) != 0;
                 double fr = bit ? c1r : c0r;
                 double fi = bit ? c1i : c0i;
-                bool pred = valid[e];
                 if constexpr (CTRL) {
-                    pred = pred && ((base[e] & cmB) == cvB);
+                    const bool pred = valid[e] && ((base[e] & cmB) == cvB);
                     fr = pred ? fr : 1.0;
                     fi = pred ? fi : 0.0;
+                    base[e] ^= pred ? flip : 0ull;
+                } else {
+                    // a padding bucket (valid false) holds zeros, so its factor
+                    // is irrelevant; its key flips too but is never stored
+                    base[e] ^= flip;
                 }
-                // a padding bucket (valid false) holds zeros: its factor is
-                // irrelevant, only its key must not change
-                const double r = pr[e] * fr - pi[e] * fi, m = pr[e] * fi + pi[e] * fr;
+                const double r = __fma_rn(pr[e], fr, -(pi[e] * fi)), m = __fma_rn(pr[e], fi, pi[e] * fr);
                 pr[e] = r;
                 pi[e] = m;
-                base[e] ^= pred ? flip : 0ull;
             });
         };
         auto base_run = [&](uint32_t jj, uint32_t n, auto ctrl_c) __attribute__((always_inline)) {
@@ -574,6 +575,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             out.norm += (double)a[e][s].x * (double)a[e][s].x + (double)a[e][s].y * (double)a[e][s].y;
         });
         if (tile[e] >= 0 && store) {
+            // (padding buckets may carry flipped keys; only valid keys are stored)
             const uint64_t b = (uint64_t)tile[e] * TILE + lane;
             static_for<S>([&](auto sc) __attribute__((always_inline)) {
                 constexpr int s = decltype(sc)::value;

@@ -57,25 +57,28 @@ struct DOp {
 //   H_SLOT_X     X on bucket qubit J: swap slot pairs
 //   H_SLOT_PERM  Y / PERM on bucket qubit J: swap slot pairs with factors
 //   H_SLOT_H     H on bucket qubit J (prunes)
-//   H_SLOT_U2    RX / RY on bucket qubit J as [[a, u01], [u10, a]] with real a
-//                (prunes)
+//   H_SLOT_RY    RY on bucket qubit J: [[c, -s], [s, c]] (prunes)
 //   H_SLOT_U     any 2x2 on bucket qubit J, 8 coefficients in a side table
 //                (prunes)
 //   H_RENORM     scale by the device-computed 1/sqrt(norm2)
 //   H_SLOT_BLOCK dense 16x16 complex block of fused bucket-qubit gates (FP32,
 //                MFMA; side table: MFMA operand table, then the FP64 matrix;
 //                c[0] = number of gates) (prunes)
+// (RX on a bucket qubit runs in H_SLOT_U: a separate handler would add a
+// path to the slot switch and push the FP64 sweep past its register budget.)
 // Coefficients c[]: DIAG f0r, f0i, f1r, f1i (Z: 1,0,-1,0; PHASE: 1,0,c,s;
 // RZ: c,-s,c,s); PERM u10r, u10i, u01r, u01i (the factor for the element
-// leaving bit 0, then for the element leaving bit 1); H f; U2 a, u01r, u01i,
-// u10r, u10i (RX: c,0,-s,0,-s; RY: c,-s,0,s,0).  With these the arithmetic of
+// leaving bit 0, then for the element leaving bit 1); H f; RY c, s (cos and
+// sin of theta/2).  With these the arithmetic of
 // a slot gate is operation for operation the reference formula of its kind
 // (products with exact 0 or +-1 only change the sign of zero results); base
 // gates differ from it only by the folding of consecutive factors (a few ulps).
 enum : uint32_t {
-    H_BASE_DIAG = 0, H_BASE_X, H_BASE_PERM, H_SLOT_DIAG, H_SLOT_X, H_SLOT_PERM, H_SLOT_H, H_SLOT_U2,
+    H_BASE_DIAG = 0, H_BASE_X, H_BASE_PERM, H_SLOT_DIAG, H_SLOT_X, H_SLOT_PERM, H_SLOT_H, H_SLOT_RY,
     H_SLOT_U, H_RENORM, H_SLOT_BLOCK
 };
+// handlers whose outputs are pruned (they mix slot pairs)
+__host__ __device__ constexpr bool branching_handler(uint32_t h) { return h >= H_SLOT_H && h != H_RENORM; }
 
 // One sweep op: 16 dwords, staged 64 at a time in the lanes of a wavefront.
 // hdr = handler | J << 4 | target << 7 | side-table index << 13; ctl = cmS | cvS << 8.

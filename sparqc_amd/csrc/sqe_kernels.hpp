@@ -31,18 +31,16 @@ __device__ __forceinline__ cf rot(float c, float s, cf v) {
     return mkf(__fmaf_rn(c, v.x, -(s * v.y)), __fmaf_rn(c, v.y, s * v.x));
 }
 
-// [[a, u01], [u10, a]] with real a: n0 = a v0 + u01 v1, n1 = u10 v0 + a v1,
-// evaluated so that RX / RY coefficients reproduce their reference formulas
-// (emulator/device/device_data.cuh:266-319) operation for operation.
-__device__ __forceinline__ void u2(double a, double u01r, double u01i, double u10r, double u10i, cd v0, cd v1,
-                                   cd& n0, cd& n1) {
-    n0 = mkd(a * v0.x + (u01r * v1.x - u01i * v1.y), a * v0.y + (u01r * v1.y + u01i * v1.x));
-    n1 = mkd((u10r * v0.x - u10i * v0.y) + a * v1.x, (u10r * v0.y + u10i * v0.x) + a * v1.y);
+// RY [[c, -s], [s, c]] with c, s of theta/2, the reference formula
+// (emulator/device/device_data.cuh:290-319) operation for operation; FP32
+// with fused multiply-add.
+__device__ __forceinline__ void ry_pair(double c, double s, cd v0, cd v1, cd& n0, cd& n1) {
+    n0 = mkd(c * v0.x - s * v1.x, c * v0.y - s * v1.y);
+    n1 = mkd(s * v0.x + c * v1.x, s * v0.y + c * v1.y);
 }
-__device__ __forceinline__ void u2(float a, float u01r, float u01i, float u10r, float u10i, cf v0, cf v1, cf& n0,
-                                   cf& n1) {
-    n0 = mkf(__fmaf_rn(a, v0.x, u01r * v1.x - u01i * v1.y), __fmaf_rn(a, v0.y, u01r * v1.y + u01i * v1.x));
-    n1 = mkf(__fmaf_rn(a, v1.x, u10r * v0.x - u10i * v0.y), __fmaf_rn(a, v1.y, u10r * v0.y + u10i * v0.x));
+__device__ __forceinline__ void ry_pair(float c, float s, cf v0, cf v1, cf& n0, cf& n1) {
+    n0 = mkf(__fmaf_rn(c, v0.x, -(s * v1.x)), __fmaf_rn(c, v0.y, -(s * v1.y)));
+    n1 = mkf(__fmaf_rn(c, v1.x, s * v0.x), __fmaf_rn(c, v1.y, s * v0.y));
 }
 __device__ __forceinline__ void hpair(double f, cd v0, cd v1, cd& n0, cd& n1) {
     n0 = mkd(f * (v0.x + v1.x), f * (v0.y + v1.y));
@@ -184,15 +182,16 @@ __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&
             });
         }, std::true_type{});
     } break;
-    case H_SLOT_U2: {
-        const R ar = (R)op.c[0], u01r = (R)op.c[1], u01i = (R)op.c[2], u10r = (R)op.c[3], u10i = (R)op.c[4];
+    case H_SLOT_RY: {
+        const R c = (R)op.c[0], sn = (R)op.c[1];
         pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
-            u2(ar, u01r, u01i, u10r, u10i, x, y, n0, n1);
+            ry_pair(c, sn, x, y, n0, n1);
             prune_pair(x, y, n0, n1, [&](cd ex, cd ey, cd& e0, cd& e1) __attribute__((always_inline)) {
-                u2(op.c[0], op.c[1], op.c[2], op.c[3], op.c[4], ex, ey, e0, e1);
+                ry_pair(op.c[0], op.c[1], ex, ey, e0, e1);
             });
         }, std::true_type{});
     } break;
+
     case H_SLOT_U: {
         DOp d{};
         for (int k = 0; k < 8; ++k) d.p[k] = coef[op.ci * 8 + k];
@@ -538,7 +537,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                         });
                     });
                 } else if constexpr (MB > 0) {
-                    const bool branch = op.h >= H_SLOT_H && op.h <= H_SLOT_BLOCK && op.h != H_RENORM;
+                    const bool branch = branching_handler(op.h);
                     if constexpr (BLOCKS && MB == 4 && sizeof(R) == 4 && EPT == 1) {
                         if (op.h == H_SLOT_BLOCK) {
                             block_apply(a[0], pr[0], pi[0], coef + (size_t)op.ci * 8, thr);

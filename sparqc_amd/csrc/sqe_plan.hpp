@@ -111,8 +111,14 @@ inline MOp encode_op(const DOp& d, std::vector<double>& coef) {
     case SQE_GATE_Y: set(0, 1, 0, -1, 0); h = slot ? H_SLOT_PERM : H_BASE_PERM; break;      // u10 = i, u01 = -i
     case SQE_GATE_PERM: set(p[2], p[3], p[0], p[1], 0); h = slot ? H_SLOT_PERM : H_BASE_PERM; break;
     case SQE_GATE_H: set(p[0], 0, 0, 0, 0); h = H_SLOT_H; break;
-    case SQE_GATE_RX: set(p[0], 0, -p[1], 0, -p[1]); h = H_SLOT_U2; break;
-    case SQE_GATE_RY: set(p[0], -p[1], 0, p[1], 0); h = H_SLOT_U2; break;
+    case SQE_GATE_RX: {
+        // the general handler: [[c, -is], [-is, c]] as 8 coefficients
+        const double u[8] = {p[0], 0.0, 0.0, -p[1], 0.0, -p[1], p[0], 0.0};
+        h = H_SLOT_U;
+        ci = (uint32_t)(coef.size() / 8);
+        coef.insert(coef.end(), u, u + 8);
+    } break;
+    case SQE_GATE_RY: set(p[0], p[1], 0, 0, 0); h = H_SLOT_RY; break;
     case SQE_GATE_U:
         h = H_SLOT_U;
         ci = (uint32_t)(coef.size() / 8);

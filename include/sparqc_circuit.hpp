@@ -12,8 +12,9 @@
 //     and the gate uses theta / 2 (table_manager.cuh:333-336; QLib.h:57-157);
 //   - errors are thrown as std::string (main.cu:44-76, table_manager.cuh:175).
 // Added: controlled gates (cx, cz, ccx and controlled forms of every gate),
-// load_state, norm2, nnz, sorted export and the unordered_map readout of
-// form_unordered_statevector_DeviceToHost (circuit.h:111-126).
+// load_state, norm2, nnz, sorted export, the unordered_map readout of
+// form_unordered_statevector_DeviceToHost (circuit.h:111-126), the pattern
+// view get_state_full (circuit.h:205-257) and restart files.
 //
 // Gates are queued by the engine and run fused on the GPU; readouts wait.
 #ifndef SPARQC_CIRCUIT_HPP
@@ -26,7 +27,9 @@
 #include <complex>
 #include <cstdio>
 #include <initializer_list>
+#include <iomanip>
 #include <iostream>
+#include <map>
 #include <sstream>
 #include <string>
 #include <unordered_map>
@@ -134,14 +137,15 @@ public:
     void sync() { check(sqe_sync(e_)); }
 
     // Nonzero entries sorted by key (entries with |re| and |im| below the
-    // prune threshold are left out, table.cuh:63-64).
-    void export_state(std::vector<uint64_t>& keys, std::vector<amp_t>& amps) {
+    // prune threshold are left out, table.cuh:63-64); with a mask, only keys
+    // with (key & mask) == value.
+    void export_state(std::vector<uint64_t>& keys, std::vector<amp_t>& amps, uint64_t mask = 0, uint64_t value = 0) {
         uint64_t n = 0;
-        const sqe_status s = sqe_export(e_, nullptr, nullptr, nullptr, 0, &n);
+        const sqe_status s = sqe_export_where(e_, mask, value, nullptr, nullptr, nullptr, 0, &n);
         if (s != SQE_OK && s != SQE_EINVAL) check(s);
         std::vector<uint64_t> k(n);
         std::vector<double> re(n), im(n);
-        check(sqe_export(e_, k.data(), re.data(), im.data(), n, &n));
+        check(sqe_export_where(e_, mask, value, k.data(), re.data(), im.data(), n, &n));
         std::vector<size_t> order(n);
         for (size_t j = 0; j < n; ++j) order[j] = j;
         std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return k[a] < k[b]; });
@@ -179,6 +183,63 @@ public:
         export_state(k, a);
         for (size_t j = 0; j < k.size(); ++j) os << amp_line(a[j], prec) << "  |" << ket(k[j]) << ">\n";
     }
+
+    // The reference's pattern view (circuit.h:205-257, QLib.cpp:300-346):
+    // state_to_choose lists one value per qubit, qubit nq-1 first (0 or 1
+    // fixes the qubit, -1 leaves it free; empty: no pattern); the matching
+    // non-negligible entries are returned in `data` (bits qubit nq-1 first ->
+    // amplitude, in key order) and printed into str_wv, one line per entry:
+    // real and imaginary parts in scientific notation with ampl_prec digits
+    // (width 9 + ampl_prec), "j", three spaces, then the ket, split into
+    // groups of organize_state[i] qubits ("|..>|..>").  The pattern is applied
+    // on the device (sqe_export_where).
+    void get_state_full(const std::vector<uint32_t>& organize_state, std::string& str_wv,
+                        std::map<std::vector<short>, amp_t>& data, const std::vector<short>& state_to_choose,
+                        uint32_t ampl_prec) {
+        if (state_to_choose.size() > nq_) throw std::string("get_state_full: pattern longer than the qubit count");
+        uint64_t mask = 0, value = 0;
+        for (size_t i = 0; i < state_to_choose.size(); ++i) {
+            if (state_to_choose[i] < 0) continue;
+            const uint32_t q = nq_ - 1 - (uint32_t)i;
+            mask |= 1ull << q;
+            if (state_to_choose[i]) value |= 1ull << q;
+        }
+        std::vector<uint64_t> k;
+        std::vector<amp_t> a;
+        export_state(k, a, mask, value);
+        data.clear();
+        for (size_t j = 0; j < k.size(); ++j) {
+            std::vector<short> bits(nq_);
+            for (uint32_t i = 0; i < nq_; ++i) bits[i] = (short)((k[j] >> (nq_ - 1 - i)) & 1ull);
+            data[bits] = a[j];
+        }
+        std::ostringstream oss;
+        oss.setf(std::ios::scientific);
+        oss.precision(ampl_prec);
+        str_wv.clear();
+        const int w = 9 + (int)ampl_prec;
+        for (const auto& kv : data) {
+            std::string st = "|";
+            uint32_t count_org = 0, prev = 0;
+            for (uint32_t jj = 0; jj < nq_; ++jj) {
+                st += std::to_string(kv.first[jj]);
+                if (!organize_state.empty() && count_org < organize_state.size() &&
+                    jj == prev + organize_state[count_org] - 1) {
+                    prev += organize_state[count_org];
+                    ++count_org;
+                    if (count_org < organize_state.size()) st += ">|";
+                }
+            }
+            st += ">";
+            oss.str(std::string());
+            oss << std::setw(w) << kv.second.real() << std::setw(w) << kv.second.imag() << "j";
+            str_wv += oss.str() + "   " + st + "\n";
+        }
+    }
+
+    // Restart file of the current state (sqe_checkpoint_save / _load).
+    void save_restart(const std::string& path) { check(sqe_checkpoint_save(e_, path.c_str())); }
+    void load_restart(const std::string& path) { check(sqe_checkpoint_load(e_, path.c_str())); }
 
     std::string ket(uint64_t key) const {
         std::string s(nq_, '0');

@@ -156,6 +156,22 @@ sqe_status sqe_norm2(sqe_engine* e, double* out);
  * Table__::get_statevector (emulator/device/table.cuh:47-75). */
 sqe_status sqe_export(sqe_engine* e, uint64_t* keys, double* re, double* im, uint64_t cap, uint64_t* n);
 
+/* sqe_export restricted to keys with (key & mask) == value, filtered on the
+ * device: the pattern view of the reference's Circuit__::get_state_full
+ * (emulator/include/circuit.h:205-257), whose state_to_choose fixes some
+ * qubits to 0 or 1 (-1: any). */
+sqe_status sqe_export_where(sqe_engine* e, uint64_t mask, uint64_t value, uint64_t* keys, double* re, double* im,
+                            uint64_t cap, uint64_t* n);
+
+/* Restart file: write the current state (keys and FP64 amplitudes of every
+ * non-negligible entry, plus the gate count) to a flat binary file / replace
+ * the engine's state with one.  Stands in for the reference's planned
+ * _RESTART.hdf5 output (emulator/include/QLib.h:39-40, 612-823); layout: a
+ * 64-byte header (magic "SQESTATE", version 1, nq, precision, count, gates),
+ * then keys[count] (u64), re[count], im[count] (f64), little-endian. */
+sqe_status sqe_checkpoint_save(sqe_engine* e, const char* path);
+sqe_status sqe_checkpoint_load(sqe_engine* e, const char* path);
+
 /* Wait for all enqueued work and surface device-side errors. */
 sqe_status sqe_sync(sqe_engine* e);
 

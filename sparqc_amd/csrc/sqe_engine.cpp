@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -421,7 +422,7 @@ sqe_status materialize(sqe_engine* e, const std::vector<sqe_gate>& gl) {
 }
 
 // Export the current store into the FP64 staging arrays; returns the count.
-sqe_status export_to_staging(sqe_engine* e, int filter, uint64_t* count) {
+sqe_status export_to_staging(sqe_engine* e, int filter, uint64_t* count, uint64_t kmask = 0, uint64_t kval = 0) {
     uint64_t nb = 0;
     SQCK(read_nb(e, &nb));
     const uint64_t bound = std::max<uint64_t>(nb * (uint64_t)e->S(), 1);
@@ -430,6 +431,7 @@ sqe_status export_to_staging(sqe_engine* e, int filter, uint64_t* count) {
     const Store& o = e->s[e->cur];
     xa.base = o.base; xa.amp = o.amp; xa.nb = &e->meta[e->cur]; xa.qm = e->qm; xa.thr = e->thr;
     xa.filter = filter; xa.keys = e->xkeys; xa.re = e->xre; xa.im = e->xim; xa.cap = e->xcap;
+    xa.kmask = kmask; xa.kval = kval;
     xa.total = &e->meta[2];
     const uint32_t g = std::max<uint32_t>(1, tiles_of(round64(nb)));
     SQCK(next_scan(e, g, &xa.scan));
@@ -971,21 +973,26 @@ sqe_status sqe_norm2(sqe_engine* e, double* out) {
     return SQE_OK;
 }
 
-sqe_status sqe_export(sqe_engine* e, uint64_t* keys, double* re, double* im, uint64_t cap, uint64_t* n) {
+sqe_status sqe_export_where(sqe_engine* e, uint64_t mask, uint64_t value, uint64_t* keys, double* re, double* im,
+                            uint64_t cap, uint64_t* n) {
     SQCK(enter(e));
     if (!n) return fail(SQE_EINVAL, "null output");
+    if (value & ~mask) return fail(SQE_EINVAL, "pattern value has bits outside its mask");
     SQCK(full_sync(e));
     if (!e->have_state) { *n = 0; return SQE_OK; }
     if (!e->on_device) {
         // nothing applied yet: filter the staged host copy
+        auto keep = [&](size_t i) {
+            return !(std::fabs(e->hre[i]) < e->thr && std::fabs(e->him[i]) < e->thr) &&
+                   (e->hkeys[i] & mask) == value;
+        };
         uint64_t c = 0;
-        for (size_t i = 0; i < e->hkeys.size(); ++i)
-            if (!(std::fabs(e->hre[i]) < e->thr && std::fabs(e->him[i]) < e->thr)) ++c;
+        for (size_t i = 0; i < e->hkeys.size(); ++i) c += keep(i) ? 1 : 0;
         *n = c;
         if (c > cap) return fail(SQE_EINVAL, "export capacity too small");
         uint64_t j = 0;
         for (size_t i = 0; i < e->hkeys.size(); ++i) {
-            if (std::fabs(e->hre[i]) < e->thr && std::fabs(e->him[i]) < e->thr) continue;
+            if (!keep(i)) continue;
             double r = e->hre[i], m = e->him[i];
             if (e->f32) { r = (double)(float)r; m = (double)(float)m; }
             keys[j] = e->hkeys[i]; re[j] = r; im[j] = m; ++j;
@@ -993,7 +1000,7 @@ sqe_status sqe_export(sqe_engine* e, uint64_t* keys, double* re, double* im, uin
         return SQE_OK;
     }
     uint64_t tot = 0;
-    SQCK(export_to_staging(e, 1, &tot));
+    SQCK(export_to_staging(e, 1, &tot, mask, value));
     *n = tot;
     if (tot > cap) return fail(SQE_EINVAL, "export capacity too small (" + std::to_string(tot) + " needed)");
     if (tot) {
@@ -1004,6 +1011,80 @@ sqe_status sqe_export(sqe_engine* e, uint64_t* keys, double* re, double* im, uin
         HIPCK(hipStreamSynchronize(e->st));
     }
     collect_times(e);
+    return SQE_OK;
+}
+
+sqe_status sqe_export(sqe_engine* e, uint64_t* keys, double* re, double* im, uint64_t cap, uint64_t* n) {
+    return sqe_export_where(e, 0, 0, keys, re, im, cap, n);
+}
+
+// Flat restart file: a 64-byte header (magic, version, qubits, precision,
+// entry count, gates applied), then the keys, the real parts and the
+// imaginary parts of the exported state, little-endian.
+namespace {
+struct CkptHeader {
+    char magic[8];
+    uint32_t version, nq, prec, pad;
+    uint64_t count, gates;
+    uint64_t reserved[3];
+};
+static_assert(sizeof(CkptHeader) == 64, "checkpoint header");
+const char CKPT_MAGIC[8] = {'S', 'Q', 'E', 'S', 'T', 'A', 'T', 'E'};
+}  // namespace
+
+sqe_status sqe_checkpoint_save(sqe_engine* e, const char* path) {
+    SQCK(enter(e));
+    if (!path) return fail(SQE_EINVAL, "null path");
+    uint64_t n = 0;
+    const sqe_status q = sqe_export(e, nullptr, nullptr, nullptr, 0, &n);   // size query
+    if (q != SQE_OK && q != SQE_EINVAL) return q;
+    std::vector<uint64_t> k(n);
+    std::vector<double> r(n), m(n);
+    uint64_t got = 0;
+    SQCK(sqe_export(e, k.data(), r.data(), m.data(), n, &got));
+    CkptHeader h{};
+    std::memcpy(h.magic, CKPT_MAGIC, 8);
+    h.version = 1;
+    h.nq = e->cfg.nq;
+    h.prec = e->cfg.prec;
+    h.count = got;
+    h.gates = e->gates;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(SQE_EINVAL, std::string("cannot open ") + path + " for writing");
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+    ok = ok && (got == 0 || (std::fwrite(k.data(), 8, got, f) == got && std::fwrite(r.data(), 8, got, f) == got &&
+                             std::fwrite(m.data(), 8, got, f) == got));
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return fail(SQE_EINVAL, std::string("write failed: ") + path);
+    return SQE_OK;
+}
+
+sqe_status sqe_checkpoint_load(sqe_engine* e, const char* path) {
+    SQCK(enter(e));
+    if (!path) return fail(SQE_EINVAL, "null path");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(SQE_EINVAL, std::string("cannot open ") + path);
+    CkptHeader h{};
+    bool ok = std::fread(&h, sizeof(h), 1, f) == 1 && std::memcmp(h.magic, CKPT_MAGIC, 8) == 0 && h.version == 1;
+    if (ok && h.nq != e->cfg.nq) {
+        std::fclose(f);
+        return fail(SQE_EINVAL, "checkpoint has " + std::to_string(h.nq) + " qubits, engine " +
+                                    std::to_string(e->cfg.nq));
+    }
+    std::vector<uint64_t> k;
+    std::vector<double> r, m;
+    if (ok) {
+        k.resize(h.count);
+        r.resize(h.count);
+        m.resize(h.count);
+        ok = h.count == 0 || (std::fread(k.data(), 8, h.count, f) == h.count &&
+                              std::fread(r.data(), 8, h.count, f) == h.count &&
+                              std::fread(m.data(), 8, h.count, f) == h.count);
+    }
+    std::fclose(f);
+    if (!ok) return fail(SQE_EINVAL, std::string("not a readable state checkpoint: ") + path);
+    SQCK(sqe_load(e, k.data(), r.data(), m.data(), h.count));
+    e->gates = h.gates;
     return SQE_OK;
 }
 

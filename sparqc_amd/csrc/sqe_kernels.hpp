@@ -900,7 +900,8 @@ __global__ __launch_bounds__(BLOCK) void k_export(ExportArgs A) {
         constexpr int s = decltype(sc)::value;
         if (valid) a[s] = amp[amp_index(b, s, S)]; else { a[s].x = 0; a[s].y = 0; }
         const cd v = mkd((double)a[s].x, (double)a[s].y);
-        const bool k = A.filter ? !negl64(v, A.thr) : nonzero(v);
+        const bool k = (A.filter ? !negl64(v, A.thr) : nonzero(v)) &&
+                       (((base | pdep_q((uint32_t)s, A.qm)) & A.kmask) == A.kval);
         if (valid && k) { keep |= 1u << s; ++count; }
     });
     unsigned long long total = 0;

@@ -92,6 +92,7 @@ struct ExportArgs {
     QMap qm;
     double thr;
     int32_t filter;
+    uint64_t kmask, kval;        // keep only keys with (key & kmask) == kval
     uint64_t* keys;
     double* re;
     double* im;

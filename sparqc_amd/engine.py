@@ -20,7 +20,8 @@ F64, F32 = 0, 1
 
 # every entry point declared in include/sqe.h
 EXPORTS = ["sqe_config_default", "sqe_create", "sqe_destroy", "sqe_set_basis", "sqe_load",
-           "sqe_apply", "sqe_apply_gates", "sqe_nnz", "sqe_norm2", "sqe_export", "sqe_sync",
+           "sqe_apply", "sqe_apply_gates", "sqe_nnz", "sqe_norm2", "sqe_export", "sqe_export_where",
+           "sqe_checkpoint_save", "sqe_checkpoint_load", "sqe_sync",
            "sqe_get_stats", "sqe_prepare", "sqe_snapshot_save", "sqe_snapshot_restore",
            "sqe_last_error",
            "sqe_batch_create", "sqe_batch_destroy", "sqe_batch_add", "sqe_batch_step", "sqe_batch_run",
@@ -107,6 +108,13 @@ def lib():
         L.sqe_norm2.restype = st
         L.sqe_export.argtypes = [vp, _u64p, _f64p, _f64p, ctypes.c_uint64, _u64p]
         L.sqe_export.restype = st
+        L.sqe_export_where.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, _u64p, _f64p, _f64p, ctypes.c_uint64,
+                                       _u64p]
+        L.sqe_export_where.restype = st
+        L.sqe_checkpoint_save.argtypes = [vp, ctypes.c_char_p]
+        L.sqe_checkpoint_save.restype = st
+        L.sqe_checkpoint_load.argtypes = [vp, ctypes.c_char_p]
+        L.sqe_checkpoint_load.restype = st
         L.sqe_sync.argtypes = [vp]
         L.sqe_sync.restype = st
         L.sqe_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
@@ -284,24 +292,32 @@ class Engine:
         _check(lib().sqe_norm2(self.h, ctypes.byref(x)))
         return x.value
 
-    def export(self, sort=True):
-        """Non-negligible entries as (keys, re, im); sorted by key by default."""
+    def export(self, sort=True, where=None):
+        """Non-negligible entries as (keys, re, im); sorted by key by default.
+        where=(mask, value): only keys with key & mask == value."""
+        mask, value = (0, 0) if where is None else (int(where[0]), int(where[1]))
         n = ctypes.c_uint64()
-        rc = lib().sqe_export(self.h, None, None, None, 0, ctypes.byref(n))
+        rc = lib().sqe_export_where(self.h, mask, value, None, None, None, 0, ctypes.byref(n))
         if rc not in (SQE_OK, SQE_EINVAL):
             _check(rc)
         cap = n.value
         k = np.zeros(cap, dtype=np.uint64)
         r = np.zeros(cap)
         i = np.zeros(cap)
-        _check(lib().sqe_export(self.h, _ptr(k, _u64p), _ptr(r, _f64p), _ptr(i, _f64p), cap,
-                                ctypes.byref(n)))
+        _check(lib().sqe_export_where(self.h, mask, value, _ptr(k, _u64p), _ptr(r, _f64p), _ptr(i, _f64p), cap,
+                                      ctypes.byref(n)))
         m = n.value
         k, r, i = k[:m], r[:m], i[:m]
         if sort:
             o = np.argsort(k, kind="stable")
             k, r, i = k[o], r[o], i[o]
         return k, r, i
+
+    def checkpoint_save(self, path):
+        _check(lib().sqe_checkpoint_save(self.h, os.fsencode(path)))
+
+    def checkpoint_load(self, path):
+        _check(lib().sqe_checkpoint_load(self.h, os.fsencode(path)))
 
     def stats(self):
         s = Stats()

@@ -7,7 +7,9 @@
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
+#include <map>
 #include <string>
+#include <unistd.h>
 #include <vector>
 
 #include "sparqc_circuit.hpp"
@@ -161,6 +163,56 @@ int main() {
                 thrown = true;
             }
             CHECK(thrown, "target out of range must throw");
+        }
+        // 7. the pattern view of get_state_full: H on 3 qubits, qubit 2 = 1,
+        //    qubit 1 free, qubit 0 = 0 -> |100> and |110>, amplitude 2^-3/2
+        {
+            Circuit__ oc(3, hp);
+            oc.h(0);
+            oc.h(1);
+            oc.h(2);
+            std::string str;
+            std::map<std::vector<short>, cd> data;
+            oc.get_state_full({1, 2}, str, data, {1, -1, 0}, 3);
+            CHECK(data.size() == 2, "pattern view: %zu entries", data.size());
+            const std::string want = "   3.536e-01   0.000e+00j   |1>|00>\n   3.536e-01   0.000e+00j   |1>|10>\n";
+            CHECK(str == want, "pattern view string:\n%s", str.c_str());
+            if (data.size() == 2) {
+                auto it = data.begin();
+                CHECK((it->first == std::vector<short>{1, 0, 0}), "first pattern entry");
+                CHECK(std::abs(it->second - cd(std::pow(2.0, -1.5), 0.0)) < 1e-15, "pattern amplitude");
+            }
+            std::string all;
+            oc.get_state_full({}, all, data, {}, 2);
+            CHECK(data.size() == 8, "no pattern: %zu entries", data.size());
+        }
+        // 8. restart files: the state and the gate count come back exactly
+        {
+            const std::string path = "/tmp/circuit_check_restart_" + std::to_string((long)getpid()) + ".bin";
+            Circuit__ a(6, hp);
+            a.h(0);
+            a.ry(3, 0.7);
+            a.cx(0, 5);
+            a.rz(5, 1.3);
+            a.save_restart(path);
+            Circuit__ b(6, hp);
+            b.load_restart(path);
+            bool same = false;
+            double d = max_diff(a, b, &same);
+            CHECK(same && d == 0.0, "restart: same keys %d, max diff %.3e", (int)same, d);
+            a.h(3);
+            b.h(3);
+            d = max_diff(a, b, &same);
+            CHECK(same && d == 0.0, "after restart and one more gate: max diff %.3e", d);
+            Circuit__ c(5, hp);
+            bool thrown = false;
+            try {
+                c.load_restart(path);
+            } catch (const std::string&) {
+                thrown = true;
+            }
+            CHECK(thrown, "a restart file of 6 qubits must not load into 5");
+            std::remove(path.c_str());
         }
     } catch (const std::string& e) {
         std::printf("FAIL exception: %s\n", e.c_str());

@@ -190,3 +190,35 @@ def test_errors_are_reported():
         e.sync()
     assert ex.value.status == E.SQE_ETABLE_FULL
     e.close()
+
+
+def test_export_where_and_checkpoint(oracle, tmp_path):
+    """The device-side pattern filter equals filtering the full export; a
+    restart file restores the state exactly and the engine continues from it."""
+    nq = 14
+    keys, re, im = random_state(77, nq, 3000)
+    gl = random_gates(78, nq, 60, p_ctrl=0.3)
+    with E.Engine(nq) as e:
+        e.load(keys, re, im)
+        e.apply(gl)
+        full = e.export()
+        for mask, value in ((0, 0), (1 << 3, 1 << 3), ((1 << 13) | 0b101, 0b100), ((1 << nq) - 1, int(full[0][0]))):
+            sel = (full[0] & np.uint64(mask)) == np.uint64(value)
+            got = e.export(where=(mask, value))
+            assert_same_state(got, tuple(x[sel] for x in full), atol=0.0, label="pattern %x/%x" % (mask, value))
+        with pytest.raises(E.SqeError):
+            e.export(where=(1, 2))
+        path = str(tmp_path / "state.bin")
+        e.checkpoint_save(path)
+        more = random_gates(79, nq, 20)
+        e.apply(more)
+        after = e.export()
+    with E.Engine(nq) as r:
+        r.checkpoint_load(path)
+        assert_same_state(r.export(), full, atol=0.0, label="restored")
+        assert r.stats()["gates"] == len(gl)
+        r.apply(more)
+        assert_same_state(r.export(), after, atol=1e-12, label="continued")
+    with E.Engine(nq + 1) as bad:
+        with pytest.raises(E.SqeError):
+            bad.checkpoint_load(path)

@@ -237,6 +237,14 @@ public:
         }
     }
 
+    // Measurement in the computational basis: `shots` keys drawn with
+    // probability |a|^2 (sqe_sample, on the device).
+    std::vector<uint64_t> sample(uint64_t shots, uint64_t seed = 0) {
+        std::vector<uint64_t> k(shots);
+        check(sqe_sample(e_, shots, seed, k.data()));
+        return k;
+    }
+
     // Restart file of the current state (sqe_checkpoint_save / _load).
     void save_restart(const std::string& path) { check(sqe_checkpoint_save(e_, path.c_str())); }
     void load_restart(const std::string& path) { check(sqe_checkpoint_load(e_, path.c_str())); }

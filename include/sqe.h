@@ -163,6 +163,14 @@ sqe_status sqe_export(sqe_engine* e, uint64_t* keys, double* re, double* im, uin
 sqe_status sqe_export_where(sqe_engine* e, uint64_t mask, uint64_t value, uint64_t* keys, double* re, double* im,
                             uint64_t cap, uint64_t* n);
 
+/* Measurement in the computational basis: `shots` keys drawn with
+ * probability |a|^2 / norm2 over the non-negligible entries, into keys[shots].
+ * Deterministic for a given state and seed (draw k uses the splitmix64 output
+ * of seed + (k+1) * 0x9E3779B97F4A7C15, top 53 bits as a uniform in [0, 1)).
+ * No counterpart in the reference (its readout is the whole state,
+ * emulator/include/circuit.h:111-126); at most 2^30 shots per call. */
+sqe_status sqe_sample(sqe_engine* e, uint64_t shots, uint64_t seed, uint64_t* keys);
+
 /* Restart file: write the current state (keys and FP64 amplitudes of every
  * non-negligible entry, plus the gate count) to a flat binary file / replace
  * the engine's state with one.  Stands in for the reference's planned

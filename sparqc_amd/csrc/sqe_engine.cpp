@@ -1018,6 +1018,37 @@ sqe_status sqe_export(sqe_engine* e, uint64_t* keys, double* re, double* im, uin
     return sqe_export_where(e, 0, 0, keys, re, im, cap, n);
 }
 
+sqe_status sqe_sample(sqe_engine* e, uint64_t shots, uint64_t seed, uint64_t* keys) {
+    SQCK(enter(e));
+    if (shots && !keys) return fail(SQE_EINVAL, "null output");
+    if (shots > (1ull << 30)) return fail(SQE_EINVAL, "at most 2^30 shots per call");
+    SQCK(full_sync(e));
+    if (!shots) return SQE_OK;
+    if (!e->have_state) return fail(SQE_EINVAL, "no state");
+    if (!e->on_device) SQCK(materialize(e, std::vector<sqe_gate>()));   // a loaded state: place it first
+    uint64_t n = 0;
+    SQCK(export_to_staging(e, 1, &n));
+    if (!n) return fail(SQE_EINVAL, "the state has no non-negligible entry");
+    const uint64_t nt = sample_tiles(n);
+    const size_t bytes = (size_t)(n + 2 * nt + 1 + shots) * 8;
+    char* scratch = nullptr;
+    SQCK(dalloc(e, (void**)&scratch, bytes));
+    double* local = (double*)scratch;
+    double* tsum = local + n;
+    double* offs = tsum + nt;
+    double* total = offs + nt;
+    uint64_t* out = (uint64_t*)(total + 1);
+    hipError_t r = launch_sample(e->st, e->xkeys, e->xre, e->xim, n, local, tsum, offs, total, seed, shots, out);
+    if (r == hipSuccess) r = hipMemcpyAsync(keys, out, shots * 8, hipMemcpyDeviceToHost, e->st);
+    if (r == hipSuccess) r = hipStreamSynchronize(e->st);
+    dfree(e, scratch, bytes);
+    if (r != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SQE_EHIP, std::string("sampling: ") + hipGetErrorString(r));
+    }
+    return SQE_OK;
+}
+
 // Flat restart file: a 64-byte header (magic, version, qubits, precision,
 // entry count, gates applied), then the keys, the real parts and the
 // imaginary parts of the exported state, little-endian.

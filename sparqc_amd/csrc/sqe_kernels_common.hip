@@ -184,6 +184,108 @@ hipError_t launch_norm_final(hipStream_t st, const double* partials, uint32_t n,
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Measurement sampling over the staged export (keys, re, im; n entries):
+// per-tile inclusive prefix sums of |a|^2, a fixed-order scan of the tile
+// sums, then one thread per shot draws a counter-based uniform and
+// binary-searches the cumulative distribution.
+// ---------------------------------------------------------------------------
+constexpr uint32_t SAMPLE_PER_THREAD = 16;
+constexpr uint32_t SAMPLE_TILE = BLOCK * SAMPLE_PER_THREAD;
+
+__global__ __launch_bounds__(BLOCK) void k_prob_tiles(const double* __restrict__ re, const double* __restrict__ im,
+                                                      uint64_t n, double* __restrict__ local,
+                                                      double* __restrict__ tile_sum) {
+    __shared__ double s_off[BLOCK];
+    const uint64_t t0 = (uint64_t)blockIdx.x * SAMPLE_TILE + (uint64_t)threadIdx.x * SAMPLE_PER_THREAD;
+    double run[SAMPLE_PER_THREAD];
+    double acc = 0.0;
+    for (uint32_t k = 0; k < SAMPLE_PER_THREAD; ++k) {
+        const uint64_t i = t0 + k;
+        if (i < n) acc += re[i] * re[i] + im[i] * im[i];
+        run[k] = acc;
+    }
+    s_off[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {                       // exclusive scan of the thread sums, in order
+        double o = 0.0;
+        for (uint32_t t = 0; t < BLOCK; ++t) {
+            const double v = s_off[t];
+            s_off[t] = o;
+            o += v;
+        }
+    }
+    __syncthreads();
+    const double off = s_off[threadIdx.x];
+    for (uint32_t k = 0; k < SAMPLE_PER_THREAD; ++k) {
+        const uint64_t i = t0 + k;
+        if (i < n) {
+            const double v = off + run[k];
+            local[i] = v;
+            if (i == n - 1 || k == SAMPLE_PER_THREAD - 1 && threadIdx.x == BLOCK - 1) tile_sum[blockIdx.x] = v;
+        }
+    }
+}
+
+// offsets[t] = sum of tile_sum[0..t) in order; total = offsets[nt-1] + tile_sum[nt-1].
+__global__ void k_prob_offsets(const double* __restrict__ tile_sum, uint32_t nt, double* __restrict__ offsets,
+                               double* __restrict__ total) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double o = 0.0;
+    for (uint32_t t = 0; t < nt; ++t) {
+        offsets[t] = o;
+        o = o + tile_sum[t];
+    }
+    *total = o;
+}
+
+__device__ __forceinline__ uint64_t splitmix_at(uint64_t seed, uint64_t k) {
+    uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sample(const uint64_t* __restrict__ keys, const double* __restrict__ local,
+                                                  const double* __restrict__ offsets, uint32_t nt,
+                                                  const double* __restrict__ total, uint64_t n, uint64_t seed,
+                                                  uint64_t shots, uint64_t* __restrict__ out) {
+    const uint64_t k = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= shots) return;
+    const double u = (double)(splitmix_at(seed, k) >> 11) * (1.0 / 9007199254740992.0);
+    const double target = u * *total;
+    // last tile whose offset is <= target
+    uint32_t lo = 0, hi = nt;                    // answer in [lo, hi)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (offsets[mid] <= target) lo = mid; else hi = mid;
+    }
+    const double base = offsets[lo];
+    const uint64_t first = (uint64_t)lo * SAMPLE_TILE;
+    const uint64_t last = first + SAMPLE_TILE < n ? first + SAMPLE_TILE : n;   // [first, last)
+    // first entry of the tile whose cumulative probability exceeds target
+    uint64_t a = first, b = last;
+    while (a < b) {
+        const uint64_t mid = (a + b) / 2;
+        if (base + local[mid] > target) b = mid; else a = mid + 1;
+    }
+    out[k] = keys[a < last ? a : last - 1];
+}
+
+hipError_t launch_sample(hipStream_t st, const uint64_t* keys, const double* re, const double* im, uint64_t n,
+                         double* local, double* tile_sum, double* offsets, double* total, uint64_t seed,
+                         uint64_t shots, uint64_t* out) {
+    if (!n || !shots) return hipSuccess;
+    const uint32_t nt = (uint32_t)((n + SAMPLE_TILE - 1) / SAMPLE_TILE);
+    hipLaunchKernelGGL(k_prob_tiles, dim3(nt), dim3(BLOCK), 0, st, re, im, n, local, tile_sum);
+    hipLaunchKernelGGL(k_prob_offsets, dim3(1), dim3(64), 0, st, (const double*)tile_sum, nt, offsets, total);
+    hipLaunchKernelGGL(k_sample, dim3((uint32_t)((shots + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, keys,
+                       (const double*)local, (const double*)offsets, nt, (const double*)total, n, seed, shots, out);
+    return hipGetLastError();
+}
+
+uint64_t sample_tiles(uint64_t n) { return (n + SAMPLE_TILE - 1) / SAMPLE_TILE; }
+
 // One wavefront per circuit: fold its items' partial sums in a fixed order.
 __global__ __launch_bounds__(BLOCK) void k_norm_batch_final(const BTask* __restrict__ tasks, uint32_t ntasks,
                                                             const double* __restrict__ partials) {

@@ -145,6 +145,12 @@ uint32_t sweep_grid(int mb, bool f32, uint64_t ntiles);   // workgroups launch_s
 hipError_t launch_pair(int mb, bool f32, uint32_t kind, hipStream_t st, uint32_t grid, const PairArgs& a);
 hipError_t launch_compact(int mb, bool f32, hipStream_t st, uint32_t grid, const CompactArgs& a);
 hipError_t launch_export(int mb, bool f32, hipStream_t st, uint32_t grid, const ExportArgs& a);
+// measurement sampling over n staged entries: `shots` keys into out; scratch
+// local[n], tile_sum / offsets[sample_tiles(n)], total[1]
+hipError_t launch_sample(hipStream_t st, const uint64_t* keys, const double* re, const double* im, uint64_t n,
+                         double* local, double* tile_sum, double* offsets, double* total, uint64_t seed,
+                         uint64_t shots, uint64_t* out);
+uint64_t sample_tiles(uint64_t n);
 hipError_t launch_norm(int mb, bool f32, hipStream_t st, uint32_t grid, const NormArgs& a, double* norm2,
                        double* scale);
 hipError_t launch_index_build(hipStream_t st, uint32_t grid, const IndexArgs& a);

@@ -21,7 +21,7 @@ F64, F32 = 0, 1
 # every entry point declared in include/sqe.h
 EXPORTS = ["sqe_config_default", "sqe_create", "sqe_destroy", "sqe_set_basis", "sqe_load",
            "sqe_apply", "sqe_apply_gates", "sqe_nnz", "sqe_norm2", "sqe_export", "sqe_export_where",
-           "sqe_checkpoint_save", "sqe_checkpoint_load", "sqe_sync",
+           "sqe_checkpoint_save", "sqe_checkpoint_load", "sqe_sample", "sqe_sync",
            "sqe_get_stats", "sqe_prepare", "sqe_snapshot_save", "sqe_snapshot_restore",
            "sqe_last_error",
            "sqe_batch_create", "sqe_batch_destroy", "sqe_batch_add", "sqe_batch_step", "sqe_batch_run",
@@ -111,6 +111,8 @@ def lib():
         L.sqe_export_where.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, _u64p, _f64p, _f64p, ctypes.c_uint64,
                                        _u64p]
         L.sqe_export_where.restype = st
+        L.sqe_sample.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, _u64p]
+        L.sqe_sample.restype = st
         L.sqe_checkpoint_save.argtypes = [vp, ctypes.c_char_p]
         L.sqe_checkpoint_save.restype = st
         L.sqe_checkpoint_load.argtypes = [vp, ctypes.c_char_p]
@@ -312,6 +314,12 @@ class Engine:
             o = np.argsort(k, kind="stable")
             k, r, i = k[o], r[o], i[o]
         return k, r, i
+
+    def sample(self, shots, seed=0):
+        """`shots` basis keys drawn with probability |a|^2 (uint64 array)."""
+        out = np.zeros(int(shots), dtype=np.uint64)
+        _check(lib().sqe_sample(self.h, int(shots), int(seed) & ((1 << 64) - 1), _ptr(out, _u64p)))
+        return out
 
     def checkpoint_save(self, path):
         _check(lib().sqe_checkpoint_save(self.h, os.fsencode(path)))

@@ -222,3 +222,74 @@ def test_export_where_and_checkpoint(oracle, tmp_path):
     with E.Engine(nq + 1) as bad:
         with pytest.raises(E.SqeError):
             bad.checkpoint_load(path)
+
+
+def _sample_restated(keys, re, im, seed, shots):
+    """The device sampler's arithmetic in numpy (same order of operations):
+    tiles of 4096 entries, 16 per thread summed in order, thread offsets and
+    tile offsets by sequential scans, a counter-based splitmix64 uniform per
+    shot, then the first entry whose cumulative |a|^2 exceeds u * total."""
+    n = keys.size
+    p = re * re + im * im
+    per_thread, tile = 16, 4096
+    nt = (n + tile - 1) // tile
+    local = np.empty(n)
+    tsum = np.empty(nt)
+    for t in range(nt):
+        seg = p[t * tile:(t + 1) * tile]
+        m = seg.size
+        pad = np.zeros(tile)
+        pad[:m] = seg
+        run = np.cumsum(pad.reshape(256, per_thread), axis=1)
+        sums = run[:, -1]
+        off = np.concatenate([[0.0], np.cumsum(sums)[:-1]])
+        loc = (off[:, None] + run).reshape(-1)[:m]
+        local[t * tile:t * tile + m] = loc
+        tsum[t] = loc[-1]
+    offs = np.concatenate([[0.0], np.cumsum(tsum)[:-1]])
+    total = np.cumsum(tsum)[-1]
+    u = (gen.SplitMix64(seed).u64_array(shots) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+    target = u * total
+    out = np.empty(shots, dtype=np.uint64)
+    for k in range(shots):
+        t = int(np.searchsorted(offs, target[k], side="right")) - 1
+        a, b = t * tile, min(n, (t + 1) * tile)
+        j = a + int(np.searchsorted(offs[t] + local[a:b], target[k], side="right"))
+        out[k] = keys[min(j, b - 1)]
+    return out
+
+
+def test_sample_measurement():
+    """Sampling: equal to the restated arithmetic draw for draw, frequencies
+    within 5 sigma of |a|^2, deterministic per seed."""
+    # three entries with probabilities 0.5 / 0.3 / 0.2
+    keys = np.array([3, 9, 12], dtype=np.uint64)
+    amp = np.sqrt(np.array([0.5, 0.3, 0.2]))
+    with E.Engine(6) as e:
+        e.load(keys, amp * np.cos([0.3, 1.0, 2.0]), amp * np.sin([0.3, 1.0, 2.0]))
+        shots = 200000
+        s = e.sample(shots, seed=5)
+        for key, p in zip(keys, [0.5, 0.3, 0.2]):
+            f = np.mean(s == key)
+            assert abs(f - p) < 5 * math.sqrt(p * (1 - p) / shots), (key, f, p)
+        assert np.array_equal(s, e.sample(shots, seed=5))
+        assert not np.array_equal(s[:1000], e.sample(1000, seed=6))
+        k, r, i = e.export(sort=False)
+        assert np.array_equal(s[:5000], _sample_restated(k, r, i, 5, 5000))
+    # a 14-qubit state of ~12000 entries (three tiles) after a circuit
+    nq = 14
+    k0, r0, i0 = random_state(91, nq, 3000)
+    gl = random_gates(92, nq, 40, kinds=[G.H, G.RY, G.X, G.RZ], p_ctrl=0.2)
+    with E.Engine(nq) as e:
+        e.load(k0, r0, i0)
+        e.apply(gl)
+        k, r, i = e.export(sort=False)
+        assert k.size > 4096
+        shots = 20000
+        s = e.sample(shots, seed=123)
+        assert np.array_equal(s, _sample_restated(k, r, i, 123, shots))
+        p = (r * r + i * i) / np.sum(r * r + i * i)
+        for q in (0, 5, 13):
+            want = float(np.sum(p[(k >> np.uint64(q)) & np.uint64(1) == 1]))
+            got = float(np.mean((s >> np.uint64(q)) & np.uint64(1)))
+            assert abs(got - want) < 5 * math.sqrt(max(want * (1 - want), 1e-6) / shots), (q, got, want)

@@ -340,7 +340,10 @@ __device__ __forceinline__ bool base_class(uint32_t h, uint32_t cmS) { return h 
 // work on the amplitude registers.  The pending factor is a scalar per bucket,
 // so it commutes with every bucket-local gate; it is multiplied in before a
 // branching op (whose prune test needs true values) and before the store.
-template <int MB, class R, int EPT, bool BLOCKS = false>
+// SREC: slot-op records are read with scalar loads from the program in
+// memory (k_sweep); otherwise from the LDS copy (k_sweep_batch, whose
+// program pointer comes from a per-wave task record).
+template <int MB, class R, int EPT, bool BLOCKS = false, bool SREC = false>
 __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* ampv, const int64_t (&tile)[EPT],
                                             const MOp* __restrict__ ops, uint32_t nops,
                                             const double* __restrict__ coef, double thr,
@@ -499,9 +502,30 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             // ---- one slot op: amplitude registers ----
             ++i;
             SOp op;
-            const uint4 q0 = sp[jj * 4], q1 = sp[jj * 4 + 1], q2 = sp[jj * 4 + 2], q3 = sp[jj * 4 + 3];
-            auto sg = [](uint32_t v) __attribute__((always_inline)) { return __builtin_amdgcn_readfirstlane(v); };
-            const uint32_t hdr = sg(q0.x), ctl = sg(q0.y);
+            uint32_t hdr, ctl;
+            if constexpr (SREC) {
+                // the record goes straight from memory into scalar registers
+                // (uniform address, program read-only during the launch)
+                const MOp* __restrict__ rec = ops + (c0 + jj);
+                hdr = rec->hdr;
+                ctl = rec->ctl;
+                op.cmB = rec->cmB;
+                op.cvB = rec->cvB;
+                for (int k = 0; k < 5; ++k) op.c[k] = rec->c[k];
+            } else {
+                // from the staged copy in LDS, moved to scalar registers
+                const uint4 q0 = sp[jj * 4], q1 = sp[jj * 4 + 1], q2 = sp[jj * 4 + 2], q3 = sp[jj * 4 + 3];
+                auto sg = [](uint32_t v) __attribute__((always_inline)) { return __builtin_amdgcn_readfirstlane(v); };
+                hdr = sg(q0.x);
+                ctl = sg(q0.y);
+                op.cmB = u64of(sg(q0.z), sg(q0.w));
+                op.cvB = u64of(sg(q1.x), sg(q1.y));
+                op.c[0] = f64of(sg(q1.z), sg(q1.w));
+                op.c[1] = f64of(sg(q2.x), sg(q2.y));
+                op.c[2] = f64of(sg(q2.z), sg(q2.w));
+                op.c[3] = f64of(sg(q3.x), sg(q3.y));
+                op.c[4] = f64of(sg(q3.z), sg(q3.w));
+            }
             op.h = hdr & 15u;
             op.cmS = ctl & 0xffu;
             {
@@ -510,13 +534,6 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                 op.j = (hdr >> 4) & 7u;
                 op.tbase = 1ull << ((hdr >> 7) & 63u);
                 op.ci = hdr >> 13;
-                op.cmB = u64of(sg(q0.z), sg(q0.w));
-                op.cvB = u64of(sg(q1.x), sg(q1.y));
-                op.c[0] = f64of(sg(q1.z), sg(q1.w));
-                op.c[1] = f64of(sg(q2.x), sg(q2.y));
-                op.c[2] = f64of(sg(q2.z), sg(q2.w));
-                op.c[3] = f64of(sg(q3.x), sg(q3.y));
-                op.c[4] = f64of(sg(q3.z), sg(q3.w));
                 bool pred[EPT];
                 static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                     constexpr int e = decltype(ec)::value;
@@ -622,7 +639,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
         any |= t < ntiles;
     });
     SweepCount cnt{0, 0, 0, 0, 0.0};
-    if (any) sweep_group<MB, R, EPT, BLOCKS>(A.base, A.amp, tile, A.ops, A.nops, A.coef, A.thr, A.scale, A.store, cnt);
+    if (any)
+        sweep_group<MB, R, EPT, BLOCKS, true>(A.base, A.amp, tile, A.ops, A.nops, A.coef, A.thr, A.scale, A.store, cnt);
     block_stats(A.bstats, cnt.upd, cnt.nnz_out, cnt.empty, A.store ? cnt.nnz_in + cnt.nnz_out : 0ull);
     if (A.norm_out) {
         // the FP64 norm of the stored state, one partial per workgroup (for a

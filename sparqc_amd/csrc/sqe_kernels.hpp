@@ -30,6 +30,13 @@ __device__ __forceinline__ cd rot(double c, double s, cd v) {
 __device__ __forceinline__ cf rot(float c, float s, cf v) {
     return mkf(__fmaf_rn(c, v.x, -(s * v.y)), __fmaf_rn(c, v.y, s * v.x));
 }
+// The same rotation with fused multiply-adds in FP64 too: for the folded
+// per-bucket factor, whose rounding already differs from gate-by-gate
+// evaluation by a few ulps.
+__device__ __forceinline__ cd rot_fma(double c, double s, cd v) {
+    return mkd(__fma_rn(c, v.x, -(s * v.y)), __fma_rn(c, v.y, s * v.x));
+}
+__device__ __forceinline__ cf rot_fma(float c, float s, cf v) { return rot(c, s, v); }
 
 // RY [[c, -s], [s, c]] with c, s of theta/2, the reference formula
 // (emulator/device/device_data.cuh:290-319) operation for operation; FP32
@@ -120,8 +127,8 @@ __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&
                 C x = a[e][s0], y = a[e][s1];
                 if constexpr (FLUSH) {
                     const R fr = (R)pr[e], fi = (R)pi[e];
-                    x = rot(fr, fi, x);
-                    y = rot(fr, fi, y);
+                    x = rot_fma(fr, fi, x);
+                    y = rot_fma(fr, fi, y);
                 }
                 C n0, n1;
                 f(x, y, n0, n1);

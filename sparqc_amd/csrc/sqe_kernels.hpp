@@ -390,12 +390,14 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
         pi[e] = 0.0;
     });
     // The program is staged 64 ops at a time through LDS (4 KB per wavefront,
-    // one coalesced 1 KB load per 16 dwords): a gate on a base qubit then reads
-    // its record with four broadcast LDS loads straight into registers that
-    // the VALU uses as operands, and a bucket-qubit gate moves its fields to
-    // scalar registers with v_readfirstlane.  The nonzero count that every gate
-    // adds to the update counter changes only at branching gates, so gates are
-    // counted in a scalar and multiplied in when the count changes.
+    // one coalesced 1 KB load per 16 dwords), with ballot masks of the
+    // base-class, base-controlled and renormalisation ops: a gate on a base
+    // qubit then reads its record with broadcast LDS loads straight into
+    // registers that the VALU uses as operands; a bucket-qubit gate takes its
+    // record into scalar registers (scalar loads with SREC, else LDS and
+    // v_readfirstlane).  The nonzero count that every gate adds to the update
+    // counter changes only at branching gates, so gates are counted in a
+    // scalar and multiplied in when the count changes.
     __shared__ uint4 s_prog[BLOCK / 64][64 * 4];
     uint4* __restrict__ sp = s_prog[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
     const uint4* __restrict__ recs = (const uint4*)ops;

@@ -293,3 +293,48 @@ def test_sample_measurement():
             want = float(np.sum(p[(k >> np.uint64(q)) & np.uint64(1) == 1]))
             got = float(np.mean((s >> np.uint64(q)) & np.uint64(1)))
             assert abs(got - want) < 5 * math.sqrt(max(want * (1 - want), 1e-6) / shots), (q, got, want)
+
+
+def test_edge_cases(oracle):
+    """Empty and all-negligible states, zero amplitudes, the top qubit of a
+    63-qubit key, keys outside the register."""
+    # empty load: gates run on nothing, export is empty
+    with E.Engine(12) as e:
+        e.load(np.zeros(0, dtype=np.uint64), np.zeros(0), np.zeros(0))
+        e.apply(G.gate_array([G.make_gate(G.H, 3), G.make_gate(G.X, 5, controls=(3,))]))
+        k, r, i = e.export()
+        assert k.size == 0 and e.nnz() == 0
+    # zero amplitudes are not entries; entries below the threshold are pruned
+    # by the first branching gate that touches them and never exported
+    with E.Engine(12) as e:
+        keys = np.array([1, 2, 4, 8], dtype=np.uint64)
+        e.load(keys, np.array([0.0, 1e-15, 0.6, 0.8]), np.array([0.0, 0.0, 0.0, 0.0]))
+        k, _, _ = e.export()
+        assert list(k) == [4, 8]
+        e.apply(G.gate_array([G.make_gate(G.H, 0), G.make_gate(G.H, 0)]))
+        got = e.export()
+        o = oracle.Sparse(12)
+        o.load(keys, np.array([0.0, 1e-15, 0.6, 0.8]), np.zeros(4))
+        o.apply(G.gate_array([G.make_gate(G.H, 0), G.make_gate(G.H, 0)]))
+        assert_same_state(got, o.export(), atol=1e-12, label="negligible entries")
+    # 63 qubits: gates on and controlled by the top qubit, against the oracle
+    nq = 63
+    top = 1 << 62
+    keys = np.array([0, 5, top, top | 5, top | (1 << 40), (1 << 61) | 3], dtype=np.uint64)
+    amp = np.linspace(0.2, 0.7, keys.size)
+    amp = amp / np.linalg.norm(amp)
+    gl = G.gate_array([G.make_gate(G.H, 62), G.make_gate(G.X, 61, controls=(62,)), G.make_gate(G.RY, 0, theta=0.4),
+                       G.make_gate(G.Z, 62, controls=(0,)), G.make_gate(G.X, 62), G.make_gate(G.H, 40)])
+    with E.Engine(nq) as e:
+        e.load(keys, amp, np.zeros(keys.size))
+        e.apply(gl)
+        got = e.export()
+    o = oracle.Sparse(nq)
+    o.load(keys, amp, np.zeros(keys.size))
+    o.apply(gl)
+    assert_same_state(got, o.export(), atol=1e-12, label="63 qubits")
+    # a key outside the register is rejected
+    with E.Engine(10) as e:
+        with pytest.raises(E.SqeError) as ex:
+            e.load(np.array([1 << 10], dtype=np.uint64), np.ones(1), np.zeros(1))
+        assert ex.value.status == E.SQE_EINVAL

@@ -157,6 +157,8 @@ struct sqe_batch {
     BTask* htask = nullptr;
     BItem* hitem = nullptr;
     uint64_t hcap_t = 0, hcap_i = 0;
+    char* hup = nullptr;                     // page-locked staging of an admission's uploads
+    uint64_t hup_cap = 0;
     double* partials = nullptr;
     uint64_t pcap = 0;
     BTask* dtask = nullptr;
@@ -676,6 +678,7 @@ sqe_status sqe_batch_destroy(sqe_batch* b) {
     drop_graph(b);
     if (b->htask) (void)hipHostFree(b->htask);
     if (b->hitem) (void)hipHostFree(b->hitem);
+    if (b->hup) (void)hipHostFree(b->hup);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->st) (void)hipStreamDestroy(b->st);
@@ -701,15 +704,11 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         if (!gate_local(gates[j], qm))
             return bfail(SQE_EINVAL, "gate " + std::to_string(j) +
                                          " crosses buckets: run this circuit with the single-circuit engine");
-    std::vector<uint64_t> k;
-    std::vector<double> r, i;
-    k.reserve(n); r.reserve(n); i.reserve(n);
+    uint64_t m = 0;                          // entries with a nonzero amplitude
     for (uint64_t j = 0; j < n; ++j) {
         if (keys[j] & ~lim) return bfail(SQE_EINVAL, "key has bits beyond nq");
-        if (re[j] == 0.0 && im[j] == 0.0) continue;
-        k.push_back(keys[j]); r.push_back(re[j]); i.push_back(im[j]);
+        m += (re[j] != 0.0 || im[j] != 0.0) ? 1 : 0;
     }
-    const uint64_t m = k.size();
     // program: gates, with a RENORM op after every renorm_every-th gate
     std::vector<MOp> prog;
     std::vector<double> coef;
@@ -772,18 +771,46 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
     ++b->generation;
     ++b->layout;
     Circ& cc = b->c[id];
-    // upload program and coefficients
-    if (cc.nops) BHIP(hipMemcpyAsync(b->prog.p + cc.prog0 * sizeof(MOp), prog.data(), cc.nops * sizeof(MOp),
-                                     hipMemcpyHostToDevice, b->st));
-    if (cc.ncoef) BHIP(hipMemcpyAsync(b->coef.p + cc.coef0 * 8, coef.data(), cc.ncoef * 8, hipMemcpyHostToDevice,
-                                      b->st));
-    // load the state into the region (insert bases, number buckets, scatter)
+    // the uploads go through one page-locked staging buffer: program,
+    // coefficients, then the nonzero entries' keys, real and imaginary parts
+    // (no copy from an earlier admission may still be reading it)
     BCK(stage_fit(b, std::max<uint64_t>(m, 1)));
-    if (m) {
-        BHIP(hipMemcpyAsync(b->xkeys, k.data(), m * 8, hipMemcpyHostToDevice, b->st));
-        BHIP(hipMemcpyAsync(b->xre, r.data(), m * 8, hipMemcpyHostToDevice, b->st));
-        BHIP(hipMemcpyAsync(b->xim, i.data(), m * 8, hipMemcpyHostToDevice, b->st));
+    BHIP(hipStreamSynchronize(b->st));
+    const uint64_t up = cc.nops * sizeof(MOp) + cc.ncoef * 8 + 24 * m;
+    if (up > b->hup_cap) {
+        if (b->hup) (void)hipHostFree(b->hup);
+        b->hup = nullptr;
+        b->hup_cap = 0;
+        const uint64_t cap = std::max<uint64_t>(p2(up), 1 << 20);
+        BHIP(hipHostMalloc((void**)&b->hup, cap, hipHostMallocDefault));
+        b->hup_cap = cap;
     }
+    char* hp = b->hup;
+    if (cc.nops) {
+        std::memcpy(hp, prog.data(), cc.nops * sizeof(MOp));
+        BHIP(hipMemcpyAsync(b->prog.p + cc.prog0 * sizeof(MOp), hp, cc.nops * sizeof(MOp), hipMemcpyHostToDevice,
+                            b->st));
+        hp += cc.nops * sizeof(MOp);
+    }
+    if (cc.ncoef) {
+        std::memcpy(hp, coef.data(), cc.ncoef * 8);
+        BHIP(hipMemcpyAsync(b->coef.p + cc.coef0 * 8, hp, cc.ncoef * 8, hipMemcpyHostToDevice, b->st));
+        hp += cc.ncoef * 8;
+    }
+    if (m) {
+        uint64_t* hk = (uint64_t*)hp;
+        double* hr = (double*)(hk + m);
+        double* hi = hr + m;
+        uint64_t q = 0;
+        for (uint64_t j = 0; j < n; ++j) {
+            if (re[j] == 0.0 && im[j] == 0.0) continue;
+            hk[q] = keys[j]; hr[q] = re[j]; hi[q] = im[j]; ++q;
+        }
+        BHIP(hipMemcpyAsync(b->xkeys, hk, m * 8, hipMemcpyHostToDevice, b->st));
+        BHIP(hipMemcpyAsync(b->xre, hr, m * 8, hipMemcpyHostToDevice, b->st));
+        BHIP(hipMemcpyAsync(b->xim, hi, m * 8, hipMemcpyHostToDevice, b->st));
+    }
+    // load the state into the region (insert bases, number buckets, scatter)
     const double ml = b->cfg.max_load > 0 && b->cfg.max_load < 0.95 ? b->cfg.max_load : 0.5;
     const uint64_t tneed = std::max<uint64_t>(1024, p2((uint64_t)std::ceil((double)m / ml) + 8));
     if (tneed > b->tcap) {

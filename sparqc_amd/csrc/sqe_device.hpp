@@ -428,14 +428,17 @@ __device__ __forceinline__ unsigned long long block_scan(uint32_t count, ScanCtl
         // nearer predecessor has not published its aggregate.  Every
         // predecessor holds an earlier ticket and is running, so the wait
         // ends; the bound only turns a broken invariant into a reported error.
+        // A status word carries its value with its flag and epoch, so the
+        // device-scope atomics can be relaxed: release / acquire would write
+        // back / invalidate the XCD's whole L2 at every publication and poll.
         unsigned long long prefix = 0;
         if (tile == 0) {
             if (lane == 0)
-                __hip_atomic_store(&sc.status[0], st_pack(ST_PRE, sc.epoch, agg), __ATOMIC_RELEASE,
+                __hip_atomic_store(&sc.status[0], st_pack(ST_PRE, sc.epoch, agg), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
         } else {
             if (lane == 0)
-                __hip_atomic_store(&sc.status[tile], st_pack(ST_AGG, sc.epoch, agg), __ATOMIC_RELEASE,
+                __hip_atomic_store(&sc.status[tile], st_pack(ST_AGG, sc.epoch, agg), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             int hi = (int)tile - 1;
             for (unsigned long long spins = 0;; ++spins) {
@@ -445,7 +448,7 @@ __device__ __forceinline__ unsigned long long block_scan(uint32_t count, ScanCtl
                 }
                 const int j = hi - lane;
                 unsigned long long st = st_pack(ST_PRE, sc.epoch, 0);     // before tile 0: empty prefix
-                if (j >= 0) st = __hip_atomic_load(&sc.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if (j >= 0) st = __hip_atomic_load(&sc.status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const uint32_t ep = (uint32_t)((st >> 40) & 0x3fffff);
                 const unsigned long long fl = st & (3ull << 62);
                 const bool ready = ep == (sc.epoch & 0x3fffff) && fl != 0;
@@ -461,7 +464,7 @@ __device__ __forceinline__ unsigned long long block_scan(uint32_t count, ScanCtl
                 hi -= 64;
             }
             if (lane == 0)
-                __hip_atomic_store(&sc.status[tile], st_pack(ST_PRE, sc.epoch, prefix + agg), __ATOMIC_RELEASE,
+                __hip_atomic_store(&sc.status[tile], st_pack(ST_PRE, sc.epoch, prefix + agg), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0) {

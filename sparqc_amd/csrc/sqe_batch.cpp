@@ -821,7 +821,9 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         b->tcap = tneed;
     }
     BCK(buf_fit(b, &b->occ, &b->occ_cap, tiles * 64));
-    BHIP(launch_fill_slots(b->st, b->tab, b->tcap));
+    // the index of this admission uses the first tneed slots of the table
+    // (kept at its largest size), so its cost follows the circuit's size
+    BHIP(launch_fill_slots(b->st, b->tab, tneed));
     BHIP(launch_fill_u64(b->st, base_ptr(b, cc), tiles * 64, EMPTY_KEY));
     BHIP(launch_fill_u64(b->st, (uint64_t*)amp_ptr(b, cc), rows * 64 * b->csize / 8, 0));
     BHIP(launch_fill_u64(b->st, (uint64_t*)b->occ, (tiles * 64 + 1) / 2, 0));
@@ -829,11 +831,11 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
     BHIP(launch_fill_u64(b->st, (uint64_t*)(b->nnz + id), 1, 0));
     LoadArgs la{};
     la.keys = b->xkeys; la.re = b->xre; la.im = b->xim; la.n = m; la.qm = qm;
-    la.tab = b->tab; la.tmask = b->tcap - 1; la.amp = amp_ptr(b, cc); la.occ = b->occ; la.stats = b->stats;
+    la.tab = b->tab; la.tmask = tneed - 1; la.amp = amp_ptr(b, cc); la.occ = b->occ; la.stats = b->stats;
     EnumArgs ea{};
-    ea.tab = b->tab; ea.tcap = b->tcap; ea.out_base = base_ptr(b, cc); ea.out_nb = b->meta + id;
+    ea.tab = b->tab; ea.tcap = tneed; ea.out_base = base_ptr(b, cc); ea.out_nb = b->meta + id;
     ea.out_cap = tiles * 64; ea.stats = b->stats;
-    const uint32_t eg = (uint32_t)((b->tcap + BLOCK - 1) / BLOCK);
+    const uint32_t eg = (uint32_t)((tneed + BLOCK - 1) / BLOCK);
     BCK(next_scan(b, eg, &ea.scan));
     BHIP(launch_load(b->f32, b->st, la, ea, eg));
     const sqe_status e = check_err(b);

@@ -94,7 +94,8 @@ struct sqe_engine {
     bool snap_valid = false;
     QMap snap_qm{};
 
-    unsigned long long* meta = nullptr;     // [0],[1] bucket counts of the stores, [2] export total, [3] snapshot count
+    unsigned long long* meta = nullptr;     // [0],[1] bucket counts of the stores, [2] export total, [3] snapshot count,
+                                            // [4] finished-workgroup count of a norm-folding sweep
     Stats* stats = nullptr;
     BlockStats* bstats = nullptr;           // per-workgroup counter slots
     uint64_t bstats_cap = 0;
@@ -518,6 +519,9 @@ sqe_status run_sweep(sqe_engine* e, const MOp* mops, const double* coef, uint64_
     if (norm_grid) {
         SQCK(ensure_partials(e, g));
         a.norm_out = e->partials;
+        a.norm_done = (unsigned int*)&e->meta[4];
+        a.norm2 = &e->dscal[0];
+        a.scale_out = &e->dscal[1];
         *norm_grid = g;
     }
     {
@@ -717,12 +721,7 @@ sqe_status flush(sqe_engine* e) {
         } break;
         case ST_PAIR: SQCK(run_pair(e, s.op)); fused_norm = 0; break;
         case ST_NORM:
-            if (fused_norm) {
-                TimeScope ts(e, 2);
-                HIPCK(launch_norm_final(e->st, e->partials, fused_norm, &e->dscal[0], &e->dscal[1]));
-            } else {
-                SQCK(run_norm(e));
-            }
+            if (!fused_norm) SQCK(run_norm(e));   // else the sweep's last workgroup folded the norm
             fused_norm = 0;
             break;
         }

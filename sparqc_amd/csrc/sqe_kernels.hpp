@@ -659,10 +659,39 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
         for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
         if ((threadIdx.x & 63) == 0) s_norm[threadIdx.x >> 6] = v;
         __syncthreads();
+        __shared__ unsigned int s_last;
         if (threadIdx.x == 0) {
             double t = 0.0;
             for (int w = 0; w < BLOCK / 64; ++w) t += s_norm[w];
-            A.norm_out[blockIdx.x] = t;
+            // The partial goes out with a device-coherent store and is
+            // complete (vmcnt 0) before this workgroup counts itself done;
+            // the last workgroup reads the partials with device-coherent loads.
+            // (A release fence would also write back the XCD's whole L2.)
+            __hip_atomic_store(&A.norm_out[blockIdx.x], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0x0F70);                  // vmcnt(0)
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            s_last = __hip_atomic_fetch_add(A.norm_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                     gridDim.x - 1;
+        }
+        __syncthreads();
+        if (s_last) {
+            // every other workgroup's partial is in: fold them here (same
+            // fixed order as a separate reduction kernel)
+            __shared__ double s_w[BLOCK / 64];
+            double acc = 0.0;
+            for (uint32_t i = threadIdx.x; i < gridDim.x; i += BLOCK)
+                acc += __hip_atomic_load(&A.norm_out[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
+            if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double t = 0.0;
+                for (int i = 0; i < BLOCK / 64; ++i) t += s_w[i];
+                *A.norm2 = t;
+                *A.scale_out = t > 0.0 ? 1.0 / sqrt(t) : 1.0;
+                __hip_atomic_store(A.norm_done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
     }
 }

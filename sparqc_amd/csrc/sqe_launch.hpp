@@ -17,7 +17,11 @@ struct SweepArgs {
     uint32_t nops;
     int32_t store;
     int32_t blocks;         // the program holds fused blocks (FP32, 16 slots)
-    double* norm_out;       // non-null: per-workgroup FP64 partial norms of the stored state
+    double* norm_out;       // non-null: per-workgroup FP64 partial norms of the stored state;
+                            // the last workgroup to finish folds them into *norm2 / *scale_out
+    unsigned int* norm_done;// workgroups finished (zero between launches; the last one resets it)
+    double* norm2;
+    double* scale_out;
     double thr;
     const double* scale;
     Stats* stats;

@@ -69,6 +69,10 @@ __device__ __forceinline__ bool near32(cf n, double thr) {
     const float m = fmaxf(fabsf(n.x), fabsf(n.y));
     return m < (float)(4.0 * thr) && m >= (float)(0.25 * thr);
 }
+__device__ __forceinline__ bool keep32(cf n, double thr, bool keep_exact) {
+    const float m = fmaxf(fabsf(n.x), fabsf(n.y));
+    return m >= (float)(4.0 * thr) ? true : (m < (float)(0.25 * thr) ? false : keep_exact);
+}
 __device__ __forceinline__ cf prune32(cf n, double thr, bool keep_exact) {
     const float m = fmaxf(fabsf(n.x), fabsf(n.y));
     const bool keep = m >= (float)(4.0 * thr) ? true : (m < (float)(0.25 * thr) ? false : keep_exact);
@@ -105,10 +109,14 @@ struct PairIdx {
 // base controls (the lane predicate): the pair's new values are computed and
 // selected, still without branches, so no pair leaves a merge of old and new
 // register copies behind.
+// Returns true when it counted the bucket's nonzero outputs into nzc (FP64
+// H / RY handlers without controls, from their prune decisions; in FP32 the
+// extra live counter cost more than the recount it saves).
 template <int MB, class R, int EPT, int J, bool MASK>
-__device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
+__device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
                                              const bool (&pred)[EPT], const double (&pr)[EPT],
-                                             const double (&pi)[EPT], double thr, const double* __restrict__ coef) {
+                                             const double (&pi)[EPT], double thr, const double* __restrict__ coef,
+                                             uint32_t (&nzc)[EPT]) {
     using C = typename CT<R>::T;
     constexpr int HALF = (1 << MB) / 2;
     constexpr bool F32 = sizeof(R) == 4;
@@ -131,7 +139,7 @@ __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&
                     y = rot_fma(fr, fi, y);
                 }
                 C n0, n1;
-                f(x, y, n0, n1);
+                f(x, y, n0, n1, nzc[e]);
                 if constexpr (MASK) {
                     const bool m = (((uint32_t)s0 & op.cmS) == op.cvS) && (pall || pred[e]);
                     a[e][s0] = csel(m, n0, x);
@@ -143,94 +151,107 @@ __device__ __forceinline__ void slot_handler(const SOp& op, typename CT<R>::T (&
             });
         });
     };
-    auto prune_pair = [&](C x, C y, C& n0, C& n1, auto&& exact) __attribute__((always_inline)) {
+    // prune both outputs and count the ones kept (an exact zero is never kept)
+    auto prune_pair = [&](C x, C y, C& n0, C& n1, uint32_t& kept, auto&& exact) __attribute__((always_inline)) {
         if constexpr (F32) {
+            bool k0e = true, k1e = true;
             if (near32(n0, thr) || near32(n1, thr)) {
                 cd e0, e1;
                 exact(mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
-                n0 = prune32(n0, thr, !negl64(e0, thr));
-                n1 = prune32(n1, thr, !negl64(e1, thr));
-            } else {
-                n0 = prune32(n0, thr, true);
-                n1 = prune32(n1, thr, true);
+                k0e = !negl64(e0, thr);
+                k1e = !negl64(e1, thr);
             }
+            const bool k0 = keep32(n0, thr, k0e), k1 = keep32(n1, thr, k1e);
+            n0 = k0 ? n0 : mkf(0.0f, 0.0f);
+            n1 = k1 ? n1 : mkf(0.0f, 0.0f);
+            kept += (k0 ? 1u : 0u) + (k1 ? 1u : 0u);
         } else {
-            n0 = prune(n0, thr);
-            n1 = prune(n1, thr);
+            const bool k0 = !negl64(n0, thr), k1 = !negl64(n1, thr);
+            n0 = k0 ? n0 : mkd(0.0, 0.0);
+            n1 = k1 ? n1 : mkd(0.0, 0.0);
+            kept += (k0 ? 1u : 0u) + (k1 ? 1u : 0u);
         }
     };
     switch (op.h) {
     case H_SLOT_DIAG: {
         const R f0r = (R)op.c[0], f0i = (R)op.c[1], f1r = (R)op.c[2], f1i = (R)op.c[3];
-        pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+        pairs([&](C x, C y, C& n0, C& n1, uint32_t&) __attribute__((always_inline)) {
             n0 = rot(f0r, f0i, x);
             n1 = rot(f1r, f1i, y);
         }, std::false_type{});
     } break;
     case H_SLOT_X:
-        pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+        pairs([&](C x, C y, C& n0, C& n1, uint32_t&) __attribute__((always_inline)) {
             n0 = y;
             n1 = x;
         }, std::false_type{});
         break;
     case H_SLOT_PERM: {
         const R u10r = (R)op.c[0], u10i = (R)op.c[1], u01r = (R)op.c[2], u01i = (R)op.c[3];
-        pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+        pairs([&](C x, C y, C& n0, C& n1, uint32_t&) __attribute__((always_inline)) {
             n0 = rot(u01r, u01i, y);
             n1 = rot(u10r, u10i, x);
         }, std::false_type{});
     } break;
     case H_SLOT_H: {
         const R f = (R)op.c[0];
-        pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) { nzc[decltype(ec)::value] = 0; });
+        pairs([&](C x, C y, C& n0, C& n1, uint32_t& k) __attribute__((always_inline)) {
             hpair(f, x, y, n0, n1);
-            prune_pair(x, y, n0, n1, [&](cd ex, cd ey, cd& e0, cd& e1) __attribute__((always_inline)) {
+            prune_pair(x, y, n0, n1, k, [&](cd ex, cd ey, cd& e0, cd& e1) __attribute__((always_inline)) {
                 hpair(op.c[0], ex, ey, e0, e1);
             });
         }, std::true_type{});
-    } break;
+        return !MASK && !F32;
+    }
     case H_SLOT_RY: {
         const R c = (R)op.c[0], sn = (R)op.c[1];
-        pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) { nzc[decltype(ec)::value] = 0; });
+        pairs([&](C x, C y, C& n0, C& n1, uint32_t& k) __attribute__((always_inline)) {
             ry_pair(c, sn, x, y, n0, n1);
-            prune_pair(x, y, n0, n1, [&](cd ex, cd ey, cd& e0, cd& e1) __attribute__((always_inline)) {
+            prune_pair(x, y, n0, n1, k, [&](cd ex, cd ey, cd& e0, cd& e1) __attribute__((always_inline)) {
                 ry_pair(op.c[0], op.c[1], ex, ey, e0, e1);
             });
         }, std::true_type{});
-    } break;
+        return !MASK && !F32;
+    }
 
     case H_SLOT_U: {
         DOp d{};
         for (int k = 0; k < 8; ++k) d.p[k] = coef[op.ci * 8 + k];
-        pairs([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+        pairs([&](C x, C y, C& n0, C& n1, uint32_t&) __attribute__((always_inline)) {
             branch_pair<KU>(d, thr, x, y, n0, n1);
         }, std::true_type{});
     } break;
     default:
         break;
     }
+    return false;
 }
 
 template <int MB, class R, int EPT, bool MASK>
-__device__ __forceinline__ void slot_dispatch_j(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
+__device__ __forceinline__ bool slot_dispatch_j(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
                                                 const bool (&pred)[EPT], const double (&pr)[EPT],
-                                                const double (&pi)[EPT], double thr, const double* __restrict__ coef) {
+                                                const double (&pi)[EPT], double thr, const double* __restrict__ coef,
+                                                uint32_t (&nzc)[EPT]) {
     switch (op.j) {
-    case 0: if constexpr (MB > 0) slot_handler<MB, R, EPT, 0, MASK>(op, a, pred, pr, pi, thr, coef); break;
-    case 1: if constexpr (MB > 1) slot_handler<MB, R, EPT, 1, MASK>(op, a, pred, pr, pi, thr, coef); break;
-    case 2: if constexpr (MB > 2) slot_handler<MB, R, EPT, 2, MASK>(op, a, pred, pr, pi, thr, coef); break;
-    case 3: if constexpr (MB > 3) slot_handler<MB, R, EPT, 3, MASK>(op, a, pred, pr, pi, thr, coef); break;
-    case 4: if constexpr (MB > 4) slot_handler<MB, R, EPT, 4, MASK>(op, a, pred, pr, pi, thr, coef); break;
+    case 0: if constexpr (MB > 0) return slot_handler<MB, R, EPT, 0, MASK>(op, a, pred, pr, pi, thr, coef, nzc); break;
+    case 1: if constexpr (MB > 1) return slot_handler<MB, R, EPT, 1, MASK>(op, a, pred, pr, pi, thr, coef, nzc); break;
+    case 2: if constexpr (MB > 2) return slot_handler<MB, R, EPT, 2, MASK>(op, a, pred, pr, pi, thr, coef, nzc); break;
+    case 3: if constexpr (MB > 3) return slot_handler<MB, R, EPT, 3, MASK>(op, a, pred, pr, pi, thr, coef, nzc); break;
+    case 4: if constexpr (MB > 4) return slot_handler<MB, R, EPT, 4, MASK>(op, a, pred, pr, pi, thr, coef, nzc); break;
     default: break;
     }
+    return false;
 }
 
 template <int MB, class R, int EPT>
-__device__ __forceinline__ void slot_dispatch(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
+__device__ __forceinline__ bool slot_dispatch(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
                                               const bool (&pred)[EPT], const double (&pr)[EPT],
-                                              const double (&pi)[EPT], double thr, const double* __restrict__ coef) {
-    if (op.cmS == 0 && op.cmB == 0) slot_dispatch_j<MB, R, EPT, false>(op, a, pred, pr, pi, thr, coef);
-    else slot_dispatch_j<MB, R, EPT, true>(op, a, pred, pr, pi, thr, coef);
+                                              const double (&pi)[EPT], double thr, const double* __restrict__ coef,
+                                              uint32_t (&nzc)[EPT]) {
+    if (op.cmS == 0 && op.cmB == 0) return slot_dispatch_j<MB, R, EPT, false>(op, a, pred, pr, pi, thr, coef, nzc);
+    return slot_dispatch_j<MB, R, EPT, true>(op, a, pred, pr, pi, thr, coef, nzc);
 }
 
 // Per-lane factor of a base-qubit DIAG / PERM op (identity where the base
@@ -564,15 +585,17 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                     });
                 } else if constexpr (MB > 0) {
                     const bool branch = branching_handler(op.h);
+                    uint32_t nzc[EPT];
+                    bool counted = false;
                     if constexpr (BLOCKS && MB == 4 && sizeof(R) == 4 && EPT == 1) {
                         if (op.h == H_SLOT_BLOCK) {
                             block_apply(a[0], pr[0], pi[0], coef + (size_t)op.ci * 8, thr);
                             pend += (uint32_t)op.c[0] - 1;       // the block's gates (lower bound of their nonzeros)
                         } else {
-                            slot_dispatch<MB, R, EPT>(op, a, pred, pr, pi, thr, coef);
+                            counted = slot_dispatch<MB, R, EPT>(op, a, pred, pr, pi, thr, coef, nzc);
                         }
                     } else {
-                        slot_dispatch<MB, R, EPT>(op, a, pred, pr, pi, thr, coef);
+                        counted = slot_dispatch<MB, R, EPT>(op, a, pred, pr, pi, thr, coef, nzc);
                     }
                     if (branch) {
                         // the branching handlers multiplied the pending factor in
@@ -581,7 +604,8 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                             constexpr int e = decltype(ec)::value;
                             pr[e] = 1.0;
                             pi[e] = 0.0;
-                            cnt[e] = valid[e] ? count_nz<MB, R>(a[e]) : 0u;
+                            // (a padding bucket holds zeros: its kept count is 0)
+                            cnt[e] = counted ? nzc[e] : (valid[e] ? count_nz<MB, R>(a[e]) : 0u);
                         });
                     }
                 }

@@ -938,7 +938,9 @@ sqe_status sqe_batch_export(sqe_batch* b, int64_t id, uint64_t* keys, double* re
 sqe_status sqe_batch_remove(sqe_batch* b, int64_t id) {
     if (!b || id < 0 || (uint64_t)id >= b->c.size() || !b->c[id].live) return bfail(SQE_EINVAL, "unknown circuit");
     BHIP(hipSetDevice(b->cfg.device));
-    BHIP(hipStreamSynchronize(b->st));
+    // no wait: work already enqueued on the region (an export) runs before
+    // anything a later admission enqueues there, and pool reallocation
+    // synchronises before it frees
     release(b, (uint32_t)id);
     return SQE_OK;
 }

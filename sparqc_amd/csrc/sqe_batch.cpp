@@ -187,6 +187,8 @@ struct sqe_batch {
     unsigned long long* gcounts = nullptr;
     double* gpartials = nullptr;
     uint64_t gcap_t = 0, gcap_i = 0, gcap_c = 0, gcap_p = 0;
+    std::vector<hipStream_t> side;           // branches of the captured graph (default priority)
+    std::vector<hipEvent_t> gev;             // fork / join events of the capture
     // tuning and counters
     uint32_t block = BLOCK, gran = 1;
     double sweep_ms = 0.0;
@@ -431,27 +433,27 @@ void pack(std::vector<Group>& gs, std::vector<BTask>& T, std::vector<BItem>& I, 
     }
 }
 
-// Enqueue one group on the stream (no host synchronisation unless `timed`).
-sqe_status launch_group(sqe_batch* b, const Group& g, const BTask* dt, const BItem* di,
+// Enqueue one group on stream st (no host synchronisation unless `timed`).
+sqe_status launch_group(sqe_batch* b, hipStream_t st, const Group& g, const BTask* dt, const BItem* di,
                         unsigned long long* counts, double* partials, bool timed) {
     const BTask* t = dt + g.task_off;
     const BItem* it = di + g.item_off;
     if (!g.norm) {
         BatchArgs a{t, it, (uint32_t)g.items.size(), b->gran, b->thr, nullptr, counts};
-        if (timed) BHIP(hipEventRecord(b->ev0, b->st));
-        BHIP(launch_sweep_batch(g.mb, b->f32, b->st, b->block, a));
+        if (timed) BHIP(hipEventRecord(b->ev0, st));
+        BHIP(launch_sweep_batch(g.mb, b->f32, st, b->block, a));
         if (timed) {
-            BHIP(hipEventRecord(b->ev1, b->st));
+            BHIP(hipEventRecord(b->ev1, st));
             BHIP(hipEventSynchronize(b->ev1));
             float ms = 0.f;
             BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
             b->sweep_ms += ms;
         }
-        BHIP(launch_batch_fold(b->st, t, (uint32_t)g.tasks.size(), counts));
+        BHIP(launch_batch_fold(st, t, (uint32_t)g.tasks.size(), counts));
     } else {
         BatchArgs a{t, it, (uint32_t)g.items.size(), NORM_TILES, b->thr, partials, nullptr};
         const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(4096, ((uint32_t)g.items.size() + 3) / 4));
-        BHIP(launch_norm_batch(g.mb, b->f32, b->st, grid, (uint32_t)g.tasks.size(), a));
+        BHIP(launch_norm_batch(g.mb, b->f32, st, grid, (uint32_t)g.tasks.size(), a));
     }
     return SQE_OK;
 }
@@ -491,7 +493,7 @@ sqe_status step_all(sqe_batch* b, uint64_t max_gates) {
     BHIP(hipMemcpyAsync(b->dtask, b->htask, T.size() * sizeof(BTask), hipMemcpyHostToDevice, b->st));
     BHIP(hipMemcpyAsync(b->ditem, b->hitem, I.size() * sizeof(BItem), hipMemcpyHostToDevice, b->st));
     for (const Group& g : gs) {
-        BCK(launch_group(b, g, b->dtask, b->ditem, b->counts, b->partials, b->cfg.timing != 0));
+        BCK(launch_group(b, b->st, g, b->dtask, b->ditem, b->counts, b->partials, b->cfg.timing != 0));
         if (!g.norm) ++b->launches;
     }
     for (size_t i = 0; i < b->c.size(); ++i) b->c[i].pos = pos[i];
@@ -532,17 +534,51 @@ sqe_status run_graph(sqe_batch* b) {
         pack(gs, T, I, &ms, &mn);
         BCK(buf_fit(b, &b->gtask, &b->gcap_t, T.size()));
         BCK(buf_fit(b, &b->gitem, &b->gcap_i, I.size()));
-        BCK(buf_fit(b, &b->gcounts, &b->gcap_c, std::max<uint64_t>(1, 2 * ms)));
-        BCK(buf_fit(b, &b->gpartials, &b->gcap_p, std::max<uint64_t>(1, mn)));
+        // every group has its own counter / partial slots (groups of
+        // different bucket sizes run concurrently)
+        BCK(buf_fit(b, &b->gcounts, &b->gcap_c, std::max<uint64_t>(1, 2 * I.size())));
+        BCK(buf_fit(b, &b->gpartials, &b->gcap_p, std::max<uint64_t>(1, I.size())));
+        (void)ms;
+        (void)mn;
         BHIP(hipStreamSynchronize(b->st));
         BHIP(hipMemcpy(b->gtask, T.data(), T.size() * sizeof(BTask), hipMemcpyHostToDevice));
         BHIP(hipMemcpy(b->gitem, I.data(), I.size() * sizeof(BItem), hipMemcpyHostToDevice));
+        // Circuits of different bucket sizes never share work, so each bucket
+        // size is its own chain of launches (steps in order) on its own
+        // branch of the graph: a small chain overlaps the large one instead
+        // of waiting for its tail.
+        std::vector<std::vector<const Group*>> chains(6);
+        for (const Group& g : gs) chains[g.mb].push_back(&g);
+        int nch = 0;
+        for (const auto& c : chains) nch += c.empty() ? 0 : 1;
+        while ((int)b->side.size() < nch - 1) {
+            hipStream_t s2 = nullptr;
+            BHIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+            b->side.push_back(s2);
+        }
+        while ((int)b->gev.size() < nch) {
+            hipEvent_t ev = nullptr;
+            BHIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            b->gev.push_back(ev);
+        }
         BHIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
         sqe_status st = SQE_OK;
-        for (const Group& g : gs) {
-            st = launch_group(b, g, b->gtask, b->gitem, b->gcounts, b->gpartials, false);
-            if (st != SQE_OK) break;
+        hipError_t he = hipEventRecord(b->gev[0], b->st);          // fork
+        int ci = 0;
+        for (const auto& c : chains) {
+            if (c.empty()) continue;
+            hipStream_t s2 = ci == 0 ? b->st : b->side[ci - 1];
+            if (ci > 0 && he == hipSuccess) he = hipStreamWaitEvent(s2, b->gev[0], 0);
+            for (const Group* g : c) {
+                if (st != SQE_OK || he != hipSuccess) break;
+                st = launch_group(b, s2, *g, b->gtask, b->gitem, b->gcounts + 2 * g->item_off,
+                                  b->gpartials + g->item_off, false);
+            }
+            if (ci > 0 && he == hipSuccess) he = hipEventRecord(b->gev[ci], s2);     // join
+            ++ci;
         }
+        for (int k = 1; k < ci && he == hipSuccess; ++k) he = hipStreamWaitEvent(b->st, b->gev[k], 0);
+        if (st == SQE_OK && he != hipSuccess) st = bfail(SQE_EHIP, std::string("graph fork/join: ") + hipGetErrorString(he));
         hipGraph_t graph = nullptr;
         const hipError_t ec = hipStreamEndCapture(b->st, &graph);
         if (st != SQE_OK || ec != hipSuccess) {
@@ -681,6 +717,8 @@ sqe_status sqe_batch_destroy(sqe_batch* b) {
     if (b->hup) (void)hipHostFree(b->hup);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (hipStream_t s2 : b->side) (void)hipStreamDestroy(s2);
+    for (hipEvent_t ev : b->gev) (void)hipEventDestroy(ev);
     if (b->st) (void)hipStreamDestroy(b->st);
     delete b;
     return SQE_OK;

@@ -753,10 +753,10 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
     std::vector<uint32_t> renorm_at;
     prog.reserve(ng + 8);
     for (uint64_t j = 0; j < ng; ++j) {
-        prog.push_back(encode_op(plan_op(gates[j], qm), coef));
+        prog.push_back(encode_op(plan_op(gates[j], qm), coef, b->f32));
         if (b->cfg.renorm_every && (j + 1) % b->cfg.renorm_every == 0) {
             renorm_at.push_back((uint32_t)prog.size());
-            prog.push_back(encode_op(renorm_op(), coef));
+            prog.push_back(encode_op(renorm_op(), coef, b->f32));
         }
     }
     // region sizes and admission

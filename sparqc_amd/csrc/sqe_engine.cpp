@@ -663,7 +663,7 @@ sqe_status flush(sqe_engine* e) {
     std::vector<double> coef;
     mops.reserve(prog.size());
     for (const auto& d : prog)
-        mops.push_back(d.kind == K_BLOCK ? encode_block(d, blocks[(size_t)d.p[0]], coef) : encode_op(d, coef));
+        mops.push_back(d.kind == K_BLOCK ? encode_block(d, blocks[(size_t)d.p[0]], coef) : encode_op(d, coef, e->f32));
     const size_t mbytes = mops.size() * sizeof(MOp);
     const size_t cbytes = coef.size() * sizeof(double);
     const size_t coff = (mbytes + 255) & ~(size_t)255;

@@ -114,8 +114,8 @@ struct PairIdx {
 // extra live counter cost more than the recount it saves).
 template <int MB, class R, int EPT, int J, bool MASK>
 __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
-                                             const bool (&pred)[EPT], const double (&pr)[EPT],
-                                             const double (&pi)[EPT], double thr, const double* __restrict__ coef,
+                                             const bool (&pred)[EPT], const R (&pr)[EPT],
+                                             const R (&pi)[EPT], double thr, const double* __restrict__ coef,
                                              uint32_t (&nzc)[EPT]) {
     using C = typename CT<R>::T;
     constexpr int HALF = (1 << MB) / 2;
@@ -231,8 +231,8 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
 
 template <int MB, class R, int EPT, bool MASK>
 __device__ __forceinline__ bool slot_dispatch_j(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
-                                                const bool (&pred)[EPT], const double (&pr)[EPT],
-                                                const double (&pi)[EPT], double thr, const double* __restrict__ coef,
+                                                const bool (&pred)[EPT], const R (&pr)[EPT],
+                                                const R (&pi)[EPT], double thr, const double* __restrict__ coef,
                                                 uint32_t (&nzc)[EPT]) {
     switch (op.j) {
     case 0: if constexpr (MB > 0) return slot_handler<MB, R, EPT, 0, MASK>(op, a, pred, pr, pi, thr, coef, nzc); break;
@@ -247,8 +247,8 @@ __device__ __forceinline__ bool slot_dispatch_j(const SOp& op, typename CT<R>::T
 
 template <int MB, class R, int EPT>
 __device__ __forceinline__ bool slot_dispatch(const SOp& op, typename CT<R>::T (&a)[EPT][1 << MB],
-                                              const bool (&pred)[EPT], const double (&pr)[EPT],
-                                              const double (&pi)[EPT], double thr, const double* __restrict__ coef,
+                                              const bool (&pred)[EPT], const R (&pr)[EPT],
+                                              const R (&pi)[EPT], double thr, const double* __restrict__ coef,
                                               uint32_t (&nzc)[EPT]) {
     if (op.cmS == 0 && op.cmB == 0) return slot_dispatch_j<MB, R, EPT, false>(op, a, pred, pr, pi, thr, coef, nzc);
     return slot_dispatch_j<MB, R, EPT, true>(op, a, pred, pr, pi, thr, coef, nzc);
@@ -267,12 +267,10 @@ __device__ __forceinline__ void base_factor(const SOp& op, bool b, bool pred, do
 // FP64: the reference's phase formula (emulator/device/device_data.cuh:221-224)
 // with the folded factor.
 template <int MB>
-__device__ __forceinline__ void apply_phase(Cplx<float> (&a)[1 << MB], double pr, double pi) {
+__device__ __forceinline__ void apply_phase(Cplx<float> (&a)[1 << MB], float pr, float pi) {
     static_for<(1 << MB)>([&](auto sc) __attribute__((always_inline)) {
         constexpr int s = decltype(sc)::value;
-        const double x = a[s].x, y = a[s].y;
-        a[s].x = (float)(pr * x - pi * y);
-        a[s].y = (float)(pr * y + pi * x);
+        a[s] = rot(pr, pi, a[s]);
     });
 }
 template <int MB>
@@ -293,14 +291,14 @@ __device__ __forceinline__ void apply_phase(Cplx<double> (&a)[1 << MB], double p
 // within 4x of the threshold are decided on an FP64 recomputation from the
 // same inputs.  Needs all 64 lanes active (wavefront-uniform control flow).
 typedef float v16f __attribute__((ext_vector_type(16)));
-__device__ __forceinline__ void block_apply(Cplx<float> (&a)[16], double pr, double pi, const double* __restrict__ side,
+__device__ __forceinline__ void block_apply(Cplx<float> (&a)[16], float pr, float pi, const double* __restrict__ side,
                                             double thr) {
     const uint32_t lane = threadIdx.x & 63;
     const float* __restrict__ tab = (const float*)side;          // 16 steps x 64 lanes
     const double* __restrict__ m64 = side + 512;                  // 16x16 complex, FP64
     float x[32];
     {
-        const float fr = (float)pr, fi = (float)pi;
+        const float fr = pr, fi = pi;
 #pragma unroll
         for (int s = 0; s < 16; ++s) {
             const Cplx<float> v = rot(fr, fi, a[s]);
@@ -384,7 +382,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
     uint64_t base[EPT], base0[EPT];
     bool valid[EPT];
     uint32_t cnt[EPT];
-    double pr[EPT], pi[EPT];          // pending per-bucket factor
+    R pr[EPT], pi[EPT];               // pending per-bucket factor (in the amplitude precision)
     out.upd = 0;
     out.nnz_in = 0;
     static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
@@ -407,8 +405,8 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
         valid[e] = base[e] != EMPTY_KEY;
         cnt[e] = valid[e] ? count_nz<MB, R>(a[e]) : 0u;
         out.nnz_in += cnt[e];
-        pr[e] = 1.0;
-        pi[e] = 0.0;
+        pr[e] = (R)1;
+        pi[e] = (R)0;
     });
     // The program is staged 64 ops at a time through LDS (4 KB per wavefront,
     // one coalesced 1 KB load per 16 dwords), with ballot masks of the
@@ -434,12 +432,19 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
         const bool ren = (q0.x & 15u) == H_RENORM;
         if (ren) {
             // a renormalisation is the diagonal factor (f, f) with the device
-            // computed f = 1/sqrt(norm2): give it that record in LDS
-            const uint64_t fb = __builtin_bit_cast(uint64_t, *scale);
-            const uint32_t lo = (uint32_t)fb, hi = (uint32_t)(fb >> 32);
+            // computed f = 1/sqrt(norm2): give it that record in LDS (FP32
+            // programs: c[0] = (f, 0), c[1] = (f, 0) as FP32 pairs)
             q0.z = 0; q0.w = 0; q1.x = 0; q1.y = 0;
-            q1.z = lo; q1.w = hi; q2.x = 0; q2.y = 0;
-            q2.z = lo; q2.w = hi; q3.x = 0; q3.y = 0;
+            if constexpr (sizeof(R) == 4) {
+                const uint32_t fb = __builtin_bit_cast(uint32_t, (float)*scale);
+                q1.z = fb; q1.w = 0; q2.x = fb; q2.y = 0;
+                q2.z = 0; q2.w = 0; q3.x = 0; q3.y = 0;
+            } else {
+                const uint64_t fb = __builtin_bit_cast(uint64_t, *scale);
+                const uint32_t lo = (uint32_t)fb, hi = (uint32_t)(fb >> 32);
+                q1.z = lo; q1.w = hi; q2.x = 0; q2.y = 0;
+                q2.z = lo; q2.w = hi; q3.x = 0; q3.y = 0;
+            }
         }
         bmask = __ballot(from + lane < nops && (ren || base_class(q0.x & 15u, q0.y & 0xffu)));
         cmask = __ballot(from + lane < nops && (q0.z | q0.w) != 0);
@@ -483,26 +488,36 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             const uint64_t tbase = 1ull << ((hdr >> 7) & 63u);
             const uint64_t flip = (h == H_BASE_X || h == H_BASE_PERM) ? tbase : 0ull;
             const uint64_t cmB = u64of(q0.z, q0.w), cvB = u64of(q1.x, q1.y);
-            const double c0r = f64of(q1.z, q1.w), c0i = f64of(q2.x, q2.y);
-            const double c1r = f64of(q2.z, q2.w), c1i = f64of(q3.x, q3.y);
+            R c0r, c0i, c1r, c1i;
+            if constexpr (sizeof(R) == 4) {                 // FP32 pairs in c[0], c[1]
+                c0r = __builtin_bit_cast(float, q1.z);
+                c0i = __builtin_bit_cast(float, q1.w);
+                c1r = __builtin_bit_cast(float, q2.x);
+                c1i = __builtin_bit_cast(float, q2.y);
+            } else {
+                c0r = f64of(q1.z, q1.w);
+                c0i = f64of(q2.x, q2.y);
+                c1r = f64of(q2.z, q2.w);
+                c1i = f64of(q3.x, q3.y);
+            }
             static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                 constexpr int e = decltype(ec)::value;
                 const bool bit = (base[e] & tbase) != 0;
-                double fr = bit ? c1r : c0r;
-                double fi = bit ? c1i : c0i;
+                R fr = bit ? c1r : c0r;
+                R fi = bit ? c1i : c0i;
                 if constexpr (CTRL) {
                     const bool pred = valid[e] && ((base[e] & cmB) == cvB);
-                    fr = pred ? fr : 1.0;
-                    fi = pred ? fi : 0.0;
+                    fr = pred ? fr : (R)1;
+                    fi = pred ? fi : (R)0;
                     base[e] ^= pred ? flip : 0ull;
                 } else {
                     // a padding bucket (valid false) holds zeros, so its factor
                     // is irrelevant; its key flips too but is never stored
                     base[e] ^= flip;
                 }
-                const double r = __fma_rn(pr[e], fr, -(pi[e] * fi)), m = __fma_rn(pr[e], fi, pi[e] * fr);
-                pr[e] = r;
-                pi[e] = m;
+                const Cplx<R> f = rot_fma(pr[e], pi[e], Cplx<R>{fr, fi});
+                pr[e] = f.x;
+                pi[e] = f.y;
             });
         };
         auto base_run = [&](uint32_t jj, uint32_t n, auto ctrl_c) __attribute__((always_inline)) {
@@ -574,9 +589,18 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                     static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                         constexpr int e = decltype(ec)::value;
                         const bool bit = (base[e] & op.tbase) != 0;
-                        double fr, fi;
-                        base_factor(op, bit, pred[e], fr, fi);
-                        const R cr = (R)fr, ci = (R)fi;
+                        R cr, ci;
+                        if constexpr (sizeof(R) == 4) {
+                            // FP32 programs carry c0 / c1 as FP32 pairs in c[0], c[1]
+                            const uint64_t w = __builtin_bit_cast(uint64_t, bit ? op.c[1] : op.c[0]);
+                            cr = pred[e] ? __builtin_bit_cast(float, (uint32_t)w) : 1.0f;
+                            ci = pred[e] ? __builtin_bit_cast(float, (uint32_t)(w >> 32)) : 0.0f;
+                        } else {
+                            double fr, fi;
+                            base_factor(op, bit, pred[e], fr, fi);
+                            cr = fr;
+                            ci = fi;
+                        }
                         static_for<S>([&](auto sc) __attribute__((always_inline)) {
                             constexpr int s = decltype(sc)::value;
                             const bool on = ((uint32_t)s & op.cmS) == op.cvS;
@@ -602,8 +626,8 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                         fold_upd();
                         static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                             constexpr int e = decltype(ec)::value;
-                            pr[e] = 1.0;
-                            pi[e] = 0.0;
+                            pr[e] = (R)1;
+                            pi[e] = (R)0;
                             // (a padding bucket holds zeros: its kept count is 0)
                             cnt[e] = counted ? nzc[e] : (valid[e] ? count_nz<MB, R>(a[e]) : 0u);
                         });

@@ -92,7 +92,10 @@ inline DOp plan_op(const sqe_gate& g, const QMap& qm) {
 // Pack a planned op into a 64-byte sweep record: pick the handler and turn the
 // gate's coefficients into the handler's (see the handler table in
 // sqe_device.hpp).  General U keeps its eight coefficients in the side table.
-inline MOp encode_op(const DOp& d, std::vector<double>& coef) {
+// f32: the program runs on FP32 amplitudes; a base-qubit op (whose factor
+// the sweep folds in FP32 there) then carries c0 / c1 as FP32 pairs in the
+// first two coefficient slots: c[0] = (f0r, f0i), c[1] = (f1r, f1i).
+inline MOp encode_op(const DOp& d, std::vector<double>& coef, bool f32 = false) {
     MOp m{};
     const bool slot = d.tslot >= 0;
     const uint32_t j = slot ? (uint32_t)d.tslot : 0u;
@@ -125,6 +128,11 @@ inline MOp encode_op(const DOp& d, std::vector<double>& coef) {
         coef.insert(coef.end(), p, p + 8);
         break;
     default: h = H_RENORM; break;                                                           // K_RENORM
+    }
+    if (f32 && h <= H_BASE_PERM) {
+        const float f[4] = {(float)m.c[0], (float)m.c[1], (float)m.c[2], (float)m.c[3]};
+        m.c[0] = m.c[1] = m.c[2] = m.c[3] = 0.0;
+        std::memcpy(&m.c[0], f, 16);
     }
     m.hdr = h | j << 4 | target << 7 | ci << 13;
     m.ctl = (d.cmS & 0xffu) | (d.cvS & 0xffu) << 8;

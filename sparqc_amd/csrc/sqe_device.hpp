@@ -91,6 +91,15 @@ struct MOp {
 constexpr uint32_t NO_COEF = 0x3ffffu;
 static_assert(sizeof(MOp) == 64, "MOp is sixteen dwords");
 
+// The same pointer in the constant address space: a load through it at a
+// wavefront-uniform address becomes a scalar load into SGPRs.  Only for memory
+// that no kernel writes while the reading kernel runs (the program and the
+// coefficient side table).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* as_const_space(const T* p) {
+    return (const __attribute__((address_space(4))) T*)p;
+}
+
 struct Slot {           // hash index slot: 16 B, 8 slots = one 128 B line
     uint64_t key;
     uint64_t val;

@@ -218,7 +218,8 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
 
     case H_SLOT_U: {
         DOp d{};
-        for (int k = 0; k < 8; ++k) d.p[k] = coef[op.ci * 8 + k];
+        const auto* __restrict__ cq = as_const_space(coef + op.ci * 8);
+        for (int k = 0; k < 8; ++k) d.p[k] = cq[k];
         pairs([&](C x, C y, C& n0, C& n1, uint32_t&) __attribute__((always_inline)) {
             branch_pair<KU>(d, thr, x, y, n0, n1);
         }, std::true_type{});
@@ -418,7 +419,13 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
     // counter changes only at branching gates, so gates are counted in a
     // scalar and multiplied in when the count changes.
     __shared__ uint4 s_prog[BLOCK / 64][64 * 4];
+    __shared__ double s_one[BLOCK / 64][2];   // the identity factor (1, 0) of failed FP64 controls
     uint4* __restrict__ sp = s_prog[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
+    double* __restrict__ one = s_one[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
+    if constexpr (sizeof(R) == 8) {
+        one[0] = 1.0;                         // every lane stores the same pair, so each
+        one[1] = 0.0;                         // lane's later read sees its own store
+    }
     const uint4* __restrict__ recs = (const uint4*)ops;
     uint32_t c0 = 0;
     uint64_t bmask = 0;               // bit k: op c0 + k is a base-class op
@@ -488,29 +495,41 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             const uint64_t tbase = 1ull << ((hdr >> 7) & 63u);
             const uint64_t flip = (h == H_BASE_X || h == H_BASE_PERM) ? tbase : 0ull;
             const uint64_t cmB = u64of(q0.z, q0.w), cvB = u64of(q1.x, q1.y);
-            R c0r, c0i, c1r, c1i;
-            if constexpr (sizeof(R) == 4) {                 // FP32 pairs in c[0], c[1]
-                c0r = __builtin_bit_cast(float, q1.z);
-                c0i = __builtin_bit_cast(float, q1.w);
-                c1r = __builtin_bit_cast(float, q2.x);
-                c1i = __builtin_bit_cast(float, q2.y);
-            } else {
-                c0r = f64of(q1.z, q1.w);
-                c0i = f64of(q2.x, q2.y);
-                c1r = f64of(q2.z, q2.w);
-                c1i = f64of(q3.x, q3.y);
-            }
+            const uint32_t tpos = (hdr >> 7) & 63u;
+            const bool thi = tpos >= 32;                    // the target bit's word (uniform)
+            // FP64: each lane reads its own factor, (c0r, c0i) at byte 24 of the
+            // staged record or (c1r, c1i) at byte 40, at an address picked by its
+            // base bit (a controlled gate's failing lanes read the identity
+            // instead): three address instructions replace eight 32-bit selects
+            // of the uniform doubles.  FP32 pairs (bytes 24 and 32) are selected
+            // as values, one instruction per float.
+            const char* __restrict__ cbase = reinterpret_cast<const char*>(&sp[jj * 4]) + 24;
             static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                 constexpr int e = decltype(ec)::value;
-                const bool bit = (base[e] & tbase) != 0;
-                R fr = bit ? c1r : c0r;
-                R fi = bit ? c1i : c0i;
-                if constexpr (CTRL) {
-                    const bool pred = valid[e] && ((base[e] & cmB) == cvB);
-                    fr = pred ? fr : (R)1;
-                    fi = pred ? fi : (R)0;
-                    base[e] ^= pred ? flip : 0ull;
+                R fr, fi;
+                if constexpr (sizeof(R) == 8) {
+                    const uint32_t w = thi ? (uint32_t)(base[e] >> 32) : (uint32_t)base[e];
+                    const uint32_t bitv = __builtin_amdgcn_ubfe(w, tpos & 31u, 1u);
+                    const R* cp = reinterpret_cast<const R*>(cbase + bitv * 16u);
+                    if constexpr (CTRL) {
+                        const bool pred = valid[e] && ((base[e] & cmB) == cvB);
+                        cp = pred ? cp : one;
+                        base[e] ^= pred ? flip : 0ull;
+                    }
+                    fr = cp[0];
+                    fi = cp[1];
                 } else {
+                    const bool bit = (base[e] & tbase) != 0;
+                    fr = bit ? __builtin_bit_cast(float, q2.x) : __builtin_bit_cast(float, q1.z);
+                    fi = bit ? __builtin_bit_cast(float, q2.y) : __builtin_bit_cast(float, q1.w);
+                    if constexpr (CTRL) {
+                        const bool pred = valid[e] && ((base[e] & cmB) == cvB);
+                        fr = pred ? fr : (R)1;
+                        fi = pred ? fi : (R)0;
+                        base[e] ^= pred ? flip : 0ull;
+                    }
+                }
+                if constexpr (!CTRL) {
                     // a padding bucket (valid false) holds zeros, so its factor
                     // is irrelevant; its key flips too but is never stored
                     base[e] ^= flip;
@@ -551,7 +570,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             if constexpr (SREC) {
                 // the record goes straight from memory into scalar registers
                 // (uniform address, program read-only during the launch)
-                const MOp* __restrict__ rec = ops + (c0 + jj);
+                const auto* __restrict__ rec = as_const_space(ops + (c0 + jj));
                 hdr = rec->hdr;
                 ctl = rec->ctl;
                 op.cmB = rec->cmB;

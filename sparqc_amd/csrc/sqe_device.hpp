@@ -271,6 +271,19 @@ __device__ __forceinline__ bool nonzero(cf v) { return v.x != 0.0f || v.y != 0.0
 // Apply a branching pair and prune its outputs (componentwise |re|,|im| < thr,
 // emulator/include/data_t.h:50-60).  FP32: decisions for outputs with
 // max(|re|,|im|) in [thr/4, 4 thr) are redone in FP64 from the FP32 inputs.
+// Band edges [thr/4, 4 thr) as FP32 values.  The round trip through a
+// scalar register keeps the compiler from widening the FP32 compares back to
+// FP64 against the double edges (FP64 compares, and two uniform doubles held
+// in VGPRs, that the FP32 sweep spilled).
+struct Band32 {
+    float lo, hi;
+};
+__device__ __forceinline__ float opaque_uniform(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(uint32_t, v)));
+}
+__device__ __forceinline__ Band32 band32(double thr) {
+    return Band32{opaque_uniform((float)(0.25 * thr)), opaque_uniform((float)(4.0 * thr))};
+}
 template <uint32_t K>
 __device__ __forceinline__ void branch_pair(const DOp& op, double thr, cd v0, cd v1, cd& n0, cd& n1) {
     pair64<K>(op.p, v0, v1, n0, n1);
@@ -280,10 +293,13 @@ __device__ __forceinline__ void branch_pair(const DOp& op, double thr, cd v0, cd
 template <uint32_t K>
 __device__ __forceinline__ void branch_pair(const DOp& op, double thr, cf v0, cf v1, cf& n0, cf& n1) {
     pair32<K>(op.p, v0, v1, n0, n1);
-    const double m0 = fmax(fabs((double)n0.x), fabs((double)n0.y));
-    const double m1 = fmax(fabs((double)n1.x), fabs((double)n1.y));
-    bool k0 = m0 >= 4.0 * thr, k1 = m1 >= 4.0 * thr;
-    const bool near0 = !k0 && m0 >= 0.25 * thr, near1 = !k1 && m1 >= 0.25 * thr;
+    // band [thr/4, 4 thr) tested in FP32: an output within an ulp of an edge
+    // is far from thr, where the FP32 and FP64 decisions agree
+    const Band32 band = band32(thr);
+    const float m0 = fmaxf(fabsf(n0.x), fabsf(n0.y));
+    const float m1 = fmaxf(fabsf(n1.x), fabsf(n1.y));
+    bool k0 = m0 >= band.hi, k1 = m1 >= band.hi;
+    const bool near0 = !k0 && m0 >= band.lo, near1 = !k1 && m1 >= band.lo;
     if (near0 || near1) {
         cd w0 = mkd(v0.x, v0.y), w1 = mkd(v1.x, v1.y), d0, d1;
         pair64<K>(op.p, w0, w1, d0, d1);

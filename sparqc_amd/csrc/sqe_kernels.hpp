@@ -65,18 +65,16 @@ __device__ __forceinline__ cd prune(cd n, double thr) { return negl64(n, thr) ? 
 // The band edges are compared in FP32: an output within an ulp of an edge is
 // far from thr, where the FP32 and FP64 decisions agree, so the edge rounding
 // cannot change a decision.
-__device__ __forceinline__ bool near32(cf n, double thr) {
+__device__ __forceinline__ bool near32(cf n, Band32 b) {
     const float m = fmaxf(fabsf(n.x), fabsf(n.y));
-    return m < (float)(4.0 * thr) && m >= (float)(0.25 * thr);
+    return m < b.hi && m >= b.lo;
 }
-__device__ __forceinline__ bool keep32(cf n, double thr, bool keep_exact) {
+__device__ __forceinline__ bool keep32(cf n, Band32 b, bool keep_exact) {
     const float m = fmaxf(fabsf(n.x), fabsf(n.y));
-    return m >= (float)(4.0 * thr) ? true : (m < (float)(0.25 * thr) ? false : keep_exact);
+    return m >= b.hi ? true : (m < b.lo ? false : keep_exact);
 }
-__device__ __forceinline__ cf prune32(cf n, double thr, bool keep_exact) {
-    const float m = fmaxf(fabsf(n.x), fabsf(n.y));
-    const bool keep = m >= (float)(4.0 * thr) ? true : (m < (float)(0.25 * thr) ? false : keep_exact);
-    return keep ? n : mkf(0.0f, 0.0f);
+__device__ __forceinline__ cf prune32(cf n, Band32 b, bool keep_exact) {
+    return keep32(n, b, keep_exact) ? n : mkf(0.0f, 0.0f);
 }
 
 template <int MB, class R>
@@ -155,13 +153,14 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
     auto prune_pair = [&](C x, C y, C& n0, C& n1, uint32_t& kept, auto&& exact) __attribute__((always_inline)) {
         if constexpr (F32) {
             bool k0e = true, k1e = true;
-            if (near32(n0, thr) || near32(n1, thr)) {
+            const Band32 band = band32(thr);
+            if (near32(n0, band) || near32(n1, band)) {
                 cd e0, e1;
                 exact(mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
                 k0e = !negl64(e0, thr);
                 k1e = !negl64(e1, thr);
             }
-            const bool k0 = keep32(n0, thr, k0e), k1 = keep32(n1, thr, k1e);
+            const bool k0 = keep32(n0, band, k0e), k1 = keep32(n1, band, k1e);
             n0 = k0 ? n0 : mkf(0.0f, 0.0f);
             n1 = k1 ? n1 : mkf(0.0f, 0.0f);
             kept += (k0 ? 1u : 0u) + (k1 ? 1u : 0u);
@@ -295,6 +294,7 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 __device__ __forceinline__ void block_apply(Cplx<float> (&a)[16], float pr, float pi, const double* __restrict__ side,
                                             double thr) {
     const uint32_t lane = threadIdx.x & 63;
+    const Band32 band = band32(thr);
     const float* __restrict__ tab = (const float*)side;          // 16 steps x 64 lanes
     const double* __restrict__ m64 = side + 512;                  // 16x16 complex, FP64
     float x[32];
@@ -331,7 +331,7 @@ __device__ __forceinline__ void block_apply(Cplx<float> (&a)[16], float pr, floa
     for (int s = 0; s < 16; ++s) {
         Cplx<float> n = mkf(y[s], y[16 + s]);
         bool keep = true;
-        if (near32(n, thr)) {
+        if (near32(n, band)) {
             double er = 0.0, ei = 0.0;
             for (int k = 0; k < 16; ++k) {
                 const double mr = m64[(s * 16 + k) * 2], mi = m64[(s * 16 + k) * 2 + 1];
@@ -341,7 +341,7 @@ __device__ __forceinline__ void block_apply(Cplx<float> (&a)[16], float pr, floa
             }
             keep = !negl64(mkd(er, ei), thr);
         }
-        a[s] = prune32(n, thr, keep);
+        a[s] = prune32(n, band, keep);
     }
 }
 

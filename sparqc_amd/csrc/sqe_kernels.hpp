@@ -684,22 +684,28 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
 // VGPRs at 512 / value): the amplitude registers of a bucket take 4 * 2^MB
 // (FP64) or 2 * 2^MB (FP32) VGPRs; with 16 slots FP64 fits 3 waves (about
 // 160 VGPRs) and FP32 runs 4 (a few spilled dwords, measured faster than 3).
+// SHORT: the FP32 16-slot sweep of a short program (a few gates, bound by
+// HBM rather than instruction issue) is compiled for 3 waves instead, where
+// it does not spill: at 4 waves every lane writes and rereads its spill slots
+// once per launch, extra traffic comparable to a tenth of the state's bytes.
 #ifndef SQE_OCC_F64_MB4
 #define SQE_OCC_F64_MB4 3
 #endif
 #ifndef SQE_OCC_F32_MB4
 #define SQE_OCC_F32_MB4 4
 #endif
-template <int MB, class R>
+template <int MB, class R, bool SHORT = false>
 struct SweepOcc {
     static constexpr int value =
-        MB >= 5 ? 1 : (MB == 4 ? (sizeof(R) == 8 ? SQE_OCC_F64_MB4 : SQE_OCC_F32_MB4) : 3);
+        MB >= 5 ? 1 : (MB == 4 ? (sizeof(R) == 8 || SHORT ? SQE_OCC_F64_MB4 : SQE_OCC_F32_MB4) : 3);
 };
+constexpr uint32_t SHORT_PROGRAM_OPS = 8;     // programs up to this length take the SHORT sweep
 
 // One program over the engine's store: wavefront g of the grid takes tiles
 // EPT*g .. EPT*g + EPT-1.
-template <int MB, class R, int EPT, bool BLOCKS = false>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<MB, R>::value))) void k_sweep(SweepArgs A) {
+template <int MB, class R, int EPT, bool BLOCKS = false, bool SHORT = false>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<MB, R, SHORT>::value))) void k_sweep(
+    SweepArgs A) {
     const uint64_t nb = *A.nb;
     const int64_t ntiles = (int64_t)((nb + 63) >> 6);
     // tile ownership is decided per wavefront from a scalar wave index, so the
@@ -1122,6 +1128,9 @@ struct Launch {
             } else {
                 return hipErrorInvalidValue;
             }
+        } else if (sizeof(R) == 4 && mb == 4 && a.nops <= SHORT_PROGRAM_OPS) {
+            if constexpr (sizeof(R) == 4)
+                hipLaunchKernelGGL((k_sweep<4, R, EPT, false, true>), dim3(grid), dim3(BLOCK), 0, st, a);
         } else {
             SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_sweep<MB_, R, EPT>), dim3(grid), dim3(BLOCK), 0, st, a));
         }

@@ -87,6 +87,17 @@ __device__ __forceinline__ uint32_t count_nz(const typename CT<R>::T (&a)[1 << M
     return c;
 }
 
+// base ^ (pred ? flip : 0): one v_bitop3 per word, S0 ^ (S1 & S2) (truth
+// table 0x78 with S0 = 0xF0, S1 = 0xCC, S2 = 0xAA), on a 0 / ~0 lane mask.
+// Written as a select, the compiler moved the uniform flip into VGPRs and
+// selected each word (six instructions instead of three).
+__device__ __forceinline__ uint64_t flip_if(uint64_t base, uint64_t flip, bool pred) {
+    const uint32_t m = pred ? ~0u : 0u;
+    const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)base, (uint32_t)flip, m, 0x78);
+    const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(base >> 32), (uint32_t)(flip >> 32), m, 0x78);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Decoded sweep op (uniform across the wavefront).
 struct SOp {
     uint32_t h, j, cmS, cvS, ci;
@@ -514,7 +525,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                     if constexpr (CTRL) {
                         const bool pred = valid[e] && ((base[e] & cmB) == cvB);
                         cp = pred ? cp : one;
-                        base[e] ^= pred ? flip : 0ull;
+                        base[e] = flip_if(base[e], flip, pred);
                     }
                     fr = cp[0];
                     fi = cp[1];
@@ -526,7 +537,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                         const bool pred = valid[e] && ((base[e] & cmB) == cvB);
                         fr = pred ? fr : (R)1;
                         fi = pred ? fi : (R)0;
-                        base[e] ^= pred ? flip : 0ull;
+                        base[e] = flip_if(base[e], flip, pred);
                     }
                 }
                 if constexpr (!CTRL) {

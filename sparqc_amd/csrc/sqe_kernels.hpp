@@ -163,15 +163,20 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
     // prune both outputs and count the ones kept (an exact zero is never kept)
     auto prune_pair = [&](C x, C y, C& n0, C& n1, uint32_t& kept, auto&& exact) __attribute__((always_inline)) {
         if constexpr (F32) {
-            bool k0e = true, k1e = true;
+            // keep above the band, prune below it, and decide the band on
+            // the FP64 recomputation; the recomputation runs when any lane of
+            // the wavefront has an output in the band (a uniform branch, so
+            // no per-pair exec-mask save and restore)
             const Band32 band = band32(thr);
-            if (near32(n0, band) || near32(n1, band)) {
+            const float m0 = fmaxf(fabsf(n0.x), fabsf(n0.y)), m1 = fmaxf(fabsf(n1.x), fabsf(n1.y));
+            bool k0 = m0 >= band.hi, k1 = m1 >= band.hi;
+            const bool near0 = !k0 && m0 >= band.lo, near1 = !k1 && m1 >= band.lo;
+            if (__builtin_amdgcn_ballot_w64(near0 || near1) != 0) {
                 cd e0, e1;
                 exact(mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
-                k0e = !negl64(e0, thr);
-                k1e = !negl64(e1, thr);
+                k0 = k0 || (near0 && !negl64(e0, thr));
+                k1 = k1 || (near1 && !negl64(e1, thr));
             }
-            const bool k0 = keep32(n0, band, k0e), k1 = keep32(n1, band, k1e);
             n0 = k0 ? n0 : mkf(0.0f, 0.0f);
             n1 = k1 ? n1 : mkf(0.0f, 0.0f);
             kept += (k0 ? 1u : 0u) + (k1 ? 1u : 0u);

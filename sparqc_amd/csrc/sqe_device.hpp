@@ -300,11 +300,13 @@ __device__ __forceinline__ void branch_pair(const DOp& op, double thr, cf v0, cf
     const float m1 = fmaxf(fabsf(n1.x), fabsf(n1.y));
     bool k0 = m0 >= band.hi, k1 = m1 >= band.hi;
     const bool near0 = !k0 && m0 >= band.lo, near1 = !k1 && m1 >= band.lo;
-    if (near0 || near1) {
+    // the FP64 recomputation runs when any lane has an output in the band (a
+    // uniform branch, no per-pair exec-mask save and restore)
+    if (__builtin_amdgcn_ballot_w64(near0 || near1) != 0) {
         cd w0 = mkd(v0.x, v0.y), w1 = mkd(v1.x, v1.y), d0, d1;
         pair64<K>(op.p, w0, w1, d0, d1);
-        if (near0) k0 = !negl64(d0, thr);
-        if (near1) k1 = !negl64(d1, thr);
+        k0 = k0 || (near0 && !negl64(d0, thr));
+        k1 = k1 || (near1 && !negl64(d1, thr));
     }
     if (!k0) n0 = mkf(0.0f, 0.0f);
     if (!k1) n1 = mkf(0.0f, 0.0f);

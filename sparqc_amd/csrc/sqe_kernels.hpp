@@ -346,8 +346,12 @@ __device__ __forceinline__ void block_apply(Cplx<float> (&a)[16], float pr, floa
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
         Cplx<float> n = mkf(y[s], y[16 + s]);
-        bool keep = true;
-        if (near32(n, band)) {
+        // keep above the band, prune below it; the FP64 row recomputation
+        // runs when any lane has this output in the band (a uniform branch)
+        const float m = fmaxf(fabsf(n.x), fabsf(n.y));
+        bool keep = m >= band.hi;
+        const bool near = !keep && m >= band.lo;
+        if (__builtin_amdgcn_ballot_w64(near) != 0) {
             double er = 0.0, ei = 0.0;
             for (int k = 0; k < 16; ++k) {
                 const double mr = m64[(s * 16 + k) * 2], mi = m64[(s * 16 + k) * 2 + 1];
@@ -355,9 +359,9 @@ __device__ __forceinline__ void block_apply(Cplx<float> (&a)[16], float pr, floa
                 er += mr * xr - mi * xi;
                 ei += mr * xi + mi * xr;
             }
-            keep = !negl64(mkd(er, ei), thr);
+            keep = keep || (near && !negl64(mkd(er, ei), thr));
         }
-        a[s] = prune32(n, band, keep);
+        a[s] = keep ? n : mkf(0.0f, 0.0f);
     }
 }
 

@@ -60,22 +60,11 @@ __device__ __forceinline__ void hpair(float f, cf v0, cf v1, cf& n0, cf& n1) {
 
 // Prune a branching output (componentwise |re|,|im| < thr,
 // emulator/include/data_t.h:50-60).  FP32: outputs with max(|re|,|im|) in
-// [thr/4, 4 thr) are decided on the FP64 recomputation `exact`.
+// [thr/4, 4 thr) are decided on an FP64 recomputation.  The band edges are
+// compared in FP32: an output within an ulp of an edge is far from thr, where
+// the FP32 and FP64 decisions agree, so the edge rounding cannot change a
+// decision.
 __device__ __forceinline__ cd prune(cd n, double thr) { return negl64(n, thr) ? mkd(0.0, 0.0) : n; }
-// The band edges are compared in FP32: an output within an ulp of an edge is
-// far from thr, where the FP32 and FP64 decisions agree, so the edge rounding
-// cannot change a decision.
-__device__ __forceinline__ bool near32(cf n, Band32 b) {
-    const float m = fmaxf(fabsf(n.x), fabsf(n.y));
-    return m < b.hi && m >= b.lo;
-}
-__device__ __forceinline__ bool keep32(cf n, Band32 b, bool keep_exact) {
-    const float m = fmaxf(fabsf(n.x), fabsf(n.y));
-    return m >= b.hi ? true : (m < b.lo ? false : keep_exact);
-}
-__device__ __forceinline__ cf prune32(cf n, Band32 b, bool keep_exact) {
-    return keep32(n, b, keep_exact) ? n : mkf(0.0f, 0.0f);
-}
 
 template <int MB, class R>
 __device__ __forceinline__ uint32_t count_nz(const typename CT<R>::T (&a)[1 << MB]) {

@@ -78,7 +78,26 @@ def build_circuit_check(verbose=False):
     return CHECK_BIN
 
 
+ENC_SRC = os.path.join(ROOT, "tests", "cpp", "encode_check.cpp")
+ENC_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "encode_check")
+
+
+def build_encode_check(verbose=False):
+    """Host-only check of the sweep-program encoder (hipcc host pass: the
+    planner headers include the HIP runtime header; no device code is built)."""
+    os.makedirs(os.path.dirname(ENC_BIN), exist_ok=True)
+    cmd = [HIPCC, "--offload-host-only", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+           ENC_SRC, "-o", ENC_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for encode_check:\n%s" % r.stderr[-4000:])
+    if verbose:
+        print("built", ENC_BIN)
+    return ENC_BIN
+
+
 if __name__ == "__main__":
     build(verbose=True)
     build_circuit_check(verbose=True)
+    build_encode_check(verbose=True)
     sys.exit(0)

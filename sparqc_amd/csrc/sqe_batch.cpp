@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <new>
+#include <stdexcept>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -32,6 +34,19 @@ thread_local std::string b_err;
 sqe_status bfail(sqe_status s, const std::string& m) {
     b_err = m;
     return s;
+}
+
+// No C++ exception crosses the C ABI: host allocation failures (and anything
+// else a standard container throws) become status codes.
+template <class F>
+sqe_status guard(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return bfail(SQE_ENOMEM, "host allocation failed");
+    } catch (const std::exception& ex) {
+        return bfail(SQE_EINVAL, std::string("host error: ") + ex.what());
+    }
 }
 
 #define BHIP(x)                                                                          \
@@ -660,372 +675,400 @@ sqe_status snap_restore(sqe_batch* b) {
 extern "C" {
 
 sqe_status sqe_batch_create(const sqe_config* cfg, sqe_batch** out) {
-    if (!cfg || !out) return bfail(SQE_EINVAL, "null argument");
-    *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
-        (void)hipGetLastError();
-        return bfail(SQE_EDEVICE, "no HIP device");
-    }
-    if (cfg->prec != SQE_F64 && cfg->prec != SQE_F32) return bfail(SQE_EINVAL, "bad precision");
-    if (cfg->device < 0 || cfg->device >= ndev) return bfail(SQE_EINVAL, "bad device ordinal");
-    BHIP(hipSetDevice(cfg->device));
-    sqe_batch* b = new (std::nothrow) sqe_batch();
-    if (!b) return bfail(SQE_ENOMEM, "host allocation");
-    b->cfg = *cfg;
-    b->f32 = cfg->prec == SQE_F32;
-    b->csize = b->f32 ? 8 : 16;
-    b->thr = cfg->prune_thr > 0 ? cfg->prune_thr : (b->f32 ? 1e-7 : 1e-14);
-    if (!b->cfg.hbm_budget_bytes) {
-        size_t fr = 0, tot = 0;
-        (void)hipMemGetInfo(&fr, &tot);
-        const double keep = b->cfg.mem_reserve_ppm ? std::min(1.0, b->cfg.mem_reserve_ppm * 1e-6) : 0.1;
-        b->cfg.hbm_budget_bytes = (uint64_t)((1.0 - keep) * (double)fr);
-    }
-    sqe_status r = SQE_OK;
-    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) r = bfail(SQE_EHIP, "stream");
-    if (r == SQE_OK) r = dal(b, (void**)&b->stats, sizeof(Stats));
-    if (r == SQE_OK) r = dal(b, (void**)&b->ticket, 64);
-    if (r == SQE_OK) r = dal(b, (void**)&b->total, 64);
-    if (r == SQE_OK && hipMemset(b->stats, 0, sizeof(Stats)) != hipSuccess) r = bfail(SQE_EHIP, "memset");
-    if (r == SQE_OK && hipMemset(b->ticket, 0, 64) != hipSuccess) r = bfail(SQE_EHIP, "memset");
-    if (r == SQE_OK && (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess))
-        r = bfail(SQE_EHIP, "event");
-    if (r == SQE_OK) r = slots_fit(b, 256);
-    if (r != SQE_OK) {
-        sqe_batch_destroy(b);
-        return r;
-    }
-    *out = b;
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        if (!cfg || !out) return bfail(SQE_EINVAL, "null argument");
+        *out = nullptr;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+            (void)hipGetLastError();
+            return bfail(SQE_EDEVICE, "no HIP device");
+        }
+        if (cfg->prec != SQE_F64 && cfg->prec != SQE_F32) return bfail(SQE_EINVAL, "bad precision");
+        if (cfg->device < 0 || cfg->device >= ndev) return bfail(SQE_EINVAL, "bad device ordinal");
+        BHIP(hipSetDevice(cfg->device));
+        sqe_batch* b = new (std::nothrow) sqe_batch();
+        if (!b) return bfail(SQE_ENOMEM, "host allocation");
+        b->cfg = *cfg;
+        b->f32 = cfg->prec == SQE_F32;
+        b->csize = b->f32 ? 8 : 16;
+        b->thr = cfg->prune_thr > 0 ? cfg->prune_thr : (b->f32 ? 1e-7 : 1e-14);
+        if (!b->cfg.hbm_budget_bytes) {
+            size_t fr = 0, tot = 0;
+            (void)hipMemGetInfo(&fr, &tot);
+            const double keep = b->cfg.mem_reserve_ppm ? std::min(1.0, b->cfg.mem_reserve_ppm * 1e-6) : 0.1;
+            b->cfg.hbm_budget_bytes = (uint64_t)((1.0 - keep) * (double)fr);
+        }
+        sqe_status r = SQE_OK;
+        if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) r = bfail(SQE_EHIP, "stream");
+        if (r == SQE_OK) r = dal(b, (void**)&b->stats, sizeof(Stats));
+        if (r == SQE_OK) r = dal(b, (void**)&b->ticket, 64);
+        if (r == SQE_OK) r = dal(b, (void**)&b->total, 64);
+        if (r == SQE_OK && hipMemset(b->stats, 0, sizeof(Stats)) != hipSuccess) r = bfail(SQE_EHIP, "memset");
+        if (r == SQE_OK && hipMemset(b->ticket, 0, 64) != hipSuccess) r = bfail(SQE_EHIP, "memset");
+        if (r == SQE_OK && (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess))
+            r = bfail(SQE_EHIP, "event");
+        if (r == SQE_OK) r = slots_fit(b, 256);
+        if (r != SQE_OK) {
+            sqe_batch_destroy(b);
+            return r;
+        }
+        *out = b;
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_batch_destroy(sqe_batch* b) {
-    if (!b) return SQE_OK;
-    (void)hipSetDevice(b->cfg.device);
-    if (b->st) (void)hipStreamSynchronize(b->st);
-    for (void* p : {(void*)b->base.p, (void*)b->amp.p, (void*)b->prog.p, (void*)b->coef.p, (void*)b->upd,
-                    (void*)b->nnz, (void*)b->meta, (void*)b->norm2, (void*)b->scale, (void*)b->tab,
-                    (void*)b->xkeys, (void*)b->xre, (void*)b->xim, (void*)b->occ, (void*)b->stats,
-                    (void*)b->scan_status, (void*)b->ticket, (void*)b->total, (void*)b->dtask, (void*)b->ditem, (void*)b->partials,
-                    (void*)b->counts, (void*)b->snap_base, (void*)b->snap_amp, (void*)b->snap_cnt, (void*)b->gtask,
-                    (void*)b->gitem, (void*)b->gcounts, (void*)b->gpartials})
-        if (p) (void)hipFree(p);
-    drop_graph(b);
-    if (b->htask) (void)hipHostFree(b->htask);
-    if (b->hitem) (void)hipHostFree(b->hitem);
-    if (b->hup) (void)hipHostFree(b->hup);
-    if (b->ev0) (void)hipEventDestroy(b->ev0);
-    if (b->ev1) (void)hipEventDestroy(b->ev1);
-    for (hipStream_t s2 : b->side) (void)hipStreamDestroy(s2);
-    for (hipEvent_t ev : b->gev) (void)hipEventDestroy(ev);
-    if (b->st) (void)hipStreamDestroy(b->st);
-    delete b;
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        if (!b) return SQE_OK;
+        (void)hipSetDevice(b->cfg.device);
+        if (b->st) (void)hipStreamSynchronize(b->st);
+        for (void* p : {(void*)b->base.p, (void*)b->amp.p, (void*)b->prog.p, (void*)b->coef.p, (void*)b->upd,
+                        (void*)b->nnz, (void*)b->meta, (void*)b->norm2, (void*)b->scale, (void*)b->tab,
+                        (void*)b->xkeys, (void*)b->xre, (void*)b->xim, (void*)b->occ, (void*)b->stats,
+                        (void*)b->scan_status, (void*)b->ticket, (void*)b->total, (void*)b->dtask, (void*)b->ditem, (void*)b->partials,
+                        (void*)b->counts, (void*)b->snap_base, (void*)b->snap_amp, (void*)b->snap_cnt, (void*)b->gtask,
+                        (void*)b->gitem, (void*)b->gcounts, (void*)b->gpartials})
+            if (p) (void)hipFree(p);
+        drop_graph(b);
+        if (b->htask) (void)hipHostFree(b->htask);
+        if (b->hitem) (void)hipHostFree(b->hitem);
+        if (b->hup) (void)hipHostFree(b->hup);
+        if (b->ev0) (void)hipEventDestroy(b->ev0);
+        if (b->ev1) (void)hipEventDestroy(b->ev1);
+        for (hipStream_t s2 : b->side) (void)hipStreamDestroy(s2);
+        for (hipEvent_t ev : b->gev) (void)hipEventDestroy(ev);
+        if (b->st) (void)hipStreamDestroy(b->st);
+        delete b;
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const double* re, const double* im,
                          uint64_t n, const sqe_gate* gates, uint64_t ng, int64_t* id_out) {
-    if (!b || !id_out) return bfail(SQE_EINVAL, "null argument");
-    if (nq < 1 || nq > 63) return bfail(SQE_EINVAL, "nq must be in 1..63");
-    if ((n && (!keys || !re || !im)) || (ng && !gates)) return bfail(SQE_EINVAL, "null array");
-    BHIP(hipSetDevice(b->cfg.device));
-    const uint64_t lim = (1ull << nq) - 1;
-    for (uint64_t j = 0; j < ng; ++j) {
-        const sqe_gate& x = gates[j];
-        if (x.kind > SQE_GATE_PERM || x.target >= nq || (x.ctrl_mask & ~lim) || ((x.ctrl_mask >> x.target) & 1ull) ||
-            (x.ctrl_val & ~x.ctrl_mask))
-            return bfail(SQE_EINVAL, "gate " + std::to_string(j) + " is invalid");
-    }
-    const QMap qm = choose_map((int)nq, b->cfg.bucket_bits, gates, ng);
-    for (uint64_t j = 0; j < ng; ++j)
-        if (!gate_local(gates[j], qm))
-            return bfail(SQE_EINVAL, "gate " + std::to_string(j) +
-                                         " crosses buckets: run this circuit with the single-circuit engine");
-    uint64_t m = 0;                          // entries with a nonzero amplitude
-    for (uint64_t j = 0; j < n; ++j) {
-        if (keys[j] & ~lim) return bfail(SQE_EINVAL, "key has bits beyond nq");
-        m += (re[j] != 0.0 || im[j] != 0.0) ? 1 : 0;
-    }
-    // program: gates, with a RENORM op after every renorm_every-th gate
-    std::vector<MOp> prog;
-    std::vector<double> coef;
-    std::vector<uint32_t> renorm_at;
-    prog.reserve(ng + 8);
-    for (uint64_t j = 0; j < ng; ++j) {
-        prog.push_back(encode_op(plan_op(gates[j], qm), coef, b->f32));
-        if (b->cfg.renorm_every && (j + 1) % b->cfg.renorm_every == 0) {
-            renorm_at.push_back((uint32_t)prog.size());
-            prog.push_back(encode_op(renorm_op(), coef, b->f32));
+    return guard([&]() -> sqe_status {
+        if (!b || !id_out) return bfail(SQE_EINVAL, "null argument");
+        if (nq < 1 || nq > 63) return bfail(SQE_EINVAL, "nq must be in 1..63");
+        if ((n && (!keys || !re || !im)) || (ng && !gates)) return bfail(SQE_EINVAL, "null array");
+        BHIP(hipSetDevice(b->cfg.device));
+        // program positions and per-launch op counts are 32-bit
+        if (ng > (1ull << 30)) return bfail(SQE_EINVAL, "at most 2^30 gates per circuit");
+        const uint64_t lim = (1ull << nq) - 1;
+        for (uint64_t j = 0; j < ng; ++j) {
+            const sqe_gate& x = gates[j];
+            if (x.kind > SQE_GATE_PERM || x.target >= nq || (x.ctrl_mask & ~lim) || ((x.ctrl_mask >> x.target) & 1ull) ||
+                (x.ctrl_val & ~x.ctrl_mask))
+                return bfail(SQE_EINVAL, "gate " + std::to_string(j) + " is invalid");
         }
-    }
-    // region sizes and admission
-    const uint64_t S = 1ull << qm.mb;
-    const uint64_t tiles = std::max<uint64_t>(1, (m + 63) / 64);
-    const uint64_t rows = tiles * S;
-    const uint64_t bytes = tiles * 64 * 8 + rows * 64 * b->csize + prog.size() * sizeof(MOp) + coef.size() * 8;
-    if (b->used + bytes > b->cfg.hbm_budget_bytes)
-        return bfail(SQE_ETABLE_FULL, "admission: circuit needs " + std::to_string(bytes) + " bytes, " +
-                                          std::to_string(b->cfg.hbm_budget_bytes - b->used) + " left in the budget");
-    uint32_t id;
-    if (!b->free_slots.empty()) { id = b->free_slots.back(); b->free_slots.pop_back(); }
-    else { id = (uint32_t)b->c.size(); b->c.emplace_back(); }
-    BCK(slots_fit(b, id + 1));
-    Circ c;
-    c.nq = nq;
-    c.qm = qm;
-    c.tiles = tiles;
-    c.rows = rows;
-    c.nops = prog.size();
-    c.ncoef = coef.size();
-    c.renorm_at = renorm_at;
-    c.gates = ng;
-    c.bytes = bytes;
-    auto place = [&](RangeAlloc& al, Pool& pl, uint64_t units, uint64_t unit_bytes, uint64_t* at) -> sqe_status {
-        if (!units) { *at = 0; return SQE_OK; }
-        if (!al.take(units, at)) {
-            const uint64_t want = std::max<uint64_t>(al.cap * 2, al.cap + units);
-            BCK(pool_fit(b, pl, want * unit_bytes, al.used_extent() * unit_bytes));
-            al.grow(pl.bytes / unit_bytes);
-            if (!al.take(units, at)) return bfail(SQE_ENOMEM, "pool allocation");
-        }
-        return SQE_OK;
-    };
-    int placed = 0;
-    sqe_status st = place(b->ta, b->base, tiles, 64 * 8, &c.tile0);
-    if (st == SQE_OK && ++placed) st = place(b->ra, b->amp, rows, 64 * b->csize, &c.row0);
-    if (st == SQE_OK && ++placed) st = place(b->pa, b->prog, c.nops, sizeof(MOp), &c.prog0);
-    if (st == SQE_OK && ++placed) st = place(b->ca, b->coef, c.ncoef, 8, &c.coef0);
-    if (st != SQE_OK) {
-        if (placed > 0) b->ta.give(c.tile0, c.tiles);
-        if (placed > 1) b->ra.give(c.row0, c.rows);
-        if (placed > 2 && c.nops) b->pa.give(c.prog0, c.nops);
-        b->free_slots.push_back(id);
-        return st;
-    }
-    c.live = true;
-    b->c[id] = c;
-    b->used += bytes;
-    ++b->generation;
-    ++b->layout;
-    Circ& cc = b->c[id];
-    // the uploads go through one page-locked staging buffer: program,
-    // coefficients, then the nonzero entries' keys, real and imaginary parts
-    // (no copy from an earlier admission may still be reading it)
-    BCK(stage_fit(b, std::max<uint64_t>(m, 1)));
-    BHIP(hipStreamSynchronize(b->st));
-    const uint64_t up = cc.nops * sizeof(MOp) + cc.ncoef * 8 + 24 * m;
-    if (up > b->hup_cap) {
-        if (b->hup) (void)hipHostFree(b->hup);
-        b->hup = nullptr;
-        b->hup_cap = 0;
-        const uint64_t cap = std::max<uint64_t>(p2(up), 1 << 20);
-        BHIP(hipHostMalloc((void**)&b->hup, cap, hipHostMallocDefault));
-        b->hup_cap = cap;
-    }
-    char* hp = b->hup;
-    if (cc.nops) {
-        std::memcpy(hp, prog.data(), cc.nops * sizeof(MOp));
-        BHIP(hipMemcpyAsync(b->prog.p + cc.prog0 * sizeof(MOp), hp, cc.nops * sizeof(MOp), hipMemcpyHostToDevice,
-                            b->st));
-        hp += cc.nops * sizeof(MOp);
-    }
-    if (cc.ncoef) {
-        std::memcpy(hp, coef.data(), cc.ncoef * 8);
-        BHIP(hipMemcpyAsync(b->coef.p + cc.coef0 * 8, hp, cc.ncoef * 8, hipMemcpyHostToDevice, b->st));
-        hp += cc.ncoef * 8;
-    }
-    if (m) {
-        uint64_t* hk = (uint64_t*)hp;
-        double* hr = (double*)(hk + m);
-        double* hi = hr + m;
-        uint64_t q = 0;
+        const QMap qm = choose_map((int)nq, b->cfg.bucket_bits, gates, ng);
+        for (uint64_t j = 0; j < ng; ++j)
+            if (!gate_local(gates[j], qm))
+                return bfail(SQE_EINVAL, "gate " + std::to_string(j) +
+                                             " crosses buckets: run this circuit with the single-circuit engine");
+        uint64_t m = 0;                          // entries with a nonzero amplitude
         for (uint64_t j = 0; j < n; ++j) {
-            if (re[j] == 0.0 && im[j] == 0.0) continue;
-            hk[q] = keys[j]; hr[q] = re[j]; hi[q] = im[j]; ++q;
+            if (keys[j] & ~lim) return bfail(SQE_EINVAL, "key has bits beyond nq");
+            m += (re[j] != 0.0 || im[j] != 0.0) ? 1 : 0;
         }
-        BHIP(hipMemcpyAsync(b->xkeys, hk, m * 8, hipMemcpyHostToDevice, b->st));
-        BHIP(hipMemcpyAsync(b->xre, hr, m * 8, hipMemcpyHostToDevice, b->st));
-        BHIP(hipMemcpyAsync(b->xim, hi, m * 8, hipMemcpyHostToDevice, b->st));
-    }
-    // load the state into the region (insert bases, number buckets, scatter)
-    const double ml = b->cfg.max_load > 0 && b->cfg.max_load < 0.95 ? b->cfg.max_load : 0.5;
-    const uint64_t tneed = std::max<uint64_t>(1024, p2((uint64_t)std::ceil((double)m / ml) + 8));
-    if (tneed > b->tcap) {
+        // program: gates, with a RENORM op after every renorm_every-th gate
+        std::vector<MOp> prog;
+        std::vector<double> coef;
+        std::vector<uint32_t> renorm_at;
+        prog.reserve(ng + 8);
+        for (uint64_t j = 0; j < ng; ++j) {
+            prog.push_back(encode_op(plan_op(gates[j], qm), coef, b->f32));
+            if (b->cfg.renorm_every && (j + 1) % b->cfg.renorm_every == 0) {
+                renorm_at.push_back((uint32_t)prog.size());
+                prog.push_back(encode_op(renorm_op(), coef, b->f32));
+            }
+        }
+        // region sizes and admission
+        const uint64_t S = 1ull << qm.mb;
+        const uint64_t tiles = std::max<uint64_t>(1, (m + 63) / 64);
+        const uint64_t rows = tiles * S;
+        const uint64_t bytes = tiles * 64 * 8 + rows * 64 * b->csize + prog.size() * sizeof(MOp) + coef.size() * 8;
+        if (b->used + bytes > b->cfg.hbm_budget_bytes)
+            return bfail(SQE_ETABLE_FULL, "admission: circuit needs " + std::to_string(bytes) + " bytes, " +
+                                              std::to_string(b->cfg.hbm_budget_bytes - b->used) + " left in the budget");
+        uint32_t id;
+        if (!b->free_slots.empty()) { id = b->free_slots.back(); b->free_slots.pop_back(); }
+        else { id = (uint32_t)b->c.size(); b->c.emplace_back(); }
+        BCK(slots_fit(b, id + 1));
+        Circ c;
+        c.nq = nq;
+        c.qm = qm;
+        c.tiles = tiles;
+        c.rows = rows;
+        c.nops = prog.size();
+        c.ncoef = coef.size();
+        c.renorm_at = renorm_at;
+        c.gates = ng;
+        c.bytes = bytes;
+        auto place = [&](RangeAlloc& al, Pool& pl, uint64_t units, uint64_t unit_bytes, uint64_t* at) -> sqe_status {
+            if (!units) { *at = 0; return SQE_OK; }
+            if (!al.take(units, at)) {
+                const uint64_t want = std::max<uint64_t>(al.cap * 2, al.cap + units);
+                BCK(pool_fit(b, pl, want * unit_bytes, al.used_extent() * unit_bytes));
+                al.grow(pl.bytes / unit_bytes);
+                if (!al.take(units, at)) return bfail(SQE_ENOMEM, "pool allocation");
+            }
+            return SQE_OK;
+        };
+        int placed = 0;
+        sqe_status st = place(b->ta, b->base, tiles, 64 * 8, &c.tile0);
+        if (st == SQE_OK && ++placed) st = place(b->ra, b->amp, rows, 64 * b->csize, &c.row0);
+        if (st == SQE_OK && ++placed) st = place(b->pa, b->prog, c.nops, sizeof(MOp), &c.prog0);
+        if (st == SQE_OK && ++placed) st = place(b->ca, b->coef, c.ncoef, 8, &c.coef0);
+        if (st != SQE_OK) {
+            if (placed > 0) b->ta.give(c.tile0, c.tiles);
+            if (placed > 1) b->ra.give(c.row0, c.rows);
+            if (placed > 2 && c.nops) b->pa.give(c.prog0, c.nops);
+            b->free_slots.push_back(id);
+            return st;
+        }
+        c.live = true;
+        b->c[id] = c;
+        b->used += bytes;
+        ++b->generation;
+        ++b->layout;
+        Circ& cc = b->c[id];
+        // the uploads go through one page-locked staging buffer: program,
+        // coefficients, then the nonzero entries' keys, real and imaginary parts
+        // (no copy from an earlier admission may still be reading it)
+        BCK(stage_fit(b, std::max<uint64_t>(m, 1)));
         BHIP(hipStreamSynchronize(b->st));
-        dfr(b, b->tab, b->tcap * sizeof(Slot));
-        b->tab = nullptr;
-        BCK(dal(b, (void**)&b->tab, tneed * sizeof(Slot)));
-        b->tcap = tneed;
-    }
-    BCK(buf_fit(b, &b->occ, &b->occ_cap, tiles * 64));
-    // the index of this admission uses the first tneed slots of the table
-    // (kept at its largest size), so its cost follows the circuit's size
-    BHIP(launch_fill_slots(b->st, b->tab, tneed));
-    BHIP(launch_fill_u64(b->st, base_ptr(b, cc), tiles * 64, EMPTY_KEY));
-    BHIP(launch_fill_u64(b->st, (uint64_t*)amp_ptr(b, cc), rows * 64 * b->csize / 8, 0));
-    BHIP(launch_fill_u64(b->st, (uint64_t*)b->occ, (tiles * 64 + 1) / 2, 0));
-    BHIP(launch_fill_u64(b->st, (uint64_t*)(b->upd + id), 1, 0));
-    BHIP(launch_fill_u64(b->st, (uint64_t*)(b->nnz + id), 1, 0));
-    LoadArgs la{};
-    la.keys = b->xkeys; la.re = b->xre; la.im = b->xim; la.n = m; la.qm = qm;
-    la.tab = b->tab; la.tmask = tneed - 1; la.amp = amp_ptr(b, cc); la.occ = b->occ; la.stats = b->stats;
-    EnumArgs ea{};
-    ea.tab = b->tab; ea.tcap = tneed; ea.out_base = base_ptr(b, cc); ea.out_nb = b->meta + id;
-    ea.out_cap = tiles * 64; ea.stats = b->stats;
-    const uint32_t eg = (uint32_t)((tneed + BLOCK - 1) / BLOCK);
-    BCK(next_scan(b, eg, &ea.scan));
-    BHIP(launch_load(b->f32, b->st, la, ea, eg));
-    const sqe_status e = check_err(b);
-    if (e != SQE_OK) {
-        release(b, id);
-        return e;
-    }
-    *id_out = id;
-    return SQE_OK;
+        const uint64_t up = cc.nops * sizeof(MOp) + cc.ncoef * 8 + 24 * m;
+        if (up > b->hup_cap) {
+            if (b->hup) (void)hipHostFree(b->hup);
+            b->hup = nullptr;
+            b->hup_cap = 0;
+            const uint64_t cap = std::max<uint64_t>(p2(up), 1 << 20);
+            BHIP(hipHostMalloc((void**)&b->hup, cap, hipHostMallocDefault));
+            b->hup_cap = cap;
+        }
+        char* hp = b->hup;
+        if (cc.nops) {
+            std::memcpy(hp, prog.data(), cc.nops * sizeof(MOp));
+            BHIP(hipMemcpyAsync(b->prog.p + cc.prog0 * sizeof(MOp), hp, cc.nops * sizeof(MOp), hipMemcpyHostToDevice,
+                                b->st));
+            hp += cc.nops * sizeof(MOp);
+        }
+        if (cc.ncoef) {
+            std::memcpy(hp, coef.data(), cc.ncoef * 8);
+            BHIP(hipMemcpyAsync(b->coef.p + cc.coef0 * 8, hp, cc.ncoef * 8, hipMemcpyHostToDevice, b->st));
+            hp += cc.ncoef * 8;
+        }
+        if (m) {
+            uint64_t* hk = (uint64_t*)hp;
+            double* hr = (double*)(hk + m);
+            double* hi = hr + m;
+            uint64_t q = 0;
+            for (uint64_t j = 0; j < n; ++j) {
+                if (re[j] == 0.0 && im[j] == 0.0) continue;
+                hk[q] = keys[j]; hr[q] = re[j]; hi[q] = im[j]; ++q;
+            }
+            BHIP(hipMemcpyAsync(b->xkeys, hk, m * 8, hipMemcpyHostToDevice, b->st));
+            BHIP(hipMemcpyAsync(b->xre, hr, m * 8, hipMemcpyHostToDevice, b->st));
+            BHIP(hipMemcpyAsync(b->xim, hi, m * 8, hipMemcpyHostToDevice, b->st));
+        }
+        // load the state into the region (insert bases, number buckets, scatter)
+        const double ml = b->cfg.max_load > 0 && b->cfg.max_load < 0.95 ? b->cfg.max_load : 0.5;
+        const uint64_t tneed = std::max<uint64_t>(1024, p2((uint64_t)std::ceil((double)m / ml) + 8));
+        if (tneed > b->tcap) {
+            BHIP(hipStreamSynchronize(b->st));
+            dfr(b, b->tab, b->tcap * sizeof(Slot));
+            b->tab = nullptr;
+            BCK(dal(b, (void**)&b->tab, tneed * sizeof(Slot)));
+            b->tcap = tneed;
+        }
+        BCK(buf_fit(b, &b->occ, &b->occ_cap, tiles * 64));
+        // the index of this admission uses the first tneed slots of the table
+        // (kept at its largest size), so its cost follows the circuit's size
+        BHIP(launch_fill_slots(b->st, b->tab, tneed));
+        BHIP(launch_fill_u64(b->st, base_ptr(b, cc), tiles * 64, EMPTY_KEY));
+        BHIP(launch_fill_u64(b->st, (uint64_t*)amp_ptr(b, cc), rows * 64 * b->csize / 8, 0));
+        BHIP(launch_fill_u64(b->st, (uint64_t*)b->occ, (tiles * 64 + 1) / 2, 0));
+        BHIP(launch_fill_u64(b->st, (uint64_t*)(b->upd + id), 1, 0));
+        BHIP(launch_fill_u64(b->st, (uint64_t*)(b->nnz + id), 1, 0));
+        LoadArgs la{};
+        la.keys = b->xkeys; la.re = b->xre; la.im = b->xim; la.n = m; la.qm = qm;
+        la.tab = b->tab; la.tmask = tneed - 1; la.amp = amp_ptr(b, cc); la.occ = b->occ; la.stats = b->stats;
+        EnumArgs ea{};
+        ea.tab = b->tab; ea.tcap = tneed; ea.out_base = base_ptr(b, cc); ea.out_nb = b->meta + id;
+        ea.out_cap = tiles * 64; ea.stats = b->stats;
+        const uint32_t eg = (uint32_t)((tneed + BLOCK - 1) / BLOCK);
+        BCK(next_scan(b, eg, &ea.scan));
+        BHIP(launch_load(b->f32, b->st, la, ea, eg));
+        const sqe_status e = check_err(b);
+        if (e != SQE_OK) {
+            release(b, id);
+            return e;
+        }
+        *id_out = id;
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_batch_step(sqe_batch* b, uint32_t max_gates) {
-    if (!b) return bfail(SQE_EINVAL, "null batch");
-    BHIP(hipSetDevice(b->cfg.device));
-    return step_all(b, max_gates);
+    return guard([&]() -> sqe_status {
+        if (!b) return bfail(SQE_EINVAL, "null batch");
+        BHIP(hipSetDevice(b->cfg.device));
+        return step_all(b, max_gates);
+    });
 }
 
 sqe_status sqe_batch_run(sqe_batch* b) {
-    if (!b) return bfail(SQE_EINVAL, "null batch");
-    BHIP(hipSetDevice(b->cfg.device));
-    if (!b->cfg.batch_eager) return run_graph(b);
-    for (;;) {
-        bool more = false;
-        for (const Circ& c : b->c) more |= c.live && c.pos < c.nops;
-        if (!more) break;
-        BCK(step_all(b, 0));
-    }
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        if (!b) return bfail(SQE_EINVAL, "null batch");
+        BHIP(hipSetDevice(b->cfg.device));
+        if (!b->cfg.batch_eager) return run_graph(b);
+        for (;;) {
+            bool more = false;
+            for (const Circ& c : b->c) more |= c.live && c.pos < c.nops;
+            if (!more) break;
+            BCK(step_all(b, 0));
+        }
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_batch_sync(sqe_batch* b) {
-    if (!b) return bfail(SQE_EINVAL, "null batch");
-    BHIP(hipSetDevice(b->cfg.device));
-    BHIP(hipStreamSynchronize(b->st));
-    return check_err(b);
+    return guard([&]() -> sqe_status {
+        if (!b) return bfail(SQE_EINVAL, "null batch");
+        BHIP(hipSetDevice(b->cfg.device));
+        BHIP(hipStreamSynchronize(b->st));
+        return check_err(b);
+    });
 }
 
 sqe_status sqe_batch_info(sqe_batch* b, int64_t id, sqe_batch_circuit* out) {
-    if (!b || !out || id < 0 || (uint64_t)id >= b->c.size() || !b->c[id].live)
-        return bfail(SQE_EINVAL, "unknown circuit");
-    BHIP(hipSetDevice(b->cfg.device));
-    BHIP(hipStreamSynchronize(b->st));
-    const Circ& c = b->c[id];
-    unsigned long long v[3] = {0, 0, 0};
-    BHIP(hipMemcpy(&v[0], b->upd + id, 8, hipMemcpyDeviceToHost));
-    BHIP(hipMemcpy(&v[1], b->nnz + id, 8, hipMemcpyDeviceToHost));
-    BHIP(hipMemcpy(&v[2], b->meta + id, 8, hipMemcpyDeviceToHost));
-    out->done = c.pos >= c.nops ? 1 : 0;
-    out->gates_total = c.gates;
-    uint64_t gd = 0;
-    for (uint64_t j = 0; j < c.pos; ++j) {
-        bool rn = false;
-        for (uint32_t r : c.renorm_at) rn |= (r == j);
-        gd += rn ? 0 : 1;
-    }
-    out->gates_done = gd;
-    out->updates = v[0];
-    out->nnz = v[1];
-    out->buckets = v[2];
-    out->bucket_bits = c.qm.mb;
-    out->bytes = c.bytes;
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        if (!b || !out || id < 0 || (uint64_t)id >= b->c.size() || !b->c[id].live)
+            return bfail(SQE_EINVAL, "unknown circuit");
+        BHIP(hipSetDevice(b->cfg.device));
+        BHIP(hipStreamSynchronize(b->st));
+        const Circ& c = b->c[id];
+        unsigned long long v[3] = {0, 0, 0};
+        BHIP(hipMemcpy(&v[0], b->upd + id, 8, hipMemcpyDeviceToHost));
+        BHIP(hipMemcpy(&v[1], b->nnz + id, 8, hipMemcpyDeviceToHost));
+        BHIP(hipMemcpy(&v[2], b->meta + id, 8, hipMemcpyDeviceToHost));
+        out->done = c.pos >= c.nops ? 1 : 0;
+        out->gates_total = c.gates;
+        uint64_t gd = 0;
+        for (uint64_t j = 0; j < c.pos; ++j) {
+            bool rn = false;
+            for (uint32_t r : c.renorm_at) rn |= (r == j);
+            gd += rn ? 0 : 1;
+        }
+        out->gates_done = gd;
+        out->updates = v[0];
+        out->nnz = v[1];
+        out->buckets = v[2];
+        out->bucket_bits = c.qm.mb;
+        out->bytes = c.bytes;
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_batch_export(sqe_batch* b, int64_t id, uint64_t* keys, double* re, double* im, uint64_t cap,
                             uint64_t* n) {
-    if (!b || !n || id < 0 || (uint64_t)id >= b->c.size() || !b->c[id].live)
-        return bfail(SQE_EINVAL, "unknown circuit");
-    BHIP(hipSetDevice(b->cfg.device));
-    const Circ& c = b->c[id];
-    // staging for every amplitude slot of the circuit's region, so the bucket
-    // count need not be read back first; the total and the error word come
-    // back together
-    const uint64_t nbk = c.tiles * 64;
-    BCK(stage_fit(b, std::max<uint64_t>(nbk << c.qm.mb, 1)));
-    ExportArgs xa{};
-    xa.base = base_ptr(b, c); xa.amp = amp_ptr(b, c); xa.nb = b->meta + id; xa.qm = c.qm; xa.thr = b->thr;
-    xa.filter = 1; xa.keys = b->xkeys; xa.re = b->xre; xa.im = b->xim; xa.cap = b->xcap; xa.total = b->total;
-    const uint32_t g = std::max<uint32_t>(1, (uint32_t)((nbk + BLOCK - 1) / BLOCK));
-    BCK(next_scan(b, g, &xa.scan));
-    BHIP(launch_export(c.qm.mb, b->f32, b->st, g, xa));
-    unsigned long long hv[2] = {0, 0};
-    BHIP(hipMemcpyAsync(&hv[0], b->total, 8, hipMemcpyDeviceToHost, b->st));
-    BHIP(hipMemcpyAsync(&hv[1], &b->stats->err, 8, hipMemcpyDeviceToHost, b->st));
-    BHIP(hipStreamSynchronize(b->st));
-    if (hv[1]) BCK(check_err(b));
-    const uint64_t tot = hv[0];
-    *n = tot;
-    if (tot > cap) return bfail(SQE_EINVAL, "export capacity too small (" + std::to_string(tot) + " needed)");
-    if (tot) {
-        if (!keys || !re || !im) return bfail(SQE_EINVAL, "null array");
-        BHIP(hipMemcpyAsync(keys, b->xkeys, tot * 8, hipMemcpyDeviceToHost, b->st));
-        BHIP(hipMemcpyAsync(re, b->xre, tot * 8, hipMemcpyDeviceToHost, b->st));
-        BHIP(hipMemcpyAsync(im, b->xim, tot * 8, hipMemcpyDeviceToHost, b->st));
+    return guard([&]() -> sqe_status {
+        if (!b || !n || id < 0 || (uint64_t)id >= b->c.size() || !b->c[id].live)
+            return bfail(SQE_EINVAL, "unknown circuit");
+        BHIP(hipSetDevice(b->cfg.device));
+        const Circ& c = b->c[id];
+        // staging for every amplitude slot of the circuit's region, so the bucket
+        // count need not be read back first; the total and the error word come
+        // back together
+        const uint64_t nbk = c.tiles * 64;
+        BCK(stage_fit(b, std::max<uint64_t>(nbk << c.qm.mb, 1)));
+        ExportArgs xa{};
+        xa.base = base_ptr(b, c); xa.amp = amp_ptr(b, c); xa.nb = b->meta + id; xa.qm = c.qm; xa.thr = b->thr;
+        xa.filter = 1; xa.keys = b->xkeys; xa.re = b->xre; xa.im = b->xim; xa.cap = b->xcap; xa.total = b->total;
+        const uint32_t g = std::max<uint32_t>(1, (uint32_t)((nbk + BLOCK - 1) / BLOCK));
+        BCK(next_scan(b, g, &xa.scan));
+        BHIP(launch_export(c.qm.mb, b->f32, b->st, g, xa));
+        unsigned long long hv[2] = {0, 0};
+        BHIP(hipMemcpyAsync(&hv[0], b->total, 8, hipMemcpyDeviceToHost, b->st));
+        BHIP(hipMemcpyAsync(&hv[1], &b->stats->err, 8, hipMemcpyDeviceToHost, b->st));
         BHIP(hipStreamSynchronize(b->st));
-    }
-    return SQE_OK;
+        if (hv[1]) BCK(check_err(b));
+        const uint64_t tot = hv[0];
+        *n = tot;
+        if (tot > cap) return bfail(SQE_EINVAL, "export capacity too small (" + std::to_string(tot) + " needed)");
+        if (tot) {
+            if (!keys || !re || !im) return bfail(SQE_EINVAL, "null array");
+            BHIP(hipMemcpyAsync(keys, b->xkeys, tot * 8, hipMemcpyDeviceToHost, b->st));
+            BHIP(hipMemcpyAsync(re, b->xre, tot * 8, hipMemcpyDeviceToHost, b->st));
+            BHIP(hipMemcpyAsync(im, b->xim, tot * 8, hipMemcpyDeviceToHost, b->st));
+            BHIP(hipStreamSynchronize(b->st));
+        }
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_batch_remove(sqe_batch* b, int64_t id) {
-    if (!b || id < 0 || (uint64_t)id >= b->c.size() || !b->c[id].live) return bfail(SQE_EINVAL, "unknown circuit");
-    BHIP(hipSetDevice(b->cfg.device));
-    // no wait: work already enqueued on the region (an export) runs before
-    // anything a later admission enqueues there, and pool reallocation
-    // synchronises before it frees
-    release(b, (uint32_t)id);
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        if (!b || id < 0 || (uint64_t)id >= b->c.size() || !b->c[id].live) return bfail(SQE_EINVAL, "unknown circuit");
+        BHIP(hipSetDevice(b->cfg.device));
+        // no wait: work already enqueued on the region (an export) runs before
+        // anything a later admission enqueues there, and pool reallocation
+        // synchronises before it frees
+        release(b, (uint32_t)id);
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_batch_finished(sqe_batch* b, int64_t* ids, uint64_t cap, uint64_t* n) {
-    if (!b || !n) return bfail(SQE_EINVAL, "null argument");
-    uint64_t k = 0;
-    for (uint32_t id = 0; id < b->c.size(); ++id) {
-        const Circ& c = b->c[id];
-        if (!c.live || c.pos < c.nops) continue;
-        if (ids && k < cap) ids[k] = id;
-        ++k;
-    }
-    *n = k;
-    return k > cap ? bfail(SQE_EINVAL, "id capacity too small") : SQE_OK;
+    return guard([&]() -> sqe_status {
+        if (!b || !n) return bfail(SQE_EINVAL, "null argument");
+        uint64_t k = 0;
+        for (uint32_t id = 0; id < b->c.size(); ++id) {
+            const Circ& c = b->c[id];
+            if (!c.live || c.pos < c.nops) continue;
+            if (ids && k < cap) ids[k] = id;
+            ++k;
+        }
+        *n = k;
+        return k > cap ? bfail(SQE_EINVAL, "id capacity too small") : SQE_OK;
+    });
 }
 
 sqe_status sqe_batch_usage(sqe_batch* b, uint64_t* used, uint64_t* budget, uint64_t* active) {
-    if (!b) return bfail(SQE_EINVAL, "null batch");
-    if (used) *used = b->used;
-    if (budget) *budget = b->cfg.hbm_budget_bytes;
-    if (active) {
-        uint64_t a = 0;
-        for (const Circ& c : b->c) a += c.live && c.pos < c.nops ? 1 : 0;
-        *active = a;
-    }
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        if (!b) return bfail(SQE_EINVAL, "null batch");
+        if (used) *used = b->used;
+        if (budget) *budget = b->cfg.hbm_budget_bytes;
+        if (active) {
+            uint64_t a = 0;
+            for (const Circ& c : b->c) a += c.live && c.pos < c.nops ? 1 : 0;
+            *active = a;
+        }
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_batch_get_tuning(sqe_batch* b, sqe_batch_tuning* t) {
-    if (!b || !t) return bfail(SQE_EINVAL, "null argument");
-    t->block = b->block;
-    t->tiles_per_item = b->gran;
-    t->load_factor = b->cfg.max_load > 0 ? b->cfg.max_load : 0.5;
-    t->sweep_ms = b->sweep_ms;
-    t->launches = b->launches;
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        if (!b || !t) return bfail(SQE_EINVAL, "null argument");
+        t->block = b->block;
+        t->tiles_per_item = b->gran;
+        t->load_factor = b->cfg.max_load > 0 ? b->cfg.max_load : 0.5;
+        t->sweep_ms = b->sweep_ms;
+        t->launches = b->launches;
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t) {
-    if (!b || !t) return bfail(SQE_EINVAL, "null argument");
-    if (t->block != 64 && t->block != 128 && t->block != 256) return bfail(SQE_EINVAL, "block must be 64/128/256");
-    if (t->tiles_per_item < 1 || t->tiles_per_item > 64) return bfail(SQE_EINVAL, "tiles_per_item in 1..64");
-    b->block = t->block;
-    b->gran = t->tiles_per_item;
-    if (t->load_factor > 0 && t->load_factor < 0.95) b->cfg.max_load = t->load_factor;
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        if (!b || !t) return bfail(SQE_EINVAL, "null argument");
+        if (t->block != 64 && t->block != 128 && t->block != 256) return bfail(SQE_EINVAL, "block must be 64/128/256");
+        if (t->tiles_per_item < 1 || t->tiles_per_item > 64) return bfail(SQE_EINVAL, "tiles_per_item in 1..64");
+        b->block = t->block;
+        b->gran = t->tiles_per_item;
+        if (t->load_factor > 0 && t->load_factor < 0.95) b->cfg.max_load = t->load_factor;
+        return SQE_OK;
+    });
 }
 
 // Time every (workgroup size, tiles per item) candidate on the
@@ -1033,78 +1076,88 @@ sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t) {
 // trial -- and keep the fastest.  With captured runs, each candidate is timed
 // on a replay of its graph.
 sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
-    if (!b) return bfail(SQE_EINVAL, "null batch");
-    BHIP(hipSetDevice(b->cfg.device));
-    BCK(snap_save(b));
-    struct Cand { uint32_t block, gran; };
-    std::vector<Cand> cands;
-    for (uint32_t bl : {64u, 128u, 256u})
-        for (uint32_t g : {1u, 2u, 4u, 8u}) cands.push_back(Cand{bl, g});
-    double best = 1e300;
-    Cand bc{b->block, b->gran};
-    const uint32_t timing = b->cfg.timing;
-    b->cfg.timing = 0;
-    for (const Cand& cd : cands) {
-        b->block = cd.block;
-        b->gran = cd.gran;
-        if (!b->cfg.batch_eager) {
-            // capture and instantiate this candidate's graph outside the timing
+    return guard([&]() -> sqe_status {
+        if (!b) return bfail(SQE_EINVAL, "null batch");
+        BHIP(hipSetDevice(b->cfg.device));
+        BCK(snap_save(b));
+        struct Cand { uint32_t block, gran; };
+        std::vector<Cand> cands;
+        for (uint32_t bl : {64u, 128u, 256u})
+            for (uint32_t g : {1u, 2u, 4u, 8u}) cands.push_back(Cand{bl, g});
+        double best = 1e300;
+        Cand bc{b->block, b->gran};
+        const uint32_t timing = b->cfg.timing;
+        b->cfg.timing = 0;
+        for (const Cand& cd : cands) {
+            b->block = cd.block;
+            b->gran = cd.gran;
+            if (!b->cfg.batch_eager) {
+                // capture and instantiate this candidate's graph outside the timing
+                BCK(sqe_batch_run(b));
+                BCK(snap_restore(b));
+                BHIP(hipStreamSynchronize(b->st));
+            }
+            BHIP(hipEventRecord(b->ev0, b->st));
             BCK(sqe_batch_run(b));
+            BHIP(hipEventRecord(b->ev1, b->st));
+            BHIP(hipEventSynchronize(b->ev1));
+            float ms = 0.f;
+            BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            if (ms < best) { best = ms; bc = cd; }
             BCK(snap_restore(b));
             BHIP(hipStreamSynchronize(b->st));
         }
-        BHIP(hipEventRecord(b->ev0, b->st));
-        BCK(sqe_batch_run(b));
-        BHIP(hipEventRecord(b->ev1, b->st));
-        BHIP(hipEventSynchronize(b->ev1));
-        float ms = 0.f;
-        BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-        if (ms < best) { best = ms; bc = cd; }
-        BCK(snap_restore(b));
-        BHIP(hipStreamSynchronize(b->st));
-    }
-    b->cfg.timing = timing;
-    b->block = bc.block;
-    b->gran = bc.gran;
-    if (best_out) {
-        BCK(sqe_batch_get_tuning(b, best_out));
-        best_out->sweep_ms = best;
-    }
-    return SQE_OK;
+        b->cfg.timing = timing;
+        b->block = bc.block;
+        b->gran = bc.gran;
+        if (best_out) {
+            BCK(sqe_batch_get_tuning(b, best_out));
+            best_out->sweep_ms = best;
+        }
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_batch_snapshot_save(sqe_batch* b) {
-    if (!b) return bfail(SQE_EINVAL, "null batch");
-    BHIP(hipSetDevice(b->cfg.device));
-    return snap_save(b);
+    return guard([&]() -> sqe_status {
+        if (!b) return bfail(SQE_EINVAL, "null batch");
+        BHIP(hipSetDevice(b->cfg.device));
+        return snap_save(b);
+    });
 }
 
 sqe_status sqe_batch_snapshot_restore(sqe_batch* b) {
-    if (!b) return bfail(SQE_EINVAL, "null batch");
-    BHIP(hipSetDevice(b->cfg.device));
-    return snap_restore(b);
+    return guard([&]() -> sqe_status {
+        if (!b) return bfail(SQE_EINVAL, "null batch");
+        BHIP(hipSetDevice(b->cfg.device));
+        return snap_restore(b);
+    });
 }
 
 const char* sqe_batch_last_error(void) { return b_err.c_str(); }
 
 sqe_status sqe_host_alloc(uint64_t bytes, void** out) {
-    if (!out) return bfail(SQE_EINVAL, "null argument");
-    *out = nullptr;
-    if (!bytes) return SQE_OK;
-    if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) {
-        (void)hipGetLastError();
+    return guard([&]() -> sqe_status {
+        if (!out) return bfail(SQE_EINVAL, "null argument");
         *out = nullptr;
-        return bfail(SQE_ENOMEM, "hipHostMalloc(" + std::to_string(bytes) + ") failed");
-    }
-    return SQE_OK;
+        if (!bytes) return SQE_OK;
+        if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            *out = nullptr;
+            return bfail(SQE_ENOMEM, "hipHostMalloc(" + std::to_string(bytes) + ") failed");
+        }
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_host_free(void* p) {
-    if (p && hipHostFree(p) != hipSuccess) {
-        (void)hipGetLastError();
-        return bfail(SQE_EHIP, "hipHostFree failed");
-    }
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        if (p && hipHostFree(p) != hipSuccess) {
+            (void)hipGetLastError();
+            return bfail(SQE_EHIP, "hipHostFree failed");
+        }
+        return SQE_OK;
+    });
 }
 
 }  // extern "C"

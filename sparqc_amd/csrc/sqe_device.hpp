@@ -81,15 +81,19 @@ enum : uint32_t {
 __host__ __device__ constexpr bool branching_handler(uint32_t h) { return h >= H_SLOT_H && h != H_RENORM; }
 
 // One sweep op: 16 dwords, staged 64 at a time in the lanes of a wavefront.
-// hdr = handler | J << 4 | target << 7 | side-table index << 13; ctl = cmS | cvS << 8.
+// hdr = handler | J << 4 | target << 7; ctl = cmS | cvS << 8.  Ops that read
+// the coefficient side table (H_SLOT_U, H_SLOT_BLOCK) carry its index, in
+// units of 8 doubles, as a full 64-bit integer in the bits of c[4]: a field of
+// its own, so no program is too long for it.
 struct MOp {
     uint32_t hdr;
     uint32_t ctl;
     uint64_t cmB, cvB;
     double c[5];
 };
-constexpr uint32_t NO_COEF = 0x3ffffu;
 static_assert(sizeof(MOp) == 64, "MOp is sixteen dwords");
+__host__ __device__ inline double coef_index_bits(uint64_t ci) { return __builtin_bit_cast(double, ci); }
+__host__ __device__ inline uint64_t coef_index_of(double c4) { return __builtin_bit_cast(uint64_t, c4); }
 
 // The same pointer in the constant address space: a load through it at a
 // wavefront-uniform address becomes a scalar load into SGPRs.  Only for memory

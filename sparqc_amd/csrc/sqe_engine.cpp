@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <new>
+#include <stdexcept>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -27,6 +29,10 @@
 
 using namespace sqe;
 
+#ifndef SQE_NORM_IN_SWEEP
+#define SQE_NORM_IN_SWEEP 1
+#endif
+
 namespace {
 
 thread_local std::string g_err;
@@ -34,6 +40,19 @@ thread_local std::string g_err;
 sqe_status fail(sqe_status s, const std::string& msg) {
     g_err = msg;
     return s;
+}
+
+// No C++ exception crosses the C ABI: host allocation failures (and anything
+// else a standard container throws) become status codes.
+template <class F>
+sqe_status guard(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(SQE_ENOMEM, "host allocation failed");
+    } catch (const std::exception& ex) {
+        return fail(SQE_EINVAL, std::string("host error: ") + ex.what());
+    }
 }
 
 #define HIPCK(x)                                                                         \
@@ -93,6 +112,7 @@ struct sqe_engine {
     Store snap;
     bool snap_valid = false;
     QMap snap_qm{};
+    uint64_t snap_gates = 0;                // gate count of the snapshot (renormalisation points follow it)
 
     unsigned long long* meta = nullptr;     // [0],[1] bucket counts of the stores, [2] export total, [3] snapshot count,
                                             // [4] finished-workgroup count of a norm-folding sweep
@@ -708,8 +728,10 @@ sqe_status flush(sqe_engine* e) {
         case ST_SWEEP: {
             bool has_block = false;
             for (uint64_t i = s.off; i < s.off + s.n; ++i) has_block |= prog[i].kind == K_BLOCK;
-            // a renormalisation right after this sweep takes the norm from its epilogue
-            const bool norm_next = si + 1 < steps.size() && steps[si + 1].kind == ST_NORM;
+            // a renormalisation right after this sweep takes the norm from its
+            // epilogue (SQE_NORM_IN_SWEEP=0 at build time: a separate
+            // reduction instead, for debugging)
+            const bool norm_next = SQE_NORM_IN_SWEEP && si + 1 < steps.size() && steps[si + 1].kind == ST_NORM;
             SQCK(run_sweep(e, dm + s.off, dc, s.n, 1, has_block, norm_next ? &fused_norm : nullptr));
             ++e->n_sweeps;
             for (uint64_t i = s.off; i < s.off + s.n; ++i) {
@@ -800,252 +822,278 @@ void sqe_config_default(sqe_config* c, uint32_t nq) {
 }
 
 sqe_status sqe_create(const sqe_config* cfg, sqe_engine** out) {
-    if (!cfg || !out) return fail(SQE_EINVAL, "null argument");
-    *out = nullptr;
-    if (cfg->nq < 1 || cfg->nq > 63) return fail(SQE_EINVAL, "nq must be in 1..63");
-    if (cfg->prec != SQE_F64 && cfg->prec != SQE_F32) return fail(SQE_EINVAL, "bad precision");
-    if (cfg->bucket_bits > (cfg->prec == SQE_F32 ? 5 : 4)) return fail(SQE_EINVAL, "bucket_bits too large");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
-        (void)hipGetLastError();
-        return fail(SQE_EDEVICE, "no HIP device");
-    }
-    if (cfg->device < 0 || cfg->device >= ndev) return fail(SQE_EINVAL, "bad device ordinal");
-    HIPCK(hipSetDevice(cfg->device));
-    sqe_engine* e = new (std::nothrow) sqe_engine();
-    if (!e) return fail(SQE_ENOMEM, "host allocation");
-    e->cfg = *cfg;
-    e->f32 = cfg->prec == SQE_F32;
-    e->thr = cfg->prune_thr > 0.0 ? cfg->prune_thr : (e->f32 ? 1e-7 : 1e-14);
-    if (e->cfg.hbm_budget_bytes == 0) {
-        size_t fr = 0, tot = 0;
-        if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete e; return fail(SQE_EHIP, "hipMemGetInfo"); }
-        const double keep = e->cfg.mem_reserve_ppm ? std::min(1.0, e->cfg.mem_reserve_ppm * 1e-6) : 0.1;
-        e->cfg.hbm_budget_bytes = (uint64_t)((1.0 - keep) * (double)fr);
-    }
-    sqe_status r = SQE_OK;
-    if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) r = fail(SQE_EHIP, "stream");
-    if (r == SQE_OK && hipEventCreateWithFlags(&e->upload_done, hipEventDisableTiming) != hipSuccess)
-        r = fail(SQE_EHIP, "event");
-    if (r == SQE_OK) r = dalloc(e, (void**)&e->meta, 64);
-    if (r == SQE_OK) r = dalloc(e, (void**)&e->stats, sizeof(Stats));
-    if (r == SQE_OK) r = dalloc(e, (void**)&e->dscal, 16);
-    if (r == SQE_OK) r = dalloc(e, (void**)&e->ticket, 64);
-    if (r == SQE_OK && hipMemsetAsync(e->meta, 0, 64, e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
-    if (r == SQE_OK && hipMemsetAsync(e->stats, 0, sizeof(Stats), e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
-    if (r == SQE_OK && hipMemsetAsync(e->dscal, 0, 16, e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
-    if (r == SQE_OK && hipMemsetAsync(e->ticket, 0, 64, e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
-    if (r == SQE_OK && hipStreamSynchronize(e->st) != hipSuccess) r = fail(SQE_EHIP, "sync");
-    if (r != SQE_OK) {
-        destroy_all(e);
-        delete e;
-        return r;
-    }
-    *out = e;
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        if (!cfg || !out) return fail(SQE_EINVAL, "null argument");
+        *out = nullptr;
+        if (cfg->nq < 1 || cfg->nq > 63) return fail(SQE_EINVAL, "nq must be in 1..63");
+        if (cfg->prec != SQE_F64 && cfg->prec != SQE_F32) return fail(SQE_EINVAL, "bad precision");
+        if (cfg->bucket_bits > (cfg->prec == SQE_F32 ? 5 : 4)) return fail(SQE_EINVAL, "bucket_bits too large");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+            (void)hipGetLastError();
+            return fail(SQE_EDEVICE, "no HIP device");
+        }
+        if (cfg->device < 0 || cfg->device >= ndev) return fail(SQE_EINVAL, "bad device ordinal");
+        HIPCK(hipSetDevice(cfg->device));
+        sqe_engine* e = new (std::nothrow) sqe_engine();
+        if (!e) return fail(SQE_ENOMEM, "host allocation");
+        e->cfg = *cfg;
+        e->f32 = cfg->prec == SQE_F32;
+        e->thr = cfg->prune_thr > 0.0 ? cfg->prune_thr : (e->f32 ? 1e-7 : 1e-14);
+        if (e->cfg.hbm_budget_bytes == 0) {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete e; return fail(SQE_EHIP, "hipMemGetInfo"); }
+            const double keep = e->cfg.mem_reserve_ppm ? std::min(1.0, e->cfg.mem_reserve_ppm * 1e-6) : 0.1;
+            e->cfg.hbm_budget_bytes = (uint64_t)((1.0 - keep) * (double)fr);
+        }
+        sqe_status r = SQE_OK;
+        if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) r = fail(SQE_EHIP, "stream");
+        if (r == SQE_OK && hipEventCreateWithFlags(&e->upload_done, hipEventDisableTiming) != hipSuccess)
+            r = fail(SQE_EHIP, "event");
+        if (r == SQE_OK) r = dalloc(e, (void**)&e->meta, 64);
+        if (r == SQE_OK) r = dalloc(e, (void**)&e->stats, sizeof(Stats));
+        if (r == SQE_OK) r = dalloc(e, (void**)&e->dscal, 16);
+        if (r == SQE_OK) r = dalloc(e, (void**)&e->ticket, 64);
+        if (r == SQE_OK && hipMemsetAsync(e->meta, 0, 64, e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
+        if (r == SQE_OK && hipMemsetAsync(e->stats, 0, sizeof(Stats), e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
+        if (r == SQE_OK && hipMemsetAsync(e->dscal, 0, 16, e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
+        if (r == SQE_OK && hipMemsetAsync(e->ticket, 0, 64, e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
+        if (r == SQE_OK && hipStreamSynchronize(e->st) != hipSuccess) r = fail(SQE_EHIP, "sync");
+        if (r != SQE_OK) {
+            destroy_all(e);
+            delete e;
+            return r;
+        }
+        *out = e;
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_destroy(sqe_engine* e) {
-    if (!e) return SQE_OK;
-    (void)hipSetDevice(e->cfg.device);
-    destroy_all(e);
-    delete e;
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        if (!e) return SQE_OK;
+        (void)hipSetDevice(e->cfg.device);
+        destroy_all(e);
+        delete e;
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_load(sqe_engine* e, const uint64_t* keys, const double* re, const double* im, uint64_t n) {
-    SQCK(enter(e));
-    if (n && (!keys || !re || !im)) return fail(SQE_EINVAL, "null array");
-    const uint64_t lim = (e->cfg.nq >= 64) ? ~0ull : ((1ull << e->cfg.nq) - 1);
-    std::vector<uint64_t> k;
-    std::vector<double> r, i;
-    k.reserve(n); r.reserve(n); i.reserve(n);
-    for (uint64_t j = 0; j < n; ++j) {
-        if (keys[j] & ~lim) return fail(SQE_EINVAL, "key has bits beyond nq");
-        if (re[j] == 0.0 && im[j] == 0.0) continue;
-        k.push_back(keys[j]); r.push_back(re[j]); i.push_back(im[j]);
-    }
-    // queued gates belonged to the state being replaced
-    e->pending.clear();
-    HIPCK(hipStreamSynchronize(e->st));
-    e->hkeys.swap(k);
-    e->hre.swap(r);
-    e->him.swap(i);
-    e->have_state = true;
-    e->on_device = false;
-    e->tab_valid = false;
-    e->renorm_pending = false;
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        if (n && (!keys || !re || !im)) return fail(SQE_EINVAL, "null array");
+        const uint64_t lim = (e->cfg.nq >= 64) ? ~0ull : ((1ull << e->cfg.nq) - 1);
+        std::vector<uint64_t> k;
+        std::vector<double> r, i;
+        k.reserve(n); r.reserve(n); i.reserve(n);
+        for (uint64_t j = 0; j < n; ++j) {
+            if (keys[j] & ~lim) return fail(SQE_EINVAL, "key has bits beyond nq");
+            if (re[j] == 0.0 && im[j] == 0.0) continue;
+            k.push_back(keys[j]); r.push_back(re[j]); i.push_back(im[j]);
+        }
+        // queued gates belonged to the state being replaced
+        e->pending.clear();
+        HIPCK(hipStreamSynchronize(e->st));
+        e->hkeys.swap(k);
+        e->hre.swap(r);
+        e->him.swap(i);
+        e->have_state = true;
+        e->on_device = false;
+        e->tab_valid = false;
+        e->renorm_pending = false;
+        // a new state starts a new gate count (renormalisation points follow it)
+        e->gates = 0;
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_set_basis(sqe_engine* e, uint64_t key) {
-    const double one = 1.0, zero = 0.0;
-    return sqe_load(e, &key, &one, &zero, 1);
+    return guard([&]() -> sqe_status {
+        const double one = 1.0, zero = 0.0;
+        return sqe_load(e, &key, &one, &zero, 1);
+    });
 }
 
 sqe_status sqe_apply_gates(sqe_engine* e, const sqe_gate* g, uint64_t n) {
-    SQCK(enter(e));
-    if (n && !g) return fail(SQE_EINVAL, "null gates");
-    const uint64_t lim = (1ull << e->cfg.nq) - 1;
-    for (uint64_t j = 0; j < n; ++j) {
-        const sqe_gate& x = g[j];
-        if (x.kind > SQE_GATE_PERM) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": bad kind");
-        if (x.target >= e->cfg.nq) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": target out of range");
-        if (x.ctrl_mask & ~lim) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": control out of range");
-        if ((x.ctrl_mask >> x.target) & 1ull) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": target is a control");
-        if (x.ctrl_val & ~x.ctrl_mask) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": ctrl_val outside ctrl_mask");
-    }
-    e->pending.insert(e->pending.end(), g, g + n);
-    // bounded batches keep the coefficient-table index within its 18 bits
-    if (e->pending.size() >= (1u << 17)) return flush(e);
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        if (n && !g) return fail(SQE_EINVAL, "null gates");
+        const uint64_t lim = (1ull << e->cfg.nq) - 1;
+        for (uint64_t j = 0; j < n; ++j) {
+            const sqe_gate& x = g[j];
+            if (x.kind > SQE_GATE_PERM) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": bad kind");
+            if (x.target >= e->cfg.nq) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": target out of range");
+            if (x.ctrl_mask & ~lim) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": control out of range");
+            if ((x.ctrl_mask >> x.target) & 1ull) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": target is a control");
+            if (x.ctrl_val & ~x.ctrl_mask) return fail(SQE_EINVAL, "gate " + std::to_string(j) + ": ctrl_val outside ctrl_mask");
+        }
+        e->pending.insert(e->pending.end(), g, g + n);
+        // bounded batches bound the host planning memory and the program upload
+        if (e->pending.size() >= (1u << 17)) return flush(e);
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_apply(sqe_engine* e, uint32_t target, uint64_t ctrl_mask, uint64_t ctrl_val, const double u[8],
                      sqe_class cls) {
-    if (!u) return fail(SQE_EINVAL, "null matrix");
-    sqe_gate g{};
-    g.target = target;
-    g.ctrl_mask = ctrl_mask;
-    g.ctrl_val = ctrl_val;
-    switch (cls) {
-    case SQE_DIAG:
-        g.kind = SQE_GATE_DIAG;
-        g.p[0] = u[0]; g.p[1] = u[1]; g.p[2] = u[6]; g.p[3] = u[7];
-        break;
-    case SQE_PERM:
-        g.kind = SQE_GATE_PERM;
-        g.p[0] = u[2]; g.p[1] = u[3]; g.p[2] = u[4]; g.p[3] = u[5];
-        break;
-    case SQE_BRANCH:
-        g.kind = SQE_GATE_U;
-        for (int k = 0; k < 8; ++k) g.p[k] = u[k];
-        break;
-    default:
-        return fail(SQE_EINVAL, "bad gate class");
-    }
-    return sqe_apply_gates(e, &g, 1);
+    return guard([&]() -> sqe_status {
+        if (!u) return fail(SQE_EINVAL, "null matrix");
+        sqe_gate g{};
+        g.target = target;
+        g.ctrl_mask = ctrl_mask;
+        g.ctrl_val = ctrl_val;
+        switch (cls) {
+        case SQE_DIAG:
+            g.kind = SQE_GATE_DIAG;
+            g.p[0] = u[0]; g.p[1] = u[1]; g.p[2] = u[6]; g.p[3] = u[7];
+            break;
+        case SQE_PERM:
+            g.kind = SQE_GATE_PERM;
+            g.p[0] = u[2]; g.p[1] = u[3]; g.p[2] = u[4]; g.p[3] = u[5];
+            break;
+        case SQE_BRANCH:
+            g.kind = SQE_GATE_U;
+            for (int k = 0; k < 8; ++k) g.p[k] = u[k];
+            break;
+        default:
+            return fail(SQE_EINVAL, "bad gate class");
+        }
+        return sqe_apply_gates(e, &g, 1);
+    });
 }
 
 sqe_status sqe_sync(sqe_engine* e) {
-    SQCK(enter(e));
-    SQCK(full_sync(e));
-    if (e->on_device) {
-        // drop buckets left all-zero by pruning once they are a quarter of the store
-        Stats h;
-        SQCK(read_stats(e, &h));
-        uint64_t nb = 0;
-        SQCK(read_nb(e, &nb));
-        if (h.empty * 4 > nb && nb > 1024) {
-            SQCK(compact(e));
-            HIPCK(hipStreamSynchronize(e->st));
-            SQCK(read_nb(e, &e->nb_upper));
-            collect_times(e);
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        SQCK(full_sync(e));
+        if (e->on_device) {
+            // drop buckets left all-zero by pruning once they are a quarter of the store
+            Stats h;
+            SQCK(read_stats(e, &h));
+            uint64_t nb = 0;
+            SQCK(read_nb(e, &nb));
+            if (h.empty * 4 > nb && nb > 1024) {
+                SQCK(compact(e));
+                HIPCK(hipStreamSynchronize(e->st));
+                SQCK(read_nb(e, &e->nb_upper));
+                collect_times(e);
+            }
         }
-    }
-    return SQE_OK;
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_nnz(sqe_engine* e, uint64_t* n) {
-    SQCK(enter(e));
-    if (!n) return fail(SQE_EINVAL, "null output");
-    Stats h;
-    SQCK(count_now(e, &h));
-    *n = h.nnz;
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        if (!n) return fail(SQE_EINVAL, "null output");
+        Stats h;
+        SQCK(count_now(e, &h));
+        *n = h.nnz;
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_norm2(sqe_engine* e, double* out) {
-    SQCK(enter(e));
-    if (!out) return fail(SQE_EINVAL, "null output");
-    SQCK(full_sync(e));
-    if (!e->on_device) {
-        double s = 0.0;
-        for (size_t i = 0; i < e->hkeys.size(); ++i) s += e->hre[i] * e->hre[i] + e->him[i] * e->him[i];
-        *out = s;
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        if (!out) return fail(SQE_EINVAL, "null output");
+        SQCK(full_sync(e));
+        if (!e->on_device) {
+            double s = 0.0;
+            for (size_t i = 0; i < e->hkeys.size(); ++i) s += e->hre[i] * e->hre[i] + e->him[i] * e->him[i];
+            *out = s;
+            return SQE_OK;
+        }
+        SQCK(run_norm(e));
+        HIPCK(hipMemcpyAsync(out, &e->dscal[0], 8, hipMemcpyDeviceToHost, e->st));
+        HIPCK(hipStreamSynchronize(e->st));
+        collect_times(e);
         return SQE_OK;
-    }
-    SQCK(run_norm(e));
-    HIPCK(hipMemcpyAsync(out, &e->dscal[0], 8, hipMemcpyDeviceToHost, e->st));
-    HIPCK(hipStreamSynchronize(e->st));
-    collect_times(e);
-    return SQE_OK;
+    });
 }
 
 sqe_status sqe_export_where(sqe_engine* e, uint64_t mask, uint64_t value, uint64_t* keys, double* re, double* im,
                             uint64_t cap, uint64_t* n) {
-    SQCK(enter(e));
-    if (!n) return fail(SQE_EINVAL, "null output");
-    if (value & ~mask) return fail(SQE_EINVAL, "pattern value has bits outside its mask");
-    SQCK(full_sync(e));
-    if (!e->have_state) { *n = 0; return SQE_OK; }
-    if (!e->on_device) {
-        // nothing applied yet: filter the staged host copy
-        auto keep = [&](size_t i) {
-            return !(std::fabs(e->hre[i]) < e->thr && std::fabs(e->him[i]) < e->thr) &&
-                   (e->hkeys[i] & mask) == value;
-        };
-        uint64_t c = 0;
-        for (size_t i = 0; i < e->hkeys.size(); ++i) c += keep(i) ? 1 : 0;
-        *n = c;
-        if (c > cap) return fail(SQE_EINVAL, "export capacity too small");
-        uint64_t j = 0;
-        for (size_t i = 0; i < e->hkeys.size(); ++i) {
-            if (!keep(i)) continue;
-            double r = e->hre[i], m = e->him[i];
-            if (e->f32) { r = (double)(float)r; m = (double)(float)m; }
-            keys[j] = e->hkeys[i]; re[j] = r; im[j] = m; ++j;
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        if (!n) return fail(SQE_EINVAL, "null output");
+        if (value & ~mask) return fail(SQE_EINVAL, "pattern value has bits outside its mask");
+        SQCK(full_sync(e));
+        if (!e->have_state) { *n = 0; return SQE_OK; }
+        if (!e->on_device) {
+            // nothing applied yet: filter the staged host copy
+            auto keep = [&](size_t i) {
+                return !(std::fabs(e->hre[i]) < e->thr && std::fabs(e->him[i]) < e->thr) &&
+                       (e->hkeys[i] & mask) == value;
+            };
+            uint64_t c = 0;
+            for (size_t i = 0; i < e->hkeys.size(); ++i) c += keep(i) ? 1 : 0;
+            *n = c;
+            if (c > cap) return fail(SQE_EINVAL, "export capacity too small");
+            uint64_t j = 0;
+            for (size_t i = 0; i < e->hkeys.size(); ++i) {
+                if (!keep(i)) continue;
+                double r = e->hre[i], m = e->him[i];
+                if (e->f32) { r = (double)(float)r; m = (double)(float)m; }
+                keys[j] = e->hkeys[i]; re[j] = r; im[j] = m; ++j;
+            }
+            return SQE_OK;
         }
+        uint64_t tot = 0;
+        SQCK(export_to_staging(e, 1, &tot, mask, value));
+        *n = tot;
+        if (tot > cap) return fail(SQE_EINVAL, "export capacity too small (" + std::to_string(tot) + " needed)");
+        if (tot) {
+            if (!keys || !re || !im) return fail(SQE_EINVAL, "null array");
+            HIPCK(hipMemcpyAsync(keys, e->xkeys, tot * 8, hipMemcpyDeviceToHost, e->st));
+            HIPCK(hipMemcpyAsync(re, e->xre, tot * 8, hipMemcpyDeviceToHost, e->st));
+            HIPCK(hipMemcpyAsync(im, e->xim, tot * 8, hipMemcpyDeviceToHost, e->st));
+            HIPCK(hipStreamSynchronize(e->st));
+        }
+        collect_times(e);
         return SQE_OK;
-    }
-    uint64_t tot = 0;
-    SQCK(export_to_staging(e, 1, &tot, mask, value));
-    *n = tot;
-    if (tot > cap) return fail(SQE_EINVAL, "export capacity too small (" + std::to_string(tot) + " needed)");
-    if (tot) {
-        if (!keys || !re || !im) return fail(SQE_EINVAL, "null array");
-        HIPCK(hipMemcpyAsync(keys, e->xkeys, tot * 8, hipMemcpyDeviceToHost, e->st));
-        HIPCK(hipMemcpyAsync(re, e->xre, tot * 8, hipMemcpyDeviceToHost, e->st));
-        HIPCK(hipMemcpyAsync(im, e->xim, tot * 8, hipMemcpyDeviceToHost, e->st));
-        HIPCK(hipStreamSynchronize(e->st));
-    }
-    collect_times(e);
-    return SQE_OK;
+    });
 }
 
 sqe_status sqe_export(sqe_engine* e, uint64_t* keys, double* re, double* im, uint64_t cap, uint64_t* n) {
-    return sqe_export_where(e, 0, 0, keys, re, im, cap, n);
+    return guard([&]() -> sqe_status {
+        return sqe_export_where(e, 0, 0, keys, re, im, cap, n);
+    });
 }
 
 sqe_status sqe_sample(sqe_engine* e, uint64_t shots, uint64_t seed, uint64_t* keys) {
-    SQCK(enter(e));
-    if (shots && !keys) return fail(SQE_EINVAL, "null output");
-    if (shots > (1ull << 30)) return fail(SQE_EINVAL, "at most 2^30 shots per call");
-    SQCK(full_sync(e));
-    if (!shots) return SQE_OK;
-    if (!e->have_state) return fail(SQE_EINVAL, "no state");
-    if (!e->on_device) SQCK(materialize(e, std::vector<sqe_gate>()));   // a loaded state: place it first
-    uint64_t n = 0;
-    SQCK(export_to_staging(e, 1, &n));
-    if (!n) return fail(SQE_EINVAL, "the state has no non-negligible entry");
-    const uint64_t nt = sample_tiles(n);
-    const size_t bytes = (size_t)(n + 2 * nt + 1 + shots) * 8;
-    char* scratch = nullptr;
-    SQCK(dalloc(e, (void**)&scratch, bytes));
-    double* local = (double*)scratch;
-    double* tsum = local + n;
-    double* offs = tsum + nt;
-    double* total = offs + nt;
-    uint64_t* out = (uint64_t*)(total + 1);
-    hipError_t r = launch_sample(e->st, e->xkeys, e->xre, e->xim, n, local, tsum, offs, total, seed, shots, out);
-    if (r == hipSuccess) r = hipMemcpyAsync(keys, out, shots * 8, hipMemcpyDeviceToHost, e->st);
-    if (r == hipSuccess) r = hipStreamSynchronize(e->st);
-    dfree(e, scratch, bytes);
-    if (r != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(SQE_EHIP, std::string("sampling: ") + hipGetErrorString(r));
-    }
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        if (shots && !keys) return fail(SQE_EINVAL, "null output");
+        if (shots > (1ull << 30)) return fail(SQE_EINVAL, "at most 2^30 shots per call");
+        SQCK(full_sync(e));
+        if (!shots) return SQE_OK;
+        if (!e->have_state) return fail(SQE_EINVAL, "no state");
+        if (!e->on_device) SQCK(materialize(e, std::vector<sqe_gate>()));   // a loaded state: place it first
+        uint64_t n = 0;
+        SQCK(export_to_staging(e, 1, &n));
+        if (!n) return fail(SQE_EINVAL, "the state has no non-negligible entry");
+        const uint64_t nt = sample_tiles(n);
+        const size_t bytes = (size_t)(n + 2 * nt + 1 + shots) * 8;
+        char* scratch = nullptr;
+        SQCK(dalloc(e, (void**)&scratch, bytes));
+        double* local = (double*)scratch;
+        double* tsum = local + n;
+        double* offs = tsum + nt;
+        double* total = offs + nt;
+        uint64_t* out = (uint64_t*)(total + 1);
+        hipError_t r = launch_sample(e->st, e->xkeys, e->xre, e->xim, n, local, tsum, offs, total, seed, shots, out);
+        if (r == hipSuccess) r = hipMemcpyAsync(keys, out, shots * 8, hipMemcpyDeviceToHost, e->st);
+        if (r == hipSuccess) r = hipStreamSynchronize(e->st);
+        dfree(e, scratch, bytes);
+        if (r != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SQE_EHIP, std::string("sampling: ") + hipGetErrorString(r));
+        }
+        return SQE_OK;
+    });
 }
 
 // Flat restart file: a 64-byte header (magic, version, qubits, precision,
@@ -1063,140 +1111,165 @@ const char CKPT_MAGIC[8] = {'S', 'Q', 'E', 'S', 'T', 'A', 'T', 'E'};
 }  // namespace
 
 sqe_status sqe_checkpoint_save(sqe_engine* e, const char* path) {
-    SQCK(enter(e));
-    if (!path) return fail(SQE_EINVAL, "null path");
-    uint64_t n = 0;
-    const sqe_status q = sqe_export(e, nullptr, nullptr, nullptr, 0, &n);   // size query
-    if (q != SQE_OK && q != SQE_EINVAL) return q;
-    std::vector<uint64_t> k(n);
-    std::vector<double> r(n), m(n);
-    uint64_t got = 0;
-    SQCK(sqe_export(e, k.data(), r.data(), m.data(), n, &got));
-    CkptHeader h{};
-    std::memcpy(h.magic, CKPT_MAGIC, 8);
-    h.version = 1;
-    h.nq = e->cfg.nq;
-    h.prec = e->cfg.prec;
-    h.count = got;
-    h.gates = e->gates;
-    FILE* f = std::fopen(path, "wb");
-    if (!f) return fail(SQE_EINVAL, std::string("cannot open ") + path + " for writing");
-    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
-    ok = ok && (got == 0 || (std::fwrite(k.data(), 8, got, f) == got && std::fwrite(r.data(), 8, got, f) == got &&
-                             std::fwrite(m.data(), 8, got, f) == got));
-    ok = (std::fclose(f) == 0) && ok;
-    if (!ok) return fail(SQE_EINVAL, std::string("write failed: ") + path);
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        if (!path) return fail(SQE_EINVAL, "null path");
+        uint64_t n = 0;
+        const sqe_status q = sqe_export(e, nullptr, nullptr, nullptr, 0, &n);   // size query
+        if (q != SQE_OK && q != SQE_EINVAL) return q;
+        std::vector<uint64_t> k(n);
+        std::vector<double> r(n), m(n);
+        uint64_t got = 0;
+        SQCK(sqe_export(e, k.data(), r.data(), m.data(), n, &got));
+        CkptHeader h{};
+        std::memcpy(h.magic, CKPT_MAGIC, 8);
+        h.version = 1;
+        h.nq = e->cfg.nq;
+        h.prec = e->cfg.prec;
+        h.count = got;
+        h.gates = e->gates;
+        FILE* f = std::fopen(path, "wb");
+        if (!f) return fail(SQE_EINVAL, std::string("cannot open ") + path + " for writing");
+        bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+        ok = ok && (got == 0 || (std::fwrite(k.data(), 8, got, f) == got && std::fwrite(r.data(), 8, got, f) == got &&
+                                 std::fwrite(m.data(), 8, got, f) == got));
+        ok = (std::fclose(f) == 0) && ok;
+        if (!ok) return fail(SQE_EINVAL, std::string("write failed: ") + path);
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_checkpoint_load(sqe_engine* e, const char* path) {
-    SQCK(enter(e));
-    if (!path) return fail(SQE_EINVAL, "null path");
-    FILE* f = std::fopen(path, "rb");
-    if (!f) return fail(SQE_EINVAL, std::string("cannot open ") + path);
-    CkptHeader h{};
-    bool ok = std::fread(&h, sizeof(h), 1, f) == 1 && std::memcmp(h.magic, CKPT_MAGIC, 8) == 0 && h.version == 1;
-    if (ok && h.nq != e->cfg.nq) {
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        if (!path) return fail(SQE_EINVAL, "null path");
+        FILE* f = std::fopen(path, "rb");
+        if (!f) return fail(SQE_EINVAL, std::string("cannot open ") + path);
+        CkptHeader h{};
+        bool ok = std::fread(&h, sizeof(h), 1, f) == 1 && std::memcmp(h.magic, CKPT_MAGIC, 8) == 0 && h.version == 1 &&
+                  (h.prec == SQE_F64 || h.prec == SQE_F32);
+        // the header's count must match the file's size before anything is sized by it
+        long fsize = -1;
+        if (ok && std::fseek(f, 0, SEEK_END) == 0) fsize = std::ftell(f);
+        ok = ok && fsize >= (long)sizeof(h) && h.count <= ((uint64_t)fsize - sizeof(h)) / 24 &&
+             (uint64_t)fsize == sizeof(h) + 24 * h.count && std::fseek(f, (long)sizeof(h), SEEK_SET) == 0;
+        if (ok && h.nq != e->cfg.nq) {
+            std::fclose(f);
+            return fail(SQE_EINVAL, "checkpoint has " + std::to_string(h.nq) + " qubits, engine " +
+                                        std::to_string(e->cfg.nq));
+        }
+        std::vector<uint64_t> k;
+        std::vector<double> r, m;
+        if (ok) {
+            try {
+                k.resize(h.count);
+                r.resize(h.count);
+                m.resize(h.count);
+            } catch (const std::exception&) {
+                std::fclose(f);
+                return fail(SQE_ENOMEM, "host memory for " + std::to_string(h.count) + " checkpoint entries");
+            }
+            ok = h.count == 0 || (std::fread(k.data(), 8, h.count, f) == h.count &&
+                                  std::fread(r.data(), 8, h.count, f) == h.count &&
+                                  std::fread(m.data(), 8, h.count, f) == h.count);
+        }
         std::fclose(f);
-        return fail(SQE_EINVAL, "checkpoint has " + std::to_string(h.nq) + " qubits, engine " +
-                                    std::to_string(e->cfg.nq));
-    }
-    std::vector<uint64_t> k;
-    std::vector<double> r, m;
-    if (ok) {
-        k.resize(h.count);
-        r.resize(h.count);
-        m.resize(h.count);
-        ok = h.count == 0 || (std::fread(k.data(), 8, h.count, f) == h.count &&
-                              std::fread(r.data(), 8, h.count, f) == h.count &&
-                              std::fread(m.data(), 8, h.count, f) == h.count);
-    }
-    std::fclose(f);
-    if (!ok) return fail(SQE_EINVAL, std::string("not a readable state checkpoint: ") + path);
-    SQCK(sqe_load(e, k.data(), r.data(), m.data(), h.count));
-    e->gates = h.gates;
-    return SQE_OK;
+        if (!ok) return fail(SQE_EINVAL, std::string("not a readable state checkpoint: ") + path);
+        SQCK(sqe_load(e, k.data(), r.data(), m.data(), h.count));
+        e->gates = h.gates;
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_get_stats(sqe_engine* e, sqe_stats* out) {
-    SQCK(enter(e));
-    if (!out) return fail(SQE_EINVAL, "null output");
-    Stats h;
-    SQCK(count_now(e, &h));
-    std::memset(out, 0, sizeof(*out));
-    out->gates = e->gates;
-    out->updates = h.updates;
-    out->nnz = h.nnz;
-    uint64_t nb = 0;
-    if (e->on_device) SQCK(read_nb(e, &nb));
-    out->buckets = nb;
-    out->capacity = e->s[e->cur].cap;
-    out->bytes_device = e->bytes;
-    out->bucket_bits = e->qm.mb;
-    out->bucket_mask = e->qm.mask;
-    out->sweeps = e->n_sweeps;
-    out->sweep_ops = e->n_sweep_ops;
-    out->pairs = e->n_pairs;
-    out->remaps = e->n_remaps;
-    out->moved = h.moved;
-    out->sweep_ms = e->t_ms[0];
-    out->pair_ms = e->t_ms[1];
-    out->other_ms = e->t_ms[2];
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        if (!out) return fail(SQE_EINVAL, "null output");
+        Stats h;
+        SQCK(count_now(e, &h));
+        std::memset(out, 0, sizeof(*out));
+        out->gates = e->gates;
+        out->updates = h.updates;
+        out->nnz = h.nnz;
+        uint64_t nb = 0;
+        if (e->on_device) SQCK(read_nb(e, &nb));
+        out->buckets = nb;
+        out->capacity = e->s[e->cur].cap;
+        out->bytes_device = e->bytes;
+        out->bucket_bits = e->qm.mb;
+        out->bucket_mask = e->qm.mask;
+        out->sweeps = e->n_sweeps;
+        out->sweep_ops = e->n_sweep_ops;
+        out->pairs = e->n_pairs;
+        out->remaps = e->n_remaps;
+        out->moved = h.moved;
+        out->sweep_ms = e->t_ms[0];
+        out->pair_ms = e->t_ms[1];
+        out->other_ms = e->t_ms[2];
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_prepare(sqe_engine* e, const sqe_gate* g, uint64_t n) {
-    SQCK(enter(e));
-    if (n && !g) return fail(SQE_EINVAL, "null gates");
-    SQCK(full_sync(e));
-    if (!e->have_state) return fail(SQE_EINVAL, "no state");
-    std::vector<sqe_gate> gl(g, g + n);
-    if (!e->on_device) return materialize(e, gl);
-    const QMap best = choose_map((int)e->cfg.nq, e->cfg.bucket_bits, gl.data(), gl.size());
-    if (best.mask != e->qm.mask) SQCK(remap(e, best));
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        if (n && !g) return fail(SQE_EINVAL, "null gates");
+        SQCK(full_sync(e));
+        if (!e->have_state) return fail(SQE_EINVAL, "no state");
+        std::vector<sqe_gate> gl(g, g + n);
+        if (!e->on_device) return materialize(e, gl);
+        const QMap best = choose_map((int)e->cfg.nq, e->cfg.bucket_bits, gl.data(), gl.size());
+        if (best.mask != e->qm.mask) SQCK(remap(e, best));
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_snapshot_save(sqe_engine* e) {
-    SQCK(enter(e));
-    SQCK(full_sync(e));
-    if (!e->have_state) return fail(SQE_EINVAL, "no state");
-    if (!e->on_device) SQCK(materialize(e, std::vector<sqe_gate>()));
-    uint64_t nb = 0;
-    SQCK(read_nb(e, &nb));
-    const uint64_t nbp = round64(std::max<uint64_t>(nb, 1));
-    if (e->snap.cap < nbp || e->snap.mb != e->qm.mb) {
-        free_store(e, e->snap);
-        SQCK(alloc_store(e, e->snap, nbp));
-    }
-    HIPCK(hipMemcpyAsync(e->snap.base, e->s[e->cur].base, nbp * 8, hipMemcpyDeviceToDevice, e->st));
-    HIPCK(hipMemcpyAsync(e->snap.amp, e->s[e->cur].amp, e->amp_bytes(nbp), hipMemcpyDeviceToDevice, e->st));
-    HIPCK(hipMemcpyAsync(&e->meta[3], &e->meta[e->cur], 8, hipMemcpyDeviceToDevice, e->st));
-    HIPCK(hipStreamSynchronize(e->st));
-    e->snap_qm = e->qm;
-    e->snap_valid = true;
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        SQCK(full_sync(e));
+        if (!e->have_state) return fail(SQE_EINVAL, "no state");
+        if (!e->on_device) SQCK(materialize(e, std::vector<sqe_gate>()));
+        uint64_t nb = 0;
+        SQCK(read_nb(e, &nb));
+        const uint64_t nbp = round64(std::max<uint64_t>(nb, 1));
+        if (e->snap.cap < nbp || e->snap.mb != e->qm.mb) {
+            free_store(e, e->snap);
+            SQCK(alloc_store(e, e->snap, nbp));
+        }
+        HIPCK(hipMemcpyAsync(e->snap.base, e->s[e->cur].base, nbp * 8, hipMemcpyDeviceToDevice, e->st));
+        HIPCK(hipMemcpyAsync(e->snap.amp, e->s[e->cur].amp, e->amp_bytes(nbp), hipMemcpyDeviceToDevice, e->st));
+        HIPCK(hipMemcpyAsync(&e->meta[3], &e->meta[e->cur], 8, hipMemcpyDeviceToDevice, e->st));
+        HIPCK(hipStreamSynchronize(e->st));
+        e->snap_qm = e->qm;
+        e->snap_gates = e->gates;
+        e->snap_valid = true;
+        return SQE_OK;
+    });
 }
 
 sqe_status sqe_snapshot_restore(sqe_engine* e) {
-    SQCK(enter(e));
-    if (!e->snap_valid) return fail(SQE_EINVAL, "no snapshot");
-    e->pending.clear();
-    HIPCK(hipStreamSynchronize(e->st));
-    unsigned long long nb = 0;
-    HIPCK(hipMemcpy(&nb, &e->meta[3], 8, hipMemcpyDeviceToHost));
-    const uint64_t nbp = round64(std::max<uint64_t>(nb, 1));
-    e->qm = e->snap_qm;
-    SQCK(ensure_stores(e, nbp));
-    HIPCK(hipMemcpyAsync(e->s[e->cur].base, e->snap.base, nbp * 8, hipMemcpyDeviceToDevice, e->st));
-    HIPCK(hipMemcpyAsync(e->s[e->cur].amp, e->snap.amp, e->amp_bytes(nbp), hipMemcpyDeviceToDevice, e->st));
-    HIPCK(hipMemcpyAsync(&e->meta[e->cur], &e->meta[3], 8, hipMemcpyDeviceToDevice, e->st));
-    e->nb_upper = nb;
-    e->tab_valid = false;
-    e->renorm_pending = false;
-    e->on_device = true;
-    e->have_state = true;
-    return SQE_OK;
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        if (!e->snap_valid) return fail(SQE_EINVAL, "no snapshot");
+        e->pending.clear();
+        HIPCK(hipStreamSynchronize(e->st));
+        unsigned long long nb = 0;
+        HIPCK(hipMemcpy(&nb, &e->meta[3], 8, hipMemcpyDeviceToHost));
+        const uint64_t nbp = round64(std::max<uint64_t>(nb, 1));
+        e->qm = e->snap_qm;
+        SQCK(ensure_stores(e, nbp));
+        HIPCK(hipMemcpyAsync(e->s[e->cur].base, e->snap.base, nbp * 8, hipMemcpyDeviceToDevice, e->st));
+        HIPCK(hipMemcpyAsync(e->s[e->cur].amp, e->snap.amp, e->amp_bytes(nbp), hipMemcpyDeviceToDevice, e->st));
+        HIPCK(hipMemcpyAsync(&e->meta[e->cur], &e->meta[3], 8, hipMemcpyDeviceToDevice, e->st));
+        e->nb_upper = nb;
+        e->tab_valid = false;
+        e->renorm_pending = false;
+        e->gates = e->snap_gates;
+        e->on_device = true;
+        e->have_state = true;
+        return SQE_OK;
+    });
 }
 
 const char* sqe_last_error(void) { return g_err.c_str(); }

@@ -89,7 +89,8 @@ __device__ __forceinline__ uint64_t flip_if(uint64_t base, uint64_t flip, bool p
 
 // Decoded sweep op (uniform across the wavefront).
 struct SOp {
-    uint32_t h, j, cmS, cvS, ci;
+    uint32_t h, j, cmS, cvS;
+    uint64_t ci;                      // coefficient side-table index (H_SLOT_U, H_SLOT_BLOCK)
     uint64_t tbase, cmB, cvB;
     double c[5];
 };
@@ -606,7 +607,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                 op.cvS = (ctl >> 8) & 0xffu;
                 op.j = (hdr >> 4) & 7u;
                 op.tbase = 1ull << ((hdr >> 7) & 63u);
-                op.ci = hdr >> 13;
+                op.ci = coef_index_of(op.c[4]);
                 bool pred[EPT];
                 static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                     constexpr int e = decltype(ec)::value;
@@ -689,6 +690,27 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
     });
 }
 
+// Wait until this wave's vector memory operations have completed
+// (s_waitcnt vmcnt(0); the immediate is the gfx9-family encoding, whose
+// vmcnt field is bits 3:0 and 15:14 with expcnt / lgkmcnt left at their
+// maxima).  The last-workgroup norm fold relies on it: a partial written with
+// a device-scope relaxed atomic store is globally visible once the store has
+// completed, which is what vmcnt(0) waits for on gfx950, so the relaxed
+// counter increment that follows it cannot overtake it.  This is the ordering
+// the LLVM AMDGPU memory model gives GFX940+ for agent-scope monotonic
+// atomics (stores with sc1 write through to memory; loads of the partials
+// by the last workgroup are agent-scope monotonic loads that miss the
+// non-coherent caches).  The compiler-level fences keep the store and the
+// count on their sides of the wait.  Built for gfx950 only.
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(__gfx950__)
+#error "the sweep's last-workgroup norm fold is written for gfx950's memory model"
+#endif
+__device__ __forceinline__ void wait_vector_memory() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // Minimum wavefronts per SIMD the sweep kernels are compiled for (caps the
 // VGPRs at 512 / value): the amplitude registers of a bucket take 4 * 2^MB
 // (FP64) or 2 * 2^MB (FP32) VGPRs; with 16 slots FP64 fits 3 waves (about
@@ -750,9 +772,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
             // the last workgroup reads the partials with device-coherent loads.
             // (A release fence would also write back the XCD's whole L2.)
             __hip_atomic_store(&A.norm_out[blockIdx.x], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            __builtin_amdgcn_s_waitcnt(0x0F70);                  // vmcnt(0)
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            wait_vector_memory();
             s_last = __hip_atomic_fetch_add(A.norm_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
                      gridDim.x - 1;
         }

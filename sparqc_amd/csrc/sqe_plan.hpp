@@ -100,7 +100,9 @@ inline MOp encode_op(const DOp& d, std::vector<double>& coef, bool f32 = false) 
     const bool slot = d.tslot >= 0;
     const uint32_t j = slot ? (uint32_t)d.tslot : 0u;
     const uint32_t target = (!slot && d.tbase) ? (uint32_t)__builtin_ctzll(d.tbase) : 0u;
-    uint32_t h = H_RENORM, ci = NO_COEF;
+    uint32_t h = H_RENORM;
+    uint64_t ci = 0;
+    bool side = false;
     const double* p = d.p;
     auto set = [&](double a, double b, double c, double e, double f) {
         m.c[0] = a; m.c[1] = b; m.c[2] = c; m.c[3] = e; m.c[4] = f;
@@ -118,13 +120,15 @@ inline MOp encode_op(const DOp& d, std::vector<double>& coef, bool f32 = false) 
         // the general handler: [[c, -is], [-is, c]] as 8 coefficients
         const double u[8] = {p[0], 0.0, 0.0, -p[1], 0.0, -p[1], p[0], 0.0};
         h = H_SLOT_U;
-        ci = (uint32_t)(coef.size() / 8);
+        ci = coef.size() / 8;
+        side = true;
         coef.insert(coef.end(), u, u + 8);
     } break;
     case SQE_GATE_RY: set(p[0], p[1], 0, 0, 0); h = H_SLOT_RY; break;
     case SQE_GATE_U:
         h = H_SLOT_U;
-        ci = (uint32_t)(coef.size() / 8);
+        ci = coef.size() / 8;
+        side = true;
         coef.insert(coef.end(), p, p + 8);
         break;
     default: h = H_RENORM; break;                                                           // K_RENORM
@@ -134,7 +138,8 @@ inline MOp encode_op(const DOp& d, std::vector<double>& coef, bool f32 = false) 
         m.c[0] = m.c[1] = m.c[2] = m.c[3] = 0.0;
         std::memcpy(&m.c[0], f, 16);
     }
-    m.hdr = h | j << 4 | target << 7 | ci << 13;
+    m.hdr = h | j << 4 | target << 7;
+    if (side) m.c[4] = coef_index_bits(ci);
     m.ctl = (d.cmS & 0xffu) | (d.cvS & 0xffu) << 8;
     m.cmB = d.cmB;
     m.cvB = d.cvB;
@@ -233,7 +238,7 @@ struct BlockBuilder {
 // for the FP64 recomputation of near-threshold outputs.
 inline MOp encode_block(const DOp& d, const std::array<cx, 256>& m, std::vector<double>& coef) {
     MOp r{};
-    const uint32_t ci = (uint32_t)(coef.size() / 8);
+    const uint64_t ci = coef.size() / 8;
     std::vector<float> tab(1024);
     auto R = [&](int i, int k) -> double {
         const cx v = m[(i & 15) * 16 + (k & 15)];
@@ -249,9 +254,10 @@ inline MOp encode_block(const DOp& d, const std::array<cx, 256>& m, std::vector<
         coef.push_back(m[k].real());
         coef.push_back(m[k].imag());
     }
-    r.hdr = H_SLOT_BLOCK | ci << 13;
+    r.hdr = H_SLOT_BLOCK;
     r.ctl = 0;
     r.c[0] = d.p[1];                               // gates in the block
+    r.c[4] = coef_index_bits(ci);
     return r;
 }
 

@@ -214,6 +214,44 @@ int main() {
             CHECK(thrown, "a restart file of 6 qubits must not load into 5");
             std::remove(path.c_str());
         }
+        // 9. unitary(): the matrices of ry / rx / h / cry given as 2x2 unitaries
+        //    produce the named gates' states (random 8-qubit state)
+        {
+            const uint32_t nq = 8;
+            std::vector<uint64_t> keys;
+            std::vector<cd> amps;
+            double n2 = 0;
+            for (uint64_t k = 0; k < (1u << nq); k += 1 + (sm_next() % 5)) {
+                const cd a((double)(sm_next() >> 11) / 9007199254740992.0 - 0.5,
+                           (double)(sm_next() >> 11) / 9007199254740992.0 - 0.5);
+                keys.push_back(k);
+                amps.push_back(a);
+                n2 += std::norm(a);
+            }
+            for (auto& a : amps) a /= std::sqrt(n2);
+            Circuit__ a(nq, hp), b(nq, hp);
+            a.load_state(keys, amps);
+            b.load_state(keys, amps);
+            const double th = 0.9, c = std::cos(th / 2), sn = std::sin(th / 2), f = 1.0 / std::sqrt(2.0);
+            const cd ry_m[4] = {c, -sn, sn, c};
+            const cd rx_m[4] = {c, cd(0, -sn), cd(0, -sn), c};
+            const cd h_m[4] = {f, f, f, -f};
+            a.ry(2, th);
+            b.unitary(2, ry_m);
+            a.rx(5, th);
+            b.unitary(5, rx_m);
+            a.h(7);
+            b.unitary(7, h_m);
+            a.cry(1, 6, th);
+            b.unitary(6, ry_m, {1});
+            b.unitary(3, ry_m, {0}, 0);      // control value 0: the gate acts where qubit 0 is 0
+            a.x(0);                          // the same through X conjugation of the control
+            a.cry(0, 3, th);
+            a.x(0);
+            bool same = false;
+            const double d = max_diff(a, b, &same);
+            CHECK(same && d < 1e-12, "unitary(): same keys %d, max diff %.3e", (int)same, d);
+        }
     } catch (const std::string& e) {
         std::printf("FAIL exception: %s\n", e.c_str());
         return 1;

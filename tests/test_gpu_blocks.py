@@ -110,3 +110,47 @@ def test_blocks_ignored_for_fp64():
         f.apply(gl)
         a, b = e.export(), f.export()
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
+def test_more_than_4096_blocks_in_one_flush(oracle):
+    """4200 fused blocks in one program: each block's side-table entry (128
+    units of 8 doubles) lies beyond 2^19 units after block 4096, so a short
+    index field would hand later blocks another block's matrix.  The state
+    must follow the FP64 oracle through all 29400 gates."""
+    nq = 16
+    bq = [1, 4, 7, 10]
+    others = [q for q in range(nq) if q not in bq]
+    rng = gen.SplitMix64(77)
+    gl = []
+    n_windows = 4200
+    for _ in range(n_windows):
+        for _k in range(6):
+            q = bq[rng.randint(4)]
+            k = [G.H, G.RY, G.RZ, G.PHASE, G.X, G.RX][rng.randint(6)]
+            gl.append(G.make_gate(k, q, theta=rng.uniform() * 6.28))
+        # CZ between a bucket qubit and another qubit closes the window
+        gl.append(G.make_gate(G.Z, bq[rng.randint(4)], controls=(others[rng.randint(len(others))],)))
+    gl = G.gate_array(gl)
+    keys, re, im = random_state(91, nq, 300)
+    say("blocks regression: %d gates, %d windows of 6" % (len(gl), n_windows))
+    with E.Engine(nq, prec="f32", fuse_blocks=6, bucket_bits=4) as e:
+        e.load(keys, re, im)
+        e.apply(gl)
+        got = e.export()
+        st = e.stats()
+    assert st["sweeps"] == 1, st["sweeps"]          # one flush, one program
+    o64 = oracle.Sparse(nq)
+    o64.load(keys, re, im)
+    o64.apply(gl)
+    want = o64.export()
+    o32 = oracle.Sparse(nq, f32=True, thr=1e-7)
+    o32.load(keys, re, im)
+    o32.apply(gl)
+    w32 = o32.export()
+    # keys agree away from the FP32 threshold
+    big = want[0][np.maximum(np.abs(want[1]), np.abs(want[2])) > 1e-5]
+    assert np.all(np.isin(big, got[0])), "keys with |a| > 1e-5 missing"
+    d64 = compare(got, want, 1e-4, max(2, got[0].size // 100))
+    d32 = compare(w32, want, 1e-4, max(2, got[0].size // 100))
+    say("max |a - a64|: fused blocks %.2e, FP32 gate by gate %.2e" % (d64, d32))
+    assert d64 <= 4 * d32 + 1e-6

@@ -62,6 +62,63 @@ def test_single_gates_every_kind(oracle, bits):
                 assert_same_state(got, want, atol=0.0, label="%s t=%d c=%s" % (G.KIND_NAMES[kind], t, ctrls))
 
 
+def _class_cases():
+    """sqe_apply's three classes: (class, 2x2 matrix, equivalent gate kind and its matrix argument)."""
+    d0, d1 = np.exp(0.3j), np.exp(-1.1j)
+    u01, u10 = np.exp(0.7j), -np.exp(-0.2j)
+    c, s = math.cos(0.45), math.sin(0.45)
+    ub = np.array([[c * np.exp(0.4j), -s * np.exp(1.3j)], [s * np.exp(-1.3j), c * np.exp(-0.4j)]])
+    return [(G.CLS_DIAG, np.array([[d0, 0], [0, d1]]), G.DIAG, [d0, d1]),
+            (G.CLS_PERM, np.array([[0, u01], [u10, 0]]), G.PERM, [u01, u10]),
+            (G.CLS_BRANCH, ub, G.U, ub)]
+
+
+@pytest.mark.parametrize("bits", [0, 2, 4])
+def test_sqe_apply_each_class(oracle, bits):
+    """The boundary call sqe_apply (a host matrix and a class, the reference's
+    Mapping__ gate seam) against the oracle's DIAG / PERM / U records: one call
+    at a time agrees to the bit, on bucket and base targets, with controls of
+    either value."""
+    nq = 9
+    keys, re, im = random_state(41 + bits, nq, 200)
+    for cls, m, kind, arg in _class_cases():
+        for t in (0, 5, 8):
+            for ctrls, cv in (((), None), ((2,), None), ((1, 6), 1 << 6), ((3,), 0)):
+                if t in ctrls:
+                    continue
+                say("sqe_apply class", cls, "t", t, "ctrls", ctrls, "cv", cv, "bits", bits)
+                g = G.gate_array([G.make_gate(kind, t, ctrls, matrix=arg, ctrl_val=cv)])
+                want, _ = oracle_run(oracle, nq, keys, re, im, g)
+                with E.Engine(nq, bucket_bits=bits) as e:
+                    e.load(keys, re, im)
+                    e.apply_matrix(t, m, cls, controls=ctrls, ctrl_val=cv)
+                    got = e.export()
+                assert_same_state(got, want, atol=0.0, label="class %d t=%d c=%s" % (cls, t, ctrls))
+
+
+def test_sqe_apply_sequence(oracle):
+    """A run of sqe_apply calls (one planned program) against the oracle."""
+    nq = 12
+    keys, re, im = random_state(43, nq, 300)
+    rng = gen.SplitMix64(44)
+    cases = _class_cases()
+    recs = []
+    with E.Engine(nq) as e:
+        e.load(keys, re, im)
+        for _ in range(150):
+            cls, m, kind, arg = cases[rng.randint(3)]
+            t = rng.randint(nq)
+            c = (t + 1 + rng.randint(nq - 1)) % nq
+            ctrls = (c,) if rng.uniform() < 0.5 else ()
+            e.apply_matrix(t, m, cls, controls=ctrls)
+            recs.append(G.make_gate(kind, t, ctrls, matrix=arg))
+        got = e.export()
+        st = e.stats()
+    want, orc = oracle_run(oracle, nq, keys, re, im, G.gate_array(recs))
+    assert_same_state(got, want, atol=1e-12, label="sqe_apply sequence")
+    assert st["updates"] == orc.updates()
+
+
 @pytest.mark.parametrize("bits,window", [(-1, 0), (0, 0), (1, 3), (2, 1), (3, 0), (4, 7)])
 def test_random_circuits(oracle, bits, window):
     """Random mixed circuits with random controls (cross-bucket and local paths)."""

@@ -110,3 +110,18 @@ def test_bad_config_rejected_before_device():
     with pytest.raises(E.SqeError) as ex:
         E.Engine(20, bucket_bits=5)          # FP64 buckets hold at most 2^4 slots
     assert ex.value.status == E.SQE_EINVAL
+
+
+def test_program_encoder_side_table_index():
+    """Fused blocks and general-U records address their side-table entry by a
+    64-bit index of their own: more than 4096 blocks (each takes 128 units)
+    and more than 2^19 U records in one program keep their own coefficients."""
+    import subprocess
+    from sparqc_amd import build as B
+    srcs = [B.ENC_SRC, os.path.join(B.CSRC, "sqe_plan.hpp"), os.path.join(B.CSRC, "sqe_device.hpp")]
+    if not os.path.exists(B.ENC_BIN) or os.path.getmtime(B.ENC_BIN) < max(os.path.getmtime(p) for p in srcs):
+        B.build_encode_check()
+    r = subprocess.run([B.ENC_BIN, "4200", str((1 << 19) + 300)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "encode_check ok" in r.stdout
+    assert ">= 2^19" in r.stdout

@@ -150,6 +150,17 @@ sqe_status sqe_nnz(sqe_engine* e, uint64_t* n);
 /* Sum of |a|^2 in FP64 (no reference counterpart). */
 sqe_status sqe_norm2(sqe_engine* e, double* out);
 
+/* sqe_norm2 without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (tight serving loops, code that
+ * must not call hipMalloc).  Its per-workgroup partial sums go to a fixed
+ * device array of 16384 doubles (128 KB per GPU, held until the process
+ * ends) instead of engine-allocated memory, so states of at most
+ * 16384 * 256 = 4194304 buckets are accepted (SQE_EINVAL beyond).  Gates must
+ * not be queued (SQE_EINVAL: call sqe_sync first, since planning them may
+ * allocate).  The array is shared by every engine on the GPU: calls that
+ * overlap on one GPU (engines driven from several host threads) race. */
+sqe_status sqe_norm2_noalloc(sqe_engine* e, double* out);
+
 /* Copy the non-negligible entries (|re| >= thr or |im| >= thr) to host arrays
  * of capacity cap, in the engine's bucket order.  *n receives the entry count;
  * when it exceeds cap nothing is copied and SQE_EINVAL is returned.  Replaces

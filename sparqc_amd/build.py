@@ -33,7 +33,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
          "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
 SOURCES = ["sqe_kernels_f64.hip", "sqe_kernels_f32.hip", "sqe_kernels_common.hip", "sqe_engine.cpp",
-           "sqe_batch.cpp"]
+           "sqe_batch.cpp", "sqe_noalloc.hip"]
 
 
 def _compile(src):

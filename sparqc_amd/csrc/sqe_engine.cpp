@@ -859,6 +859,8 @@ sqe_status sqe_create(const sqe_config* cfg, sqe_engine** out) {
         if (r == SQE_OK && hipMemsetAsync(e->dscal, 0, 16, e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
         if (r == SQE_OK && hipMemsetAsync(e->ticket, 0, 64, e->st) != hipSuccess) r = fail(SQE_EHIP, "memset");
         if (r == SQE_OK && hipStreamSynchronize(e->st) != hipSuccess) r = fail(SQE_EHIP, "sync");
+    // the allocation-free entry points' scratch on this device
+    if (r == SQE_OK && noalloc_scratch_init(cfg->device) != hipSuccess) r = fail(SQE_EHIP, "device scratch lookup");
         if (r != SQE_OK) {
             destroy_all(e);
             delete e;
@@ -1010,6 +1012,38 @@ sqe_status sqe_norm2(sqe_engine* e, double* out) {
         HIPCK(hipMemcpyAsync(out, &e->dscal[0], 8, hipMemcpyDeviceToHost, e->st));
         HIPCK(hipStreamSynchronize(e->st));
         collect_times(e);
+        return SQE_OK;
+    });
+}
+
+sqe_status sqe_norm2_noalloc(sqe_engine* e, double* out) {
+    return guard([&]() -> sqe_status {
+        SQCK(enter(e));
+        if (!out) return fail(SQE_EINVAL, "null output");
+        // queued gates would have to be planned and uploaded, which can allocate
+        if (!e->pending.empty())
+            return fail(SQE_EINVAL, "sqe_norm2_noalloc: gates are queued; call sqe_sync first");
+        if (!e->have_state || !e->on_device) {
+            double s = 0.0;
+            for (size_t i = 0; i < e->hkeys.size(); ++i) s += e->hre[i] * e->hre[i] + e->him[i] * e->him[i];
+            *out = s;
+            return SQE_OK;
+        }
+        const uint32_t g = std::max<uint32_t>(1, tiles_of(round64(e->nb_upper)));
+        if (g > NOALLOC_PARTIALS)
+            return fail(SQE_EINVAL, "sqe_norm2_noalloc: the state spans " + std::to_string(e->nb_upper) +
+                                        " buckets; at most " + std::to_string((uint64_t)NOALLOC_PARTIALS * BLOCK) +
+                                        " fit its fixed scratch (use sqe_norm2)");
+        double* scratch = noalloc_scratch(e->cfg.device);
+        if (!scratch) return fail(SQE_EHIP, "sqe_norm2_noalloc: device scratch not set up");
+        // every partial the final fold reads is written first by this call
+        NormArgs a{};
+        a.amp = e->s[e->cur].amp;
+        a.nb = &e->meta[e->cur];
+        a.partials = scratch;
+        HIPCK(launch_norm(e->qm.mb, e->f32, e->st, g, a, &e->dscal[0], &e->dscal[1]));
+        HIPCK(hipMemcpyAsync(out, &e->dscal[0], 8, hipMemcpyDeviceToHost, e->st));
+        HIPCK(hipStreamSynchronize(e->st));
         return SQE_OK;
     });
 }

@@ -170,4 +170,12 @@ hipError_t launch_batch_fold(hipStream_t st, const BTask* tasks, uint32_t ntasks
 hipError_t launch_stats_reduce(hipStream_t st, BlockStats* bs, uint32_t n_acc, uint32_t n_last, Stats* stats);
 hipError_t launch_load(bool f32, hipStream_t st, const LoadArgs& a, const EnumArgs& e, uint32_t enum_grid);
 
+// Fixed per-device scratch of the allocation-free entry points: room for
+// NOALLOC_PARTIALS FP64 per-workgroup partials (128 KB).  noalloc_scratch_init
+// looks up its device address on `device` once (at engine creation);
+// noalloc_scratch returns it (nullptr before the lookup).
+constexpr uint32_t NOALLOC_PARTIALS = 16384;
+hipError_t noalloc_scratch_init(int device);
+double* noalloc_scratch(int device);
+
 }  // namespace sqe

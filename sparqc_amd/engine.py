@@ -20,7 +20,7 @@ F64, F32 = 0, 1
 
 # every entry point declared in include/sqe.h
 EXPORTS = ["sqe_config_default", "sqe_create", "sqe_destroy", "sqe_set_basis", "sqe_load",
-           "sqe_apply", "sqe_apply_gates", "sqe_nnz", "sqe_norm2", "sqe_export", "sqe_export_where",
+           "sqe_apply", "sqe_apply_gates", "sqe_nnz", "sqe_norm2", "sqe_norm2_noalloc", "sqe_export", "sqe_export_where",
            "sqe_checkpoint_save", "sqe_checkpoint_load", "sqe_sample", "sqe_sync",
            "sqe_get_stats", "sqe_prepare", "sqe_snapshot_save", "sqe_snapshot_restore",
            "sqe_last_error",
@@ -106,6 +106,8 @@ def lib():
         L.sqe_nnz.restype = st
         L.sqe_norm2.argtypes = [vp, _f64p]
         L.sqe_norm2.restype = st
+        L.sqe_norm2_noalloc.argtypes = [vp, _f64p]
+        L.sqe_norm2_noalloc.restype = st
         L.sqe_export.argtypes = [vp, _u64p, _f64p, _f64p, ctypes.c_uint64, _u64p]
         L.sqe_export.restype = st
         L.sqe_export_where.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, _u64p, _f64p, _f64p, ctypes.c_uint64,

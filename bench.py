@@ -51,6 +51,11 @@ def parse():
     ap.add_argument("--circuit", default="mix", choices=["mix", "layers"],
                     help="mix: the 60/35/5 diagonal/permutation/branching circuit; layers: layered "
                          "single-qubit gates on the 4 branching qubits (long fusable windows)")
+    ap.add_argument("--frames", default="on", choices=["on", "off"],
+                    help="Pauli gates on bucket qubits through the per-bucket frame (on) or the amplitude "
+                         "registers (off)")
+    ap.add_argument("--renorm-mode", type=int, default=0, choices=[0, 1],
+                    help="FP32 renormalisation: 0 one launch per period, 1 one pass per run of gates (checked)")
     ap.add_argument("--tail-gates", type=int, default=200,
                     help="also run the last N gates one gate per launch (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -342,7 +347,8 @@ def main():
         print("--fused-blocks needs --prec f32", file=sys.stderr)
         sys.exit(2)
     eng = E.Engine(48, prec=a.prec, timing=True, fuse_window=a.fuse_window, bucket_bits=a.bucket_bits,
-                   renorm_every=100 if a.prec == "f32" else 0, fuse_blocks=a.fused_blocks)
+                   renorm_every=100 if a.prec == "f32" else 0, fuse_blocks=a.fused_blocks,
+                   frames=a.frames == "on", renorm_mode=a.renorm_mode)
     eng.load(w["keys"], w["re"], w["im"])
     eng.prepare(w["gates"])
     eng.snapshot_save()

@@ -89,7 +89,18 @@ typedef struct sqe_config {
     uint32_t batch_eager;       /* batch: 1 launches each step of sqe_batch_run
                                    directly; 0 replays the run as one captured
                                    hipGraph (planned once per set of circuits)    */
-    uint32_t reserved[4];
+    uint32_t renorm_mode;       /* FP32 with renorm_every: 0 stops at every
+                                   renormalisation point (one launch per period);
+                                   1 applies a queued run of bucket-local gates in one
+                                   pass that takes the norm at each renormalisation
+                                   point and applies the scale afterwards, checked: the
+                                   run is redone step by step when a prune decision lay
+                                   within 2^-10 of the threshold or a norm moved by
+                                   more than 2^-11; 2 as 1 with the check failing
+                                   (tests the rerun)                                   */
+    uint32_t frame_off;         /* 1: Pauli gates on bucket qubits update the amplitude
+                                   registers instead of the per-bucket frame           */
+    uint32_t reserved[2];
 } sqe_config;
 
 typedef struct sqe_stats {
@@ -100,7 +111,7 @@ typedef struct sqe_stats {
     uint64_t capacity;          /* bucket capacity of one store                        */
     uint64_t bytes_device;      /* device bytes held by the engine                     */
     int32_t bucket_bits;        /* slots per bucket = 2^bucket_bits                    */
-    uint32_t pad;
+    uint32_t reruns;            /* one-pass renormalised runs rerun step by step       */
     uint64_t bucket_mask;       /* bucket qubits                                       */
     uint64_t sweeps;            /* fused local sweep launches                          */
     uint64_t sweep_ops;         /* gates executed inside sweeps                        */

@@ -128,8 +128,10 @@ struct Circ {
     uint64_t coef0 = 0, ncoef = 0;        // coefficient pool doubles
     uint64_t pos = 0;                     // next op
     std::vector<uint32_t> renorm_at;      // op indices of RENORM ops
+    std::vector<uint32_t> gate_at;        // op index of each gate's record (increasing)
     uint64_t gates = 0;
     uint64_t bytes = 0;
+    bool frames = false;                  // the program holds frame records
 };
 
 struct Pool {
@@ -397,6 +399,8 @@ bool plan_step(sqe_batch* b, uint64_t max_ops, std::vector<uint64_t>& pos, std::
             t.norm2 = b->norm2 + s.id;
             t.nops = (uint32_t)(s.to - s.from);
             t.ntiles = (uint32_t)c.tiles;
+            t.qm = c.qm;
+            t.frames = c.frames ? 1 : 0;
             t.item0 = (uint32_t)g.items.size();
             const uint32_t ti = (uint32_t)g.tasks.size();
             for (uint64_t tl = 0; tl < c.tiles; tl += b->gran) g.items.push_back(BItem{ti, (uint32_t)tl});
@@ -770,17 +774,28 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
             m += (re[j] != 0.0 || im[j] != 0.0) ? 1 : 0;
         }
         // program: gates, with a RENORM op after every renorm_every-th gate
-        std::vector<MOp> prog;
-        std::vector<double> coef;
-        std::vector<uint32_t> renorm_at;
-        prog.reserve(ng + 8);
+        // (Pauli gates on bucket qubits as frame records when the keys leave
+        // room for the frame)
+        std::vector<DOp> dprog;
+        std::vector<size_t> gate_idx, ren_idx;
+        dprog.reserve(ng + 8);
         for (uint64_t j = 0; j < ng; ++j) {
-            prog.push_back(encode_op(plan_op(gates[j], qm), coef, b->f32));
+            gate_idx.push_back(dprog.size());
+            dprog.push_back(plan_op(gates[j], qm));
             if (b->cfg.renorm_every && (j + 1) % b->cfg.renorm_every == 0) {
-                renorm_at.push_back((uint32_t)prog.size());
-                prog.push_back(encode_op(renorm_op(), coef, b->f32));
+                ren_idx.push_back(dprog.size());
+                dprog.push_back(renorm_op());
             }
         }
+        const bool frames = !b->cfg.frame_off && nq <= (uint32_t)FRAME_Z_BIT0 && qm.mb > 0;
+        Encoded enc;
+        encode_program(dprog, {}, b->f32, frames ? &qm : nullptr, enc);
+        const std::vector<MOp>& prog = enc.ops;
+        const std::vector<double>& coef = enc.coef;
+        std::vector<uint32_t> renorm_at, gate_at;
+        for (size_t r : ren_idx) renorm_at.push_back((uint32_t)enc.start[r]);
+        for (size_t g : gate_idx) gate_at.push_back((uint32_t)(enc.start[g + 1] - 1));   // the gate's own record
+        const bool framed = enc.framed;
         // region sizes and admission
         const uint64_t S = 1ull << qm.mb;
         const uint64_t tiles = std::max<uint64_t>(1, (m + 63) / 64);
@@ -801,8 +816,10 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         c.nops = prog.size();
         c.ncoef = coef.size();
         c.renorm_at = renorm_at;
+        c.gate_at = gate_at;
         c.gates = ng;
         c.bytes = bytes;
+        c.frames = framed;
         auto place = [&](RangeAlloc& al, Pool& pl, uint64_t units, uint64_t unit_bytes, uint64_t* at) -> sqe_status {
             if (!units) { *at = 0; return SQE_OK; }
             if (!al.take(units, at)) {
@@ -953,13 +970,8 @@ sqe_status sqe_batch_info(sqe_batch* b, int64_t id, sqe_batch_circuit* out) {
         BHIP(hipMemcpy(&v[2], b->meta + id, 8, hipMemcpyDeviceToHost));
         out->done = c.pos >= c.nops ? 1 : 0;
         out->gates_total = c.gates;
-        uint64_t gd = 0;
-        for (uint64_t j = 0; j < c.pos; ++j) {
-            bool rn = false;
-            for (uint32_t r : c.renorm_at) rn |= (r == j);
-            gd += rn ? 0 : 1;
-        }
-        out->gates_done = gd;
+        out->gates_done = (uint64_t)(std::lower_bound(c.gate_at.begin(), c.gate_at.end(), (uint32_t)c.pos) -
+                                     c.gate_at.begin());
         out->updates = v[0];
         out->nnz = v[1];
         out->buckets = v[2];

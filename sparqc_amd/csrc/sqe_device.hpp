@@ -73,12 +73,40 @@ struct DOp {
 // a slot gate is operation for operation the reference formula of its kind
 // (products with exact 0 or +-1 only change the sign of zero results); base
 // gates differ from it only by the folding of consecutive factors (a few ulps).
+//   H_FRAME      apply the per-bucket Pauli frame (below) to the amplitude
+//                registers; the planner puts one before every op that reads or
+//                writes them after frame records
+//   H_NORM_POINT FP64 sum of |a|^2 of the bucket (pending factor included),
+//                written per wavefront for checkpoint hdr >> 13 (the one-pass
+//                renormalisation of the FP32 form); no amplitude changes
 enum : uint32_t {
     H_BASE_DIAG = 0, H_BASE_X, H_BASE_PERM, H_SLOT_DIAG, H_SLOT_X, H_SLOT_PERM, H_SLOT_H, H_SLOT_RY,
-    H_SLOT_U, H_RENORM, H_SLOT_BLOCK
+    H_SLOT_U, H_RENORM, H_SLOT_BLOCK, H_NORM_POINT, H_FRAME
 };
 // handlers whose outputs are pruned (they mix slot pairs)
-__host__ __device__ constexpr bool branching_handler(uint32_t h) { return h >= H_SLOT_H && h != H_RENORM; }
+__host__ __device__ constexpr bool branching_handler(uint32_t h) {
+    return h == H_SLOT_H || h == H_SLOT_RY || h == H_SLOT_U || h == H_SLOT_BLOCK;
+}
+
+// Base-qubit records (H_BASE_*) select their factor c0 / c1 by the base-key
+// bit at `target` (hdr bits 7-12) and then XOR the key with the 64-bit mask
+// in the bits of c[4] (zero for diagonal gates, 1 << target for X / Y / PERM).
+//
+// Per-bucket Pauli frame.  A Pauli gate on a bucket qubit whose controls are
+// base qubits only (CX / CCX from base qubits onto a bucket qubit, CZ between
+// a base and a bucket qubit, X / Y / Z on a bucket qubit) is applied as such a
+// base record: it toggles frame bits kept in the bucket's base-key register,
+// which are zero in every stored key -- the X part at the bucket qubits'
+// own positions and the Z part at bits FRAME_Z_BIT0 + j -- and multiplies the
+// pending factor by the sign or phase the reordering produces.  The logical
+// bucket is g * Z^fz X^fx (registers): before any op that reads or writes the
+// amplitude registers, and before the store, the sweep applies the frame (slot
+// swaps and sign flips, exact) and clears the bits.  Keys of at most 59 qubits
+// leave room for the Z part.
+constexpr int FRAME_Z_BIT0 = 59;
+__host__ __device__ constexpr uint64_t frame_z_mask(int mb) {
+    return mb <= 0 ? 0ull : (((1ull << mb) - 1) << FRAME_Z_BIT0);
+}
 
 // One sweep op: 16 dwords, staged 64 at a time in the lanes of a wavefront.
 // hdr = handler | J << 4 | target << 7; ctl = cmS | cvS << 8.  Ops that read
@@ -288,6 +316,20 @@ __device__ __forceinline__ float opaque_uniform(float v) {
 __device__ __forceinline__ Band32 band32(double thr) {
     return Band32{opaque_uniform((float)(0.25 * thr)), opaque_uniform((float)(4.0 * thr))};
 }
+// Lazy-renormalisation check of the FP32 form: a prune decision taken on an
+// FP64 value within 2^-10 (relative) of the threshold could flip under the
+// renormalisation scale a one-pass run leaves pending, so such decisions are
+// flagged (a workgroup-wide flag the sweep kernel reports; other kernels
+// ignore it).
+__device__ __forceinline__ uint32_t* near_threshold_flag() {
+    __shared__ uint32_t flag;
+    return &flag;
+}
+__device__ __forceinline__ void note_near_threshold(cd v, double thr) {
+    const double m = fmax(fabs(v.x), fabs(v.y));
+    if (m >= thr * (1.0 - 0x1p-10) && m < thr * (1.0 + 0x1p-10)) *near_threshold_flag() = 1u;
+}
+
 template <uint32_t K>
 __device__ __forceinline__ void branch_pair(const DOp& op, double thr, cd v0, cd v1, cd& n0, cd& n1) {
     pair64<K>(op.p, v0, v1, n0, n1);
@@ -309,6 +351,8 @@ __device__ __forceinline__ void branch_pair(const DOp& op, double thr, cf v0, cf
     if (__builtin_amdgcn_ballot_w64(near0 || near1) != 0) {
         cd w0 = mkd(v0.x, v0.y), w1 = mkd(v1.x, v1.y), d0, d1;
         pair64<K>(op.p, w0, w1, d0, d1);
+        if (near0) note_near_threshold(d0, thr);
+        if (near1) note_near_threshold(d1, thr);
         k0 = k0 || (near0 && !negl64(d0, thr));
         k1 = k1 || (near1 && !negl64(d1, thr));
     }

@@ -90,6 +90,13 @@ struct Timed {
 };
 
 enum StepKind { ST_SWEEP, ST_PAIR, ST_NORM };
+
+struct LazyRun {
+    bool pending = false;               // a verdict is outstanding
+    int in = 0;                         // the store holding the run's input
+    uint64_t gates0 = 0, sweeps0 = 0, sweep_ops0 = 0;
+    std::vector<sqe_gate> gl;           // the run's gates
+};
 struct Step {
     StepKind kind;
     uint64_t off, n;    // sweep: ops [off, off+n) of the uploaded program
@@ -151,6 +158,12 @@ struct sqe_engine {
     uint64_t gates = 0;
     uint64_t bytes = 0;
     bool renorm_pending = false;
+    bool prog_frames = false;               // the uploaded program holds frame records
+    LazyRun lz;                             // last one-pass renormalised run, until its verdict is read
+    unsigned long long* lz_dev = nullptr;   // its verdict flag, near-threshold count and Stats backup
+    double* ckpt = nullptr;                 // norm-point partials
+    uint64_t ckpt_cap = 0;
+    uint64_t n_reruns = 0;                  // one-pass runs rerun step by step
 
     uint64_t n_sweeps = 0, n_sweep_ops = 0, n_pairs = 0, n_remaps = 0;
     std::vector<Timed> timed;
@@ -525,24 +538,44 @@ sqe_status ensure_partials(sqe_engine* e, uint32_t g) {
 
 // A sweep with norm_grid != nullptr also writes per-workgroup norm partials
 // (into e->partials) and returns their count.
+// Options of one sweep launch.
+struct SweepOpt {
+    bool blocks = false;                // the program holds fused blocks
+    uint32_t* norm_grid = nullptr;      // non-null: per-workgroup norm partials into e->partials (count returned)
+    int out_store = -1;                 // >= 0: write the result to this store (out of place)
+    double* ckpt = nullptr;             // norm-point partials, ckpt_stride per checkpoint
+    uint32_t* ckpt_stride = nullptr;    // receives the wavefront count of the grid
+    unsigned long long* ambig = nullptr;
+};
+
 sqe_status run_sweep(sqe_engine* e, const MOp* mops, const double* coef, uint64_t nops, int store,
-                     bool blocks = false, uint32_t* norm_grid = nullptr) {
+                     const SweepOpt& opt = SweepOpt{}) {
     SweepArgs a{};
-    a.blocks = blocks ? 1 : 0;
+    a.blocks = opt.blocks ? 1 : 0;
     Store& o = e->s[e->cur];
     a.base = o.base; a.amp = o.amp; a.nb = &e->meta[e->cur]; a.ops = mops; a.coef = coef; a.nops = (uint32_t)nops;
     a.store = store; a.thr = e->thr; a.scale = &e->dscal[1]; a.stats = e->stats;
+    a.qm = e->qm;
+    a.frames = (mops && e->prog_frames) ? 1 : 0;
+    if (opt.out_store >= 0) {
+        a.out_base = e->s[opt.out_store].base;
+        a.out_amp = e->s[opt.out_store].amp;
+    }
     const uint64_t ntiles = std::max<uint64_t>(1, round64(e->nb_upper) / 64);
     const uint32_t g = sweep_grid(e->qm.mb, e->f32, ntiles);
+    a.ckpt = opt.ckpt;
+    a.ckpt_stride = g * (BLOCK / 64);
+    if (opt.ckpt_stride) *opt.ckpt_stride = a.ckpt_stride;
+    a.ambig = opt.ambig;
     SQCK(prep_bstats(e, g));
     a.bstats = e->bstats;
-    if (norm_grid) {
+    if (opt.norm_grid) {
         SQCK(ensure_partials(e, g));
         a.norm_out = e->partials;
         a.norm_done = (unsigned int*)&e->meta[4];
         a.norm2 = &e->dscal[0];
         a.scale_out = &e->dscal[1];
-        *norm_grid = g;
+        *opt.norm_grid = g;
     }
     {
         TimeScope ts(e, 0);
@@ -609,28 +642,71 @@ sqe_status compact(sqe_engine* e) {
     return SQE_OK;
 }
 
-// Plan and launch the queued gates.
-sqe_status flush(sqe_engine* e) {
-    if (e->pending.empty()) return SQE_OK;
-    if (!e->have_state) return fail(SQE_EINVAL, "no state: call sqe_set_basis or sqe_load first");
-    std::vector<sqe_gate> gl;
-    gl.swap(e->pending);
-    if (!e->on_device) {
-        SQCK(materialize(e, gl));
-    } else if (e->cfg.bucket_bits < 0) {
-        const QMap best = choose_map((int)e->cfg.nq, e->cfg.bucket_bits, gl.data(), gl.size());
-        const uint64_t now = count_cross(gl, e->qm), then = count_cross(gl, best);
-        if (best.mask != e->qm.mask && now >= 8 && then * 2 < now) SQCK(remap(e, best));
+// Upload an encoded program (records, then the coefficient table) through
+// the pinned staging buffer; *dm / *dc receive its device addresses.
+sqe_status upload_program(sqe_engine* e, const Encoded& enc, const MOp** dm, const double** dc) {
+    const size_t mbytes = enc.ops.size() * sizeof(MOp);
+    const size_t cbytes = enc.coef.size() * sizeof(double);
+    const size_t coff = (mbytes + 255) & ~(size_t)255;
+    const size_t total = coff + cbytes;
+    *dm = nullptr;
+    *dc = nullptr;
+    if (enc.ops.empty()) return SQE_OK;
+    if (e->upload_pending) {
+        HIPCK(hipEventSynchronize(e->upload_done));
+        e->upload_pending = false;
     }
-    // plan into steps over one uploaded program
+    if (total > e->hprog_cap) {
+        if (e->hprog) (void)hipHostFree(e->hprog);
+        e->hprog = nullptr;
+        e->hprog_cap = 0;
+        const uint64_t cap = pow2_at_least(total);
+        if (hipHostMalloc((void**)&e->hprog, cap, hipHostMallocDefault) != hipSuccess)
+            return fail(SQE_ENOMEM, "pinned program buffer");
+        e->hprog_cap = cap;
+    }
+    if (total > e->dprog_cap) {
+        HIPCK(hipStreamSynchronize(e->st));
+        dfree(e, e->dprog, e->dprog_cap);
+        e->dprog = nullptr;
+        e->dprog_cap = 0;
+        const uint64_t cap = pow2_at_least(total);
+        SQCK(dalloc(e, (void**)&e->dprog, cap));
+        e->dprog_cap = cap;
+    }
+    std::memcpy(e->hprog, enc.ops.data(), mbytes);
+    if (cbytes) std::memcpy(e->hprog + coff, enc.coef.data(), cbytes);
+    HIPCK(hipMemcpyAsync(e->dprog, e->hprog, total, hipMemcpyHostToDevice, e->st));
+    HIPCK(hipEventRecord(e->upload_done, e->st));
+    e->upload_pending = true;
+    e->prog_frames = enc.framed;
+    *dm = (const MOp*)e->dprog;
+    *dc = (const double*)(e->dprog + coff);
+    return SQE_OK;
+}
+
+bool frames_on(const sqe_engine* e) {
+    // keys of at most 59 qubits leave the frame's Z bits free
+    return !e->cfg.frame_off && e->cfg.nq <= (uint32_t)FRAME_Z_BIT0 && e->qm.mb > 0;
+}
+
+void setup_blocks(sqe_engine* e, BlockBuilder& bb, std::vector<std::array<cx, 256>>* blocks) {
+    bb.on = e->cfg.fuse_blocks && e->f32 && e->qm.mb == 4;
+    if (e->cfg.fuse_blocks >= 2) bb.min_gates = e->cfg.fuse_blocks;
+    bb.frames = frames_on(e);
+    bb.qm = e->qm;
+    bb.blocks = blocks;
+}
+
+// Plan and launch gates step by step: fused sweeps, cross-bucket launches and,
+// with renorm_every, one launch per renormalisation period (norm, then the
+// scale enters the next sweep).
+sqe_status flush_exact(sqe_engine* e, const std::vector<sqe_gate>& gl) {
     std::vector<DOp> prog;
     std::vector<Step> steps;
     std::vector<std::array<cx, 256>> blocks;
     BlockBuilder bb;
-    bb.on = e->cfg.fuse_blocks && e->f32 && e->qm.mb == 4;
-    if (e->cfg.fuse_blocks >= 2) bb.min_gates = e->cfg.fuse_blocks;
-    bb.qm = e->qm;
-    bb.blocks = &blocks;
+    setup_blocks(e, bb, &blocks);
     const uint64_t W = e->cfg.fuse_window ? e->cfg.fuse_window : ~0ull;
     uint64_t run_start = 0;
     auto close_run = [&]() {
@@ -653,17 +729,9 @@ sqe_status flush(sqe_engine* e) {
     };
     for (const auto& g : gl) {
         if (gate_local(g, e->qm)) {
-            if (prog.size() == run_start && bb.count == 0) open_run();
-            if (!bb.on) {
-                prog.push_back(plan_op(g, e->qm));
-            } else if (BlockBuilder::plain_slot(g, e->qm)) {
-                bb.add(g);
-            } else if (BlockBuilder::commutes(g, e->qm)) {
-                prog.push_back(plan_op(g, e->qm));          // ahead of the open window
-            } else {
-                bb.flush(prog);
-                prog.push_back(plan_op(g, e->qm));
-            }
+            if (prog.size() == run_start && !bb.open()) open_run();
+            if (!bb.on) prog.push_back(plan_op(g, e->qm));
+            else bb.route(g, prog);
         } else {
             if (e->renorm_pending) open_run();
             close_run();
@@ -678,49 +746,11 @@ sqe_status flush(sqe_engine* e) {
     }
     if (e->renorm_pending) open_run();
     close_run();
-    // upload the program: MOp records, then the coefficient table
-    std::vector<MOp> mops;
-    std::vector<double> coef;
-    mops.reserve(prog.size());
-    for (const auto& d : prog)
-        mops.push_back(d.kind == K_BLOCK ? encode_block(d, blocks[(size_t)d.p[0]], coef) : encode_op(d, coef, e->f32));
-    const size_t mbytes = mops.size() * sizeof(MOp);
-    const size_t cbytes = coef.size() * sizeof(double);
-    const size_t coff = (mbytes + 255) & ~(size_t)255;
-    const size_t total = coff + cbytes;
+    Encoded enc;
+    encode_program(prog, blocks, e->f32, frames_on(e) ? &e->qm : nullptr, enc);
     const MOp* dm = nullptr;
     const double* dc = nullptr;
-    if (!prog.empty()) {
-        if (e->upload_pending) {
-            HIPCK(hipEventSynchronize(e->upload_done));
-            e->upload_pending = false;
-        }
-        if (total > e->hprog_cap) {
-            if (e->hprog) (void)hipHostFree(e->hprog);
-            e->hprog = nullptr;
-            e->hprog_cap = 0;
-            const uint64_t cap = pow2_at_least(total);
-            if (hipHostMalloc((void**)&e->hprog, cap, hipHostMallocDefault) != hipSuccess)
-                return fail(SQE_ENOMEM, "pinned program buffer");
-            e->hprog_cap = cap;
-        }
-        if (total > e->dprog_cap) {
-            HIPCK(hipStreamSynchronize(e->st));
-            dfree(e, e->dprog, e->dprog_cap);
-            e->dprog = nullptr;
-            e->dprog_cap = 0;
-            const uint64_t cap = pow2_at_least(total);
-            SQCK(dalloc(e, (void**)&e->dprog, cap));
-            e->dprog_cap = cap;
-        }
-        std::memcpy(e->hprog, mops.data(), mbytes);
-        if (cbytes) std::memcpy(e->hprog + coff, coef.data(), cbytes);
-        HIPCK(hipMemcpyAsync(e->dprog, e->hprog, total, hipMemcpyHostToDevice, e->st));
-        HIPCK(hipEventRecord(e->upload_done, e->st));
-        e->upload_pending = true;
-        dm = (const MOp*)e->dprog;
-        dc = (const double*)(e->dprog + coff);
-    }
+    SQCK(upload_program(e, enc, &dm, &dc));
     uint32_t fused_norm = 0;              // partials of the last sweep, when it computed them
     for (size_t si = 0; si < steps.size(); ++si) {
         const Step& s = steps[si];
@@ -732,12 +762,15 @@ sqe_status flush(sqe_engine* e) {
             // epilogue (SQE_NORM_IN_SWEEP=0 at build time: a separate
             // reduction instead, for debugging)
             const bool norm_next = SQE_NORM_IN_SWEEP && si + 1 < steps.size() && steps[si + 1].kind == ST_NORM;
-            SQCK(run_sweep(e, dm + s.off, dc, s.n, 1, has_block, norm_next ? &fused_norm : nullptr));
+            SweepOpt so;
+            so.blocks = has_block;
+            so.norm_grid = norm_next ? &fused_norm : nullptr;
+            SQCK(run_sweep(e, dm + enc.start[s.off], dc, enc.start[s.off + s.n] - enc.start[s.off], 1, so));
             ++e->n_sweeps;
             for (uint64_t i = s.off; i < s.off + s.n; ++i) {
                 const DOp& d = prog[i];
                 if (d.kind == K_BLOCK) e->n_sweep_ops += (uint64_t)d.p[1];
-                else if (d.kind != K_RENORM) ++e->n_sweep_ops;
+                else if (d.kind != K_RENORM && d.kind != K_NORM_POINT) ++e->n_sweep_ops;
                 if (d.tbase && kind_is_perm(d.kind)) e->tab_valid = false;
             }
         } break;
@@ -751,6 +784,157 @@ sqe_status flush(sqe_engine* e) {
     return reduce_stats(e);
 }
 
+// The one-pass form applies when every gate is bucket-local (FP32 with
+// renormalisation) and the other store can take the result.
+bool lazy_applies(const sqe_engine* e, const std::vector<sqe_gate>& gl) {
+    if (!e->f32 || !e->cfg.renorm_every || e->cfg.renorm_mode == 0 || e->renorm_pending) return false;
+    const Store& in = e->s[e->cur];
+    const Store& out = e->s[e->cur ^ 1];
+    if (!out.base || out.cap < round64(e->nb_upper) || out.mb != in.mb || out.mb != e->qm.mb) return false;
+    for (const auto& g : gl)
+        if (!gate_local(g, e->qm)) return false;
+    return true;
+}
+
+// One pass over a run of bucket-local gates with renormalisation every R
+// gates: the sweep reads the state once (out of place, so the input stays
+// for a rerun) and records the FP64 norm at each renormalisation point
+// without rescaling; a fold kernel takes the norms, checks that no prune
+// decision could have changed under the scales the step-by-step run applies,
+// and leaves 1/sqrt(norm) of the last point, which one more in-place pass
+// applies.  lazy_settle() reads the verdict and reruns step by step if needed.
+sqe_status flush_lazy(sqe_engine* e, const std::vector<sqe_gate>& gl) {
+    LazyRun& L = e->lz;
+    L.gates0 = e->gates;
+    L.in = e->cur;
+    L.sweeps0 = e->n_sweeps;
+    L.sweep_ops0 = e->n_sweep_ops;
+    std::vector<DOp> prog;
+    std::vector<std::array<cx, 256>> blocks;
+    BlockBuilder bb;
+    setup_blocks(e, bb, &blocks);
+    uint32_t k = 0;
+    for (const auto& g : gl) {
+        if (!bb.on) prog.push_back(plan_op(g, e->qm));
+        else bb.route(g, prog);
+        ++e->gates;
+        if (e->gates % e->cfg.renorm_every == 0) {
+            bb.flush(prog);
+            prog.push_back(norm_point_op(k++));
+        }
+    }
+    bb.flush(prog);
+    const size_t n_main = prog.size();
+    if (k) prog.push_back(renorm_op());
+    Encoded enc;
+    encode_program(prog, blocks, e->f32, frames_on(e) ? &e->qm : nullptr, enc);
+    const MOp* dm = nullptr;
+    const double* dc = nullptr;
+    SQCK(upload_program(e, enc, &dm, &dc));
+    // device words: [0] verdict flag, [1] near-threshold count, then the Stats backup
+    if (!e->lz_dev) SQCK(dalloc(e, (void**)&e->lz_dev, 256));
+    SQCK(reduce_stats(e));
+    HIPCK(hipMemcpyAsync(e->lz_dev + 2, e->stats, sizeof(Stats), hipMemcpyDeviceToDevice, e->st));
+    HIPCK(launch_fill_u64(e->st, (uint64_t*)e->lz_dev, 2, 0));
+    const int out = e->cur ^ 1;
+    HIPCK(hipMemcpyAsync(&e->meta[out], &e->meta[e->cur], 8, hipMemcpyDeviceToDevice, e->st));
+    // checkpoint partials: one row of wavefronts per norm point
+    const uint64_t ntiles = std::max<uint64_t>(1, round64(e->nb_upper) / 64);
+    const uint64_t waves = (uint64_t)sweep_grid(e->qm.mb, e->f32, ntiles) * (BLOCK / 64);
+    if ((uint64_t)k * waves > e->ckpt_cap) {
+        HIPCK(hipStreamSynchronize(e->st));
+        dfree(e, e->ckpt, e->ckpt_cap * 8);
+        e->ckpt = nullptr;
+        e->ckpt_cap = 0;
+        const uint64_t cap = pow2_at_least((uint64_t)k * waves);
+        SQCK(dalloc(e, (void**)&e->ckpt, cap * 8));
+        e->ckpt_cap = cap;
+    }
+    // wavefronts of the grid without a tile write no partials: start from zeros
+    if (k) HIPCK(launch_fill_u64(e->st, (uint64_t*)e->ckpt, (uint64_t)k * waves, 0));
+    const uint64_t W = e->cfg.fuse_window ? e->cfg.fuse_window : ~0ull;
+    bool first = true;
+    uint32_t stride = 0;
+    for (uint64_t off = 0; off < n_main || first;) {
+        const uint64_t n = std::min<uint64_t>(n_main - off, W);
+        bool has_block = false;
+        for (uint64_t i = off; i < off + n; ++i) has_block |= prog[i].kind == K_BLOCK;
+        SweepOpt so;
+        so.blocks = has_block;
+        so.out_store = first ? out : -1;
+        so.ckpt = e->ckpt;
+        so.ckpt_stride = &stride;
+        so.ambig = e->lz_dev + 1;
+        SQCK(run_sweep(e, dm + enc.start[off], dc, enc.start[off + n] - enc.start[off], 1, so));
+        if (first) e->cur = out;
+        first = false;
+        ++e->n_sweeps;
+        for (uint64_t i = off; i < off + n; ++i) {
+            const DOp& d = prog[i];
+            if (d.kind == K_BLOCK) e->n_sweep_ops += (uint64_t)d.p[1];
+            else if (d.kind != K_RENORM && d.kind != K_NORM_POINT) ++e->n_sweep_ops;
+        }
+        off += n;
+        if (n == 0) break;
+    }
+    e->tab_valid = false;
+    if (k) {
+        HIPCK(launch_ckpt_fold(e->st, e->ckpt, stride, k, e->lz_dev + 1, 0x1p-11, e->cfg.renorm_mode == 2 ? 1 : 0,
+                               &e->dscal[0], &e->dscal[1], e->lz_dev));
+        SQCK(run_sweep(e, dm + enc.start[n_main], dc, enc.start[n_main + 1] - enc.start[n_main], 1));
+        ++e->n_sweeps;
+    } else if (e->cfg.renorm_mode == 2) {
+        HIPCK(launch_fill_u64(e->st, (uint64_t*)e->lz_dev, 1, 1));
+    }
+    L.gl = gl;
+    L.pending = true;
+    return reduce_stats(e);
+}
+
+// Read the verdict of the last one-pass run; rerun its gates step by step
+// from its input (counters restored) when it does not stand.
+sqe_status lazy_settle(sqe_engine* e) {
+    LazyRun& L = e->lz;
+    if (!L.pending) return SQE_OK;
+    L.pending = false;
+    unsigned long long flag = 0;
+    HIPCK(hipMemcpyAsync(&flag, e->lz_dev, 8, hipMemcpyDeviceToHost, e->st));
+    HIPCK(hipStreamSynchronize(e->st));
+    if (!flag) return SQE_OK;
+    ++e->n_reruns;
+    e->cur = L.in;
+    e->gates = L.gates0;
+    e->n_sweeps = L.sweeps0;
+    e->n_sweep_ops = L.sweep_ops0;
+    e->renorm_pending = false;
+    e->tab_valid = false;
+    HIPCK(hipMemcpyAsync(e->stats, e->lz_dev + 2, sizeof(Stats), hipMemcpyDeviceToDevice, e->st));
+    if (e->bstats_cap) HIPCK(launch_fill_u64(e->st, (uint64_t*)e->bstats, e->bstats_cap * (sizeof(BlockStats) / 8), 0));
+    e->acc_hi = 0;
+    e->last_grid = 0;
+    std::vector<sqe_gate> gl;
+    gl.swap(L.gl);
+    return flush_exact(e, gl);
+}
+
+// Plan and launch the queued gates.
+sqe_status flush(sqe_engine* e) {
+    SQCK(lazy_settle(e));
+    if (e->pending.empty()) return SQE_OK;
+    if (!e->have_state) return fail(SQE_EINVAL, "no state: call sqe_set_basis or sqe_load first");
+    std::vector<sqe_gate> gl;
+    gl.swap(e->pending);
+    if (!e->on_device) {
+        SQCK(materialize(e, gl));
+    } else if (e->cfg.bucket_bits < 0) {
+        const QMap best = choose_map((int)e->cfg.nq, e->cfg.bucket_bits, gl.data(), gl.size());
+        const uint64_t now = count_cross(gl, e->qm), then = count_cross(gl, best);
+        if (best.mask != e->qm.mask && now >= 8 && then * 2 < now) SQCK(remap(e, best));
+    }
+    if (lazy_applies(e, gl)) return flush_lazy(e, gl);
+    return flush_exact(e, gl);
+}
+
 sqe_status enter(sqe_engine* e) {
     if (!e) return fail(SQE_EINVAL, "null engine");
     HIPCK(hipSetDevice(e->cfg.device));
@@ -759,6 +943,7 @@ sqe_status enter(sqe_engine* e) {
 
 sqe_status full_sync(sqe_engine* e) {
     SQCK(flush(e));
+    SQCK(lazy_settle(e));
     HIPCK(hipStreamSynchronize(e->st));
     SQCK(check_device_errors(e));
     collect_times(e);
@@ -796,6 +981,8 @@ void destroy_all(sqe_engine* e) {
     dfree(e, e->xkeys, e->xcap * 8);
     dfree(e, e->xre, e->xcap * 8);
     dfree(e, e->xim, e->xcap * 8);
+    dfree(e, e->lz_dev, 256);
+    dfree(e, e->ckpt, e->ckpt_cap * 8);
     if (e->hprog) (void)hipHostFree(e->hprog);
     for (auto& t : e->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
@@ -894,8 +1081,11 @@ sqe_status sqe_load(sqe_engine* e, const uint64_t* keys, const double* re, const
             if (re[j] == 0.0 && im[j] == 0.0) continue;
             k.push_back(keys[j]); r.push_back(re[j]); i.push_back(im[j]);
         }
-        // queued gates belonged to the state being replaced
+        // queued gates, and the verdict of a one-pass run, belonged to the
+        // state being replaced
         e->pending.clear();
+        e->lz.pending = false;
+        e->lz.gl.clear();
         HIPCK(hipStreamSynchronize(e->st));
         e->hkeys.swap(k);
         e->hre.swap(r);
@@ -1021,7 +1211,7 @@ sqe_status sqe_norm2_noalloc(sqe_engine* e, double* out) {
         SQCK(enter(e));
         if (!out) return fail(SQE_EINVAL, "null output");
         // queued gates would have to be planned and uploaded, which can allocate
-        if (!e->pending.empty())
+        if (!e->pending.empty() || e->lz.pending)
             return fail(SQE_EINVAL, "sqe_norm2_noalloc: gates are queued; call sqe_sync first");
         if (!e->have_state || !e->on_device) {
             double s = 0.0;
@@ -1236,6 +1426,7 @@ sqe_status sqe_get_stats(sqe_engine* e, sqe_stats* out) {
         out->sweep_ops = e->n_sweep_ops;
         out->pairs = e->n_pairs;
         out->remaps = e->n_remaps;
+        out->reruns = (uint32_t)e->n_reruns;
         out->moved = h.moved;
         out->sweep_ms = e->t_ms[0];
         out->pair_ms = e->t_ms[1];
@@ -1287,6 +1478,8 @@ sqe_status sqe_snapshot_restore(sqe_engine* e) {
         SQCK(enter(e));
         if (!e->snap_valid) return fail(SQE_EINVAL, "no snapshot");
         e->pending.clear();
+        e->lz.pending = false;
+        e->lz.gl.clear();
         HIPCK(hipStreamSynchronize(e->st));
         unsigned long long nb = 0;
         HIPCK(hipMemcpy(&nb, &e->meta[3], 8, hipMemcpyDeviceToHost));

@@ -87,6 +87,19 @@ __device__ __forceinline__ uint64_t flip_if(uint64_t base, uint64_t flip, bool p
     return ((uint64_t)hi << 32) | lo;
 }
 
+// -v where c holds (a sign flip of both components: one XOR per FP64 high
+// word or FP32 word with a lane mask).
+__device__ __forceinline__ Cplx<double> negate_if(Cplx<double> v, bool c) {
+    const uint64_t m = c ? 0x8000000000000000ull : 0ull;
+    return Cplx<double>{__builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v.x) ^ m),
+                        __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v.y) ^ m)};
+}
+__device__ __forceinline__ Cplx<float> negate_if(Cplx<float> v, bool c) {
+    const uint32_t m = c ? 0x80000000u : 0u;
+    return Cplx<float>{__builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v.x) ^ m),
+                       __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v.y) ^ m)};
+}
+
 // Decoded sweep op (uniform across the wavefront).
 struct SOp {
     uint32_t h, j, cmS, cvS;
@@ -164,6 +177,8 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
             if (__builtin_amdgcn_ballot_w64(near0 || near1) != 0) {
                 cd e0, e1;
                 exact(mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
+                if (near0) note_near_threshold(e0, thr);
+                if (near1) note_near_threshold(e1, thr);
                 k0 = k0 || (near0 && !negl64(e0, thr));
                 k1 = k1 || (near1 && !negl64(e1, thr));
             }
@@ -349,10 +364,63 @@ __device__ __forceinline__ void block_apply(Cplx<float> (&a)[16], float pr, floa
                 er += mr * xr - mi * xi;
                 ei += mr * xi + mi * xr;
             }
+            if (near) note_near_threshold(mkd(er, ei), thr);
             keep = keep || (near && !negl64(mkd(er, ei), thr));
         }
         a[s] = keep ? n : mkf(0.0f, 0.0f);
     }
+}
+
+// Apply the per-bucket Pauli frame held in the base-key registers (see
+// sqe_device.hpp) to the amplitude registers and clear it: for each bucket
+// qubit j, swap the slot pairs along bit j in buckets whose frame has X_j, then
+// negate the slots with bit j set in buckets whose frame has Z_j.  Both are
+// exact.  Wavefront-uniform branches skip the bits no bucket of the
+// wavefront has set.
+template <int MB, class R, int EPT>
+__device__ __forceinline__ void frame_apply(typename CT<R>::T (&a)[EPT][1 << MB], uint64_t (&base)[EPT],
+                                            const QMap& qm) {
+    using C = typename CT<R>::T;
+    constexpr int HALF = (1 << MB) / 2;
+    const uint64_t fmask = qm.mask | frame_z_mask(MB);
+    bool any = false;
+    static_for<EPT>([&](auto ec) __attribute__((always_inline)) { any |= (base[ec] & fmask) != 0; });
+    if (__builtin_amdgcn_ballot_w64(any) == 0) return;
+    static_for<MB>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int J = decltype(jc)::value;
+        const uint32_t qj = qm.q[J];
+        bool bx[EPT], bz[EPT];
+        bool anyx = false, anyz = false;
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+            constexpr int e = decltype(ec)::value;
+            bx[e] = ((base[e] >> qj) & 1ull) != 0;
+            bz[e] = ((base[e] >> (FRAME_Z_BIT0 + J)) & 1ull) != 0;
+            anyx |= bx[e];
+            anyz |= bz[e];
+        });
+        if (__builtin_amdgcn_ballot_w64(anyx) != 0) {
+            static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                constexpr int e = decltype(ec)::value;
+                static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+                    constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+                    constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+                    const C x = a[e][s0], y = a[e][s1];
+                    a[e][s0] = csel(bx[e], y, x);
+                    a[e][s1] = csel(bx[e], x, y);
+                });
+            });
+        }
+        if (__builtin_amdgcn_ballot_w64(anyz) != 0) {
+            static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                constexpr int e = decltype(ec)::value;
+                static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+                    constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+                    a[e][s1] = negate_if(a[e][s1], bz[e]);
+                });
+            });
+        }
+    });
+    static_for<EPT>([&](auto ec) __attribute__((always_inline)) { base[ec] &= ~fmask; });
 }
 
 // Per-thread counters of one sweep group.
@@ -384,7 +452,8 @@ template <int MB, class R, int EPT, bool BLOCKS = false, bool SREC = false>
 __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* ampv, const int64_t (&tile)[EPT],
                                             const MOp* __restrict__ ops, uint32_t nops,
                                             const double* __restrict__ coef, double thr,
-                                            const double* __restrict__ scale, int store, SweepCount& out) {
+                                            const double* __restrict__ scale, int store, SweepCount& out,
+                                            const SweepCtx& ctx) {
     using C = typename CT<R>::T;
     constexpr int S = 1 << MB;
     const uint32_t lane = threadIdx.x & 63;
@@ -441,6 +510,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
     uint64_t bmask = 0;               // bit k: op c0 + k is a base-class op
     uint64_t cmask = 0;               // bit k: op c0 + k has base controls
     uint64_t rmask = 0;               // bit k: op c0 + k is a renormalisation
+    uint64_t nmask = 0;               // bit k: op c0 + k is a norm point
     auto stage = [&](uint32_t from) __attribute__((always_inline)) {
         c0 = from;
         const uint32_t mine = from + lane < nops ? from + lane : nops - 1;
@@ -466,6 +536,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
         bmask = __ballot(from + lane < nops && (ren || base_class(q0.x & 15u, q0.y & 0xffu)));
         cmask = __ballot(from + lane < nops && (q0.z | q0.w) != 0);
         rmask = __ballot(from + lane < nops && ren);
+        nmask = __ballot(from + lane < nops && (q0.x & 15u) == H_NORM_POINT);
         sp[lane * 4] = q0;
         sp[lane * 4 + 1] = q1;
         sp[lane * 4 + 2] = q2;
@@ -501,9 +572,8 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             constexpr bool CTRL = decltype(ctrl_c)::value;
             const uint4 q0 = sp[jj * 4], q1 = sp[jj * 4 + 1], q2 = sp[jj * 4 + 2], q3 = sp[jj * 4 + 3];
             const uint32_t hdr = __builtin_amdgcn_readfirstlane(q0.x);
-            const uint32_t h = hdr & 15u;
             const uint64_t tbase = 1ull << ((hdr >> 7) & 63u);
-            const uint64_t flip = (h == H_BASE_X || h == H_BASE_PERM) ? tbase : 0ull;
+            const uint64_t flip = u64of(q3.z, q3.w);          // key bits to toggle (c[4])
             const uint64_t cmB = u64of(q0.z, q0.w), cvB = u64of(q1.x, q1.y);
             const uint32_t tpos = (hdr >> 7) & 63u;
             const bool thi = tpos >= 32;                    // the target bit's word (uniform)
@@ -572,6 +642,24 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             else base_run(jj, n, std::false_type{});
             i += n;
             pend += n - (uint32_t)__builtin_popcountll((rmask >> jj) & run);   // renormalisations are not gates
+        } else if ((nmask >> jj) & 1ull) {
+            // ---- norm point: FP64 norm of the state at this point of the
+            // program (the frame and the pending factor's phase leave |a|
+            // unchanged); reads the amplitude registers only ----
+            const uint32_t k = __builtin_amdgcn_readfirstlane(sp[jj * 4].x) >> 13;
+            double v = 0.0;
+            static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                constexpr int e = decltype(ec)::value;
+                double t = 0.0;
+                static_for<S>([&](auto sc) __attribute__((always_inline)) {
+                    constexpr int s = decltype(sc)::value;
+                    t += (double)a[e][s].x * (double)a[e][s].x + (double)a[e][s].y * (double)a[e][s].y;
+                });
+                v += t * ((double)pr[e] * (double)pr[e] + (double)pi[e] * (double)pi[e]);
+            });
+            for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+            if (lane == 0 && ctx.ckpt) ctx.ckpt[(size_t)k * ctx.ckpt_stride + ctx.wave] = v;
+            ++i;
         } else {
             // ---- one slot op: amplitude registers ----
             ++i;
@@ -602,7 +690,10 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             }
             op.h = hdr & 15u;
             op.cmS = ctl & 0xffu;
-            {
+            if (op.h == H_FRAME) {
+                // apply the per-bucket frame before the next amplitude op
+                frame_apply<MB, R, EPT>(a, base, ctx.qm);
+            } else {
                 ++pend;
                 op.cvS = (ctl >> 8) & 0xffu;
                 op.j = (hdr >> 4) & 7u;
@@ -666,6 +757,7 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
         }
     }
     fold_upd();
+    if (ctx.frames) frame_apply<MB, R, EPT>(a, base, ctx.qm);
     out.nnz_out = 0;
     out.empty = 0;
     out.norm = 0.0;
@@ -679,11 +771,13 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
         if (tile[e] >= 0 && store) {
             // (padding buckets may carry flipped keys; only valid keys are stored)
             const uint64_t b = (uint64_t)tile[e] * TILE + lane;
+            C* __restrict__ dst = ctx.out_amp ? (C*)ctx.out_amp : amp;
             static_for<S>([&](auto sc) __attribute__((always_inline)) {
                 constexpr int s = decltype(sc)::value;
-                amp[amp_index(b, s, S)] = a[e][s];
+                dst[amp_index(b, s, S)] = a[e][s];
             });
-            if (valid[e] && base[e] != base0[e]) basep[b] = base[e];
+            if (ctx.out_base) ctx.out_base[b] = valid[e] ? base[e] : EMPTY_KEY;
+            else if (valid[e] && base[e] != base0[e]) basep[b] = base[e];
         }
         out.nnz_out += cnt[e];
         out.empty += (valid[e] && cnt[e] == 0) ? 1ull : 0ull;
@@ -752,9 +846,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
         any |= t < ntiles;
     });
     SweepCount cnt{0, 0, 0, 0, 0.0};
+    if (A.ambig) {
+        if (threadIdx.x == 0) *near_threshold_flag() = 0u;
+        __syncthreads();
+    }
+    const SweepCtx ctx{A.qm, A.frames, A.ckpt_stride, (uint32_t)g, A.out_base, A.out_amp, A.ckpt};
     if (any)
-        sweep_group<MB, R, EPT, BLOCKS, true>(A.base, A.amp, tile, A.ops, A.nops, A.coef, A.thr, A.scale, A.store, cnt);
+        sweep_group<MB, R, EPT, BLOCKS, true>(A.base, A.amp, tile, A.ops, A.nops, A.coef, A.thr, A.scale, A.store, cnt,
+                                              ctx);
     block_stats(A.bstats, cnt.upd, cnt.nnz_out, cnt.empty, A.store ? cnt.nnz_in + cnt.nnz_out : 0ull);
+    // (block_stats ends with a workgroup barrier, after every wave's flag write)
+    if (A.ambig && threadIdx.x == 0 && *near_threshold_flag()) atomicAdd(A.ambig, 1ull);
     if (A.norm_out) {
         // the FP64 norm of the stored state, one partial per workgroup (for a
         // renormalisation right after this launch)
@@ -807,7 +909,8 @@ __global__ __launch_bounds__(BLOCK) void k_count(SweepArgs A) {
     const int64_t t = (int64_t)blockIdx.x * (BLOCK / 64) + wave;
     const int64_t tile[1] = {t < ntiles ? t : -1};
     SweepCount cnt{0, 0, 0, 0, 0.0};
-    if (t < ntiles) sweep_group<MB, R, 1>(A.base, A.amp, tile, nullptr, 0, nullptr, A.thr, nullptr, 0, cnt);
+    const SweepCtx ctx{A.qm, 0, 0, 0, nullptr, nullptr, nullptr};
+    if (t < ntiles) sweep_group<MB, R, 1>(A.base, A.amp, tile, nullptr, 0, nullptr, A.thr, nullptr, 0, cnt, ctx);
     block_stats(A.bstats, 0ull, cnt.nnz_out, cnt.empty, 0ull);
 }
 
@@ -834,7 +937,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
         if (tile >= t.ntiles) break;
         const int64_t tl[1] = {(int64_t)tile};
         SweepCount c{0, 0, 0, 0};
-        sweep_group<MB, R, 1>(t.base, t.amp, tl, t.ops, t.nops, t.coef, A.thr, t.scale, 1, c);
+        const SweepCtx ctx{t.qm, t.frames, 0, it, nullptr, nullptr, nullptr};
+        sweep_group<MB, R, 1>(t.base, t.amp, tl, t.ops, t.nops, t.coef, A.thr, t.scale, 1, c, ctx);
         upd += c.upd;
         nnz += c.nnz_out;
     }

@@ -26,6 +26,24 @@ struct SweepArgs {
     const double* scale;
     Stats* stats;
     BlockStats* bstats;
+    QMap qm;                // bucket qubits (positions of the frame's X part)
+    int32_t frames;         // the program holds frame records: apply the frame before amplitude ops
+    uint32_t ckpt_stride;   // wavefronts of the grid (row length of ckpt)
+    uint64_t* out_base;     // non-null: store the result here (out of place) instead of in place
+    void* out_amp;
+    double* ckpt;           // [checkpoint * ckpt_stride + wavefront]: FP64 norms of H_NORM_POINT ops
+    unsigned long long* ambig;  // FP32 prune decisions within 2^-10 of the threshold (counted per workgroup)
+};
+
+// Per-launch context of sweep_group beyond the program.
+struct SweepCtx {
+    QMap qm;
+    int32_t frames;
+    uint32_t ckpt_stride;
+    uint32_t wave;          // this wavefront's index in the grid
+    uint64_t* out_base;
+    void* out_amp;
+    double* ckpt;
 };
 
 // One circuit's share of a batched launch.
@@ -42,6 +60,9 @@ struct BTask {
     uint32_t ntiles;
     uint32_t item0;              // first work item of this circuit (items are grouped by circuit)
     uint32_t nitem;
+    QMap qm;                     // the circuit's bucket qubits
+    int32_t frames;              // its program holds frame records
+    int32_t pad;
 };
 
 struct BItem {                   // a circuit and the first of `gran` consecutive tiles
@@ -169,6 +190,11 @@ hipError_t launch_norm_batch(int mb, bool f32, hipStream_t st, uint32_t grid, ui
 hipError_t launch_batch_fold(hipStream_t st, const BTask* tasks, uint32_t ntasks, const unsigned long long* counts);
 hipError_t launch_stats_reduce(hipStream_t st, BlockStats* bs, uint32_t n_acc, uint32_t n_last, Stats* stats);
 hipError_t launch_load(bool f32, hipStream_t st, const LoadArgs& a, const EnumArgs& e, uint32_t enum_grid);
+// Checkpoint norms of a one-pass renormalised run -> norm2 / scale of the last
+// checkpoint and the verdict flag (0: the run stands); see k_ckpt_fold.
+hipError_t launch_ckpt_fold(hipStream_t st, const double* ckpt, uint32_t nw, uint32_t nk,
+                            const unsigned long long* ambig, double tol, int force_fail, double* norm2, double* scale,
+                            unsigned long long* flag);
 
 // Fixed per-device scratch of the allocation-free entry points: room for
 // NOALLOC_PARTIALS FP64 per-workgroup partials (128 KB).  noalloc_scratch_init

@@ -89,13 +89,69 @@ inline DOp plan_op(const sqe_gate& g, const QMap& qm) {
     return d;
 }
 
+// A Pauli gate on a bucket qubit that the sweep applies to the per-bucket
+// frame (sqe_device.hpp) as a base record: X / Y / Z on bucket qubit j with
+// controls on base qubits only (or none), or Z on a base qubit b with exactly
+// one bucket-qubit control q_j -- a CZ, which acts on buckets whose base bit b
+// is 1 as Z_j (control value 1) or -Z_j (control value 0).  With the frame
+// logical = g Z^fz X^fx (registers):
+//   X_j: fx ^= e_j and g *= (-1)^fz_j    (factor selected by the fz_j bit)
+//   Z_j: fz ^= e_j
+//   Y_j = i X_j Z_j: fz ^= e_j, fx ^= e_j, g *= (fz_j ? i : -i) (fz_j before)
+// Returns false when the gate is not of this form.
+inline bool frame_record(const DOp& d, const QMap& qm, bool f32, MOp* out) {
+    int j = -1;
+    uint64_t cm = d.cmB, cv = d.cvB;
+    double f0r = 1, f0i = 0, f1r = 1, f1i = 0;
+    uint64_t flip = 0;
+    if (d.tslot >= 0 && d.cmS == 0 && (d.kind == KX || d.kind == KY || d.kind == KZ)) {
+        j = d.tslot;
+        const uint64_t xb = 1ull << qm.q[j], zb = 1ull << (FRAME_Z_BIT0 + j);
+        if (d.kind == KX) { flip = xb; f1r = -1; }
+        else if (d.kind == KZ) { flip = zb; }
+        else { flip = xb | zb; f0r = 0; f0i = -1; f1r = 0; f1i = 1; }
+    } else if (d.tslot < 0 && d.kind == KZ && d.cmS != 0 && (d.cmS & (d.cmS - 1)) == 0) {
+        j = __builtin_ctz(d.cmS);
+        cm |= d.tbase;
+        cv |= d.tbase;
+        flip = 1ull << (FRAME_Z_BIT0 + j);
+        if (!(d.cvS & d.cmS)) { f0r = -1; f1r = -1; }
+    } else {
+        return false;
+    }
+    MOp m{};
+    m.hdr = H_BASE_PERM | (uint32_t)(FRAME_Z_BIT0 + j) << 7;
+    m.ctl = 0;
+    m.cmB = cm;
+    m.cvB = cv;
+    if (f32) {
+        const float f[4] = {(float)f0r, (float)f0i, (float)f1r, (float)f1i};
+        std::memcpy(&m.c[0], f, 16);
+    } else {
+        m.c[0] = f0r; m.c[1] = f0i; m.c[2] = f1r; m.c[3] = f1i;
+    }
+    m.c[4] = coef_index_bits(flip);
+    *out = m;
+    return true;
+}
+
 // Pack a planned op into a 64-byte sweep record: pick the handler and turn the
 // gate's coefficients into the handler's (see the handler table in
 // sqe_device.hpp).  General U keeps its eight coefficients in the side table.
 // f32: the program runs on FP32 amplitudes; a base-qubit op (whose factor
 // the sweep folds in FP32 there) then carries c0 / c1 as FP32 pairs in the
 // first two coefficient slots: c[0] = (f0r, f0i), c[1] = (f1r, f1i).
-inline MOp encode_op(const DOp& d, std::vector<double>& coef, bool f32 = false) {
+// qm non-null: Pauli gates of the frame_record form become frame records
+// (*framed is set when one does).
+inline MOp encode_op(const DOp& d, std::vector<double>& coef, bool f32 = false, const QMap* qm = nullptr,
+                     bool* framed = nullptr) {
+    if (qm) {
+        MOp fr;
+        if (frame_record(d, *qm, f32, &fr)) {
+            if (framed) *framed = true;
+            return fr;
+        }
+    }
     MOp m{};
     const bool slot = d.tslot >= 0;
     const uint32_t j = slot ? (uint32_t)d.tslot : 0u;
@@ -140,6 +196,7 @@ inline MOp encode_op(const DOp& d, std::vector<double>& coef, bool f32 = false) 
     }
     m.hdr = h | j << 4 | target << 7;
     if (side) m.c[4] = coef_index_bits(ci);
+    if (h <= H_BASE_PERM) m.c[4] = coef_index_bits((h == H_BASE_X || h == H_BASE_PERM) ? d.tbase : 0ull);
     m.ctl = (d.cmS & 0xffu) | (d.cvS & 0xffu) << 8;
     m.cmB = d.cmB;
     m.cvB = d.cvB;
@@ -179,14 +236,49 @@ inline void gate_matrix(const sqe_gate& g, cx m[4]) {
 }
 
 constexpr uint32_t K_BLOCK = 17;                  // engine-internal op: fused dense block
+constexpr uint32_t K_NORM_POINT = 18;             // engine-internal op: norm checkpoint p[0]
 
+// The planned op becomes a frame record (see frame_record).
+inline bool is_frame_op(const DOp& d) {
+    if (d.tslot >= 0) return d.cmS == 0 && (d.kind == KX || d.kind == KY || d.kind == KZ);
+    return d.kind == KZ && d.cmS != 0 && (d.cmS & (d.cmS - 1)) == 0;
+}
+
+// The op reads or writes the amplitude registers (so a pending frame must be
+// applied first): gates on bucket qubits, base-qubit gates restricted by
+// bucket controls and fused blocks.  Base-class gates, renormalisations and
+// norm points only touch the base key and the pending factor.
+inline bool touches_amplitudes(const DOp& d) {
+    if (d.kind == K_BLOCK) return true;
+    if (d.kind == K_RENORM || d.kind == K_NORM_POINT) return false;
+    return d.tslot >= 0 || d.cmS != 0;
+}
+
+// Fused dense blocks (FP32, 16-slot buckets) built from a program's gates in
+// order.  Plain gates on bucket qubits (no controls) go into the open window,
+// whose product is one 16x16 matrix.  Base-class gates commute with the
+// window (they scale or relabel whole buckets) and are emitted ahead of it.
+// Frame gates (Pauli gates on a bucket qubit whose controls are base
+// qubits, see frame_record) act on one bucket qubit with a per-bucket
+// condition, so they cannot enter a shared matrix; they are held back and
+// emitted after the block, which is exact as long as the window's later gates
+// commute with them: a held gate on qubit j blocks only later window gates on
+// j that it does not commute with (a Z-type gate commutes with diagonal
+// gates, an X-type gate with X), and later base gates that flip one of its
+// control bits.  Any other gate closes the window.  A window of fewer than
+// min_gates gates runs gate by gate; prune decisions are made once, on the
+// block's outputs.
 struct BlockBuilder {
     bool on = false;                              // fusion enabled for this program
+    bool frames = false;                          // frame gates may be held back past the window
     uint32_t min_gates = 6;                       // smaller windows run gate by gate
     QMap qm{};
     std::array<cx, 256> m{};                      // 16x16, row-major, out x in
     uint32_t count = 0;
     std::vector<DOp> window;                      // the window's gates, in order
+    std::vector<DOp> held;                        // frame gates held back behind the window
+    uint32_t held_x = 0, held_z = 0;              // bucket slots with held X-type / Z-type gates
+    uint64_t held_ctl = 0;                        // base bits the held gates' conditions read
     std::vector<std::array<cx, 256>>* blocks = nullptr;
 
     static bool plain_slot(const sqe_gate& g, const QMap& qm) {
@@ -195,6 +287,31 @@ struct BlockBuilder {
     static bool commutes(const sqe_gate& g, const QMap& qm) {   // base-class gate
         return !((qm.mask >> g.target) & 1ull) && (g.ctrl_mask & qm.mask) == 0;
     }
+    bool open() const { return count > 0 || !held.empty(); }
+    // the window can take plain slot gate g past the held gates
+    bool passes_held(const sqe_gate& g, int j) const {
+        const bool diag = g.kind == KZ || g.kind == KRZ || g.kind == KPHASE || g.kind == KDIAG;
+        if ((held_z >> j) & 1u) { if (!diag) return false; }
+        if ((held_x >> j) & 1u) { if (g.kind != KX) return false; }
+        return true;
+    }
+    // base-class gate g may be emitted ahead of the held gates
+    bool base_passes_held(const sqe_gate& g) const {
+        const bool flips = g.kind == KX || g.kind == KY || g.kind == KPERM;
+        return !(flips && ((held_ctl >> g.target) & 1ull));
+    }
+    void hold(const DOp& d) {
+        int j = d.tslot >= 0 ? d.tslot : __builtin_ctz(d.cmS);
+        if (d.kind == KX || d.kind == KY) held_x |= 1u << j;
+        if (d.kind == KZ || d.kind == KY) held_z |= 1u << j;
+        held_ctl |= d.cmB | d.tbase;
+        held.push_back(d);
+    }
+    int slot_of(const sqe_gate& g) const {
+        for (int k = 0; k < qm.mb; ++k)
+            if (qm.q[k] == g.target) return k;
+        return 0;
+    }
     void add(const sqe_gate& g) {
         if (count == 0) {
             for (int r = 0; r < 16; ++r)
@@ -202,9 +319,7 @@ struct BlockBuilder {
             window.clear();
         }
         window.push_back(plan_op(g, qm));
-        int j = 0;
-        for (int k = 0; k < qm.mb; ++k)
-            if (qm.q[k] == g.target) j = k;
+        const int j = slot_of(g);
         cx u[4];
         gate_matrix(g, u);
         std::array<cx, 256> out{};
@@ -229,7 +344,44 @@ struct BlockBuilder {
             prog.push_back(d);
         }
         count = 0;
+        prog.insert(prog.end(), held.begin(), held.end());
+        held.clear();
+        held_x = held_z = 0;
+        held_ctl = 0;
     }
+    // Route one bucket-local gate: into the window, held back, ahead of the
+    // window, or (after closing it) straight into the program.
+    void route(const sqe_gate& g, std::vector<DOp>& prog) {
+        if (plain_slot(g, qm)) {
+            if (!passes_held(g, slot_of(g))) flush(prog);
+            add(g);
+            return;
+        }
+        const DOp d = plan_op(g, qm);
+        if (commutes(g, qm)) {
+            if (!base_passes_held(g)) flush(prog);
+            prog.push_back(d);                     // ahead of the open window
+            return;
+        }
+        if (frames && is_frame_op(d)) {
+            hold(d);
+            return;
+        }
+        flush(prog);
+        prog.push_back(d);
+    }
+};
+
+// A program encoded into sweep records.  start[i] is the first record of
+// planned op i (start[n] = the record count), so a run of planned ops maps to
+// a run of records.  With frames, Pauli gates on bucket qubits become frame
+// records and an H_FRAME record is put before the next op that touches the
+// amplitude registers while frame records are outstanding.
+struct Encoded {
+    std::vector<MOp> ops;
+    std::vector<double> coef;
+    std::vector<size_t> start;
+    bool framed = false;
 };
 
 // Record of a block: the side table gets the 32x32 real form R = [[Mr, -Mi],
@@ -266,6 +418,51 @@ inline DOp renorm_op() {
     d.kind = K_RENORM;
     d.tslot = -1;
     return d;
+}
+
+inline DOp norm_point_op(uint32_t k) {
+    DOp d{};
+    d.kind = K_NORM_POINT;
+    d.tslot = -1;
+    d.p[0] = (double)k;
+    return d;
+}
+
+inline MOp frame_apply_record() {
+    MOp m{};
+    m.hdr = H_FRAME;
+    return m;
+}
+
+// Encode planned ops (blocks index into `blocks`); qm non-null enables frames.
+inline void encode_program(const std::vector<DOp>& prog, const std::vector<std::array<cx, 256>>& blocks, bool f32,
+                           const QMap* qm, Encoded& out) {
+    out.ops.clear();
+    out.coef.clear();
+    out.start.assign(prog.size() + 1, 0);
+    out.framed = false;
+    out.ops.reserve(prog.size() + 8);
+    bool outstanding = false;
+    for (size_t i = 0; i < prog.size(); ++i) {
+        const DOp& d = prog[i];
+        out.start[i] = out.ops.size();
+        if (qm && outstanding && touches_amplitudes(d) && !is_frame_op(d)) {
+            out.ops.push_back(frame_apply_record());
+            outstanding = false;
+        }
+        if (d.kind == K_BLOCK) {
+            out.ops.push_back(encode_block(d, blocks[(size_t)d.p[0]], out.coef));
+        } else if (d.kind == K_NORM_POINT) {
+            MOp m{};
+            m.hdr = H_NORM_POINT | (uint32_t)d.p[0] << 13;
+            out.ops.push_back(m);
+        } else {
+            bool fr = false;
+            out.ops.push_back(encode_op(d, out.coef, f32, qm, &fr));
+            if (fr) outstanding = out.framed = true;
+        }
+    }
+    out.start[prog.size()] = out.ops.size();
 }
 
 }  // namespace sqe

@@ -44,14 +44,15 @@ class Config(ctypes.Structure):
                 ("renorm_every", ctypes.c_uint32), ("fuse_window", ctypes.c_uint32),
                 ("timing", ctypes.c_uint32), ("mem_reserve_ppm", ctypes.c_uint32),
                 ("fuse_blocks", ctypes.c_uint32), ("batch_eager", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 4)]
+                ("renorm_mode", ctypes.c_uint32), ("frame_off", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 2)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("gates", ctypes.c_uint64), ("updates", ctypes.c_uint64), ("nnz", ctypes.c_uint64),
                 ("buckets", ctypes.c_uint64), ("capacity", ctypes.c_uint64),
                 ("bytes_device", ctypes.c_uint64), ("bucket_bits", ctypes.c_int32),
-                ("pad", ctypes.c_uint32), ("bucket_mask", ctypes.c_uint64), ("sweeps", ctypes.c_uint64),
+                ("reruns", ctypes.c_uint32), ("bucket_mask", ctypes.c_uint64), ("sweeps", ctypes.c_uint64),
                 ("sweep_ops", ctypes.c_uint64), ("pairs", ctypes.c_uint64), ("remaps", ctypes.c_uint64),
                 ("moved", ctypes.c_uint64), ("sweep_ms", ctypes.c_double), ("pair_ms", ctypes.c_double), ("other_ms", ctypes.c_double)]
 
@@ -216,11 +217,13 @@ class Engine:
 
     def __init__(self, nq, prec="f64", prune_thr=0.0, bucket_bits=-1, renorm_every=0,
                  fuse_window=0, timing=False, device=0, hbm_budget_bytes=0, max_load=0.5,
-                 fuse_blocks=False):
+                 fuse_blocks=False, renorm_mode=0, frames=True):
         L = lib()
         cfg = Config()
         L.sqe_config_default(ctypes.byref(cfg), nq)
         cfg.fuse_blocks = int(fuse_blocks) if fuse_blocks is not True else 1
+        cfg.renorm_mode = int(renorm_mode)
+        cfg.frame_off = 0 if frames else 1
         cfg.prec = F32 if prec in ("f32", F32) else F64
         cfg.prune_thr = prune_thr
         cfg.bucket_bits = bucket_bits
@@ -332,7 +335,7 @@ class Engine:
     def stats(self):
         s = Stats()
         _check(lib().sqe_get_stats(self.h, ctypes.byref(s)))
-        return {f: getattr(s, f) for f, _ in Stats._fields_ if f != "pad"}
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
 
     def snapshot_save(self):
         _check(lib().sqe_snapshot_save(self.h))
@@ -345,13 +348,14 @@ class Batch:
     """Many independent circuits on one GPU, advanced together (sqe_batch_*)."""
 
     def __init__(self, prec="f64", prune_thr=0.0, bucket_bits=-1, renorm_every=0, timing=False,
-                 device=0, hbm_budget_bytes=0, max_load=0.5, eager=False):
+                 device=0, hbm_budget_bytes=0, max_load=0.5, eager=False, frames=True):
         """eager: run() launches every step directly instead of replaying one
         captured hipGraph of the whole run."""
         L = lib()
         cfg = Config()
         L.sqe_config_default(ctypes.byref(cfg), 1)
         cfg.batch_eager = 1 if eager else 0
+        cfg.frame_off = 0 if frames else 1
         cfg.prec = F32 if prec in ("f32", F32) else F64
         cfg.prune_thr = prune_thr
         cfg.bucket_bits = bucket_bits

@@ -395,3 +395,28 @@ def test_edge_cases(oracle):
         with pytest.raises(E.SqeError) as ex:
             e.load(np.array([1 << 10], dtype=np.uint64), np.ones(1), np.zeros(1))
         assert ex.value.status == E.SQE_EINVAL
+
+
+@pytest.mark.parametrize("bits", [2, 4])
+def test_frames_on_and_off_agree(oracle, bits):
+    """Pauli gates on bucket qubits through the per-bucket frame against the
+    same gates through the amplitude registers: both are exact, so the
+    states differ only where the pending factor, which then carries the
+    frame's signs, is multiplied in with a fused multiply-add (ulps)."""
+    nq = 14
+    keys, re, im = random_state(61 + bits, nq, 400)
+    gl = random_gates(62 + bits, nq, 500, kinds=[G.X, G.Y, G.Z, G.H, G.RY, G.RZ, G.PHASE, G.U], p_ctrl=0.6)
+    for prec in ("f64", "f32"):
+        outs = []
+        for frames in (True, False):
+            with E.Engine(nq, prec=prec, bucket_bits=bits, frames=frames) as e:
+                e.load(keys, re, im)
+                e.apply(gl)
+                outs.append((e.export(), e.stats()["updates"]))
+        (a, ua), (b, ub) = outs
+        say("frames on/off", prec, "bits", bits, "entries", a[0].size)
+        assert_same_state(a, b, atol=1e-12 if prec == "f64" else 1e-6, label="frames on/off %s" % prec)
+        assert ua == ub
+        if prec == "f64":
+            want, _ = oracle_run(oracle, nq, keys, re, im, gl)
+            assert_same_state(a, want, atol=1e-12, label="frames bits %d" % bits)

@@ -58,11 +58,17 @@ def parse():
                     help="FP32 renormalisation: 0 one launch per period, 1 one pass per run of gates (checked)")
     ap.add_argument("--tail-gates", type=int, default=200,
                     help="also run the last N gates one gate per launch (0: skip)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-chunk", type=int, default=25,
+                    help="gates per CPU-baseline chunk (chunks start at gates 0, 500, 1000, 1500)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", default="large", choices=["large", "batch", "serve"],
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the FP32, batched and serving measurements of the default line")
+    ap.add_argument("--mode", default="large", choices=["large", "batch", "serve", "c1", "wide"],
                     help="large: the 48-qubit 2^20-key circuit (default, the headline metric); "
-                         "batch: 256 independent circuits in one batch; serve: Poisson arrivals")
+                         "batch: 256 independent circuits in one batch; serve: Poisson arrivals; "
+                         "c1: the 20-qubit 200-gate circuit from |0...0> (CPU baseline: the FC engine); "
+                         "wide: the 48-qubit circuit branching on --branch qubits (cross-bucket gates)")
+    ap.add_argument("--branch", type=int, default=6, help="wide mode: branching qubits")
     ap.add_argument("--circuits", type=int, default=256, help="batch mode: circuits")
     ap.add_argument("--eager", action="store_true",
                     help="batch mode: launch every step directly instead of replaying a captured hipGraph")
@@ -134,35 +140,56 @@ def cpu_model():
     return None
 
 
-def cpu_baseline(w, seconds):
-    """Oracle (FP64 sparse map, OpenMP) on the first gates of the same workload,
-    on up to 16 host cores, then on one core over a shorter sample."""
+def cpu_baseline(w, seconds, chunk=25, points=(0, 500, 1000, 1500)):
+    """The CPU oracle (FP64 sparse engine, OpenMP) timed on a sample spread
+    across the circuit's nonzero growth: `chunk` gates from each of the gate
+    positions in `points`, each started from the exact state the circuit has
+    there (taken from the GPU engine, which matches the oracle bit for bit on
+    keys and within 1e-12 on amplitudes).  The value is the sampled updates
+    over the sampled time, on up to 16 host cores; one core on the first two
+    chunks.  Sampling, not the whole circuit, keeps the bench within minutes
+    (the whole circuit takes the oracle about a minute on 16 cores)."""
     from oracle import oracle as O
+    from sparqc_amd import engine as E
     try:
         avail = len(os.sched_getaffinity(0))
     except AttributeError:
         avail = os.cpu_count() or 1
     cores = min(16, avail)
     gl = w["gates"]
-
-    def timed(threads, limit):
-        O.set_threads(threads)
-        s = O.Sparse(w["nq"])
-        s.load(w["keys"], w["re"], w["im"])
-        t0 = time.perf_counter()
+    pts = [p for p in points if p < len(gl)]
+    starts = []
+    with E.Engine(w["nq"], prec="f64") as e:
+        e.load(w["keys"], w["re"], w["im"])
+        e.prepare(gl)
         done = 0
-        while done < len(gl) and time.perf_counter() - t0 < limit:
-            s.apply(gl[done:done + 25])
-            done = min(len(gl), done + 25)
-        return s.updates() / (time.perf_counter() - t0), done, time.perf_counter() - t0
+        for p in pts:
+            e.apply(gl[done:p])
+            done = p
+            starts.append(e.export(sort=False))
 
-    v, done, dt = timed(cores, seconds)
-    v1, done1, dt1 = timed(1, seconds / 3)
-    return {"value": v, "unit": "updates/s", "cores": cores, "kind": "port",
-            "sample": "CPU oracle (FP64 sparse map) on the first %d of %d gates of the same "
-                      "workload from the same 2^%d-key state, %.1f s" % (
-                          done, len(gl), int(round(__import__("math").log2(w["keys"].size))), dt),
-            "one_core": {"value": v1, "gates": done1, "seconds": dt1},
+    def timed(threads, which):
+        O.set_threads(threads)
+        upd, secs, nnz = 0, 0.0, []
+        for i in which:
+            k, r, im = starts[i]
+            s = O.Sparse(w["nq"])
+            s.load(k, r, im)
+            nnz.append(int(k.size))
+            t0 = time.perf_counter()
+            s.apply(gl[pts[i]:pts[i] + chunk])
+            secs += time.perf_counter() - t0
+            upd += s.updates()
+        return upd, secs, nnz
+
+    upd, secs, nnz = timed(cores, range(len(pts)))
+    upd1, secs1, _ = timed(1, range(min(2, len(pts))))
+    return {"value": upd / secs, "unit": "updates/s", "cores": cores, "kind": "port",
+            "sample": "CPU oracle (FP64 sparse engine) on %d-gate chunks at gates %s of the same 2000-gate "
+                      "workload, each from the exact state there (%s nonzeros): %d updates in %.1f s"
+                      % (chunk, ",".join(str(p) for p in pts), "/".join(str(n) for n in nnz), upd, secs),
+            "one_core": {"value": upd1 / secs1 if secs1 > 0 else None, "chunks": min(2, len(pts)),
+                         "seconds": secs1},
             "cpu_model": cpu_model(), "cores_available": avail}
 
 
@@ -205,16 +232,12 @@ def unfused_tail(a, w, eng):
             "updates_per_s": (s1["updates"] - s0["updates"]) / ((s1["sweep_ms"] - s0["sweep_ms"]) * 1e-3)}
 
 
-def precision_report(w, eng32):
-    """FP32 final state against the FP64 engine on the same workload: infidelity
-    1 - |<psi64|psi32>|^2 / (|psi64|^2 |psi32|^2) and max |a32 - a64| over the
-    key union (a key missing on one side counts with amplitude 0)."""
-    from sparqc_amd import engine as E
-    k32, r32, i32 = eng32.export()
-    with E.Engine(48, prec="f64") as e:
-        e.load(w["keys"], w["re"], w["im"])
-        e.apply(w["gates"])
-        k64, r64, i64 = e.export()
+def precision_report(got32, want64):
+    """FP32 final state against the FP64 engine's on the same workload:
+    infidelity 1 - |<psi64|psi32>|^2 / (|psi64|^2 |psi32|^2) and max |a32 - a64|
+    over the key union (a key missing on one side counts with amplitude 0)."""
+    k32, r32, i32 = got32
+    k64, r64, i64 = want64
     keys = np.union1d(k32, k64)
     a32 = np.zeros(keys.size, dtype=np.complex128)
     a64 = np.zeros(keys.size, dtype=np.complex128)
@@ -225,6 +248,42 @@ def precision_report(w, eng32):
     return {"infidelity": float(infid), "max_abs_diff": float(np.max(np.abs(a32 - a64))),
             "keys_fp32": int(k32.size), "keys_fp64": int(k64.size),
             "keys_only_one_side": int(keys.size * 2 - k32.size - k64.size)}
+
+
+def run_timed(w, prec, steps, warmup, **kw):
+    """Steps of the workload (restore the initial snapshot, apply the circuit,
+    sync) on one engine; returns (updates per step, seconds per step, engine)."""
+    from sparqc_amd import engine as E
+    eng = E.Engine(w["nq"], prec=prec, timing=True, renorm_every=100 if prec == "f32" else 0, **kw)
+    eng.load(w["keys"], w["re"], w["im"])
+    eng.prepare(w["gates"])
+    eng.snapshot_save()
+    for _ in range(warmup):
+        eng.snapshot_restore()
+        eng.apply(w["gates"])
+        eng.sync()
+    s0 = eng.stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.snapshot_restore()
+        eng.apply(w["gates"])
+        eng.sync()
+    dt = (time.perf_counter() - t0) / steps
+    s1 = eng.stats()
+    return (s1["updates"] - s0["updates"]) / steps, dt, eng, s0, s1
+
+
+def fp32_detail(w, fp64_value, fp64_state, steps=10, warmup=2):
+    """The FP32 form on the same workload: updates/s, its ratio to the FP64
+    line's value, and its precision against the FP64 final state."""
+    upd, dt, eng, s0, s1 = run_timed(w, "f32", steps, warmup)
+    got = eng.export()
+    eng.close()
+    out = {"updates_per_s": upd / dt, "ms_per_step": dt * 1e3, "ratio_to_fp64": upd / dt / fp64_value,
+           "renormalise_every": 100, "sweep_launches_per_step": (s1["sweeps"] - s0["sweeps"]) / steps,
+           "steps": steps}
+    out.update(precision_report(got, fp64_state))
+    return out
 
 
 def traffic_from_profiles(prec, window):
@@ -260,64 +319,169 @@ def valu_from_profiles(a, sweep_s_per_step):
             "insts_per_step": v["valu_insts"]}
 
 
-def batch_mode(a):
-    """256 independent circuits of uneven size and depth advanced together."""
+def run_batch(seed, n_circuits, prec, steps, warmup, eager=False):
+    """n_circuits independent circuits of the batched workload advanced
+    together: autotuned, then timed over whole runs from a snapshot."""
     from sparqc_amd import engine as E
     from sparqc_amd import gen
-    cs = gen.batch_circuits(a.seed, n_circuits=a.circuits)
-    with E.Batch(prec=a.prec, renorm_every=100 if a.prec == "f32" else 0, timing=False, eager=a.eager) as b:
+    cs = gen.batch_circuits(seed, n_circuits=n_circuits)
+    with E.Batch(prec=prec, renorm_every=100 if prec == "f32" else 0, timing=False, eager=eager) as b:
+        t0 = time.perf_counter()
         ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        admit_s = time.perf_counter() - t0
         tune = b.autotune()
         b.snapshot_save()
-        for _ in range(a.warmup):
+        for _ in range(warmup):
             b.snapshot_restore()
             b.run()
             b.sync()
         t0 = time.perf_counter()
-        for _ in range(a.steps):
+        for _ in range(steps):
             b.snapshot_restore()
             b.run()
             b.sync()
-        dt = time.perf_counter() - t0
+        dt = (time.perf_counter() - t0) / steps
         upd = sum(b.info(i)["updates"] for i in ids)
         nnz = sum(b.info(i)["nnz"] for i in ids)
         used = b.usage()["used"]
-    out = {"metric": "nonzero-amp gate updates/s (batched)", "value": upd / (dt / a.steps), "unit": "updates/s",
-           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt / a.steps * 1e3,
+    return {"updates_per_s": upd / dt, "ms_per_run": dt * 1e3, "circuits": n_circuits, "prec": prec,
+            "updates_per_run": upd, "nnz_final": nnz, "initial_keys": int(sum(c["keys"].size for c in cs)),
+            "gates": int(sum(len(c["gates"]) for c in cs)), "pool_bytes": used, "admission_s": admit_s,
+            "autotune": tune, "circuits_per_s": n_circuits / dt,
+            "launch": "eager steps" if eager else "captured hipGraph replay", "runs_timed": steps}
+
+
+def batch_mode(a):
+    """256 independent circuits of uneven size and depth advanced together."""
+    r = run_batch(a.seed, a.circuits, a.prec, a.steps, a.warmup, eager=a.eager)
+    out = {"metric": "nonzero-amp gate updates/s (batched)", "value": r["updates_per_s"], "unit": "updates/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["ms_per_run"],
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": a.prec,
            "data": "synthetic (seeded splitmix64 circuits and states)",
            "config": {"workload": "%d circuits, 16-40 qubits, 2^8-2^18 initial keys, depth 50-500, "
                                   "60/35/5 diag/perm/branch mix on 3 branching qubits each" % a.circuits,
-                      "circuits": a.circuits, "seed": a.seed, "prec": a.prec,
-                      "launch": "eager steps" if a.eager else "captured hipGraph replay"},
-           "detail": {"updates_per_step": upd, "nnz_final": nnz, "initial_keys": int(sum(c["keys"].size for c in cs)),
-                      "gates": int(sum(len(c["gates"]) for c in cs)), "pool_bytes": used,
-                      "autotune": tune, "circuits_per_s": a.circuits / (dt / a.steps)}}
+                      "circuits": a.circuits, "seed": a.seed, "prec": a.prec, "launch": r["launch"]},
+           "detail": r}
     print(json.dumps(out), flush=True)
 
 
-def serve_mode(a):
-    """Poisson arrivals at each rate, continuous batching, p50/p99 latency."""
+def run_serve(seed, n_requests, rates, prec, step_gates):
+    """Poisson arrivals at each rate into the serving loop: p50/p99 latency."""
     from sparqc_amd import gen
     from sparqc_amd.serve import ServeLoop
-    reqs = gen.batch_circuits(a.seed, n_circuits=a.requests)
+    reqs = gen.batch_circuits(seed, n_circuits=n_requests)
     rows = []
-    for rate in [float(x) for x in a.rates.split(",")]:
-        arr = gen.poisson_arrivals(a.seed + int(rate), rate, len(reqs))
-        with ServeLoop(prec=a.prec, step_gates=a.step_gates, hbm_budget_bytes=200 * 10**9) as s:
+    for rate in rates:
+        arr = gen.poisson_arrivals(seed + int(rate), rate, len(reqs))
+        with ServeLoop(prec=prec, step_gates=step_gates, hbm_budget_bytes=200 * 10**9) as s:
             s.run(reqs[:8], np.zeros(8))                       # warm-up (kernels, pools)
             rep = s.run(reqs, arr)
         rows.append({"rate_per_s": rate, "p50_latency_ms": rep["p50_latency_s"] * 1e3,
                      "p99_latency_ms": rep["p99_latency_s"] * 1e3, "updates_per_s": rep["updates_per_s"],
+                     "served_per_s": len(reqs) / rep["wall_s"],
                      "wall_s": rep["wall_s"], "steps": rep["steps"], "peak_live": rep["peak_live"],
                      "peak_pool_bytes": rep["peak_pool_bytes"]})
-        print(json.dumps({"serve": rows[-1], "prec": a.prec}), flush=True)
+    return {"requests_per_rate": n_requests, "prec": prec, "step_gates": step_gates, "hbm_budget_bytes": 200 * 10**9,
+            "rows": rows}
+
+
+def serve_mode(a):
+    """Poisson arrivals at each rate, continuous batching, p50/p99 latency."""
+    r = run_serve(a.seed, a.requests, [float(x) for x in a.rates.split(",")], a.prec, a.step_gates)
     out = {"metric": "serving latency p50/p99 and nonzero-amp gate updates/s", "unit": "ms / updates/s",
            "n_gpus": 1, "dtype": a.prec, "data": "synthetic (seeded splitmix64 circuits, Poisson arrivals)",
            "config": {"workload": "%d requests per rate drawn like the batched workload, HBM budget 200 GB, "
                                   "%d gates per batched step" % (a.requests, a.step_gates),
                       "rates": a.rates, "seed": a.seed, "prec": a.prec},
-           "rows": rows}
+           "rows": r["rows"]}
+    print(json.dumps(out), flush=True)
+
+
+def kernel_classes(s0, s1, steps):
+    """Per-step kernel time by class (engine timing) and cross-bucket traffic."""
+    return {"sweep_ms_per_step": (s1["sweep_ms"] - s0["sweep_ms"]) / steps,
+            "pair_ms_per_step": (s1["pair_ms"] - s0["pair_ms"]) / steps,
+            "other_ms_per_step": (s1["other_ms"] - s0["other_ms"]) / steps,
+            "sweeps_per_step": (s1["sweeps"] - s0["sweeps"]) / steps,
+            "pairs_per_step": (s1["pairs"] - s0["pairs"]) / steps,
+            "pair_moved_per_step": (s1["moved_pair"] - s0["moved_pair"]) / steps}
+
+
+def c1_mode(a):
+    """The 20-qubit, 200-gate circuit from |0...0> (H / RY on 10 active
+    qubits, X / CX / RZ / CZ anywhere): more branching qubits than bucket bits,
+    so cross-bucket launches and the hash index carry part of it.  CPU
+    baseline: the FC engine (the reference's dense-slot table on the CPU)."""
+    from oracle import oracle as O
+    from sparqc_amd import gen
+    w = gen.circuit_small(a.seed)
+    steps = max(a.steps, 50)
+    upd, dt, eng, s0, s1 = run_timed(w, "f64", steps, a.warmup)
+    nnz1 = s1["nnz"]
+    eng.close()
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    cores = min(16, avail)
+
+    def fc_rate(threads, reps):
+        O.set_threads(threads)
+        u, t = 0, 0.0
+        for _ in range(reps):
+            f = O.FC(20)
+            f.load(w["keys"], w["re"], w["im"])
+            t0 = time.perf_counter()
+            f.apply(w["gates"])
+            t += time.perf_counter() - t0
+            u += f.updates()
+        return u / t, t
+    v, t = fc_rate(cores, 20)
+    v1, t1 = fc_rate(1, 5)
+    out = {"metric": "nonzero-amp gate updates/s", "value": upd / dt, "unit": "updates/s", "n_gpus": 1,
+           "steps": steps, "warmup": a.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded splitmix64 circuit)",
+           "config": {"workload": "sparse statevector, 20 qubits from |0...0>, 200 gates: H / RY on 10 active "
+                                  "qubits, X / CX / RZ / CZ on any, prune 1e-14", "nq": 20, "gates": 200,
+                      "seed": a.seed},
+           "detail": dict(kernel_classes(s0, s1, steps), updates_per_step=upd, nnz_final=nnz1,
+                          bucket_bits=s1["bucket_bits"]),
+           "cpu_baseline": {"value": v, "unit": "updates/s", "cores": cores, "kind": "port",
+                            "sample": "FC engine (dense slots addressed by key, address book, two buffers: the "
+                                      "reference's table restated on the CPU) on the whole circuit, 20 runs, %.2f s"
+                                      % t,
+                            "one_core": {"value": v1, "runs": 5, "seconds": t1}, "cpu_model": cpu_model()}}
+    print(json.dumps(out), flush=True)
+
+
+def wide_mode(a):
+    """The 48-qubit circuit branching on more qubits than a bucket holds:
+    2^18 keys, 2000 gates of the 60/35/5 mix with branching on --branch
+    qubits, so branching gates on the extra qubits run as cross-bucket launches
+    (pairing through the hash index, prune and scan compaction)."""
+    from sparqc_amd import gen
+    w = gen.circuit_large(a.seed, nq=48, n_init=1 << 18, n_gates=2000, n_branch=a.branch)
+    upd, dt, eng, s0, s1 = run_timed(w, "f64", a.steps, a.warmup)
+    nnz1 = s1["nnz"]
+    eng.close()
+    kc = kernel_classes(s0, s1, a.steps)
+    pair_bytes = kc["pair_moved_per_step"] * (KEY_B + 16)
+    out = {"metric": "nonzero-amp gate updates/s", "value": upd / dt, "unit": "updates/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded splitmix64 circuit and state)",
+           "config": {"workload": "sparse statevector, 48 qubits, 2^18 initial keys, 2000-gate mix 60%% diag / "
+                                  "35%% perm / 5%% branch on %d qubits, prune 1e-14" % a.branch,
+                      "nq": 48, "n_init": 1 << 18, "gates": 2000, "branch": a.branch, "seed": a.seed},
+           "roofline": {"bound": "hbm", "kernel": "k_pair (cross-bucket gate)",
+                        "achieved": pair_bytes / (kc["pair_ms_per_step"] * 1e-3) / 1e9 if kc["pair_ms_per_step"] else None,
+                        "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": pair_bytes / (kc["pair_ms_per_step"] * 1e-3) / HBM_PEAK if kc["pair_ms_per_step"] else None,
+                        "traffic": None,
+                        "note": "model bytes (nonzeros read + written) x 24 B over the summed cross-bucket launch "
+                                "time; measured HBM bytes per kernel are in the rocprof summaries"},
+           "detail": dict(kc, updates_per_step=upd, nnz_final=nnz1, bucket_bits=s1["bucket_bits"])}
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_chunk)
     print(json.dumps(out), flush=True)
 
 
@@ -335,6 +499,12 @@ def main():
         return
     if a.mode == "serve":
         serve_mode(a)
+        return
+    if a.mode == "c1":
+        c1_mode(a)
+        return
+    if a.mode == "wide":
+        wide_mode(a)
         return
     from sparqc_amd import engine as E
     from sparqc_amd import gen
@@ -427,14 +597,26 @@ def main():
                    "sweep_kernel_ms_per_step": sweep_s / a.steps * 1e3,
                    "device_bytes": s1["bytes_device"]},
     }
+    final = eng.export()
+    eng.close()
     if a.prec == "f32":
-        out["detail"]["precision_vs_fp64"] = precision_report(w, eng)
+        with E.Engine(48, prec="f64") as ref:
+            ref.load(w["keys"], w["re"], w["im"])
+            ref.apply(w["gates"])
+            out["detail"]["precision_vs_fp64"] = precision_report(final, ref.export())
+    elif not a.no_extras and a.circuit == "mix" and not a.fused_blocks:
+        # the FP32 form, the batched form and the serving loop, measured in
+        # the same run (their own fixed sizes, a few seconds each)
+        out["detail"]["fp32"] = fp32_detail(w, upd / dt, final)
+        final = None
+        out["detail"]["batch"] = {p: run_batch(a.seed, 256, p, steps=3, warmup=1) for p in ("f64", "f32")}
+        out["detail"]["serve"] = run_serve(a.seed, 300, [200.0, 1000.0, 5000.0], "f64", a.step_gates)
+    final = None
     out["detail"]["stream_copy"] = stream_copy()
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_chunk)
     else:
         out["cpu_baseline"] = None
-    eng.close()
     print(json.dumps(out), flush=True)
 
 

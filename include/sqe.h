@@ -122,6 +122,7 @@ typedef struct sqe_stats {
     double sweep_ms;            /* summed sweep kernel time (timing = 1)               */
     double pair_ms;             /* summed cross-bucket kernel time (timing = 1)        */
     double other_ms;            /* summed time of all other kernels (timing = 1)       */
+    uint64_t moved_pair;        /* the part of `moved` from cross-bucket launches      */
 } sqe_stats;
 
 /* Fill *cfg with defaults for nq qubits in FP64. */

@@ -507,4 +507,156 @@ int orc_dense_run(int nq, const uint64_t* keys, const double* re, const double* 
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// FC engine: the reference's full-capacity table restated on the CPU, as the
+// CPU baseline of the 20-qubit workload (timing only; checked against the
+// map engine above).  emulator/device/device_table_FC.cuh and
+// device_data.cuh:
+//   * two dense buffers of 2^nq amplitudes addressed by the key itself
+//     (identity hash, table.cuh:13-26, device_table_FC.cuh:31-36) and an
+//     address book of the occupied keys (ah_);
+//   * every gate walks the address book (gate_sq, device_table_FC.cuh:113-168)
+//     and writes the other buffer; diagonal and permutation gates keep their
+//     book position (update_ah_diag / update_ah_swap, device_data.cuh:108-131),
+//     branching gates append their non-negligible outputs (update_ah_sup,
+//     device_data.cuh:134-156); the old buffer's entries are then cleared
+//     (device_zero_arrays, device_data.cuh:62-105) and the buffers swap;
+//   * pairing (find_elements, device_table_FC.cuh:62-110): the key with the
+//     target bit 0 owns the pair; the bit-1 key skips when its companion is
+//     in the table.  (The reference tests the companion's value for
+//     negligibility; a present companion with a negligible value would make
+//     both keys process the pair and append it twice, a defect the map
+//     engine and the GPU engine do not reproduce.  Presence is a nonzero
+//     slot here: pruned outputs are never stored.)
+// Appends use per-thread lists concatenated in thread order instead of the
+// reference's atomic counter.
+// ---------------------------------------------------------------------------
+namespace {
+struct FC {
+    int nq = 0;
+    double thr = 1e-14;
+    uint64_t updates = 0;
+    std::vector<Cx<double>> buf[2];
+    int cur = 0;
+    std::vector<uint64_t> book;
+};
+
+inline bool is_zero(const Cx<double>& v) { return v.r == 0.0 && v.i == 0.0; }
+
+void fc_gate(FC& f, const Gate& g) {
+    auto& A = f.buf[f.cur];
+    auto& B = f.buf[f.cur ^ 1];
+    const uint64_t tb = 1ULL << g.target;
+    const int64_t n = (int64_t)f.book.size();
+    const int cls = gate_class(g.kind);
+    f.updates += (uint64_t)n;
+    std::vector<uint64_t> nb;
+    if (cls != 2) {
+        nb.resize(f.book.size());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            const uint64_t k = f.book[i];
+            const bool on = (k & g.cmask) == g.cval;
+            const uint64_t k2 = (on && cls == 1) ? (k ^ tb) : k;
+            B[k2] = on ? single<double>(g, A[k], (int)((k >> g.target) & 1)) : A[k];
+            nb[i] = k2;
+        }
+    } else {
+        const int T = nthreads();
+        std::vector<std::vector<uint64_t>> part(T);
+#pragma omp parallel
+        {
+            auto& P = part[tid()];
+#pragma omp for schedule(static)
+            for (int64_t i = 0; i < n; ++i) {
+                const uint64_t k = f.book[i];
+                if ((k & g.cmask) != g.cval) {
+                    B[k] = A[k];
+                    P.push_back(k);
+                    continue;
+                }
+                const int b = (int)((k >> g.target) & 1);
+                const uint64_t c = k ^ tb;
+                if (b == 1 && !is_zero(A[c])) continue;         // the bit-0 key owns the pair
+                const Cx<double> v0 = b ? A[c] : A[k], v1 = b ? A[k] : A[c];
+                Cx<double> n0, n1;
+                op64(g, v0, v1, n0, n1);
+                const uint64_t k0 = k & ~tb, k1 = k | tb;
+                if (!negl(n0.r, n0.i, f.thr)) { B[k0] = n0; P.push_back(k0); }
+                if (!negl(n1.r, n1.i, f.thr)) { B[k1] = n1; P.push_back(k1); }
+            }
+        }
+        size_t tot = 0;
+        for (auto& P : part) tot += P.size();
+        nb.reserve(tot);
+        for (auto& P : part) nb.insert(nb.end(), P.begin(), P.end());
+    }
+    // clear the old buffer's entries, then swap
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) A[f.book[i]] = Cx<double>{0.0, 0.0};
+    f.book.swap(nb);
+    f.cur ^= 1;
+}
+}  // namespace
+
+void* fc_create(int nq, double thr) {
+    if (nq < 1 || nq > 26) return nullptr;
+    FC* f = new FC();
+    f->nq = nq;
+    f->thr = thr;
+    f->buf[0].assign(1ULL << nq, Cx<double>{0.0, 0.0});
+    f->buf[1].assign(1ULL << nq, Cx<double>{0.0, 0.0});
+    return f;
+}
+
+void fc_destroy(void* h) { delete (FC*)h; }
+
+int fc_load(void* h, const uint64_t* keys, const double* re, const double* im, uint64_t n) {
+    FC* f = (FC*)h;
+    for (uint64_t k : f->book) f->buf[f->cur][k] = Cx<double>{0.0, 0.0};
+    f->book.clear();
+    const uint64_t N = 1ULL << f->nq;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (keys[i] >= N) return -1;
+        if (re[i] == 0.0 && im[i] == 0.0) continue;
+        if (!is_zero(f->buf[f->cur][keys[i]])) return -2;
+        f->buf[f->cur][keys[i]] = Cx<double>{re[i], im[i]};
+        f->book.push_back(keys[i]);
+    }
+    f->updates = 0;
+    return 0;
+}
+
+int fc_apply(void* h, const void* gates, uint64_t ng) {
+    FC* f = (FC*)h;
+    const Gate* g = (const Gate*)gates;
+    for (uint64_t i = 0; i < ng; ++i) {
+        if (g[i].target >= (uint32_t)f->nq || (g[i].cmask >> g[i].target & 1) || g[i].kind > KPERM) return -1;
+        fc_gate(*f, g[i]);
+    }
+    return 0;
+}
+
+uint64_t fc_updates(void* h) { return ((FC*)h)->updates; }
+uint64_t fc_nnz(void* h) { return ((FC*)h)->book.size(); }
+
+// Non-negligible entries sorted by key (the reference's readout filter,
+// table.cuh:58-73); returns the count, writing nothing when cap is too small.
+uint64_t fc_export(void* h, uint64_t* keys, double* re, double* im, uint64_t cap) {
+    FC* f = (FC*)h;
+    std::vector<uint64_t> ks;
+    for (uint64_t k : f->book) {
+        const Cx<double>& v = f->buf[f->cur][k];
+        if (!negl(v.r, v.i, f->thr)) ks.push_back(k);
+    }
+    if (ks.size() > cap) return ks.size();
+    std::sort(ks.begin(), ks.end());
+    for (size_t j = 0; j < ks.size(); ++j) {
+        keys[j] = ks[j];
+        re[j] = f->buf[f->cur][ks[j]].r;
+        im[j] = f->buf[f->cur][ks[j]].i;
+    }
+    return ks.size();
+}
+
 }  // extern "C"

@@ -44,6 +44,19 @@ def lib():
         L.orc_norm2.argtypes = [ctypes.c_void_p]
         L.orc_export.restype = ctypes.c_uint64
         L.orc_export.argtypes = [ctypes.c_void_p, _u64p, _f64p, _f64p, ctypes.c_uint64]
+        L.fc_create.restype = ctypes.c_void_p
+        L.fc_create.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.fc_destroy.argtypes = [ctypes.c_void_p]
+        L.fc_load.restype = ctypes.c_int
+        L.fc_load.argtypes = [ctypes.c_void_p, _u64p, _f64p, _f64p, ctypes.c_uint64]
+        L.fc_apply.restype = ctypes.c_int
+        L.fc_apply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+        L.fc_updates.restype = ctypes.c_uint64
+        L.fc_updates.argtypes = [ctypes.c_void_p]
+        L.fc_nnz.restype = ctypes.c_uint64
+        L.fc_nnz.argtypes = [ctypes.c_void_p]
+        L.fc_export.restype = ctypes.c_uint64
+        L.fc_export.argtypes = [ctypes.c_void_p, _u64p, _f64p, _f64p, ctypes.c_uint64]
         L.orc_dense_run.restype = ctypes.c_int
         L.orc_dense_run.argtypes = [ctypes.c_int, _u64p, _f64p, _f64p, ctypes.c_uint64,
                                     ctypes.c_void_p, ctypes.c_uint64, _f64p, _f64p]
@@ -107,6 +120,50 @@ class Sparse:
         r = np.zeros(n)
         i = np.zeros(n)
         m = lib().orc_export(self.h, _p(k, _u64p), _p(r, _f64p), _p(i, _f64p), n)
+        return k[:m], r[:m], i[:m]
+
+
+class FC:
+    """The reference's full-capacity table on the CPU (dense slots addressed by
+    key, an address book, two buffers): the CPU baseline of small-qubit
+    workloads, nq <= 26."""
+
+    def __init__(self, nq, thr=1e-14):
+        self.h = lib().fc_create(nq, thr)
+        if not self.h:
+            raise ValueError("FC engine holds at most 26 qubits")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().fc_destroy(self.h)
+            self.h = None
+
+    def load(self, keys, re, im):
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        r = np.ascontiguousarray(re, dtype=np.float64)
+        i = np.ascontiguousarray(im, dtype=np.float64)
+        rc = lib().fc_load(self.h, _p(k, _u64p), _p(r, _f64p), _p(i, _f64p), k.size)
+        if rc != 0:
+            raise ValueError("FC load failed (%d)" % rc)
+
+    def apply(self, gates):
+        from sparqc_amd.gates import gate_array
+        g = gate_array(gates)
+        if lib().fc_apply(self.h, g.ctypes.data, g.size) != 0:
+            raise ValueError("FC apply failed")
+
+    def updates(self):
+        return int(lib().fc_updates(self.h))
+
+    def nnz(self):
+        return int(lib().fc_nnz(self.h))
+
+    def export(self):
+        n = self.nnz()
+        k = np.zeros(n, dtype=np.uint64)
+        r = np.zeros(n)
+        i = np.zeros(n)
+        m = lib().fc_export(self.h, _p(k, _u64p), _p(r, _f64p), _p(i, _f64p), n)
         return k[:m], r[:m], i[:m]
 
 

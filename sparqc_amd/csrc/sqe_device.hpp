@@ -150,14 +150,16 @@ struct Stats {
     unsigned long long nnz;          // nonzero slots after the last counting launch
     unsigned long long empty;        // all-zero live buckets after the last counting launch
     unsigned long long err;          // sticky error bits
-    unsigned long long pad[3];
+    unsigned long long moved_pair;   // the part of `moved` from cross-bucket launches (monotonic)
+    unsigned long long pad[2];
 };
 
 // One slot per workgroup: counting kernels write their block's totals here
 // with plain stores (no same-address atomics across thousands of blocks);
 // upd / moved accumulate across launches until k_stats_reduce folds them.
 struct BlockStats {
-    unsigned long long upd, moved, nnz, empty;
+    unsigned long long upd, moved, nnz, empty, moved_pair;
+    unsigned long long pad[3];
 };
 
 enum : unsigned long long { ERR_PROBE = 1, ERR_DUP = 2, ERR_MISSING = 4, ERR_CAP = 8, ERR_SCAN = 16 };
@@ -565,7 +567,8 @@ __device__ __forceinline__ uint32_t take_ticket(ScanCtl sc) {
 
 // Block reduction of the counters into this workgroup's BlockStats slot.
 __device__ __forceinline__ void block_stats(BlockStats* bs, unsigned long long upd, unsigned long long nnz,
-                                            unsigned long long empty, unsigned long long moved = 0) {
+                                            unsigned long long empty, unsigned long long moved = 0,
+                                            bool pair = false) {
     __shared__ unsigned long long s_red[4][BLOCK / 64];
     for (int d = 32; d >= 1; d >>= 1) {
         upd += __shfl_xor(upd, d);
@@ -588,6 +591,7 @@ __device__ __forceinline__ void block_stats(BlockStats* bs, unsigned long long u
         BlockStats& b = bs[blockIdx.x];
         b.upd += t[0];
         b.moved += t[3];
+        if (pair) b.moved_pair += t[3];
         b.nnz = t[1];
         b.empty = t[2];
     }

@@ -587,8 +587,9 @@ sqe_status run_sweep(sqe_engine* e, const MOp* mops, const double* coef, uint64_
 
 sqe_status run_pair(sqe_engine* e, const DOp& op) {
     if (!e->tab_valid) SQCK(rebuild_index(e));
+    // (grow() moves the state to the other store: pick the output store after it)
+    if (2 * e->nb_upper + 64 > e->s[e->cur ^ 1].cap) SQCK(grow(e, 0));
     const int other = e->cur ^ 1;
-    if (2 * e->nb_upper + 64 > e->s[other].cap) SQCK(grow(e, 0));
     PairArgs a{};
     const Store& i = e->s[e->cur];
     const Store& o = e->s[other];
@@ -1428,6 +1429,7 @@ sqe_status sqe_get_stats(sqe_engine* e, sqe_stats* out) {
         out->remaps = e->n_remaps;
         out->reruns = (uint32_t)e->n_reruns;
         out->moved = h.moved;
+        out->moved_pair = h.moved_pair;
         out->sweep_ms = e->t_ms[0];
         out->pair_ms = e->t_ms[1];
         out->other_ms = e->t_ms[2];

@@ -1096,7 +1096,7 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
             }
         }
     }
-    block_stats(A.bstats, nz_in, nz_out, 0ull, (unsigned long long)nz_in + nz_out);
+    block_stats(A.bstats, nz_in, nz_out, 0ull, (unsigned long long)nz_in + nz_out, true);
     (void)nb;
 }
 

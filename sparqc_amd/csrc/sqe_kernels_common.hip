@@ -131,13 +131,15 @@ __global__ __launch_bounds__(BLOCK) void k_index_enumerate(EnumArgs A) {
 // fold (then cleared), nnz / empty over the slots of the last counting launch.
 __global__ __launch_bounds__(BLOCK) void k_stats_reduce(BlockStats* __restrict__ bs, uint32_t n_acc,
                                                         uint32_t n_last, Stats* __restrict__ st) {
-    __shared__ unsigned long long s_red[4][BLOCK / 64];
-    unsigned long long u = 0, m = 0, z = 0, e = 0;
+    __shared__ unsigned long long s_red[5][BLOCK / 64];
+    unsigned long long u = 0, m = 0, z = 0, e = 0, mp = 0;
     for (uint32_t i = threadIdx.x; i < n_acc; i += BLOCK) {
         u += bs[i].upd;
         m += bs[i].moved;
+        mp += bs[i].moved_pair;
         bs[i].upd = 0;
         bs[i].moved = 0;
+        bs[i].moved_pair = 0;
     }
     for (uint32_t i = threadIdx.x; i < n_last; i += BLOCK) {
         z += bs[i].nnz;
@@ -148,6 +150,7 @@ __global__ __launch_bounds__(BLOCK) void k_stats_reduce(BlockStats* __restrict__
         m += __shfl_xor(m, d);
         z += __shfl_xor(z, d);
         e += __shfl_xor(e, d);
+        mp += __shfl_xor(mp, d);
     }
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
@@ -155,14 +158,16 @@ __global__ __launch_bounds__(BLOCK) void k_stats_reduce(BlockStats* __restrict__
         s_red[1][w] = m;
         s_red[2][w] = z;
         s_red[3][w] = e;
+        s_red[4][w] = mp;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long t[4] = {0, 0, 0, 0};
-        for (int k = 0; k < 4; ++k)
+        unsigned long long t[5] = {0, 0, 0, 0, 0};
+        for (int k = 0; k < 5; ++k)
             for (int i = 0; i < BLOCK / 64; ++i) t[k] += s_red[k][i];
         st->updates += t[0];
         st->moved += t[1];
+        st->moved_pair += t[4];
         if (n_last) {
             st->nnz = t[2];
             st->empty = t[3];

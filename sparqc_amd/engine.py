@@ -54,7 +54,8 @@ class Stats(ctypes.Structure):
                 ("bytes_device", ctypes.c_uint64), ("bucket_bits", ctypes.c_int32),
                 ("reruns", ctypes.c_uint32), ("bucket_mask", ctypes.c_uint64), ("sweeps", ctypes.c_uint64),
                 ("sweep_ops", ctypes.c_uint64), ("pairs", ctypes.c_uint64), ("remaps", ctypes.c_uint64),
-                ("moved", ctypes.c_uint64), ("sweep_ms", ctypes.c_double), ("pair_ms", ctypes.c_double), ("other_ms", ctypes.c_double)]
+                ("moved", ctypes.c_uint64), ("sweep_ms", ctypes.c_double), ("pair_ms", ctypes.c_double), ("other_ms", ctypes.c_double),
+                ("moved_pair", ctypes.c_uint64)]
 
 
 class BatchCircuit(ctypes.Structure):
@@ -69,7 +70,7 @@ class BatchTuning(ctypes.Structure):
 
 
 assert ctypes.sizeof(Config) == 80
-assert ctypes.sizeof(Stats) == 128
+assert ctypes.sizeof(Stats) == 136
 assert ctypes.sizeof(BatchCircuit) == 56
 assert ctypes.sizeof(BatchTuning) == 32
 

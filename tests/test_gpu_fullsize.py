@@ -13,6 +13,8 @@
 * Serving at 1000 requests/s with requests drawn like the batched workload:
   every result equals the same request served alone, to the bit, and the
   oracle within 1e-12.
+* 2^18 keys branching on 6 qubits (cross-bucket launches that grow the
+  stores) against the oracle.
 
 The oracle needs about a minute on 16 host cores for the 48-qubit circuit;
 these tests carry their own time limits.
@@ -176,3 +178,24 @@ def test_serving_1000_per_s_full_size(oracle):
         assert mix["updates"] == o.updates()
         if i % 8 == 7:
             say("  %d/%d requests checked" % (i + 1, len(reqs)))
+
+
+@pytest.mark.timeout(900)
+def test_wide_branching_with_store_growth(oracle):
+    """2^18 keys branching on 6 qubits (2 of them outside the buckets): the
+    cross-bucket launches grow the stores while the state reaches 2^24
+    nonzeros, with grids larger than the device holds at once, so an output
+    store that aliased the input would corrupt the state."""
+    w = gen.circuit_large(1, nq=48, n_init=1 << 18, n_gates=2000, n_branch=6)
+    w = dict(w, gates=w["gates"][:500])
+    say("48 qubits, 2^18 keys, 6 branching qubits, 500 gates")
+    with E.Engine(48) as e:
+        e.load(w["keys"], w["re"], w["im"])
+        e.apply(w["gates"])
+        got = e.export()
+        st = e.stats()
+    s = oracle_chunked(oracle, w, "wide")
+    say("  %d nonzeros, %d cross-bucket launches" % (got[0].size, st["pairs"]))
+    assert st["pairs"] > 0
+    assert_same_state(got, s.export(), atol=1e-12, label="wide branching")
+    assert st["updates"] == s.updates()

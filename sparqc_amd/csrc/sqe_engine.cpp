@@ -367,7 +367,9 @@ sqe_status ensure_staging(sqe_engine* e, uint64_t n) {
 
 sqe_status ensure_index(sqe_engine* e, uint64_t nkeys) {
     const double ml = e->cfg.max_load > 0.0 && e->cfg.max_load < 0.95 ? e->cfg.max_load : 0.5;
-    const uint64_t want = std::max<uint64_t>(1024, pow2_at_least((uint64_t)std::ceil((double)nkeys / ml) + 8));
+    // (a power of two holding nkeys at load factor ml; an exact power of two of
+    // keys must not round up to twice the table)
+    const uint64_t want = std::max<uint64_t>(1024, pow2_at_least((uint64_t)std::ceil((double)nkeys / ml)));
     if (want > e->tcap || want * 8 < e->tcap) {
         HIPCK(hipStreamSynchronize(e->st));
         dfree(e, e->tab, e->tcap * sizeof(Slot));
@@ -488,9 +490,16 @@ sqe_status remap(sqe_engine* e, const QMap& nm) {
     return load_from_staging(e, n);
 }
 
-sqe_status rebuild_index(sqe_engine* e) {
+// Size and clear the index for the current state (before a rebuild, or
+// before a sweep that inserts its buckets as it stores them).
+sqe_status clear_index(sqe_engine* e) {
     SQCK(ensure_index(e, e->nb_upper));
     HIPCK(launch_fill_slots(e->st, e->tab, e->tcap));
+    return SQE_OK;
+}
+
+sqe_status rebuild_index(sqe_engine* e) {
+    SQCK(clear_index(e));
     IndexArgs ia{};
     const Store& o = e->s[e->cur];
     ia.base = o.base; ia.nb = &e->meta[e->cur]; ia.tab = e->tab; ia.tmask = e->tcap - 1; ia.stats = e->stats;
@@ -546,6 +555,7 @@ struct SweepOpt {
     double* ckpt = nullptr;             // norm-point partials, ckpt_stride per checkpoint
     uint32_t* ckpt_stride = nullptr;    // receives the wavefront count of the grid
     unsigned long long* ambig = nullptr;
+    bool index = false;                 // insert every stored bucket into the (cleared) hash index
 };
 
 sqe_status run_sweep(sqe_engine* e, const MOp* mops, const double* coef, uint64_t nops, int store,
@@ -567,6 +577,10 @@ sqe_status run_sweep(sqe_engine* e, const MOp* mops, const double* coef, uint64_
     a.ckpt_stride = g * (BLOCK / 64);
     if (opt.ckpt_stride) *opt.ckpt_stride = a.ckpt_stride;
     a.ambig = opt.ambig;
+    if (opt.index) {
+        a.itab = e->tab;
+        a.itmask = e->tcap - 1;
+    }
     SQCK(prep_bstats(e, g));
     a.bstats = e->bstats;
     if (opt.norm_grid) {
@@ -766,14 +780,20 @@ sqe_status flush_exact(sqe_engine* e, const std::vector<sqe_gate>& gl) {
             SweepOpt so;
             so.blocks = has_block;
             so.norm_grid = norm_next ? &fused_norm : nullptr;
+            bool flips = false;
+            for (uint64_t i = s.off; i < s.off + s.n; ++i) flips |= prog[i].tbase && kind_is_perm(prog[i].kind);
+            // a sweep that changes base keys right before a cross-bucket gate
+            // rebuilds the index as it stores its buckets (no separate pass)
+            so.index = flips && si + 1 < steps.size() && steps[si + 1].kind == ST_PAIR;
+            if (so.index) SQCK(clear_index(e));
             SQCK(run_sweep(e, dm + enc.start[s.off], dc, enc.start[s.off + s.n] - enc.start[s.off], 1, so));
             ++e->n_sweeps;
             for (uint64_t i = s.off; i < s.off + s.n; ++i) {
                 const DOp& d = prog[i];
                 if (d.kind == K_BLOCK) e->n_sweep_ops += (uint64_t)d.p[1];
                 else if (d.kind != K_RENORM && d.kind != K_NORM_POINT) ++e->n_sweep_ops;
-                if (d.tbase && kind_is_perm(d.kind)) e->tab_valid = false;
             }
+            if (flips) e->tab_valid = so.index;
         } break;
         case ST_PAIR: SQCK(run_pair(e, s.op)); fused_norm = 0; break;
         case ST_NORM:

@@ -779,6 +779,12 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             if (ctx.out_base) ctx.out_base[b] = valid[e] ? base[e] : EMPTY_KEY;
             else if (valid[e] && base[e] != base0[e]) basep[b] = base[e];
         }
+        if (ctx.itab) {
+            // the index of the stored state, built here instead of by a
+            // separate pass (all 64 lanes take part in the cooperative insert)
+            const uint64_t b = (uint64_t)(tile[e] >= 0 ? tile[e] : 0) * TILE + lane;
+            coop_insert(ctx.itab, ctx.itmask, base[e], b, tile[e] >= 0 && store && valid[e], ctx.err);
+        }
         out.nnz_out += cnt[e];
         out.empty += (valid[e] && cnt[e] == 0) ? 1ull : 0ull;
     });
@@ -850,7 +856,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
         if (threadIdx.x == 0) *near_threshold_flag() = 0u;
         __syncthreads();
     }
-    const SweepCtx ctx{A.qm, A.frames, A.ckpt_stride, (uint32_t)g, A.out_base, A.out_amp, A.ckpt};
+    const SweepCtx ctx{A.qm, A.frames, A.ckpt_stride, (uint32_t)g, A.out_base, A.out_amp, A.ckpt, A.itab, A.itmask,
+                       &A.stats->err};
     if (any)
         sweep_group<MB, R, EPT, BLOCKS, true>(A.base, A.amp, tile, A.ops, A.nops, A.coef, A.thr, A.scale, A.store, cnt,
                                               ctx);
@@ -909,7 +916,7 @@ __global__ __launch_bounds__(BLOCK) void k_count(SweepArgs A) {
     const int64_t t = (int64_t)blockIdx.x * (BLOCK / 64) + wave;
     const int64_t tile[1] = {t < ntiles ? t : -1};
     SweepCount cnt{0, 0, 0, 0, 0.0};
-    const SweepCtx ctx{A.qm, 0, 0, 0, nullptr, nullptr, nullptr};
+    const SweepCtx ctx{A.qm, 0, 0, 0, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
     if (t < ntiles) sweep_group<MB, R, 1>(A.base, A.amp, tile, nullptr, 0, nullptr, A.thr, nullptr, 0, cnt, ctx);
     block_stats(A.bstats, 0ull, cnt.nnz_out, cnt.empty, 0ull);
 }
@@ -937,7 +944,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
         if (tile >= t.ntiles) break;
         const int64_t tl[1] = {(int64_t)tile};
         SweepCount c{0, 0, 0, 0};
-        const SweepCtx ctx{t.qm, t.frames, 0, it, nullptr, nullptr, nullptr};
+        const SweepCtx ctx{t.qm, t.frames, 0, it, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
         sweep_group<MB, R, 1>(t.base, t.amp, tl, t.ops, t.nops, t.coef, A.thr, t.scale, 1, c, ctx);
         upd += c.upd;
         nnz += c.nnz_out;

@@ -33,6 +33,8 @@ struct SweepArgs {
     void* out_amp;
     double* ckpt;           // [checkpoint * ckpt_stride + wavefront]: FP64 norms of H_NORM_POINT ops
     unsigned long long* ambig;  // FP32 prune decisions within 2^-10 of the threshold (counted per workgroup)
+    Slot* itab;             // non-null: insert (base key -> bucket) of every stored bucket into this cleared index
+    uint64_t itmask;
 };
 
 // Per-launch context of sweep_group beyond the program.
@@ -44,6 +46,9 @@ struct SweepCtx {
     uint64_t* out_base;
     void* out_amp;
     double* ckpt;
+    Slot* itab;             // non-null: index the stored buckets (base key -> bucket) here
+    uint64_t itmask;
+    unsigned long long* err;
 };
 
 // One circuit's share of a batched launch.

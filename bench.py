@@ -374,12 +374,14 @@ def run_serve(seed, n_requests, rates, prec, step_gates):
     for rate in rates:
         arr = gen.poisson_arrivals(seed + int(rate), rate, len(reqs))
         with ServeLoop(prec=prec, step_gates=step_gates, hbm_budget_bytes=200 * 10**9) as s:
+            s.reserve(16, 16 * max(((r["keys"].size + 63) // 64) * 64 for r in reqs))
             s.run(reqs[:8], np.zeros(8))                       # warm-up (kernels, pools)
             rep = s.run(reqs, arr)
         rows.append({"rate_per_s": rate, "p50_latency_ms": rep["p50_latency_s"] * 1e3,
                      "p99_latency_ms": rep["p99_latency_s"] * 1e3, "updates_per_s": rep["updates_per_s"],
                      "served_per_s": len(reqs) / rep["wall_s"],
                      "wall_s": rep["wall_s"], "steps": rep["steps"], "peak_live": rep["peak_live"],
+                     "peak_exports_in_flight": rep["peak_exports_in_flight"], "errors": len(rep["errors"]),
                      "peak_pool_bytes": rep["peak_pool_bytes"]})
     return {"requests_per_rate": n_requests, "prec": prec, "step_gates": step_gates, "hbm_budget_bytes": 200 * 10**9,
             "rows": rows}

@@ -263,7 +263,10 @@ sqe_status sqe_batch_destroy(sqe_batch* b);
 
 /* Admit a circuit: nq qubits, n distinct keys with amplitudes, ng gates.
  * *id receives its handle.  SQE_ETABLE_FULL when its pool bytes would exceed
- * the budget (nothing is admitted; retry after sqe_batch_remove). */
+ * the budget (nothing is admitted; retry after sqe_batch_remove).  The state
+ * is copied to the device asynchronously (the call does not wait for the
+ * device); a duplicate key in it is reported for this circuit by
+ * sqe_batch_info, sqe_batch_export and sqe_batch_export_poll (SQE_EINVAL). */
 sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const double* re, const double* im,
                          uint64_t n, const sqe_gate* gates, uint64_t ng, int64_t* id);
 
@@ -282,6 +285,27 @@ sqe_status sqe_batch_export(sqe_batch* b, int64_t id, uint64_t* keys, double* re
 
 /* Release a circuit's pool space (finished or not). */
 sqe_status sqe_batch_remove(sqe_batch* b, int64_t id);
+
+/* Asynchronous export, for serving loops: enqueue the export of circuit id
+ * (the entries the readout filter keeps, in bucket order) and its copy into
+ * keys / re / im, then return without waiting.  The arrays must be
+ * page-locked host memory from sqe_host_alloc (or device memory) of at least
+ * the circuit's slot count entries: *bound (optional) receives that count,
+ * and a smaller cap is refused with SQE_EINVAL before anything is enqueued.
+ * The circuit may be removed right after this call; the arrays belong to the
+ * export until sqe_batch_export_poll reports it done.  *ticket identifies
+ * the export.  No reference counterpart (the reference's readout,
+ * emulator/device/table.cuh:47-75, is synchronous). */
+sqe_status sqe_batch_export_start(sqe_batch* b, int64_t id, uint64_t* keys, double* re, double* im, uint64_t cap,
+                                  int64_t* ticket, uint64_t* bound);
+
+/* *done = 1 once the export's entries are in the caller's arrays (wait = 1
+ * waits for it; wait = 0 returns at once with *done = 0 while it runs).  When
+ * done, *n receives the entry count and *info (optional) the circuit's
+ * counters at export, the ticket is retired, and a duplicate key in the
+ * circuit's load is reported as SQE_EINVAL. */
+sqe_status sqe_batch_export_poll(sqe_batch* b, int64_t ticket, int32_t wait, int32_t* done, uint64_t* n,
+                                 sqe_batch_circuit* info);
 
 /* Ids of admitted circuits whose every gate has been enqueued (host state, no
  * wait); *n receives the count, SQE_EINVAL when it exceeds cap. */

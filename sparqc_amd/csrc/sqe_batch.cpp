@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <deque>
 #include <new>
 #include <stdexcept>
 #include <cmath>
@@ -118,6 +119,34 @@ struct RangeAlloc {
     }
 };
 
+// Staging that asynchronous copies read or write, used as a ring: each
+// reservation stays busy until the event recorded after the work that uses it
+// has completed, so the host never waits for the device unless the ring has
+// come round to a reservation still in use.  Page-locked host memory for
+// uploads, device memory for export output.
+struct Ring {
+    char* p = nullptr;
+    uint64_t cap = 0, head = 0;
+    bool host = false;
+    struct Span {
+        uint64_t begin, end;
+        hipEvent_t ev;
+    };
+    std::deque<Span> busy;                // in reservation order
+    std::vector<hipEvent_t> idle;         // recorded events whose work is done
+};
+
+constexpr uint64_t RING_ALIGN = 256;
+inline uint64_t ring_round(uint64_t n) { return (std::max<uint64_t>(n, 1) + RING_ALIGN - 1) & ~(RING_ALIGN - 1); }
+
+// One exported circuit on its way to the caller's arrays.
+struct XTicket {
+    uint32_t rec = 0;                     // host record slot
+    hipEvent_t ev = nullptr;              // after the copy to the caller
+    sqe_batch_circuit info{};             // host-side fields at export time
+    uint64_t cap = 0;
+};
+
 struct Circ {
     bool live = false;
     uint32_t nq = 0;
@@ -170,12 +199,20 @@ struct sqe_batch {
     unsigned int* ticket = nullptr;
     uint32_t epoch = 0;
     unsigned long long* total = nullptr;
-    // launch staging
-    BTask* htask = nullptr;
-    BItem* hitem = nullptr;
-    uint64_t hcap_t = 0, hcap_i = 0;
-    char* hup = nullptr;                     // page-locked staging of an admission's uploads
-    uint64_t hup_cap = 0;
+    // page-locked staging of admissions' and steps' uploads
+    Ring up;
+    // asynchronous exports: device staging of their output, the stream that
+    // copies it to the caller, page-locked records of entry counts and counters
+    Ring xr;
+    hipStream_t xst = nullptr;
+    std::map<int64_t, XTicket> xt;
+    int64_t next_ticket = 1;
+    unsigned long long *hrec = nullptr, *drec = nullptr;
+    std::vector<uint32_t> rec_free;
+    hipEvent_t xev = nullptr;                // export kernel done -> copy stream may start
+    std::vector<hipEvent_t> evpool;          // idle ticket events
+    unsigned long long* hsmall = nullptr;    // page-locked words for small readbacks
+    Stats* cst = nullptr;                    // per-slot load error words (err field)
     double* partials = nullptr;
     uint64_t pcap = 0;
     BTask* dtask = nullptr;
@@ -233,6 +270,105 @@ void dfr(sqe_batch* b, void* p, size_t n) {
     }
 }
 
+constexpr uint32_t REC_WORDS = 8;          // per export record: entry count, counter snapshot, spare
+constexpr uint32_t REC_SLOTS = 4096;       // exports in flight at once
+constexpr uint64_t UP_RING_MIN = 64ull << 20;
+constexpr uint64_t XR_RING_MIN = 256ull << 20;
+
+sqe_status ring_event(Ring& r, hipEvent_t* ev) {
+    if (!r.idle.empty()) {
+        *ev = r.idle.back();
+        r.idle.pop_back();
+        return SQE_OK;
+    }
+    BHIP(hipEventCreateWithFlags(ev, hipEventDisableTiming));
+    return SQE_OK;
+}
+
+// Retire the oldest reservation if its work has completed (or after waiting
+// for it, with `wait`); *popped tells whether one was retired.
+sqe_status ring_pop(Ring& r, bool wait, bool* popped) {
+    *popped = false;
+    if (r.busy.empty()) return SQE_OK;
+    const Ring::Span sp = r.busy.front();
+    if (wait) {
+        BHIP(hipEventSynchronize(sp.ev));
+    } else {
+        const hipError_t q = hipEventQuery(sp.ev);
+        if (q == hipErrorNotReady) return SQE_OK;
+        BHIP(q);
+    }
+    r.idle.push_back(sp.ev);
+    r.busy.pop_front();
+    *popped = true;
+    return SQE_OK;
+}
+
+sqe_status ring_release(sqe_batch* b, Ring& r) {
+    for (bool pp = true; pp;) BCK(ring_pop(r, true, &pp));
+    if (r.p) {
+        if (r.host) (void)hipHostFree(r.p);
+        else dfr(b, r.p, r.cap);
+    }
+    r.p = nullptr;
+    r.cap = 0;
+    r.head = 0;
+    return SQE_OK;
+}
+
+// Reserve n bytes of the ring at *off (RING_ALIGN-aligned), growing it to at
+// least min_cap bytes the first time.
+sqe_status ring_take(sqe_batch* b, Ring& r, uint64_t n, uint64_t min_cap, uint64_t* off) {
+    n = ring_round(n);
+    if (n > r.cap) {
+        BCK(ring_release(b, r));
+        const uint64_t cap = std::max<uint64_t>(p2(n), min_cap);
+        if (r.host) {
+            if (hipHostMalloc((void**)&r.p, cap, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                r.p = nullptr;
+                return bfail(SQE_ENOMEM, "hipHostMalloc(" + std::to_string(cap) + ") failed");
+            }
+        } else {
+            BCK(dal(b, (void**)&r.p, cap));
+        }
+        r.cap = cap;
+    }
+    for (bool pp = true; pp;) BCK(ring_pop(r, false, &pp));
+    for (;;) {
+        const uint64_t s0 = r.head + n <= r.cap ? r.head : 0;
+        bool clash = false;
+        for (const Ring::Span& sp : r.busy) clash |= s0 < sp.end && sp.begin < s0 + n;
+        if (!clash) {
+            *off = s0;
+            r.head = s0 + n;
+            return SQE_OK;
+        }
+        bool pp = false;
+        BCK(ring_pop(r, true, &pp));       // the oldest reservation is the next in ring order
+    }
+}
+
+// The reservation of n bytes at off stays busy until the work enqueued on st
+// so far has completed.
+sqe_status ring_mark(Ring& r, uint64_t off, uint64_t n, hipStream_t st) {
+    hipEvent_t ev = nullptr;
+    BCK(ring_event(r, &ev));
+    const hipError_t e = hipEventRecord(ev, st);
+    if (e != hipSuccess) {
+        r.idle.push_back(ev);
+        BHIP(e);
+    }
+    r.busy.push_back(Ring::Span{off, off + ring_round(n), ev});
+    return SQE_OK;
+}
+
+void ring_destroy(sqe_batch* b, Ring& r) {
+    (void)ring_release(b, r);
+    for (hipEvent_t e : r.idle) (void)hipEventDestroy(e);
+    r.idle.clear();
+}
+
 // Grow a pool so that `need` bytes fit, keeping its used contents.
 sqe_status pool_fit(sqe_batch* b, Pool& pl, uint64_t need, uint64_t keep) {
     if (need <= pl.bytes) return SQE_OK;
@@ -266,6 +402,7 @@ sqe_status slots_fit(sqe_batch* b, uint64_t need) {
     BCK(regrow((void**)&b->meta, 8));
     BCK(regrow((void**)&b->norm2, 8));
     BCK(regrow((void**)&b->scale, 8));
+    BCK(regrow((void**)&b->cst, sizeof(Stats)));
     b->nslot = n;
     ++b->layout;
     return SQE_OK;
@@ -313,6 +450,38 @@ sqe_status check_err(sqe_batch* b) {
                      (err & ERR_DUP) ? "duplicate key in load" : "device-side error in batch");
     }
     return SQE_OK;
+}
+
+// A circuit's counters (updates, nonzeros, buckets) and its load error word,
+// after the work enqueued so far.
+sqe_status read_counters(sqe_batch* b, uint32_t id, unsigned long long v[4]) {
+    const unsigned long long* src[4] = {b->upd + id, b->nnz + id, b->meta + id, &b->cst[id].err};
+    for (int k = 0; k < 4; ++k) BHIP(hipMemcpyAsync(b->hsmall + k, src[k], 8, hipMemcpyDeviceToHost, b->st));
+    BHIP(hipStreamSynchronize(b->st));
+    for (int k = 0; k < 4; ++k) v[k] = b->hsmall[k];
+    return SQE_OK;
+}
+
+sqe_status load_fail(unsigned long long err) {
+    return bfail((err & ERR_DUP) ? SQE_EINVAL : SQE_EHIP,
+                 (err & ERR_DUP) ? "duplicate key in the circuit's load" : "device-side error in the circuit's load");
+}
+
+// The device address of a caller's export array: page-locked host memory
+// (sqe_host_alloc) or device memory; pageable memory is refused, since the
+// copy runs as a kernel.
+sqe_status device_view(void* p, void** out) {
+    hipPointerAttribute_t at{};
+    const hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess) (void)hipGetLastError();
+    if (e == hipSuccess && at.type == hipMemoryTypeDevice) {
+        *out = p;
+        return SQE_OK;
+    }
+    if (e == hipSuccess && at.type == hipMemoryTypeHost && hipHostGetDevicePointer(out, p, 0) == hipSuccess)
+        return SQE_OK;
+    (void)hipGetLastError();
+    return bfail(SQE_EINVAL, "export arrays must be page-locked (sqe_host_alloc) or device memory");
 }
 
 // FP64 key / re / im staging arrays of at least `need` entries.
@@ -488,29 +657,21 @@ sqe_status step_all(sqe_batch* b, uint64_t max_gates) {
     std::vector<BItem> I;
     uint64_t ms = 0, mn = 0;
     pack(gs, T, I, &ms, &mn);
-    if (T.size() > b->hcap_t) {
-        BHIP(hipStreamSynchronize(b->st));
-        if (b->htask) (void)hipHostFree(b->htask);
-        b->htask = nullptr;
-        b->hcap_t = p2(T.size());
-        BHIP(hipHostMalloc((void**)&b->htask, b->hcap_t * sizeof(BTask), hipHostMallocDefault));
-    }
-    if (I.size() > b->hcap_i) {
-        BHIP(hipStreamSynchronize(b->st));
-        if (b->hitem) (void)hipHostFree(b->hitem);
-        b->hitem = nullptr;
-        b->hcap_i = p2(I.size());
-        BHIP(hipHostMalloc((void**)&b->hitem, b->hcap_i * sizeof(BItem), hipHostMallocDefault));
-    }
     BCK(buf_fit(b, &b->dtask, &b->dcap_t, T.size()));
     BCK(buf_fit(b, &b->ditem, &b->dcap_i, I.size()));
     BCK(buf_fit(b, &b->counts, &b->ccap, std::max<uint64_t>(1, 2 * ms)));
     BCK(buf_fit(b, &b->partials, &b->pcap, std::max<uint64_t>(1, mn)));
-    BHIP(hipStreamSynchronize(b->st));              // the pinned staging is reused
-    std::memcpy(b->htask, T.data(), T.size() * sizeof(BTask));
-    std::memcpy(b->hitem, I.data(), I.size() * sizeof(BItem));
-    BHIP(hipMemcpyAsync(b->dtask, b->htask, T.size() * sizeof(BTask), hipMemcpyHostToDevice, b->st));
-    BHIP(hipMemcpyAsync(b->ditem, b->hitem, I.size() * sizeof(BItem), hipMemcpyHostToDevice, b->st));
+    // task and item lists through the upload ring (the device lists are
+    // rewritten in stream order, after the previous step has read them)
+    const uint64_t tb = T.size() * sizeof(BTask), ib = I.size() * sizeof(BItem);
+    uint64_t off = 0;
+    BCK(ring_take(b, b->up, ring_round(tb) + ib, UP_RING_MIN, &off));
+    char* hp = b->up.p + off;
+    std::memcpy(hp, T.data(), tb);
+    std::memcpy(hp + ring_round(tb), I.data(), ib);
+    BHIP(hipMemcpyAsync(b->dtask, hp, tb, hipMemcpyHostToDevice, b->st));
+    BHIP(hipMemcpyAsync(b->ditem, hp + ring_round(tb), ib, hipMemcpyHostToDevice, b->st));
+    BCK(ring_mark(b->up, off, ring_round(tb) + ib, b->st));
     for (const Group& g : gs) {
         BCK(launch_group(b, b->st, g, b->dtask, b->ditem, b->counts, b->partials, b->cfg.timing != 0));
         if (!g.norm) ++b->launches;
@@ -709,8 +870,14 @@ sqe_status sqe_batch_create(const sqe_config* cfg, sqe_batch** out) {
         if (r == SQE_OK) r = dal(b, (void**)&b->total, 64);
         if (r == SQE_OK && hipMemset(b->stats, 0, sizeof(Stats)) != hipSuccess) r = bfail(SQE_EHIP, "memset");
         if (r == SQE_OK && hipMemset(b->ticket, 0, 64) != hipSuccess) r = bfail(SQE_EHIP, "memset");
-        if (r == SQE_OK && (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess))
+        if (r == SQE_OK && (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+                            hipEventCreateWithFlags(&b->xev, hipEventDisableTiming) != hipSuccess))
             r = bfail(SQE_EHIP, "event");
+        if (r == SQE_OK && hipStreamCreateWithFlags(&b->xst, hipStreamNonBlocking) != hipSuccess)
+            r = bfail(SQE_EHIP, "stream");
+        if (r == SQE_OK && hipHostMalloc((void**)&b->hsmall, 64, hipHostMallocDefault) != hipSuccess)
+            r = bfail(SQE_ENOMEM, "hipHostMalloc");
+        b->up.host = true;
         if (r == SQE_OK) r = slots_fit(b, 256);
         if (r != SQE_OK) {
             sqe_batch_destroy(b);
@@ -726,7 +893,16 @@ sqe_status sqe_batch_destroy(sqe_batch* b) {
         if (!b) return SQE_OK;
         (void)hipSetDevice(b->cfg.device);
         if (b->st) (void)hipStreamSynchronize(b->st);
-        for (void* p : {(void*)b->base.p, (void*)b->amp.p, (void*)b->prog.p, (void*)b->coef.p, (void*)b->upd,
+        if (b->xst) (void)hipStreamSynchronize(b->xst);
+        ring_destroy(b, b->up);
+        ring_destroy(b, b->xr);
+        for (auto& kv : b->xt) (void)hipEventDestroy(kv.second.ev);
+        for (hipEvent_t ev : b->evpool) (void)hipEventDestroy(ev);
+        if (b->xev) (void)hipEventDestroy(b->xev);
+        if (b->hrec) (void)hipHostFree(b->hrec);
+        if (b->hsmall) (void)hipHostFree(b->hsmall);
+        if (b->xst) (void)hipStreamDestroy(b->xst);
+        for (void* p : {(void*)b->cst, (void*)b->base.p, (void*)b->amp.p, (void*)b->prog.p, (void*)b->coef.p, (void*)b->upd,
                         (void*)b->nnz, (void*)b->meta, (void*)b->norm2, (void*)b->scale, (void*)b->tab,
                         (void*)b->xkeys, (void*)b->xre, (void*)b->xim, (void*)b->occ, (void*)b->stats,
                         (void*)b->scan_status, (void*)b->ticket, (void*)b->total, (void*)b->dtask, (void*)b->ditem, (void*)b->partials,
@@ -734,9 +910,6 @@ sqe_status sqe_batch_destroy(sqe_batch* b) {
                         (void*)b->gitem, (void*)b->gcounts, (void*)b->gpartials})
             if (p) (void)hipFree(p);
         drop_graph(b);
-        if (b->htask) (void)hipHostFree(b->htask);
-        if (b->hitem) (void)hipHostFree(b->hitem);
-        if (b->hup) (void)hipHostFree(b->hup);
         if (b->ev0) (void)hipEventDestroy(b->ev0);
         if (b->ev1) (void)hipEventDestroy(b->ev1);
         for (hipStream_t s2 : b->side) (void)hipStreamDestroy(s2);
@@ -848,21 +1021,14 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         ++b->generation;
         ++b->layout;
         Circ& cc = b->c[id];
-        // the uploads go through one page-locked staging buffer: program,
-        // coefficients, then the nonzero entries' keys, real and imaginary parts
-        // (no copy from an earlier admission may still be reading it)
+        // the uploads go through the page-locked upload ring: program,
+        // coefficients, then the nonzero entries' keys, real and imaginary
+        // parts; the device staging they land in is rewritten in stream order
         BCK(stage_fit(b, std::max<uint64_t>(m, 1)));
-        BHIP(hipStreamSynchronize(b->st));
         const uint64_t up = cc.nops * sizeof(MOp) + cc.ncoef * 8 + 24 * m;
-        if (up > b->hup_cap) {
-            if (b->hup) (void)hipHostFree(b->hup);
-            b->hup = nullptr;
-            b->hup_cap = 0;
-            const uint64_t cap = std::max<uint64_t>(p2(up), 1 << 20);
-            BHIP(hipHostMalloc((void**)&b->hup, cap, hipHostMallocDefault));
-            b->hup_cap = cap;
-        }
-        char* hp = b->hup;
+        uint64_t up_off = 0;
+        BCK(ring_take(b, b->up, up, UP_RING_MIN, &up_off));
+        char* hp = b->up.p + up_off;
         if (cc.nops) {
             std::memcpy(hp, prog.data(), cc.nops * sizeof(MOp));
             BHIP(hipMemcpyAsync(b->prog.p + cc.prog0 * sizeof(MOp), hp, cc.nops * sizeof(MOp), hipMemcpyHostToDevice,
@@ -887,6 +1053,7 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
             BHIP(hipMemcpyAsync(b->xre, hr, m * 8, hipMemcpyHostToDevice, b->st));
             BHIP(hipMemcpyAsync(b->xim, hi, m * 8, hipMemcpyHostToDevice, b->st));
         }
+        BCK(ring_mark(b->up, up_off, up, b->st));
         // load the state into the region (insert bases, number buckets, scatter)
         const double ml = b->cfg.max_load > 0 && b->cfg.max_load < 0.95 ? b->cfg.max_load : 0.5;
         const uint64_t tneed = std::max<uint64_t>(1024, p2((uint64_t)std::ceil((double)m / ml) + 8));
@@ -898,28 +1065,31 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
             b->tcap = tneed;
         }
         BCK(buf_fit(b, &b->occ, &b->occ_cap, tiles * 64));
-        // the index of this admission uses the first tneed slots of the table
-        // (kept at its largest size), so its cost follows the circuit's size
-        BHIP(launch_fill_slots(b->st, b->tab, tneed));
-        BHIP(launch_fill_u64(b->st, base_ptr(b, cc), tiles * 64, EMPTY_KEY));
-        BHIP(launch_fill_u64(b->st, (uint64_t*)amp_ptr(b, cc), rows * 64 * b->csize / 8, 0));
-        BHIP(launch_fill_u64(b->st, (uint64_t*)b->occ, (tiles * 64 + 1) / 2, 0));
-        BHIP(launch_fill_u64(b->st, (uint64_t*)(b->upd + id), 1, 0));
-        BHIP(launch_fill_u64(b->st, (uint64_t*)(b->nnz + id), 1, 0));
+        // one launch clears the index (its first tneed slots: the table is
+        // kept at its largest size, so the cost follows the circuit's size),
+        // the region, the occupancy bits and the circuit's counters
+        FillSet fs{};
+        fs.r[0] = FillRange{(uint64_t*)b->tab, 2 * tneed, EMPTY_KEY, NONE};
+        fs.r[1] = FillRange{base_ptr(b, cc), tiles * 64, EMPTY_KEY, EMPTY_KEY};
+        fs.r[2] = FillRange{(uint64_t*)amp_ptr(b, cc), rows * 64 * b->csize / 8, 0, 0};
+        fs.r[3] = FillRange{(uint64_t*)b->occ, (tiles * 64 + 1) / 2, 0, 0};
+        fs.r[4] = FillRange{(uint64_t*)(b->upd + id), 1, 0, 0};
+        fs.r[5] = FillRange{(uint64_t*)(b->nnz + id), 1, 0, 0};
+        fs.r[6] = FillRange{(uint64_t*)(b->cst + id), sizeof(Stats) / 8, 0, 0};
+        fs.k = 7;
+        BHIP(launch_fill_set(b->st, fs));
+        // a load error (a duplicate key) lands in the circuit's own error word
+        // and is reported by sqe_batch_info / sqe_batch_export for it, so
+        // admission never waits for the device
         LoadArgs la{};
         la.keys = b->xkeys; la.re = b->xre; la.im = b->xim; la.n = m; la.qm = qm;
-        la.tab = b->tab; la.tmask = tneed - 1; la.amp = amp_ptr(b, cc); la.occ = b->occ; la.stats = b->stats;
+        la.tab = b->tab; la.tmask = tneed - 1; la.amp = amp_ptr(b, cc); la.occ = b->occ; la.stats = b->cst + id;
         EnumArgs ea{};
         ea.tab = b->tab; ea.tcap = tneed; ea.out_base = base_ptr(b, cc); ea.out_nb = b->meta + id;
-        ea.out_cap = tiles * 64; ea.stats = b->stats;
+        ea.out_cap = tiles * 64; ea.stats = b->cst + id;
         const uint32_t eg = (uint32_t)((tneed + BLOCK - 1) / BLOCK);
         BCK(next_scan(b, eg, &ea.scan));
         BHIP(launch_load(b->f32, b->st, la, ea, eg));
-        const sqe_status e = check_err(b);
-        if (e != SQE_OK) {
-            release(b, id);
-            return e;
-        }
         *id_out = id;
         return SQE_OK;
     });
@@ -953,6 +1123,7 @@ sqe_status sqe_batch_sync(sqe_batch* b) {
         if (!b) return bfail(SQE_EINVAL, "null batch");
         BHIP(hipSetDevice(b->cfg.device));
         BHIP(hipStreamSynchronize(b->st));
+        BHIP(hipStreamSynchronize(b->xst));
         return check_err(b);
     });
 }
@@ -962,12 +1133,10 @@ sqe_status sqe_batch_info(sqe_batch* b, int64_t id, sqe_batch_circuit* out) {
         if (!b || !out || id < 0 || (uint64_t)id >= b->c.size() || !b->c[id].live)
             return bfail(SQE_EINVAL, "unknown circuit");
         BHIP(hipSetDevice(b->cfg.device));
-        BHIP(hipStreamSynchronize(b->st));
         const Circ& c = b->c[id];
-        unsigned long long v[3] = {0, 0, 0};
-        BHIP(hipMemcpy(&v[0], b->upd + id, 8, hipMemcpyDeviceToHost));
-        BHIP(hipMemcpy(&v[1], b->nnz + id, 8, hipMemcpyDeviceToHost));
-        BHIP(hipMemcpy(&v[2], b->meta + id, 8, hipMemcpyDeviceToHost));
+        unsigned long long v[4] = {0, 0, 0, 0};
+        BCK(read_counters(b, (uint32_t)id, v));
+        if (v[3]) return load_fail(v[3]);
         out->done = c.pos >= c.nops ? 1 : 0;
         out->gates_total = c.gates;
         out->gates_done = (uint64_t)(std::lower_bound(c.gate_at.begin(), c.gate_at.end(), (uint32_t)c.pos) -
@@ -999,10 +1168,12 @@ sqe_status sqe_batch_export(sqe_batch* b, int64_t id, uint64_t* keys, double* re
         const uint32_t g = std::max<uint32_t>(1, (uint32_t)((nbk + BLOCK - 1) / BLOCK));
         BCK(next_scan(b, g, &xa.scan));
         BHIP(launch_export(c.qm.mb, b->f32, b->st, g, xa));
-        unsigned long long hv[2] = {0, 0};
+        unsigned long long* hv = b->hsmall;
         BHIP(hipMemcpyAsync(&hv[0], b->total, 8, hipMemcpyDeviceToHost, b->st));
         BHIP(hipMemcpyAsync(&hv[1], &b->stats->err, 8, hipMemcpyDeviceToHost, b->st));
+        BHIP(hipMemcpyAsync(&hv[2], &b->cst[id].err, 8, hipMemcpyDeviceToHost, b->st));
         BHIP(hipStreamSynchronize(b->st));
+        if (hv[2]) return load_fail(hv[2]);
         if (hv[1]) BCK(check_err(b));
         const uint64_t tot = hv[0];
         *n = tot;
@@ -1026,6 +1197,127 @@ sqe_status sqe_batch_remove(sqe_batch* b, int64_t id) {
         // anything a later admission enqueues there, and pool reallocation
         // synchronises before it frees
         release(b, (uint32_t)id);
+        return SQE_OK;
+    });
+}
+
+sqe_status sqe_batch_export_start(sqe_batch* b, int64_t id, uint64_t* keys, double* re, double* im, uint64_t cap,
+                                  int64_t* ticket, uint64_t* bound) {
+    return guard([&]() -> sqe_status {
+        if (!b || !ticket) return bfail(SQE_EINVAL, "null argument");
+        *ticket = 0;
+        if (id < 0 || (uint64_t)id >= b->c.size() || !b->c[id].live) return bfail(SQE_EINVAL, "unknown circuit");
+        BHIP(hipSetDevice(b->cfg.device));
+        const Circ& c = b->c[id];
+        const uint64_t nbk = c.tiles * 64, slots = nbk << c.qm.mb;
+        if (bound) *bound = slots;
+        if (cap < slots)
+            return bfail(SQE_EINVAL, "export capacity below the circuit's slot count (" + std::to_string(slots) + ")");
+        if (!keys || !re || !im) return bfail(SQE_EINVAL, "null array");
+        void* dv[3] = {nullptr, nullptr, nullptr};
+        void* hv[3] = {keys, re, im};
+        for (int k = 0; k < 3; ++k) BCK(device_view(hv[k], &dv[k]));
+        if (!b->hrec) {
+            if (hipHostMalloc((void**)&b->hrec, REC_SLOTS * REC_WORDS * 8, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                b->hrec = nullptr;
+                return bfail(SQE_ENOMEM, "hipHostMalloc");
+            }
+            BHIP(hipHostGetDevicePointer((void**)&b->drec, b->hrec, 0));
+            for (uint32_t k = REC_SLOTS; k-- > 0;) b->rec_free.push_back(k);
+        }
+        if (b->rec_free.empty())
+            return bfail(SQE_EINVAL, std::to_string(REC_SLOTS) + " exports in flight: poll some before starting more");
+        // device staging: header (entry count, counter snapshot), then keys,
+        // real and imaginary parts of up to `slots` entries
+        const uint64_t arr = ring_round(slots * 8), bytes = RING_ALIGN + 3 * arr;
+        uint64_t off = 0;
+        BCK(ring_take(b, b->xr, bytes, XR_RING_MIN, &off));
+        char* sp = b->xr.p + off;
+        unsigned long long* hdr = (unsigned long long*)sp;
+        uint64_t* sk = (uint64_t*)(sp + RING_ALIGN);
+        double* sr = (double*)(sp + RING_ALIGN + arr);
+        double* si = (double*)(sp + RING_ALIGN + 2 * arr);
+        ExportArgs xa{};
+        xa.base = base_ptr(b, c); xa.amp = amp_ptr(b, c); xa.nb = b->meta + id; xa.qm = c.qm; xa.thr = b->thr;
+        xa.filter = 1; xa.keys = sk; xa.re = sr; xa.im = si; xa.cap = slots; xa.total = hdr;
+        xa.snap_src[0] = b->upd + id; xa.snap_src[1] = b->nnz + id; xa.snap_src[2] = b->meta + id;
+        xa.snap_src[3] = &b->cst[id].err;
+        xa.snap = hdr + 1;
+        const uint32_t g = std::max<uint32_t>(1, (uint32_t)((nbk + BLOCK - 1) / BLOCK));
+        BCK(next_scan(b, g, &xa.scan));
+        BHIP(launch_export(c.qm.mb, b->f32, b->st, g, xa));
+        // the copy to the caller runs on the export stream, off the step stream
+        BHIP(hipEventRecord(b->xev, b->st));
+        BHIP(hipStreamWaitEvent(b->xst, b->xev, 0));
+        const uint32_t rec = b->rec_free.back();
+        CopyOutArgs ca{};
+        ca.hdr = hdr; ca.keys = sk; ca.re = sr; ca.im = si;
+        ca.okeys = (uint64_t*)dv[0]; ca.ore = (double*)dv[1]; ca.oim = (double*)dv[2];
+        ca.cap = cap; ca.rec = b->drec + (uint64_t)rec * REC_WORDS;
+        bool al = true;
+        for (const void* q : {(const void*)sk, (const void*)sr, (const void*)si, (const void*)dv[0], (const void*)dv[1], (const void*)dv[2]})
+            al &= ((uintptr_t)q & 15) == 0;
+        ca.vec = al ? 1 : 0;
+        const uint32_t cg = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(1, (slots / 2 + BLOCK - 1) / BLOCK));
+        BHIP(launch_copy_out(b->xst, cg, ca));
+        BCK(ring_mark(b->xr, off, bytes, b->xst));
+        hipEvent_t ev = nullptr;
+        if (!b->evpool.empty()) { ev = b->evpool.back(); b->evpool.pop_back(); }
+        else BHIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        const hipError_t er = hipEventRecord(ev, b->xst);
+        if (er != hipSuccess) {
+            b->evpool.push_back(ev);
+            BHIP(er);
+        }
+        b->rec_free.pop_back();
+        XTicket t;
+        t.rec = rec;
+        t.ev = ev;
+        t.cap = cap;
+        t.info.done = c.pos >= c.nops ? 1 : 0;
+        t.info.bucket_bits = c.qm.mb;
+        t.info.gates_total = c.gates;
+        t.info.gates_done = (uint64_t)(std::lower_bound(c.gate_at.begin(), c.gate_at.end(), (uint32_t)c.pos) -
+                                       c.gate_at.begin());
+        t.info.bytes = c.bytes;
+        const int64_t tk = b->next_ticket++;
+        b->xt[tk] = t;
+        *ticket = tk;
+        return SQE_OK;
+    });
+}
+
+sqe_status sqe_batch_export_poll(sqe_batch* b, int64_t ticket, int32_t wait, int32_t* done, uint64_t* n,
+                                 sqe_batch_circuit* info) {
+    return guard([&]() -> sqe_status {
+        if (!b || !done) return bfail(SQE_EINVAL, "null argument");
+        *done = 0;
+        auto it = b->xt.find(ticket);
+        if (it == b->xt.end()) return bfail(SQE_EINVAL, "unknown export ticket");
+        BHIP(hipSetDevice(b->cfg.device));
+        XTicket& t = it->second;
+        if (wait) {
+            BHIP(hipEventSynchronize(t.ev));
+        } else {
+            const hipError_t q = hipEventQuery(t.ev);
+            if (q == hipErrorNotReady) return SQE_OK;
+            BHIP(q);
+        }
+        const unsigned long long* r = b->hrec + (uint64_t)t.rec * REC_WORDS;
+        sqe_batch_circuit ci = t.info;
+        const uint64_t tot = r[0];
+        ci.updates = r[1];
+        ci.nnz = r[2];
+        ci.buckets = r[3];
+        const unsigned long long lerr = r[4];
+        b->evpool.push_back(t.ev);
+        b->rec_free.push_back(t.rec);
+        b->xt.erase(it);
+        *done = 1;
+        if (n) *n = tot;
+        if (info) *info = ci;
+        if (lerr) return load_fail(lerr);
         return SQE_OK;
     });
 }

@@ -1157,6 +1157,14 @@ __global__ __launch_bounds__(BLOCK) void k_export(ExportArgs A) {
     using C = typename CT<R>::T;
     constexpr int S = 1 << MB;
     const uint32_t tile = take_ticket(A.scan);
+    static_assert(SNAP_WORDS == 4, "snapshot source selection below");
+    if (A.snap && blockIdx.x == 0 && threadIdx.x < SNAP_WORDS) {
+        const unsigned long long* src = A.snap_src[0];
+        if (threadIdx.x == 1) src = A.snap_src[1];
+        if (threadIdx.x == 2) src = A.snap_src[2];
+        if (threadIdx.x == 3) src = A.snap_src[3];
+        A.snap[threadIdx.x] = *src;
+    }
     const uint64_t b = (uint64_t)tile * BLOCK + threadIdx.x;
     const uint64_t nb_pad = (*A.nb + 63) & ~63ull;
     const bool live = b < nb_pad;

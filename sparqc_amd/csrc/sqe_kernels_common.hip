@@ -28,6 +28,44 @@ __global__ void k_fill_slots(Slot* __restrict__ p, size_t n) {
     }
 }
 
+__global__ void k_fill_set(FillSet S) {
+    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    for (uint32_t j = 0; j < S.k && j < FILL_RANGES; ++j) {
+        uint64_t* __restrict__ p = S.r[j].p;
+        const uint64_t v0 = S.r[j].v0, v1 = S.r[j].v1;
+        for (size_t i = t0; i < S.r[j].n; i += stride) p[i] = (i & 1) ? v1 : v0;
+    }
+}
+
+// Device staging -> host-mapped arrays.  Each lane moves 16 bytes of each
+// array per pass, so a wavefront writes 1 KiB contiguous runs across the host
+// link.
+__global__ __launch_bounds__(BLOCK) void k_copy_out(CopyOutArgs A) {
+    const unsigned long long total = A.hdr[0];
+    const uint64_t n = total < A.cap ? total : A.cap;
+    const uint64_t t0 = (uint64_t)blockIdx.x * BLOCK + threadIdx.x, stride = (uint64_t)gridDim.x * BLOCK;
+    if (A.vec) {
+        const uint64_t nc = n >> 1;
+        for (uint64_t i = t0; i < nc; i += stride) {
+            ((uint4*)A.okeys)[i] = ((const uint4*)A.keys)[i];
+            ((uint4*)A.ore)[i] = ((const uint4*)A.re)[i];
+            ((uint4*)A.oim)[i] = ((const uint4*)A.im)[i];
+        }
+        if ((n & 1) && t0 == 0) {
+            A.okeys[n - 1] = A.keys[n - 1];
+            A.ore[n - 1] = A.re[n - 1];
+            A.oim[n - 1] = A.im[n - 1];
+        }
+    } else {
+        for (uint64_t i = t0; i < n; i += stride) {
+            A.okeys[i] = A.keys[i];
+            A.ore[i] = A.re[i];
+            A.oim[i] = A.im[i];
+        }
+    }
+    if (t0 <= (uint64_t)SNAP_WORDS) A.rec[t0] = A.hdr[t0];
+}
+
 __global__ __launch_bounds__(BLOCK) void k_norm_final(const double* __restrict__ partials, uint32_t n,
                                                       double* __restrict__ out_norm2, double* __restrict__ out_scale) {
     __shared__ double s_w[BLOCK / 64];
@@ -188,6 +226,19 @@ static inline uint32_t fill_grid(size_t n) {
 hipError_t launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v) {
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u64, dim3(fill_grid(n)), dim3(BLOCK), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_set(hipStream_t st, const FillSet& s) {
+    size_t most = 0;
+    for (uint32_t j = 0; j < s.k && j < FILL_RANGES; ++j) most = s.r[j].n > most ? s.r[j].n : most;
+    if (!most) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_set, dim3(fill_grid(most)), dim3(BLOCK), 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_out(hipStream_t st, uint32_t grid, const CopyOutArgs& a) {
+    hipLaunchKernelGGL(k_copy_out, dim3(grid ? grid : 1), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
 }
 

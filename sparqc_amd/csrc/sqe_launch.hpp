@@ -115,6 +115,8 @@ struct CompactArgs {
     BlockStats* bstats;
 };
 
+constexpr int SNAP_WORDS = 4;
+
 struct ExportArgs {
     const uint64_t* base;
     const void* amp;
@@ -129,6 +131,39 @@ struct ExportArgs {
     uint64_t cap;
     unsigned long long* total;
     ScanCtl scan;
+    // optional: the first workgroup copies *snap_src[k] to snap[k] (a
+    // circuit's counters, taken in stream order with its export)
+    const unsigned long long* snap_src[SNAP_WORDS];
+    unsigned long long* snap;
+};
+
+// An export's entries from device staging to host-mapped (or device)
+// arrays: hdr[0] entries (at most cap) of keys / re / im, then the header's
+// 1 + SNAP_WORDS words (entry count, counter snapshot) to rec.  vec: every
+// array 16-byte aligned (16-byte copies).
+struct CopyOutArgs {
+    const unsigned long long* hdr;
+    const uint64_t* keys;
+    const double* re;
+    const double* im;
+    uint64_t* okeys;
+    double* ore;
+    double* oim;
+    uint64_t cap;
+    unsigned long long* rec;
+    int32_t vec;
+};
+
+// Up to FILL_RANGES clears in one launch: p[i] = (i odd ? v1 : v0).
+struct FillRange {
+    uint64_t* p;
+    uint64_t n;
+    uint64_t v0, v1;
+};
+constexpr int FILL_RANGES = 8;
+struct FillSet {
+    FillRange r[FILL_RANGES];
+    uint32_t k;
 };
 
 struct NormArgs {
@@ -170,6 +205,8 @@ struct EnumArgs {
 
 hipError_t launch_fill_u64(hipStream_t st, uint64_t* p, size_t n, uint64_t v);
 hipError_t launch_fill_slots(hipStream_t st, Slot* p, size_t n);
+hipError_t launch_fill_set(hipStream_t st, const FillSet& s);
+hipError_t launch_copy_out(hipStream_t st, uint32_t grid, const CopyOutArgs& a);
 hipError_t launch_sweep(int mb, bool f32, hipStream_t st, uint64_t ntiles, const SweepArgs& a);
 uint32_t sweep_grid(int mb, bool f32, uint64_t ntiles);   // workgroups launch_sweep uses
 hipError_t launch_pair(int mb, bool f32, uint32_t kind, hipStream_t st, uint32_t grid, const PairArgs& a);

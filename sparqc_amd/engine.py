@@ -26,6 +26,7 @@ EXPORTS = ["sqe_config_default", "sqe_create", "sqe_destroy", "sqe_set_basis", "
            "sqe_last_error",
            "sqe_batch_create", "sqe_batch_destroy", "sqe_batch_add", "sqe_batch_step", "sqe_batch_run",
            "sqe_batch_sync", "sqe_batch_info", "sqe_batch_export", "sqe_batch_remove", "sqe_batch_finished",
+           "sqe_batch_export_start", "sqe_batch_export_poll",
            "sqe_batch_usage",
            "sqe_batch_get_tuning", "sqe_batch_set_tuning", "sqe_batch_autotune", "sqe_batch_snapshot_save",
            "sqe_batch_snapshot_restore", "sqe_batch_last_error", "sqe_host_alloc", "sqe_host_free"]
@@ -138,6 +139,10 @@ def lib():
         L.sqe_batch_info.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(BatchCircuit)]
         L.sqe_batch_export.argtypes = [vp, ctypes.c_int64, _u64p, _f64p, _f64p, ctypes.c_uint64, _u64p]
         L.sqe_batch_remove.argtypes = [vp, ctypes.c_int64]
+        L.sqe_batch_export_start.argtypes = [vp, ctypes.c_int64, _u64p, _f64p, _f64p, ctypes.c_uint64,
+                                             ctypes.POINTER(ctypes.c_int64), _u64p]
+        L.sqe_batch_export_poll.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _u64p,
+                                            ctypes.POINTER(BatchCircuit)]
         L.sqe_batch_usage.argtypes = [vp, _u64p, _u64p, _u64p]
         L.sqe_batch_finished.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.c_uint64, _u64p]
         L.sqe_batch_get_tuning.argtypes = [vp, ctypes.POINTER(BatchTuning)]
@@ -185,7 +190,7 @@ class PinnedArrays:
         if cap <= self.cap and self._p is not None:
             return
         self.free()
-        cap = max(cap, 1024)
+        cap = (max(cap, 1024) + 63) & ~63          # keeps re / im 16-byte aligned
         p = ctypes.c_void_p()
         _bcheck(lib().sqe_host_alloc(3 * 8 * cap, ctypes.byref(p)))
         self._p = p
@@ -456,6 +461,39 @@ class Batch:
 
     def remove(self, cid):
         _bcheck(lib().sqe_batch_remove(self.h, int(cid)))
+
+    @staticmethod
+    def export_buffer(cap=0):
+        """A page-locked buffer for export_start."""
+        return PinnedArrays(cap)
+
+    def export_start(self, cid, out):
+        """Start an asynchronous export of circuit cid into `out` (a
+        PinnedArrays, grown to the circuit's slot count when smaller); returns
+        a ticket for export_poll.  `out` belongs to the export until then."""
+        t = ctypes.c_int64()
+        bound = ctypes.c_uint64()
+        rc = lib().sqe_batch_export_start(self.h, int(cid), _ptr(out.keys, _u64p), _ptr(out.re, _f64p),
+                                          _ptr(out.im, _f64p), out.cap, ctypes.byref(t), ctypes.byref(bound))
+        if rc == SQE_EINVAL and bound.value > out.cap:
+            out.grow(bound.value)
+            rc = lib().sqe_batch_export_start(self.h, int(cid), _ptr(out.keys, _u64p), _ptr(out.re, _f64p),
+                                              _ptr(out.im, _f64p), out.cap, ctypes.byref(t), ctypes.byref(bound))
+        _bcheck(rc)
+        return t.value
+
+    def export_poll(self, ticket, wait=False):
+        """None while the export runs; then (n, info): the entry count (the
+        entries are out.keys[:n] etc. of its PinnedArrays) and the circuit's
+        counters at export."""
+        done = ctypes.c_int32()
+        n = ctypes.c_uint64()
+        c = BatchCircuit()
+        _bcheck(lib().sqe_batch_export_poll(self.h, int(ticket), 1 if wait else 0, ctypes.byref(done),
+                                            ctypes.byref(n), ctypes.byref(c)))
+        if not done.value:
+            return None
+        return n.value, {f: getattr(c, f) for f, _ in BatchCircuit._fields_}
 
     def finished(self):
         """Ids of admitted circuits that have all their gates enqueued (no wait)."""

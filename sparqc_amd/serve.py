@@ -9,15 +9,20 @@ Requests arrive at given times (seconds from the start).  The loop
   2. advances every admitted, unfinished circuit by `step_gates` gates with
      one batched launch per bucket size (circuits that arrived while others
      were running join at the next step, so their work overlaps on the GPU);
-  3. exports each circuit that finished, records arrival -> export latency and
-     releases its pool space.
-It sleeps only when nothing is admitted and the next request is in the future.
+  3. starts the export of each circuit that finished -- asynchronous, on the
+     engine's export stream, into a reused page-locked buffer -- and releases
+     its pool space at once;
+  4. collects the exports whose copies have completed and records each
+     request's arrival -> result latency.
+Only an idle loop waits for the device (for the oldest export, or sleeps until
+the next arrival), so admissions, steps and result copies overlap.
 
 Each circuit advances by the same gate step from its own first gate whatever
 else is in the batch, so its result does not depend on the mix: the solo and
 mixed runs of a circuit agree to the bit (checked by tests/test_gpu_serve.py).
 """
 import time
+from collections import deque
 
 import numpy as np
 
@@ -41,10 +46,22 @@ class ServeLoop:
             batch = E.Batch(prec=prec, renorm_every=renorm_every, hbm_budget_bytes=hbm_budget_bytes,
                             timing=timing, device=device)
         self.batch = batch
+        self._free = []                         # page-locked export buffers, reused
         self.step_gates = int(step_gates)
         self.prec = getattr(batch, "prec", prec)
 
+    def reserve(self, count, cap):
+        """Allocate `count` page-locked export buffers of `cap` entries ahead
+        of time (allocating them inside the loop stalls it)."""
+        while len(self._free) < count:
+            self._free.append(self.batch.export_buffer(cap))
+        for buf in self._free:
+            buf.grow(cap)
+
     def close(self):
+        for buf in self._free:
+            buf.free()
+        self._free = []
         self.batch.close()
 
     def __enter__(self):
@@ -56,24 +73,60 @@ class ServeLoop:
     def run(self, requests, arrivals, keep_results=False, clock=time.perf_counter, sleep=time.sleep):
         """Serve `requests` (circuit dicts: nq, keys, re, im, gates) arriving at
         `arrivals` (seconds, nondecreasing).  Returns a report dict; with
-        keep_results the exported states are in report["results"][i]."""
+        keep_results the exported states (sorted by key) are in
+        report["results"][i].
+
+        Nothing in the loop waits for the device except when it has nothing
+        else to do: admissions enqueue their uploads, steps their launches,
+        and a finished circuit's export runs on the engine's export stream
+        into a page-locked buffer while later steps run; the loop collects
+        exports whose copies have completed.  A request whose state the engine
+        rejects (a duplicate key) is reported in report["errors"]."""
         n = len(requests)
         arrivals = np.asarray(arrivals, dtype=np.float64)
         assert arrivals.size == n and np.all(np.diff(arrivals) >= 0)
         b = self.batch
         nxt = 0                                 # next request not yet admitted
         live = {}                               # circuit id -> request index
+        pending = deque()                       # (ticket, request index, buffer) of exports in flight
         latency = [None] * n
         admit_t = [None] * n
         results = [None] * n if keep_results else None
+        errors = {}
         updates = 0
         steps = 0
         peak_live = 0
         peak_bytes = 0
-        # results that are only timed land in one reused page-locked buffer
-        pinned = E.PinnedArrays(1 << 16) if (not keep_results and isinstance(b, E.Batch)) else None
+        peak_pending = 0
         t0 = clock()
-        while nxt < n or live:
+
+        def collect(wait):
+            nonlocal updates
+            while pending:
+                tk, i, buf = pending[0]
+                try:
+                    r = b.export_poll(tk, wait=wait)
+                except E.SqeError as ex:
+                    pending.popleft()
+                    self._free.append(buf)
+                    errors[i] = str(ex)
+                    latency[i] = clock() - t0 - arrivals[i]
+                    continue
+                if r is None:
+                    return
+                pending.popleft()
+                cnt, info = r
+                updates += info["updates"]
+                latency[i] = clock() - t0 - arrivals[i]
+                if keep_results:
+                    k = buf.keys[:cnt].copy()
+                    o = np.argsort(k, kind="stable")
+                    results[i] = dict(state=(k[o], buf.re[:cnt][o].copy(), buf.im[:cnt][o].copy()),
+                                      updates=info["updates"], nnz=info["nnz"])
+                self._free.append(buf)
+                wait = False                    # waited for the oldest; take the rest only if done
+
+        while nxt < n or live or pending:
             now = clock() - t0
             # 1. admission (FIFO, by HBM budget)
             while nxt < n and arrivals[nxt] <= now:
@@ -88,36 +141,35 @@ class ServeLoop:
                 admit_t[nxt] = clock() - t0
                 nxt += 1
             peak_live = max(peak_live, len(live))
-            if not live:
+            if live:
+                u = b.usage()
+                peak_bytes = max(peak_bytes, u["used"])
+                # 2. one step of every admitted circuit
+                b.step(self.step_gates)
+                steps += 1
+                # 3. finished circuits: start their exports, free their pool space
+                for cid in b.finished():
+                    i = live.pop(cid)
+                    buf = self._free.pop() if self._free else b.export_buffer()
+                    pending.append((b.export_start(cid, buf), i, buf))
+                    b.remove(cid)
+                peak_pending = max(peak_pending, len(pending))
+            # 4. exports whose copies are done
+            if pending:
+                idle = not live and (nxt >= n or arrivals[nxt] > clock() - t0)
+                collect(wait=idle)
+            elif not live and nxt < n:
                 wait = arrivals[nxt] - (clock() - t0)
                 if wait > 0:
                     sleep(min(wait, 0.05))
-                continue
-            u = b.usage()
-            peak_bytes = max(peak_bytes, u["used"])
-            # 2. one step of every admitted circuit
-            b.step(self.step_gates)
-            steps += 1
-            # 3. completions
-            for cid in b.finished():
-                i = live.pop(cid)
-                out = b.export(cid, sort=keep_results, out=pinned) if pinned is not None else \
-                    b.export(cid, sort=keep_results)
-                info = b.info(cid)
-                updates += info["updates"]
-                latency[i] = clock() - t0 - arrivals[i]
-                if keep_results:
-                    results[i] = dict(state=out, updates=info["updates"], nnz=info["nnz"])
-                b.remove(cid)
         wall = clock() - t0
-        if pinned is not None:
-            pinned.free()
         lat = [x for x in latency if x is not None]
         rep = dict(requests=n, wall_s=wall, steps=steps, step_gates=self.step_gates, prec=self.prec,
                    updates=updates, updates_per_s=updates / wall if wall > 0 else 0.0,
                    p50_latency_s=percentile(lat, 50), p99_latency_s=percentile(lat, 99),
                    mean_latency_s=float(np.mean(lat)) if lat else None,
-                   peak_live=peak_live, peak_pool_bytes=peak_bytes, latency_s=latency,
+                   peak_live=peak_live, peak_pool_bytes=peak_bytes, peak_exports_in_flight=peak_pending,
+                   latency_s=latency, errors=errors,
                    queue_wait_s=[(a - arrivals[i]) if a is not None else None for i, a in enumerate(admit_t)])
         if keep_results:
             rep["results"] = results

@@ -251,3 +251,103 @@ def test_export_into_pinned_arrays():
             assert_same_state(tuple(np.array(x) for x in got), want, atol=0.0, label="circuit %d" % i)
         assert pin.cap >= max(b.info(i)["nnz"] for i in ids)
         pin.free()
+
+
+def sorted_state(keys, re, im):
+    o = np.argsort(keys, kind="stable")
+    return keys[o], re[o], im[o]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_async_export_equals_export(oracle, prec):
+    """sqe_batch_export_start / _poll give what sqe_batch_export gives, with
+    the circuit removed right after the start and new circuits admitted into
+    its pool space (and stepped) before the copy is collected."""
+    cs = small_batch(41, 12)
+    extra = small_batch(42, 6)
+    kw = dict(prec=prec, renorm_every=100) if prec == "f32" else {}
+    with E.Batch(**kw) as b, E.Batch(**kw) as ref:
+        ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        rids = [ref.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        b.run()
+        ref.run()
+        bufs, tickets = [], []
+        for cid in ids:
+            buf = b.export_buffer()
+            tickets.append(b.export_start(cid, buf))
+            bufs.append(buf)
+            b.remove(cid)
+        more = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in extra]
+        b.step(16)
+        for k, (tk, buf) in enumerate(zip(tickets, bufs)):
+            n, info = b.export_poll(tk, wait=True)
+            got = sorted_state(buf.keys[:n].copy(), buf.re[:n].copy(), buf.im[:n].copy())
+            want = ref.export(rids[k])
+            assert_same_state(got, want, atol=0.0, label="circuit %d" % k)
+            rinfo = ref.info(rids[k])
+            assert info["updates"] == rinfo["updates"] and info["nnz"] == rinfo["nnz"]
+            assert info["done"] == 1 and info["gates_done"] == len(cs[k]["gates"])
+        b.run()
+        if prec == "f64":
+            for cid, c in zip(more, extra):
+                want, upd = oracle_of(oracle, c)
+                assert_same_state(b.export(cid), want, atol=1e-12, label="later circuit %d" % cid)
+                assert b.info(cid)["updates"] == upd
+        for buf in bufs:
+            buf.free()
+
+
+def test_async_export_argument_checks():
+    c = small_batch(43, 1)[0]
+    with E.Batch() as b:
+        cid = b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"])
+        b.run()
+        L = E.lib()
+        t = E.ctypes.c_int64()
+        bound = E.ctypes.c_uint64()
+        small = E.PinnedArrays(1024)
+        rc = L.sqe_batch_export_start(b.h, cid, E._ptr(small.keys, E._u64p), E._ptr(small.re, E._f64p),
+                                      E._ptr(small.im, E._f64p), 1, E.ctypes.byref(t), E.ctypes.byref(bound))
+        assert rc == E.SQE_EINVAL and bound.value >= c["keys"].size
+        # pageable numpy arrays are refused (the copy runs as a kernel)
+        cap = bound.value
+        k, r, i = np.zeros(cap, np.uint64), np.zeros(cap), np.zeros(cap)
+        rc = L.sqe_batch_export_start(b.h, cid, E._ptr(k, E._u64p), E._ptr(r, E._f64p), E._ptr(i, E._f64p), cap,
+                                      E.ctypes.byref(t), E.ctypes.byref(bound))
+        assert rc == E.SQE_EINVAL
+        assert b"page-locked" in L.sqe_batch_last_error()
+        with pytest.raises(E.SqeError):
+            b.export_poll(12345)
+        small.free()
+        assert_same_state(b.export(cid), b.export(cid), atol=0.0)
+
+
+def test_duplicate_key_reported_for_its_circuit(oracle):
+    """Admission does not wait for the device: a duplicate key in one
+    circuit's state is reported by info / export / export_poll for that
+    circuit, and the circuits around it are unaffected."""
+    cs = small_batch(44, 3)
+    bad = dict(cs[1])
+    bad["keys"] = bad["keys"].copy()
+    bad["keys"][-1] = bad["keys"][0]
+    with E.Batch() as b:
+        ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in (cs[0], bad, cs[2])]
+        b.run()
+        for which in (0, 2):
+            want, upd = oracle_of(oracle, cs[which])
+            assert_same_state(b.export(ids[which]), want, atol=1e-12)
+        with pytest.raises(E.SqeError) as ex:
+            b.info(ids[1])
+        assert ex.value.status == E.SQE_EINVAL
+        with pytest.raises(E.SqeError) as ex:
+            b.export(ids[1])
+        assert ex.value.status == E.SQE_EINVAL
+        buf = b.export_buffer()
+        tk = b.export_start(ids[1], buf)
+        with pytest.raises(E.SqeError) as ex:
+            b.export_poll(tk, wait=True)
+        assert ex.value.status == E.SQE_EINVAL
+        with pytest.raises(E.SqeError):
+            b.export_poll(tk)                    # the ticket was retired
+        buf.free()
+        b.sync()

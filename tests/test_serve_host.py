@@ -49,12 +49,33 @@ class ModelBatch:
         v = self.c[cid]
         return (np.arange(v["nnz"], dtype=np.uint64), np.zeros(v["nnz"]), np.zeros(v["nnz"]))
 
+    def export_buffer(self, cap=0):
+        return Buffer()
+
+    def export_start(self, cid, out):
+        """The export completes at the next poll (no device in the model)."""
+        v = self.c[cid]
+        keys, re, im = self.export(cid)
+        out.keys, out.re, out.im = keys, re, im
+        self.next_ticket = getattr(self, "next_ticket", 0) + 1
+        self.tickets = getattr(self, "tickets", {})
+        self.tickets[self.next_ticket] = (v["nnz"], {"updates": v["n"] * v["nnz"], "nnz": v["nnz"]})
+        self.log.append(("export", cid, self.clock.t))
+        return self.next_ticket
+
+    def export_poll(self, ticket, wait=False):
+        return self.tickets.pop(ticket)
+
     def info(self, cid):
         v = self.c[cid]
         return {"done": int(v["left"] == 0), "updates": v["n"] * v["nnz"], "nnz": v["nnz"]}
 
     def remove(self, cid):
         self.used -= self.c.pop(cid)["bytes"]
+
+
+class Buffer:
+    keys = re = im = None
 
 
 class Clock:
@@ -120,3 +141,33 @@ def test_request_larger_than_budget_is_an_error():
     s = ServeLoop(batch=mb, step_gates=10)
     with pytest.raises(E.SqeError):
         s.run([req(5, 10)], [0.0], clock=clk, sleep=clk.sleep)
+
+
+def test_failed_export_is_reported_and_the_rest_complete():
+    """An export the engine rejects (a duplicate key in that request's state)
+    is reported in the report's errors; the other requests are served."""
+    clk = Clock()
+    mb = ModelBatch(budget=10**9, clock=clk)
+    poll = mb.export_poll
+    bad = {}
+
+    def failing_poll(ticket, wait=False):
+        if ticket in bad:
+            mb.tickets.pop(ticket)
+            raise E.SqeError(E.SQE_EINVAL, "duplicate key in the circuit's load")
+        return poll(ticket, wait)
+
+    start = mb.export_start
+
+    def marking_start(cid, out):
+        t = start(cid, out)
+        if cid == 1:
+            bad[t] = True
+        return t
+
+    mb.export_poll, mb.export_start = failing_poll, marking_start
+    s = ServeLoop(batch=mb, step_gates=10)
+    rep = s.run([req(4, 10), req(4, 10), req(4, 20)], [0.0, 0.0, 0.0], clock=clk, sleep=clk.sleep)
+    assert list(rep["errors"]) == [1]
+    assert all(x is not None for x in rep["latency_s"])
+    assert rep["updates"] == 10 * 4 + 20 * 4

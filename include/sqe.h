@@ -91,13 +91,14 @@ typedef struct sqe_config {
                                    hipGraph (planned once per set of circuits)    */
     uint32_t renorm_mode;       /* FP32 with renorm_every: 0 stops at every
                                    renormalisation point (one launch per period);
-                                   1 applies a queued run of bucket-local gates in one
-                                   pass that takes the norm at each renormalisation
-                                   point and applies the scale afterwards, checked: the
-                                   run is redone step by step when a prune decision lay
-                                   within 2^-10 of the threshold or a norm moved by
-                                   more than 2^-11; 2 as 1 with the check failing
-                                   (tests the rerun)                                   */
+                                   1 (default) applies a queued run of bucket-local
+                                   gates in one pass that takes the FP64 norm at each
+                                   renormalisation point and applies the scale
+                                   afterwards (prune decisions then lack the pending
+                                   scale, within 2^-11 of 1), checked: the run is
+                                   redone step by step when a norm moved by more than
+                                   2^-11; 2 as 1 with the check failing (tests the
+                                   rerun)                                              */
     uint32_t frame_off;         /* 1: Pauli gates on bucket qubits update the amplitude
                                    registers instead of the per-bucket frame           */
     uint32_t reserved[2];
@@ -253,6 +254,9 @@ typedef struct sqe_batch_tuning {
     double sweep_ms;            /* summed batched sweep time (timing = 1), or the
                                    best trial time from sqe_batch_autotune             */
     uint64_t launches;
+    double load_ms;             /* sqe_batch_autotune: index part of the load path
+                                   (clear, insert, scan numbering of every admitted
+                                   circuit's buckets) at the chosen load factor        */
 } sqe_batch_tuning;
 
 /* cfg->nq is ignored (each circuit has its own); prec, prune_thr, max_load,
@@ -319,7 +323,10 @@ sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t);
 
 /* Time each (workgroup size, tiles per item) candidate on the
  * admitted circuits (their state is restored after every trial) and keep the
- * fastest; *best (optional) receives it. */
+ * fastest; then time the index part of the load path (table clear, insert of
+ * every admitted circuit's bucket bases, scan numbering into scratch) at each
+ * candidate load factor (0.25 ... 0.75) and keep the fastest for later
+ * admissions; *best (optional) receives the choice. */
 sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best);
 
 /* Keep a device-side copy of every admitted circuit's state, position and

@@ -36,8 +36,20 @@ SOURCES = ["sqe_kernels_f64.hip", "sqe_kernels_f32.hip", "sqe_kernels_common.hip
            "sqe_batch.cpp", "sqe_noalloc.hip"]
 
 
-def _compile(src):
+def _stale(src, obj):
+    """The object is older than its source or any header it may include."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
+    deps += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
+    if incremental and not _stale(src, obj):
+        return obj
     cmd = [HIPCC, "--offload-arch=" + ARCH] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -45,10 +57,12 @@ def _compile(src):
     return obj
 
 
-def build(verbose=False):
+def build(verbose=False, incremental=False):
+    """Compile every translation unit (incremental: only those whose object is
+    older than a source or header) and link libsqe.so."""
     os.makedirs(OUT_DIR, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda s: _compile(s, incremental), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -97,7 +111,7 @@ def build_encode_check(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     build_circuit_check(verbose=True)
     build_encode_check(verbose=True)
     sys.exit(0)

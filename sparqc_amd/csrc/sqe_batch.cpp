@@ -13,12 +13,18 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
 #include <deque>
+#include <functional>
+#include <mutex>
 #include <new>
+#include <thread>
 #include <stdexcept>
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -170,6 +176,68 @@ struct Pool {
 
 }  // namespace
 
+// Helper threads for the host copies of large admissions (copying a
+// request's keys and amplitudes into the page-locked upload ring is the bulk of
+// an admission's host time).  run(parts, f) calls f(0) on the calling thread
+// and f(1) .. f(parts - 1) on the helpers, and returns when all are done.
+class CopyPool {
+public:
+    static constexpr int HELPERS = 3;
+    CopyPool() {
+        for (int i = 0; i < HELPERS; ++i) th_.emplace_back([this, i] { work(i + 1); });
+    }
+    ~CopyPool() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    void run(int parts, const std::function<void(int)>& f) {
+        parts = std::max(1, std::min(parts, HELPERS + 1));
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            job_ = &f;
+            parts_ = parts;
+            left_.store(parts - 1, std::memory_order_relaxed);
+            ++gen_;
+        }
+        if (parts > 1) cv_.notify_all();
+        f(0);
+        while (left_.load(std::memory_order_acquire) > 0) std::this_thread::yield();
+    }
+
+private:
+    void work(int part) {
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(int)>* f = nullptr;
+            int parts = 0;
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                f = job_;
+                parts = parts_;
+            }
+            if (part < parts) {
+                (*f)(part);
+                left_.fetch_sub(1, std::memory_order_release);
+            }
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    const std::function<void(int)>* job_ = nullptr;
+    int parts_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+    std::atomic<int> left_{0};
+};
+
 struct sqe_batch {
     sqe_config cfg;
     bool f32 = false;
@@ -246,6 +314,8 @@ struct sqe_batch {
     // tuning and counters
     uint32_t block = BLOCK, gran = 1;
     double sweep_ms = 0.0;
+    double load_ms = 0.0;                   // index part of the load path, last autotune
+    std::unique_ptr<CopyPool> copy_pool;     // helper threads for large admission copies (made on first use)
     uint64_t launches = 0, steps = 0, bytes_dev = 0;
 };
 
@@ -484,23 +554,86 @@ sqe_status device_view(void* p, void** out) {
     return bfail(SQE_EINVAL, "export arrays must be page-locked (sqe_host_alloc) or device memory");
 }
 
-// FP64 key / re / im staging arrays of at least `need` entries.
+// FP64 key / re / im staging arrays of at least `need` entries: one
+// allocation of three arrays of xcap entries (exports use them at stride
+// xcap; an admission packs its m entries at stride m, so one copy uploads them).
 sqe_status stage_fit(sqe_batch* b, uint64_t need) {
     if (need <= b->xcap) return SQE_OK;
     BHIP(hipStreamSynchronize(b->st));
-    dfr(b, b->xkeys, b->xcap * 8);
-    dfr(b, b->xre, b->xcap * 8);
-    dfr(b, b->xim, b->xcap * 8);
+    dfr(b, b->xkeys, b->xcap * 24);
     b->xkeys = nullptr;
     b->xre = nullptr;
     b->xim = nullptr;
     b->xcap = 0;
     const uint64_t nc = std::max<uint64_t>(1024, p2(need));
-    BCK(dal(b, (void**)&b->xkeys, nc * 8));
-    BCK(dal(b, (void**)&b->xre, nc * 8));
-    BCK(dal(b, (void**)&b->xim, nc * 8));
+    BCK(dal(b, (void**)&b->xkeys, nc * 24));
+    b->xre = (double*)(b->xkeys + nc);
+    b->xim = b->xre + nc;
     b->xcap = nc;
     return SQE_OK;
+}
+
+// Copy the n input entries to k / r / i on `parts` threads when every
+// amplitude is nonzero (each part copies its range in place and checks it);
+// otherwise, or for small inputs, compact on the calling thread.  Returns
+// the count of entries kept and ORs every input key into *keybits.
+uint64_t compact_entries(const uint64_t*, const double*, const double*, uint64_t, uint64_t*, double*, double*,
+                         uint64_t*);
+uint64_t copy_entries(sqe_batch* b, const uint64_t* keys, const double* re, const double* im, uint64_t n,
+                      uint64_t* k, double* r, double* i, uint64_t* keybits) {
+    constexpr uint64_t PAR_MIN = 1 << 15;          // entries: smaller copies stay on one thread
+    if (n < PAR_MIN) return compact_entries(keys, re, im, n, k, r, i, keybits);
+    if (!b->copy_pool) b->copy_pool.reset(new CopyPool());
+    const int parts = CopyPool::HELPERS + 1;
+    uint64_t bits[parts] = {}, zeros[parts] = {};
+    b->copy_pool->run(parts, [&](int p) {
+        const uint64_t j0 = n * (uint64_t)p / parts, j1 = n * (uint64_t)(p + 1) / parts;
+        std::memcpy(k + j0, keys + j0, (j1 - j0) * 8);
+        std::memcpy(r + j0, re + j0, (j1 - j0) * 8);
+        std::memcpy(i + j0, im + j0, (j1 - j0) * 8);
+        uint64_t acc = 0, z = 0;
+        for (uint64_t j = j0; j < j1; ++j) {
+            acc |= keys[j];
+            z += (re[j] == 0.0 && im[j] == 0.0) ? 1 : 0;
+        }
+        bits[p] = acc;
+        zeros[p] = z;
+    });
+    uint64_t acc = 0, z = 0;
+    for (int p = 0; p < parts; ++p) { acc |= bits[p]; z += zeros[p]; }
+    if (z) return compact_entries(keys, re, im, n, k, r, i, keybits);
+    *keybits = acc;
+    return n;
+}
+
+// Copy the entries with a nonzero amplitude (re != 0 or im != 0) of the n
+// input entries, in input order, to k / r / i; returns their count and ORs
+// every input key into *keybits.  Blocks of 64 entries whose amplitudes are
+// all nonzero (the usual case) are copied with memcpy.
+uint64_t compact_entries(const uint64_t* keys, const double* re, const double* im, uint64_t n, uint64_t* k,
+                         double* r, double* i, uint64_t* keybits) {
+    uint64_t q = 0, acc = 0;
+    for (uint64_t j0 = 0; j0 < n; j0 += 64) {
+        const uint64_t e = std::min<uint64_t>(n - j0, 64);
+        uint64_t nz = 0;
+        for (uint64_t j = 0; j < e; ++j) {
+            acc |= keys[j0 + j];
+            nz += (re[j0 + j] != 0.0 || im[j0 + j] != 0.0) ? 1 : 0;
+        }
+        if (nz == e) {
+            std::memcpy(k + q, keys + j0, e * 8);
+            std::memcpy(r + q, re + j0, e * 8);
+            std::memcpy(i + q, im + j0, e * 8);
+            q += e;
+        } else {
+            for (uint64_t j = j0; j < j0 + e; ++j) {
+                if (re[j] == 0.0 && im[j] == 0.0) continue;
+                k[q] = keys[j]; r[q] = re[j]; i[q] = im[j]; ++q;
+            }
+        }
+    }
+    *keybits = acc;
+    return q;
 }
 
 char* amp_ptr(sqe_batch* b, const Circ& c) { return b->amp.p + c.row0 * 64 * b->csize; }
@@ -904,7 +1037,7 @@ sqe_status sqe_batch_destroy(sqe_batch* b) {
         if (b->xst) (void)hipStreamDestroy(b->xst);
         for (void* p : {(void*)b->cst, (void*)b->base.p, (void*)b->amp.p, (void*)b->prog.p, (void*)b->coef.p, (void*)b->upd,
                         (void*)b->nnz, (void*)b->meta, (void*)b->norm2, (void*)b->scale, (void*)b->tab,
-                        (void*)b->xkeys, (void*)b->xre, (void*)b->xim, (void*)b->occ, (void*)b->stats,
+                        (void*)b->xkeys, (void*)b->occ, (void*)b->stats,
                         (void*)b->scan_status, (void*)b->ticket, (void*)b->total, (void*)b->dtask, (void*)b->ditem, (void*)b->partials,
                         (void*)b->counts, (void*)b->snap_base, (void*)b->snap_amp, (void*)b->snap_cnt, (void*)b->gtask,
                         (void*)b->gitem, (void*)b->gcounts, (void*)b->gpartials})
@@ -941,11 +1074,6 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
             if (!gate_local(gates[j], qm))
                 return bfail(SQE_EINVAL, "gate " + std::to_string(j) +
                                              " crosses buckets: run this circuit with the single-circuit engine");
-        uint64_t m = 0;                          // entries with a nonzero amplitude
-        for (uint64_t j = 0; j < n; ++j) {
-            if (keys[j] & ~lim) return bfail(SQE_EINVAL, "key has bits beyond nq");
-            m += (re[j] != 0.0 || im[j] != 0.0) ? 1 : 0;
-        }
         // program: gates, with a RENORM op after every renorm_every-th gate
         // (Pauli gates on bucket qubits as frame records when the keys leave
         // room for the frame)
@@ -969,14 +1097,39 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         for (size_t r : ren_idx) renorm_at.push_back((uint32_t)enc.start[r]);
         for (size_t g : gate_idx) gate_at.push_back((uint32_t)(enc.start[g + 1] - 1));   // the gate's own record
         const bool framed = enc.framed;
+        // the uploads go through the page-locked upload ring: program,
+        // coefficients, then the keys, real and imaginary parts of the entries
+        // with a nonzero amplitude, packed (one pass over the input: key range
+        // check, zero filter and copy); the device staging they land in is
+        // rewritten in stream order
+        const uint64_t pc = prog.size() * sizeof(MOp) + coef.size() * 8;
+        uint64_t up_off = 0;
+        BCK(ring_take(b, b->up, pc + 24 * n, UP_RING_MIN, &up_off));
+        char* const hp0 = b->up.p + up_off;
+        uint64_t* hk = (uint64_t*)(hp0 + pc);
+        uint64_t keybits = 0;
+        const uint64_t m = copy_entries(b, keys, re, im, n, hk, (double*)(hk + n), (double*)(hk + 2 * n), &keybits);
+        double* hr = (double*)(hk + m);
+        double* hi = hr + m;
+        if (m < n) {                             // pack at stride m
+            std::memmove(hr, hk + n, m * 8);
+            std::memmove(hi, hk + 2 * n, m * 8);
+        }
+        auto unreserve = [&]() { b->up.head = up_off; };
+        if (keybits & ~lim) {
+            unreserve();
+            return bfail(SQE_EINVAL, "key has bits beyond nq");
+        }
         // region sizes and admission
         const uint64_t S = 1ull << qm.mb;
         const uint64_t tiles = std::max<uint64_t>(1, (m + 63) / 64);
         const uint64_t rows = tiles * S;
         const uint64_t bytes = tiles * 64 * 8 + rows * 64 * b->csize + prog.size() * sizeof(MOp) + coef.size() * 8;
-        if (b->used + bytes > b->cfg.hbm_budget_bytes)
+        if (b->used + bytes > b->cfg.hbm_budget_bytes) {
+            unreserve();
             return bfail(SQE_ETABLE_FULL, "admission: circuit needs " + std::to_string(bytes) + " bytes, " +
                                               std::to_string(b->cfg.hbm_budget_bytes - b->used) + " left in the budget");
+        }
         uint32_t id;
         if (!b->free_slots.empty()) { id = b->free_slots.back(); b->free_slots.pop_back(); }
         else { id = (uint32_t)b->c.size(); b->c.emplace_back(); }
@@ -1013,6 +1166,7 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
             if (placed > 1) b->ra.give(c.row0, c.rows);
             if (placed > 2 && c.nops) b->pa.give(c.prog0, c.nops);
             b->free_slots.push_back(id);
+            unreserve();
             return st;
         }
         c.live = true;
@@ -1021,14 +1175,10 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         ++b->generation;
         ++b->layout;
         Circ& cc = b->c[id];
-        // the uploads go through the page-locked upload ring: program,
-        // coefficients, then the nonzero entries' keys, real and imaginary
-        // parts; the device staging they land in is rewritten in stream order
         BCK(stage_fit(b, std::max<uint64_t>(m, 1)));
-        const uint64_t up = cc.nops * sizeof(MOp) + cc.ncoef * 8 + 24 * m;
-        uint64_t up_off = 0;
-        BCK(ring_take(b, b->up, up, UP_RING_MIN, &up_off));
-        char* hp = b->up.p + up_off;
+        const uint64_t up = pc + 24 * m;
+        b->up.head = up_off + ring_round(up);
+        char* hp = hp0;
         if (cc.nops) {
             std::memcpy(hp, prog.data(), cc.nops * sizeof(MOp));
             BHIP(hipMemcpyAsync(b->prog.p + cc.prog0 * sizeof(MOp), hp, cc.nops * sizeof(MOp), hipMemcpyHostToDevice,
@@ -1038,21 +1188,8 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         if (cc.ncoef) {
             std::memcpy(hp, coef.data(), cc.ncoef * 8);
             BHIP(hipMemcpyAsync(b->coef.p + cc.coef0 * 8, hp, cc.ncoef * 8, hipMemcpyHostToDevice, b->st));
-            hp += cc.ncoef * 8;
         }
-        if (m) {
-            uint64_t* hk = (uint64_t*)hp;
-            double* hr = (double*)(hk + m);
-            double* hi = hr + m;
-            uint64_t q = 0;
-            for (uint64_t j = 0; j < n; ++j) {
-                if (re[j] == 0.0 && im[j] == 0.0) continue;
-                hk[q] = keys[j]; hr[q] = re[j]; hi[q] = im[j]; ++q;
-            }
-            BHIP(hipMemcpyAsync(b->xkeys, hk, m * 8, hipMemcpyHostToDevice, b->st));
-            BHIP(hipMemcpyAsync(b->xre, hr, m * 8, hipMemcpyHostToDevice, b->st));
-            BHIP(hipMemcpyAsync(b->xim, hi, m * 8, hipMemcpyHostToDevice, b->st));
-        }
+        if (m) BHIP(hipMemcpyAsync(b->xkeys, hk, m * 24, hipMemcpyHostToDevice, b->st));
         BCK(ring_mark(b->up, up_off, up, b->st));
         // load the state into the region (insert bases, number buckets, scatter)
         const double ml = b->cfg.max_load > 0 && b->cfg.max_load < 0.95 ? b->cfg.max_load : 0.5;
@@ -1082,7 +1219,7 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         // and is reported by sqe_batch_info / sqe_batch_export for it, so
         // admission never waits for the device
         LoadArgs la{};
-        la.keys = b->xkeys; la.re = b->xre; la.im = b->xim; la.n = m; la.qm = qm;
+        la.keys = b->xkeys; la.re = (const double*)(b->xkeys + m); la.im = la.re + m; la.n = m; la.qm = qm;
         la.tab = b->tab; la.tmask = tneed - 1; la.amp = amp_ptr(b, cc); la.occ = b->occ; la.stats = b->cst + id;
         EnumArgs ea{};
         ea.tab = b->tab; ea.tcap = tneed; ea.out_base = base_ptr(b, cc); ea.out_nb = b->meta + id;
@@ -1247,20 +1384,26 @@ sqe_status sqe_batch_export_start(sqe_batch* b, int64_t id, uint64_t* keys, doub
         const uint32_t g = std::max<uint32_t>(1, (uint32_t)((nbk + BLOCK - 1) / BLOCK));
         BCK(next_scan(b, g, &xa.scan));
         BHIP(launch_export(c.qm.mb, b->f32, b->st, g, xa));
-        // the copy to the caller runs on the export stream, off the step stream
+        // the copy to the caller runs on the export stream, off the step
+        // stream: the entries by DMA (one pitched copy of the three arrays when
+        // the caller's arrays are equally spaced, as sqe_host_alloc buffers
+        // split in three are; all `slots` staging entries, of which the
+        // header's count are valid), then the header into the ticket's record
         BHIP(hipEventRecord(b->xev, b->st));
         BHIP(hipStreamWaitEvent(b->xst, b->xev, 0));
         const uint32_t rec = b->rec_free.back();
-        CopyOutArgs ca{};
-        ca.hdr = hdr; ca.keys = sk; ca.re = sr; ca.im = si;
-        ca.okeys = (uint64_t*)dv[0]; ca.ore = (double*)dv[1]; ca.oim = (double*)dv[2];
-        ca.cap = cap; ca.rec = b->drec + (uint64_t)rec * REC_WORDS;
-        bool al = true;
-        for (const void* q : {(const void*)sk, (const void*)sr, (const void*)si, (const void*)dv[0], (const void*)dv[1], (const void*)dv[2]})
-            al &= ((uintptr_t)q & 15) == 0;
-        ca.vec = al ? 1 : 0;
-        const uint32_t cg = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(1, (slots / 2 + BLOCK - 1) / BLOCK));
-        BHIP(launch_copy_out(b->xst, cg, ca));
+        const uint64_t wbytes = slots * 8;
+        const char* k0 = (const char*)keys;
+        const ptrdiff_t pitch = (const char*)re - k0;
+        if (pitch > 0 && (const char*)im - (const char*)re == pitch && (uint64_t)pitch >= wbytes) {
+            BHIP(hipMemcpy2DAsync(keys, (size_t)pitch, sk, (size_t)arr, wbytes, 3, hipMemcpyDeviceToHost, b->xst));
+        } else {
+            BHIP(hipMemcpyAsync(keys, sk, wbytes, hipMemcpyDeviceToHost, b->xst));
+            BHIP(hipMemcpyAsync(re, sr, wbytes, hipMemcpyDeviceToHost, b->xst));
+            BHIP(hipMemcpyAsync(im, si, wbytes, hipMemcpyDeviceToHost, b->xst));
+        }
+        BHIP(hipMemcpyAsync(b->hrec + (uint64_t)rec * REC_WORDS, hdr, (SNAP_WORDS + 1) * 8, hipMemcpyDeviceToHost,
+                            b->xst));
         BCK(ring_mark(b->xr, off, bytes, b->xst));
         hipEvent_t ev = nullptr;
         if (!b->evpool.empty()) { ev = b->evpool.back(); b->evpool.pop_back(); }
@@ -1351,6 +1494,78 @@ sqe_status sqe_batch_usage(sqe_batch* b, uint64_t* used, uint64_t* budget, uint6
     });
 }
 
+// Admission builds a hash index of the circuit's bucket bases sized by the
+// load factor: a lower factor shortens probe sequences but makes the table
+// (which the scan numbering reads whole) larger.  Time the index part of the
+// load path -- table clear, cooperative insert of the bases, scan numbering --
+// over every admitted circuit at each candidate factor, on the current
+// bucket bases (one per bucket, as admission inserts them), with the
+// numbering written to scratch so the circuits are untouched; keep the
+// fastest for later admissions.
+sqe_status tune_load_factor(sqe_batch* b) {
+    std::vector<uint32_t> ids;
+    uint64_t maxn = 0;
+    for (uint32_t id = 0; id < b->c.size(); ++id)
+        if (b->c[id].live) {
+            ids.push_back(id);
+            maxn = std::max<uint64_t>(maxn, b->c[id].tiles * 64);
+        }
+    if (ids.empty()) return SQE_OK;
+    const double cands[] = {0.25, 0.375, 0.5, 0.625, 0.75};
+    auto tsize = [](uint64_t n, double lf) {
+        return std::max<uint64_t>(1024, p2((uint64_t)std::ceil((double)n / lf) + 8));
+    };
+    const uint64_t tmax = tsize(maxn, cands[0]);
+    if (tmax > b->tcap) {
+        BHIP(hipStreamSynchronize(b->st));
+        dfr(b, b->tab, b->tcap * sizeof(Slot));
+        b->tab = nullptr;
+        BCK(dal(b, (void**)&b->tab, tmax * sizeof(Slot)));
+        b->tcap = tmax;
+    }
+    // scratch: numbered bases, their count and an error word
+    char* scratch = nullptr;
+    const uint64_t sbytes = (maxn + 64) * 8 + 8 + sizeof(Stats);
+    BCK(dal(b, (void**)&scratch, sbytes));
+    uint64_t* out_base = (uint64_t*)scratch;
+    unsigned long long* out_nb = (unsigned long long*)(scratch + (maxn + 64) * 8);
+    Stats* st = (Stats*)(out_nb + 1);
+    sqe_status rc = SQE_OK;
+    double best = 1e300, best_lf = b->cfg.max_load > 0 && b->cfg.max_load < 0.95 ? b->cfg.max_load : 0.5;
+    for (int rep = 0; rep < 2 && rc == SQE_OK; ++rep) {      // the first pass warms up
+        for (double lf : cands) {
+            hipError_t he = hipEventRecord(b->ev0, b->st);
+            for (uint32_t id : ids) {
+                if (he != hipSuccess || rc != SQE_OK) break;
+                const Circ& c = b->c[id];
+                const uint64_t tn = tsize(c.tiles * 64, lf);
+                he = launch_fill_slots(b->st, b->tab, tn);
+                IndexArgs ia{base_ptr(b, c), b->meta + id, b->tab, tn - 1, st};
+                if (he == hipSuccess) he = launch_index_build(b->st, (uint32_t)((c.tiles * 64 + BLOCK - 1) / BLOCK), ia);
+                EnumArgs ea{};
+                ea.tab = b->tab; ea.tcap = tn; ea.out_base = out_base; ea.out_nb = out_nb;
+                ea.out_cap = maxn + 64; ea.stats = st;
+                const uint32_t eg = (uint32_t)((tn + BLOCK - 1) / BLOCK);
+                rc = next_scan(b, eg, &ea.scan);
+                if (rc == SQE_OK && he == hipSuccess) he = launch_index_enumerate(b->st, eg, ea);
+            }
+            if (he == hipSuccess) he = hipEventRecord(b->ev1, b->st);
+            if (he == hipSuccess) he = hipEventSynchronize(b->ev1);
+            float ms = 0.f;
+            if (he == hipSuccess) he = hipEventElapsedTime(&ms, b->ev0, b->ev1);
+            if (he != hipSuccess) rc = bfail(SQE_EHIP, std::string("load factor tuning: ") + hipGetErrorString(he));
+            if (rc != SQE_OK) break;
+            if (rep == 1 && ms < best) { best = ms; best_lf = lf; }
+        }
+    }
+    (void)hipStreamSynchronize(b->st);
+    dfr(b, scratch, sbytes);
+    if (rc != SQE_OK) return rc;
+    b->cfg.max_load = best_lf;
+    b->load_ms = best;
+    return SQE_OK;
+}
+
 sqe_status sqe_batch_get_tuning(sqe_batch* b, sqe_batch_tuning* t) {
     return guard([&]() -> sqe_status {
         if (!b || !t) return bfail(SQE_EINVAL, "null argument");
@@ -1359,6 +1574,7 @@ sqe_status sqe_batch_get_tuning(sqe_batch* b, sqe_batch_tuning* t) {
         t->load_factor = b->cfg.max_load > 0 ? b->cfg.max_load : 0.5;
         t->sweep_ms = b->sweep_ms;
         t->launches = b->launches;
+        t->load_ms = b->load_ms;
         return SQE_OK;
     });
 }
@@ -1414,6 +1630,7 @@ sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
         b->cfg.timing = timing;
         b->block = bc.block;
         b->gran = bc.gran;
+        BCK(tune_load_factor(b));
         if (best_out) {
             BCK(sqe_batch_get_tuning(b, best_out));
             best_out->sweep_ms = best;

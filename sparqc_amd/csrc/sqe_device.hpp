@@ -318,20 +318,6 @@ __device__ __forceinline__ float opaque_uniform(float v) {
 __device__ __forceinline__ Band32 band32(double thr) {
     return Band32{opaque_uniform((float)(0.25 * thr)), opaque_uniform((float)(4.0 * thr))};
 }
-// Lazy-renormalisation check of the FP32 form: a prune decision taken on an
-// FP64 value within 2^-10 (relative) of the threshold could flip under the
-// renormalisation scale a one-pass run leaves pending, so such decisions are
-// flagged (a workgroup-wide flag the sweep kernel reports; other kernels
-// ignore it).
-__device__ __forceinline__ uint32_t* near_threshold_flag() {
-    __shared__ uint32_t flag;
-    return &flag;
-}
-__device__ __forceinline__ void note_near_threshold(cd v, double thr) {
-    const double m = fmax(fabs(v.x), fabs(v.y));
-    if (m >= thr * (1.0 - 0x1p-10) && m < thr * (1.0 + 0x1p-10)) *near_threshold_flag() = 1u;
-}
-
 template <uint32_t K>
 __device__ __forceinline__ void branch_pair(const DOp& op, double thr, cd v0, cd v1, cd& n0, cd& n1) {
     pair64<K>(op.p, v0, v1, n0, n1);
@@ -353,8 +339,6 @@ __device__ __forceinline__ void branch_pair(const DOp& op, double thr, cf v0, cf
     if (__builtin_amdgcn_ballot_w64(near0 || near1) != 0) {
         cd w0 = mkd(v0.x, v0.y), w1 = mkd(v1.x, v1.y), d0, d1;
         pair64<K>(op.p, w0, w1, d0, d1);
-        if (near0) note_near_threshold(d0, thr);
-        if (near1) note_near_threshold(d1, thr);
         k0 = k0 || (near0 && !negl64(d0, thr));
         k1 = k1 || (near1 && !negl64(d1, thr));
     }

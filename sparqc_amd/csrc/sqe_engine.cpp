@@ -160,7 +160,7 @@ struct sqe_engine {
     bool renorm_pending = false;
     bool prog_frames = false;               // the uploaded program holds frame records
     LazyRun lz;                             // last one-pass renormalised run, until its verdict is read
-    unsigned long long* lz_dev = nullptr;   // its verdict flag, near-threshold count and Stats backup
+    unsigned long long* lz_dev = nullptr;   // its verdict flag (word 0) and Stats backup (words 2-18)
     double* ckpt = nullptr;                 // norm-point partials
     uint64_t ckpt_cap = 0;
     uint64_t n_reruns = 0;                  // one-pass runs rerun step by step
@@ -554,7 +554,6 @@ struct SweepOpt {
     int out_store = -1;                 // >= 0: write the result to this store (out of place)
     double* ckpt = nullptr;             // norm-point partials, ckpt_stride per checkpoint
     uint32_t* ckpt_stride = nullptr;    // receives the wavefront count of the grid
-    unsigned long long* ambig = nullptr;
     bool index = false;                 // insert every stored bucket into the (cleared) hash index
 };
 
@@ -576,7 +575,6 @@ sqe_status run_sweep(sqe_engine* e, const MOp* mops, const double* coef, uint64_
     a.ckpt = opt.ckpt;
     a.ckpt_stride = g * (BLOCK / 64);
     if (opt.ckpt_stride) *opt.ckpt_stride = a.ckpt_stride;
-    a.ambig = opt.ambig;
     if (opt.index) {
         a.itab = e->tab;
         a.itmask = e->tcap - 1;
@@ -852,22 +850,23 @@ sqe_status flush_lazy(sqe_engine* e, const std::vector<sqe_gate>& gl) {
     const MOp* dm = nullptr;
     const double* dc = nullptr;
     SQCK(upload_program(e, enc, &dm, &dc));
-    // device words: [0] verdict flag, [1] near-threshold count, then the Stats backup
+    // device words: [0] verdict flag, [2..] the Stats backup
     if (!e->lz_dev) SQCK(dalloc(e, (void**)&e->lz_dev, 256));
     SQCK(reduce_stats(e));
     HIPCK(hipMemcpyAsync(e->lz_dev + 2, e->stats, sizeof(Stats), hipMemcpyDeviceToDevice, e->st));
-    HIPCK(launch_fill_u64(e->st, (uint64_t*)e->lz_dev, 2, 0));
+    HIPCK(launch_fill_u64(e->st, (uint64_t*)e->lz_dev, 1, 0));
     const int out = e->cur ^ 1;
     HIPCK(hipMemcpyAsync(&e->meta[out], &e->meta[e->cur], 8, hipMemcpyDeviceToDevice, e->st));
     // checkpoint partials: one row of wavefronts per norm point
     const uint64_t ntiles = std::max<uint64_t>(1, round64(e->nb_upper) / 64);
     const uint64_t waves = (uint64_t)sweep_grid(e->qm.mb, e->f32, ntiles) * (BLOCK / 64);
-    if ((uint64_t)k * waves > e->ckpt_cap) {
+    // (rows of partials, then the rows' sums)
+    if ((uint64_t)k * (waves + 1) > e->ckpt_cap) {
         HIPCK(hipStreamSynchronize(e->st));
         dfree(e, e->ckpt, e->ckpt_cap * 8);
         e->ckpt = nullptr;
         e->ckpt_cap = 0;
-        const uint64_t cap = pow2_at_least((uint64_t)k * waves);
+        const uint64_t cap = pow2_at_least((uint64_t)k * (waves + 1));
         SQCK(dalloc(e, (void**)&e->ckpt, cap * 8));
         e->ckpt_cap = cap;
     }
@@ -885,7 +884,6 @@ sqe_status flush_lazy(sqe_engine* e, const std::vector<sqe_gate>& gl) {
         so.out_store = first ? out : -1;
         so.ckpt = e->ckpt;
         so.ckpt_stride = &stride;
-        so.ambig = e->lz_dev + 1;
         SQCK(run_sweep(e, dm + enc.start[off], dc, enc.start[off + n] - enc.start[off], 1, so));
         if (first) e->cur = out;
         first = false;
@@ -900,7 +898,7 @@ sqe_status flush_lazy(sqe_engine* e, const std::vector<sqe_gate>& gl) {
     }
     e->tab_valid = false;
     if (k) {
-        HIPCK(launch_ckpt_fold(e->st, e->ckpt, stride, k, e->lz_dev + 1, 0x1p-11, e->cfg.renorm_mode == 2 ? 1 : 0,
+        HIPCK(launch_ckpt_fold(e->st, e->ckpt, stride, k, 0x1p-11, e->cfg.renorm_mode == 2 ? 1 : 0,
                                &e->dscal[0], &e->dscal[1], e->lz_dev));
         SQCK(run_sweep(e, dm + enc.start[n_main], dc, enc.start[n_main + 1] - enc.start[n_main], 1));
         ++e->n_sweeps;
@@ -1025,6 +1023,7 @@ void sqe_config_default(sqe_config* c, uint32_t nq) {
     c->device = 0;
     c->bucket_bits = -1;
     c->renorm_every = 0;
+    c->renorm_mode = 1;
     c->fuse_window = 0;
     c->timing = 0;
 }

@@ -177,8 +177,6 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
             if (__builtin_amdgcn_ballot_w64(near0 || near1) != 0) {
                 cd e0, e1;
                 exact(mkd(x.x, x.y), mkd(y.x, y.y), e0, e1);
-                if (near0) note_near_threshold(e0, thr);
-                if (near1) note_near_threshold(e1, thr);
                 k0 = k0 || (near0 && !negl64(e0, thr));
                 k1 = k1 || (near1 && !negl64(e1, thr));
             }
@@ -190,6 +188,45 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
             n0 = k0 ? n0 : mkd(0.0, 0.0);
             n1 = k1 ? n1 : mkd(0.0, 0.0);
             kept += (k0 ? 1u : 0u) + (k1 ? 1u : 0u);
+        }
+    };
+    // FP64 branching gate without controls: apply it to every pair, keeping
+    // per lane the smallest max(|re|,|im|) of its outputs; when no lane of
+    // the wavefront has an output below the threshold (full buckets, the
+    // usual case once the branching qubits have mixed) nothing is pruned and
+    // every slot counts as nonzero, so the componentwise prune and count run
+    // only behind that wavefront-uniform test.  The outputs are the same
+    // either way.
+    auto unprune_then_prune = [&](auto&& gate) __attribute__((always_inline)) {
+        R mm[EPT];
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) { mm[decltype(ec)::value] = (R)__builtin_inf(); });
+        pairs([&](C x, C y, C& n0, C& n1, uint32_t&) __attribute__((always_inline)) {
+            gate(x, y, n0, n1);
+        }, std::true_type{});
+        bool small = false;
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+            constexpr int e = decltype(ec)::value;
+            static_for<(1 << MB)>([&](auto sc) __attribute__((always_inline)) {
+                constexpr int sl = decltype(sc)::value;
+                mm[e] = fmin(mm[e], fmax(fabs(a[e][sl].x), fabs(a[e][sl].y)));
+            });
+            small |= !(mm[e] >= (R)thr);
+        });
+        if (__builtin_amdgcn_ballot_w64(small) != 0) {
+            static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+                constexpr int e = decltype(ec)::value;
+                uint32_t kept = 0;
+                static_for<(1 << MB)>([&](auto sc) __attribute__((always_inline)) {
+                    constexpr int sl = decltype(sc)::value;
+                    // componentwise |re|, |im| < thr (the prune predicate of negl64)
+                    const bool k = !(fabs((double)a[e][sl].x) < thr && fabs((double)a[e][sl].y) < thr);
+                    a[e][sl] = k ? a[e][sl] : C{0, 0};
+                    kept += k ? 1u : 0u;
+                });
+                nzc[e] = kept;
+            });
+        } else {
+            static_for<EPT>([&](auto ec) __attribute__((always_inline)) { nzc[decltype(ec)::value] = 1u << MB; });
         }
     };
     switch (op.h) {
@@ -215,6 +252,10 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
     } break;
     case H_SLOT_H: {
         const R f = (R)op.c[0];
+        if constexpr (!F32 && !MASK) {
+            unprune_then_prune([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) { hpair(f, x, y, n0, n1); });
+            return true;
+        }
         static_for<EPT>([&](auto ec) __attribute__((always_inline)) { nzc[decltype(ec)::value] = 0; });
         pairs([&](C x, C y, C& n0, C& n1, uint32_t& k) __attribute__((always_inline)) {
             hpair(f, x, y, n0, n1);
@@ -226,6 +267,12 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
     }
     case H_SLOT_RY: {
         const R c = (R)op.c[0], sn = (R)op.c[1];
+        if constexpr (!F32 && !MASK) {
+            unprune_then_prune([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+                ry_pair(c, sn, x, y, n0, n1);
+            });
+            return true;
+        }
         static_for<EPT>([&](auto ec) __attribute__((always_inline)) { nzc[decltype(ec)::value] = 0; });
         pairs([&](C x, C y, C& n0, C& n1, uint32_t& k) __attribute__((always_inline)) {
             ry_pair(c, sn, x, y, n0, n1);
@@ -364,7 +411,6 @@ __device__ __forceinline__ void block_apply(Cplx<float> (&a)[16], float pr, floa
                 er += mr * xr - mi * xi;
                 ei += mr * xi + mi * xr;
             }
-            if (near) note_near_threshold(mkd(er, ei), thr);
             keep = keep || (near && !negl64(mkd(er, ei), thr));
         }
         a[s] = keep ? n : mkf(0.0f, 0.0f);
@@ -852,18 +898,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
         any |= t < ntiles;
     });
     SweepCount cnt{0, 0, 0, 0, 0.0};
-    if (A.ambig) {
-        if (threadIdx.x == 0) *near_threshold_flag() = 0u;
-        __syncthreads();
-    }
     const SweepCtx ctx{A.qm, A.frames, A.ckpt_stride, (uint32_t)g, A.out_base, A.out_amp, A.ckpt, A.itab, A.itmask,
                        &A.stats->err};
     if (any)
         sweep_group<MB, R, EPT, BLOCKS, true>(A.base, A.amp, tile, A.ops, A.nops, A.coef, A.thr, A.scale, A.store, cnt,
                                               ctx);
     block_stats(A.bstats, cnt.upd, cnt.nnz_out, cnt.empty, A.store ? cnt.nnz_in + cnt.nnz_out : 0ull);
-    // (block_stats ends with a workgroup barrier, after every wave's flag write)
-    if (A.ambig && threadIdx.x == 0 && *near_threshold_flag()) atomicAdd(A.ambig, 1ull);
     if (A.norm_out) {
         // the FP64 norm of the stored state, one partial per workgroup (for a
         // renormalisation right after this launch)

@@ -83,47 +83,55 @@ __global__ __launch_bounds__(BLOCK) void k_norm_final(const double* __restrict__
 }
 
 // One-pass renormalisation of a run of gates: checkpoint j's norm N_j is the
-// fixed-order sum of its per-wavefront partials ckpt[j * nw + w].  The run
-// took every prune decision after checkpoint j at scale 1 where the
-// step-by-step run scales by 1/sqrt(N_j) (the run's amplitudes carry the
-// product of the earlier scales, which N_j includes); it stands when each
-// such scale is within tol of 1 and no decision lay within 2^-10 of the
-// threshold (*ambig == 0).  Writes the norm and the scale 1/sqrt(N) of the
-// last checkpoint, which the run's result still needs, and the verdict
-// (*flag = 0: stands, 1: rerun step by step).
-__global__ __launch_bounds__(BLOCK) void k_ckpt_fold(const double* __restrict__ ckpt, uint32_t nw, uint32_t nk,
-                                                     const unsigned long long* __restrict__ ambig, double tol,
-                                                     int force_fail, double* __restrict__ norm2,
-                                                     double* __restrict__ scale, unsigned long long* __restrict__ flag) {
+// fixed-order sum of its per-wavefront partials ckpt[j * nw + w], one
+// workgroup per checkpoint (k_ckpt_rows).  The run took every prune decision
+// after checkpoint j on amplitudes that still lack the scale 1/sqrt(N_j) the
+// step-by-step run applies there (the run's amplitudes carry the product of
+// the earlier scales, which N_j includes).  The run stands when each such
+// scale is within tol of 1 (k_ckpt_fold): then a decision can differ from the
+// step-by-step run's only for a value within that relative distance of the
+// threshold (its contribution to the state is below the threshold), while a
+// larger scale would move decisions materially and the run is redone step by
+// step.  Writes the norm and the scale 1/sqrt(N) of the last checkpoint,
+// which the run's result still needs, and the verdict (*flag = 0: stands,
+// 1: rerun step by step).
+__global__ __launch_bounds__(BLOCK) void k_ckpt_rows(const double* __restrict__ ckpt, uint32_t nw,
+                                                     double* __restrict__ norms) {
     __shared__ double s_w[BLOCK / 64];
-    double last = 1.0;
-    bool bad = false;
-    for (uint32_t j = 0; j < nk; ++j) {
-        double acc = 0.0;
-        for (uint32_t i = threadIdx.x; i < nw; i += BLOCK) acc += ckpt[(size_t)j * nw + i];
-        for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
-        if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int i = 0; i < BLOCK / 64; ++i) t += s_w[i];
-            last = t;
-            bad = bad || !(t > 0.0) || !(fabs(1.0 / sqrt(t) - 1.0) < tol);
-        }
-        __syncthreads();
-    }
+    const uint32_t j = blockIdx.x;
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < nw; i += BLOCK) acc += ckpt[(size_t)j * nw + i];
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        *norm2 = last;
-        *scale = last > 0.0 ? 1.0 / sqrt(last) : 1.0;
-        *flag = (bad || *ambig != 0 || force_fail) ? 1ull : 0ull;
+        double t = 0.0;
+        for (int i = 0; i < BLOCK / 64; ++i) t += s_w[i];
+        norms[j] = t;
     }
 }
 
-hipError_t launch_ckpt_fold(hipStream_t st, const double* ckpt, uint32_t nw, uint32_t nk,
-                            const unsigned long long* ambig, double tol, int force_fail, double* norm2, double* scale,
-                            unsigned long long* flag) {
-    hipLaunchKernelGGL(k_ckpt_fold, dim3(1), dim3(BLOCK), 0, st, ckpt, nw, nk, ambig, tol, force_fail, norm2, scale,
-                       flag);
+__global__ void k_ckpt_fold(const double* __restrict__ norms, uint32_t nk, double tol, int force_fail,
+                            double* __restrict__ norm2, double* __restrict__ scale,
+                            unsigned long long* __restrict__ flag) {
+    if (threadIdx.x != 0) return;
+    double last = 1.0;
+    bool bad = false;
+    for (uint32_t j = 0; j < nk; ++j) {
+        const double t = norms[j];
+        last = t;
+        bad = bad || !(t > 0.0) || !(fabs(1.0 / sqrt(t) - 1.0) < tol);
+    }
+    *norm2 = last;
+    *scale = last > 0.0 ? 1.0 / sqrt(last) : 1.0;
+    *flag = (bad || force_fail) ? 1ull : 0ull;
+}
+
+hipError_t launch_ckpt_fold(hipStream_t st, double* ckpt, uint32_t nw, uint32_t nk, double tol, int force_fail,
+                            double* norm2, double* scale, unsigned long long* flag) {
+    double* norms = ckpt + (size_t)nk * nw;
+    if (nk) hipLaunchKernelGGL(k_ckpt_rows, dim3(nk), dim3(BLOCK), 0, st, ckpt, nw, norms);
+    hipLaunchKernelGGL(k_ckpt_fold, dim3(1), dim3(64), 0, st, norms, nk, tol, force_fail, norm2, scale, flag);
     return hipGetLastError();
 }
 
@@ -457,6 +465,11 @@ hipError_t launch_stats_reduce(hipStream_t st, BlockStats* bs, uint32_t n_acc, u
 hipError_t launch_index_build(hipStream_t st, uint32_t grid, const IndexArgs& a) {
     if (!grid) return hipSuccess;
     hipLaunchKernelGGL(k_index_build, dim3(grid), dim3(BLOCK), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_index_enumerate(hipStream_t st, uint32_t grid, const EnumArgs& e) {
+    hipLaunchKernelGGL(k_index_enumerate, dim3(grid), dim3(BLOCK), 0, st, e);
     return hipGetLastError();
 }
 

@@ -32,7 +32,6 @@ struct SweepArgs {
     uint64_t* out_base;     // non-null: store the result here (out of place) instead of in place
     void* out_amp;
     double* ckpt;           // [checkpoint * ckpt_stride + wavefront]: FP64 norms of H_NORM_POINT ops
-    unsigned long long* ambig;  // FP32 prune decisions within 2^-10 of the threshold (counted per workgroup)
     Slot* itab;             // non-null: insert (base key -> bucket) of every stored bucket into this cleared index
     uint64_t itmask;
 };
@@ -221,6 +220,7 @@ uint64_t sample_tiles(uint64_t n);
 hipError_t launch_norm(int mb, bool f32, hipStream_t st, uint32_t grid, const NormArgs& a, double* norm2,
                        double* scale);
 hipError_t launch_index_build(hipStream_t st, uint32_t grid, const IndexArgs& a);
+hipError_t launch_index_enumerate(hipStream_t st, uint32_t grid, const EnumArgs& e);
 // norm2 and scale from n per-workgroup partials (of a sweep with norm_out)
 hipError_t launch_norm_final(hipStream_t st, const double* partials, uint32_t n, double* norm2, double* scale);
 // One wavefront per work item: grid = ceil(nitems / (block / 64)).
@@ -234,9 +234,9 @@ hipError_t launch_stats_reduce(hipStream_t st, BlockStats* bs, uint32_t n_acc, u
 hipError_t launch_load(bool f32, hipStream_t st, const LoadArgs& a, const EnumArgs& e, uint32_t enum_grid);
 // Checkpoint norms of a one-pass renormalised run -> norm2 / scale of the last
 // checkpoint and the verdict flag (0: the run stands); see k_ckpt_fold.
-hipError_t launch_ckpt_fold(hipStream_t st, const double* ckpt, uint32_t nw, uint32_t nk,
-                            const unsigned long long* ambig, double tol, int force_fail, double* norm2, double* scale,
-                            unsigned long long* flag);
+// ckpt holds nk rows of nw partials followed by room for nk row sums
+hipError_t launch_ckpt_fold(hipStream_t st, double* ckpt, uint32_t nw, uint32_t nk, double tol, int force_fail,
+                            double* norm2, double* scale, unsigned long long* flag);
 
 // Fixed per-device scratch of the allocation-free entry points: room for
 // NOALLOC_PARTIALS FP64 per-workgroup partials (128 KB).  noalloc_scratch_init

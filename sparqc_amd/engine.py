@@ -67,13 +67,14 @@ class BatchCircuit(ctypes.Structure):
 
 class BatchTuning(ctypes.Structure):
     _fields_ = [("block", ctypes.c_uint32), ("tiles_per_item", ctypes.c_uint32),
-                ("load_factor", ctypes.c_double), ("sweep_ms", ctypes.c_double), ("launches", ctypes.c_uint64)]
+                ("load_factor", ctypes.c_double), ("sweep_ms", ctypes.c_double), ("launches", ctypes.c_uint64),
+                ("load_ms", ctypes.c_double)]
 
 
 assert ctypes.sizeof(Config) == 80
 assert ctypes.sizeof(Stats) == 136
 assert ctypes.sizeof(BatchCircuit) == 56
-assert ctypes.sizeof(BatchTuning) == 32
+assert ctypes.sizeof(BatchTuning) == 40
 
 _lib = None
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -223,7 +224,7 @@ class Engine:
 
     def __init__(self, nq, prec="f64", prune_thr=0.0, bucket_bits=-1, renorm_every=0,
                  fuse_window=0, timing=False, device=0, hbm_budget_bytes=0, max_load=0.5,
-                 fuse_blocks=False, renorm_mode=0, frames=True):
+                 fuse_blocks=False, renorm_mode=1, frames=True):
         L = lib()
         cfg = Config()
         L.sqe_config_default(ctypes.byref(cfg), nq)
@@ -466,6 +467,17 @@ class Batch:
     def export_buffer(cap=0):
         """A page-locked buffer for export_start."""
         return PinnedArrays(cap)
+
+    def export_bound(self, cid):
+        """Entries an export of circuit cid may need (its slot count): the
+        capacity export_start asks of its buffer."""
+        t = ctypes.c_int64()
+        bound = ctypes.c_uint64()
+        rc = lib().sqe_batch_export_start(self.h, int(cid), None, None, None, 0, ctypes.byref(t),
+                                          ctypes.byref(bound))
+        if rc != SQE_EINVAL or bound.value == 0:
+            _bcheck(rc)
+        return bound.value
 
     def export_start(self, cid, out):
         """Start an asynchronous export of circuit cid into `out` (a

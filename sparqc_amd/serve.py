@@ -10,8 +10,9 @@ Requests arrive at given times (seconds from the start).  The loop
      one batched launch per bucket size (circuits that arrived while others
      were running join at the next step, so their work overlaps on the GPU);
   3. starts the export of each circuit that finished -- asynchronous, on the
-     engine's export stream, into a reused page-locked buffer -- and releases
-     its pool space at once;
+     engine's export stream, into a reused page-locked buffer of the
+     circuit's size class (buffers are kept for reuse, never freed inside the
+     loop) -- and releases its pool space at once;
   4. collects the exports whose copies have completed and records each
      request's arrival -> result latency.
 Only an idle loop waits for the device (for the oldest export, or sleeps until
@@ -46,22 +47,38 @@ class ServeLoop:
             batch = E.Batch(prec=prec, renorm_every=renorm_every, hbm_budget_bytes=hbm_budget_bytes,
                             timing=timing, device=device)
         self.batch = batch
-        self._free = []                         # page-locked export buffers, reused
+        self._pool = {}                         # capacity -> page-locked export buffers, reused
         self.step_gates = int(step_gates)
         self.prec = getattr(batch, "prec", prec)
 
+    @staticmethod
+    def _size_class(n):
+        """Buffer capacities are powers of two (at least 4096 entries), so a
+        freed buffer serves every later export of its class."""
+        return 1 << max(12, (int(n) - 1).bit_length())
+
+    def _acquire(self, need):
+        bufs = self._pool.get(self._size_class(need))
+        if bufs:
+            return bufs.pop()
+        return self.batch.export_buffer(self._size_class(need))
+
+    def _release(self, buf):
+        self._pool.setdefault(buf.cap, []).append(buf)
+
     def reserve(self, count, cap):
-        """Allocate `count` page-locked export buffers of `cap` entries ahead
-        of time (allocating them inside the loop stalls it)."""
-        while len(self._free) < count:
-            self._free.append(self.batch.export_buffer(cap))
-        for buf in self._free:
-            buf.grow(cap)
+        """Allocate `count` page-locked export buffers for exports of up to
+        `cap` entries ahead of time (allocating inside the loop stalls it)."""
+        c = self._size_class(cap)
+        have = len(self._pool.get(c, []))
+        for _ in range(count - have):
+            self._release(self.batch.export_buffer(c))
 
     def close(self):
-        for buf in self._free:
-            buf.free()
-        self._free = []
+        for bufs in self._pool.values():
+            for buf in bufs:
+                buf.free()
+        self._pool = {}
         self.batch.close()
 
     def __enter__(self):
@@ -108,7 +125,7 @@ class ServeLoop:
                     r = b.export_poll(tk, wait=wait)
                 except E.SqeError as ex:
                     pending.popleft()
-                    self._free.append(buf)
+                    self._release(buf)
                     errors[i] = str(ex)
                     latency[i] = clock() - t0 - arrivals[i]
                     continue
@@ -123,7 +140,7 @@ class ServeLoop:
                     o = np.argsort(k, kind="stable")
                     results[i] = dict(state=(k[o], buf.re[:cnt][o].copy(), buf.im[:cnt][o].copy()),
                                       updates=info["updates"], nnz=info["nnz"])
-                self._free.append(buf)
+                self._release(buf)
                 wait = False                    # waited for the oldest; take the rest only if done
 
         while nxt < n or live or pending:
@@ -150,7 +167,7 @@ class ServeLoop:
                 # 3. finished circuits: start their exports, free their pool space
                 for cid in b.finished():
                     i = live.pop(cid)
-                    buf = self._free.pop() if self._free else b.export_buffer()
+                    buf = self._acquire(b.export_bound(cid))
                     pending.append((b.export_start(cid, buf), i, buf))
                     b.remove(cid)
                 peak_pending = max(peak_pending, len(pending))

@@ -143,8 +143,20 @@ def test_autotune_keeps_results(oracle):
         t = b.autotune()
         say("autotune best", t)
         assert t["block"] in (64, 128, 256) and t["tiles_per_item"] >= 1
+        # the load factor is chosen among the timed candidates and kept for
+        # later admissions
+        assert t["load_factor"] in (0.25, 0.375, 0.5, 0.625, 0.75) and t["load_ms"] > 0
+        assert b.tuning()["load_factor"] == t["load_factor"]
         b.run()
         for cid, c in zip(ids, cs):
+            want, upd = oracle_of(oracle, c)
+            assert_same_state(b.export(cid), want, atol=1e-12)
+            assert b.info(cid)["updates"] == upd
+        # a circuit admitted after tuning (its index at the tuned load factor)
+        extra = small_batch(12, 2)
+        eid = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in extra]
+        b.run()
+        for cid, c in zip(eid, extra):
             want, upd = oracle_of(oracle, c)
             assert_same_state(b.export(cid), want, atol=1e-12)
             assert b.info(cid)["updates"] == upd

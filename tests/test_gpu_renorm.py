@@ -2,9 +2,10 @@
 
 With renorm_every = R and renorm_mode = 1 the FP32 engine applies a queued
 run of bucket-local gates in one pass: it records the FP64 norm at every R-th gate without
-rescaling, then applies 1/sqrt(norm) of the last point; a prune decision
-that could change under the scales the step-by-step run applies (one within
-2^-10 of the threshold, or a norm more than 2^-11 from 1) makes the engine
+rescaling, then applies 1/sqrt(norm) of the last point; its prune decisions
+are taken on amplitudes that lack the pending scale (so a decision can differ
+from the step-by-step run's only for a value within that scale's distance
+from 1 of the threshold), and a norm more than 2^-11 from 1 makes the engine
 rerun the gates step by step.  Both forms must give the same state within
 FP32 rounding (or agree to the bit, when the check made it rerun); the
 forced rerun (renorm_mode 2) must give the step-by-step state to the bit.

@@ -50,7 +50,10 @@ class ModelBatch:
         return (np.arange(v["nnz"], dtype=np.uint64), np.zeros(v["nnz"]), np.zeros(v["nnz"]))
 
     def export_buffer(self, cap=0):
-        return Buffer()
+        return Buffer(cap)
+
+    def export_bound(self, cid):
+        return self.c[cid]["nnz"]
 
     def export_start(self, cid, out):
         """The export completes at the next poll (no device in the model)."""
@@ -76,6 +79,12 @@ class ModelBatch:
 
 class Buffer:
     keys = re = im = None
+
+    def __init__(self, cap=0):
+        self.cap = cap
+
+    def free(self):
+        pass
 
 
 class Clock:

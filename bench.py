@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--circuit", default="mix", choices=["mix", "layers"],
                     help="mix: the 60/35/5 diagonal/permutation/branching circuit; layers: layered "
                          "single-qubit gates on the 4 branching qubits (long fusable windows)")
+    ap.add_argument("--layer-qubits", type=int, default=4,
+                    help="layers circuit: branching qubits the layers act on (5 with --bucket-bits 5 and "
+                         "--fused-blocks: 32x32 complex blocks)")
     ap.add_argument("--frames", default="on", choices=["on", "off"],
                     help="Pauli gates on bucket qubits through the per-bucket frame (on) or the amplitude "
                          "registers (off)")
@@ -514,7 +517,7 @@ def main():
     from sparqc_amd import gen
 
     if a.circuit == "layers":
-        w = gen.circuit_layers(a.seed, nq=48, n_init=1 << a.n_init_log2, n_gates=a.gates)
+        w = gen.circuit_layers(a.seed, nq=48, n_init=1 << a.n_init_log2, n_gates=a.gates, n_branch=a.layer_qubits)
     else:
         w = gen.circuit_large(a.seed, nq=48, n_init=1 << a.n_init_log2, n_gates=a.gates)
     if a.fused_blocks and a.prec != "f32":
@@ -577,11 +580,12 @@ def main():
                                 "60%% diag / 35%% perm / 5%% branch on 4 qubits, prune 1e-14"
                                 % (a.n_init_log2, a.gates)) if a.circuit == "mix" else
                                ("sparse statevector, 48 qubits, 2^%d initial keys, %d gates in layers of "
-                                "single-qubit gates on 4 qubits with diagonal / CX gates between"
-                                % (a.n_init_log2, a.gates)),
+                                "single-qubit gates on %d qubits with diagonal / CX gates between"
+                                % (a.n_init_log2, a.gates, a.layer_qubits)),
                    "circuit": a.circuit,
                    "nq": 48, "n_init": 1 << a.n_init_log2, "gates": a.gates, "seed": a.seed,
-                   "prec": a.prec, "fuse_window": a.fuse_window, "fused_blocks": a.fused_blocks},
+                   "prec": a.prec, "fuse_window": a.fuse_window, "fused_blocks": a.fused_blocks,
+                   "bucket_bits": a.bucket_bits},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": "k_sweep (fused local sweep)",
@@ -596,6 +600,8 @@ def main():
                                         "equivalent_frac": upd / dt * PER_UPDATE_BYTES[a.prec] / HBM_PEAK},
                      "unfused_tail": unfused},
         "detail": {"updates_per_step": upd / a.steps, "nnz_initial": nnz0, "nnz_final": nnz1,
+                   "blocks_per_step": (s1["blocks"] - s0["blocks"]) / a.steps,
+                   "block_gates_per_step": (s1["block_gates"] - s0["block_gates"]) / a.steps,
                    "buckets": s1["buckets"], "bucket_bits": s1["bucket_bits"],
                    "cross_bucket_launches": s1["pairs"] - s0["pairs"],
                    "sweep_kernel_ms_per_step": sweep_s / a.steps * 1e3,

@@ -82,10 +82,11 @@ typedef struct sqe_config {
     uint32_t timing;            /* 1: time every kernel launch with HIP events         */
     uint32_t mem_reserve_ppm;   /* with hbm_budget_bytes = 0: fraction of free device
                                    memory kept unused, parts per million (0: 10%)     */
-    uint32_t fuse_blocks;       /* FP32 with 16-slot buckets: fuse windows of at least
-                                   6 (1) or N (N >= 2) gates on bucket qubits into
-                                   dense 16x16 blocks applied with MFMA (prunes once
-                                   per block); 0: off                                 */
+    uint32_t fuse_blocks;       /* FP32 with 16- or 32-slot buckets (4 or 5 bucket
+                                   qubits): fuse windows of at least 6 (1) or N
+                                   (N >= 2) gates on bucket qubits into dense 16x16
+                                   or 32x32 complex blocks applied with MFMA (prunes
+                                   once per block); 0: off                            */
     uint32_t batch_eager;       /* batch: 1 launches each step of sqe_batch_run
                                    directly; 0 replays the run as one captured
                                    hipGraph (planned once per set of circuits)    */
@@ -124,6 +125,8 @@ typedef struct sqe_stats {
     double pair_ms;             /* summed cross-bucket kernel time (timing = 1)        */
     double other_ms;            /* summed time of all other kernels (timing = 1)       */
     uint64_t moved_pair;        /* the part of `moved` from cross-bucket launches      */
+    uint64_t blocks;            /* fused dense blocks in launched sweep programs       */
+    uint64_t block_gates;       /* gates those blocks hold                             */
 } sqe_stats;
 
 /* Fill *cfg with defaults for nq qubits in FP64. */

@@ -36,14 +36,30 @@ SOURCES = ["sqe_kernels_f64.hip", "sqe_kernels_f32.hip", "sqe_kernels_common.hip
            "sqe_batch.cpp", "sqe_noalloc.hip"]
 
 
+def _deps(path, seen=None):
+    """The file and every local header it includes, recursively."""
+    import re
+    seen = set() if seen is None else seen
+    path = os.path.normpath(path)
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    with open(path) as f:
+        for inc in re.findall(r'^\s*#\s*include\s+"([^"]+)"', f.read(), re.M):
+            for base in (os.path.dirname(path), CSRC, os.path.join(ROOT, "include")):
+                cand = os.path.join(base, inc)
+                if os.path.exists(cand):
+                    _deps(cand, seen)
+                    break
+    return seen
+
+
 def _stale(src, obj):
-    """The object is older than its source or any header it may include."""
+    """The object is older than its source or a header the source includes."""
     if not os.path.exists(obj):
         return True
     t = os.path.getmtime(obj)
-    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
-    deps += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
-    return any(os.path.getmtime(d) > t for d in deps)
+    return any(os.path.getmtime(d) > t for d in _deps(os.path.join(CSRC, src)))
 
 
 def _compile(src, incremental=False):

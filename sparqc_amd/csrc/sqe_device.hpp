@@ -61,9 +61,9 @@ struct DOp {
 //   H_SLOT_U     any 2x2 on bucket qubit J, 8 coefficients in a side table
 //                (prunes)
 //   H_RENORM     scale by the device-computed 1/sqrt(norm2)
-//   H_SLOT_BLOCK dense 16x16 complex block of fused bucket-qubit gates (FP32,
-//                MFMA; side table: MFMA operand table, then the FP64 matrix;
-//                c[0] = number of gates) (prunes)
+//   H_SLOT_BLOCK dense 2^MB x 2^MB complex block of fused bucket-qubit gates
+//                (FP32, MB = 4 or 5, MFMA; side table: MFMA operand table,
+//                then the FP64 matrix; c[0] = number of gates) (prunes)
 // (RX on a bucket qubit runs in H_SLOT_U: a separate handler would add a
 // path to the slot switch and push the FP64 sweep past its register budget.)
 // Coefficients c[]: DIAG f0r, f0i, f1r, f1i (Z: 1,0,-1,0; PHASE: 1,0,c,s;

@@ -94,7 +94,7 @@ enum StepKind { ST_SWEEP, ST_PAIR, ST_NORM };
 struct LazyRun {
     bool pending = false;               // a verdict is outstanding
     int in = 0;                         // the store holding the run's input
-    uint64_t gates0 = 0, sweeps0 = 0, sweep_ops0 = 0;
+    uint64_t gates0 = 0, sweeps0 = 0, sweep_ops0 = 0, blocks0 = 0, block_gates0 = 0;
     std::vector<sqe_gate> gl;           // the run's gates
 };
 struct Step {
@@ -166,6 +166,7 @@ struct sqe_engine {
     uint64_t n_reruns = 0;                  // one-pass runs rerun step by step
 
     uint64_t n_sweeps = 0, n_sweep_ops = 0, n_pairs = 0, n_remaps = 0;
+    uint64_t n_blocks = 0, n_block_gates = 0;   // fused dense blocks launched, gates they hold
     std::vector<Timed> timed;
     std::vector<hipEvent_t> ev_pool;
     double t_ms[3] = {0, 0, 0};
@@ -703,8 +704,8 @@ bool frames_on(const sqe_engine* e) {
     return !e->cfg.frame_off && e->cfg.nq <= (uint32_t)FRAME_Z_BIT0 && e->qm.mb > 0;
 }
 
-void setup_blocks(sqe_engine* e, BlockBuilder& bb, std::vector<std::array<cx, 256>>* blocks) {
-    bb.on = e->cfg.fuse_blocks && e->f32 && e->qm.mb == 4;
+void setup_blocks(sqe_engine* e, BlockBuilder& bb, std::vector<BlockMat>* blocks) {
+    bb.on = e->cfg.fuse_blocks && e->f32 && (e->qm.mb == 4 || e->qm.mb == 5);
     if (e->cfg.fuse_blocks >= 2) bb.min_gates = e->cfg.fuse_blocks;
     bb.frames = frames_on(e);
     bb.qm = e->qm;
@@ -717,7 +718,7 @@ void setup_blocks(sqe_engine* e, BlockBuilder& bb, std::vector<std::array<cx, 25
 sqe_status flush_exact(sqe_engine* e, const std::vector<sqe_gate>& gl) {
     std::vector<DOp> prog;
     std::vector<Step> steps;
-    std::vector<std::array<cx, 256>> blocks;
+    std::vector<BlockMat> blocks;
     BlockBuilder bb;
     setup_blocks(e, bb, &blocks);
     const uint64_t W = e->cfg.fuse_window ? e->cfg.fuse_window : ~0ull;
@@ -788,8 +789,13 @@ sqe_status flush_exact(sqe_engine* e, const std::vector<sqe_gate>& gl) {
             ++e->n_sweeps;
             for (uint64_t i = s.off; i < s.off + s.n; ++i) {
                 const DOp& d = prog[i];
-                if (d.kind == K_BLOCK) e->n_sweep_ops += (uint64_t)d.p[1];
-                else if (d.kind != K_RENORM && d.kind != K_NORM_POINT) ++e->n_sweep_ops;
+                if (d.kind == K_BLOCK) {
+                    e->n_sweep_ops += (uint64_t)d.p[1];
+                    ++e->n_blocks;
+                    e->n_block_gates += (uint64_t)d.p[1];
+                } else if (d.kind != K_RENORM && d.kind != K_NORM_POINT) {
+                    ++e->n_sweep_ops;
+                }
             }
             if (flips) e->tab_valid = so.index;
         } break;
@@ -828,8 +834,10 @@ sqe_status flush_lazy(sqe_engine* e, const std::vector<sqe_gate>& gl) {
     L.in = e->cur;
     L.sweeps0 = e->n_sweeps;
     L.sweep_ops0 = e->n_sweep_ops;
+    L.blocks0 = e->n_blocks;
+    L.block_gates0 = e->n_block_gates;
     std::vector<DOp> prog;
-    std::vector<std::array<cx, 256>> blocks;
+    std::vector<BlockMat> blocks;
     BlockBuilder bb;
     setup_blocks(e, bb, &blocks);
     uint32_t k = 0;
@@ -890,8 +898,13 @@ sqe_status flush_lazy(sqe_engine* e, const std::vector<sqe_gate>& gl) {
         ++e->n_sweeps;
         for (uint64_t i = off; i < off + n; ++i) {
             const DOp& d = prog[i];
-            if (d.kind == K_BLOCK) e->n_sweep_ops += (uint64_t)d.p[1];
-            else if (d.kind != K_RENORM && d.kind != K_NORM_POINT) ++e->n_sweep_ops;
+            if (d.kind == K_BLOCK) {
+                e->n_sweep_ops += (uint64_t)d.p[1];
+                ++e->n_blocks;
+                e->n_block_gates += (uint64_t)d.p[1];
+            } else if (d.kind != K_RENORM && d.kind != K_NORM_POINT) {
+                ++e->n_sweep_ops;
+            }
         }
         off += n;
         if (n == 0) break;
@@ -925,6 +938,8 @@ sqe_status lazy_settle(sqe_engine* e) {
     e->gates = L.gates0;
     e->n_sweeps = L.sweeps0;
     e->n_sweep_ops = L.sweep_ops0;
+    e->n_blocks = L.blocks0;
+    e->n_block_gates = L.block_gates0;
     e->renorm_pending = false;
     e->tab_valid = false;
     HIPCK(hipMemcpyAsync(e->stats, e->lz_dev + 2, sizeof(Stats), hipMemcpyDeviceToDevice, e->st));
@@ -1449,6 +1464,8 @@ sqe_status sqe_get_stats(sqe_engine* e, sqe_stats* out) {
         out->reruns = (uint32_t)e->n_reruns;
         out->moved = h.moved;
         out->moved_pair = h.moved_pair;
+        out->blocks = e->n_blocks;
+        out->block_gates = e->n_block_gates;
         out->sweep_ms = e->t_ms[0];
         out->pair_ms = e->t_ms[1];
         out->other_ms = e->t_ms[2];

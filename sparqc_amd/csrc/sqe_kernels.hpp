@@ -349,55 +349,84 @@ __device__ __forceinline__ void apply_phase(Cplx<double> (&a)[1 << MB], double p
     });
 }
 
-// Fused dense block on a 16-slot FP32 bucket (one bucket per lane, 64 per
-// wavefront): Y = M X with M the 16x16 complex matrix of the block, computed
-// as the real product R X' (R = [[Mr, -Mi], [Mi, Mr]] 32x32, X' = 16 real
-// parts over 16 imaginary parts, 32 x 64 buckets) by two v_mfma_f32_32x32x2f32
-// chains (buckets 0-31 and 32-63) over 16 steps of two rows.  The lane-per-bucket
-// registers become B operands with one v_permlane32_swap per row pair, and
-// the outputs go back the same way.  Outputs are pruned componentwise; those
-// within 4x of the threshold are decided on an FP64 recomputation from the
-// same inputs.  Needs all 64 lanes active (wavefront-uniform control flow).
+// Fused dense block on a D-slot FP32 bucket (D = 16 or 32; one bucket per
+// lane, 64 per wavefront): Y = M X with M the D x D complex matrix of the
+// block, computed as the real product R X' (R = [[Mr, -Mi], [Mi, Mr]],
+// 2D x 2D; X' = D real parts over D imaginary parts, 2D x 64 buckets) by
+// v_mfma_f32_32x32x2f32 chains: one per 32-row tile of R (D / 16 of them) and
+// per half of the wavefront's buckets (0-31, 32-63), over D steps of two
+// input rows.  The lane-per-bucket registers become B operands with one
+// v_permlane32_swap per row pair, and the outputs go back the same way.  The
+// A operands come from the side table in the lane order the instruction reads
+// (tile q, step t, lane l: R[32 q + l % 32][2t + l / 32]).  Outputs are pruned
+// componentwise; those within 4x of the threshold are decided on an FP64
+// recomputation from the same inputs.  Needs all 64 lanes active
+// (wavefront-uniform control flow).
 typedef float v16f __attribute__((ext_vector_type(16)));
-__device__ __forceinline__ void block_apply(Cplx<float> (&a)[16], float pr, float pi, const double* __restrict__ side,
+template <int D>
+__device__ __forceinline__ void block_apply(Cplx<float> (&a)[D], float pr, float pi, const double* __restrict__ side,
                                             double thr) {
+    static_assert(D == 16 || D == 32, "blocks of 4 or 5 bucket qubits");
+    constexpr int Q = D / 16;                                     // 32-row tiles of R
     const uint32_t lane = threadIdx.x & 63;
     const Band32 band = band32(thr);
-    const float* __restrict__ tab = (const float*)side;          // 16 steps x 64 lanes
-    const double* __restrict__ m64 = side + 512;                  // 16x16 complex, FP64
-    float x[32];
+    const float* __restrict__ tab = (const float*)side;          // Q tiles x D steps x 64 lanes
+    const double* __restrict__ m64 = side + Q * D * 32;           // D x D complex, FP64
+    float x[2 * D];
     {
         const float fr = pr, fi = pi;
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
+        for (int s = 0; s < D; ++s) {
             const Cplx<float> v = rot(fr, fi, a[s]);
             x[s] = v.x;
-            x[16 + s] = v.y;
+            x[D + s] = v.y;
         }
     }
-    float av[16];
+    float y[2 * D];
+    // register r of lane l holds output row 8 (r / 4) + 4 (l / 32) + r % 4 of
+    // its tile for bucket l % 32 of its half; one swap per register pair
+    // brings rows back to the lane of their bucket
+    auto unpack = [&](const v16f& c0, const v16f& c1, int row0) __attribute__((always_inline)) {
 #pragma unroll
-    for (int t = 0; t < 16; ++t) av[t] = tab[t * 64 + lane];
-    v16f acc0 = {}, acc1 = {};
+        for (int r = 0; r < 16; ++r) {
+            const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(c0[r]), __float_as_uint(c1[r]), false,
+                                                             false);
+            const int i0 = row0 + 8 * (r / 4) + (r % 4);
+            y[i0] = __uint_as_float(sw[0]);
+            y[i0 + 4] = __uint_as_float(sw[1]);
+        }
+    };
+    if constexpr (D == 16) {
+        float av[16];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[2 * t]), __float_as_uint(x[2 * t + 1]),
-                                                         false, false);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], __uint_as_float(sw[0]), acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], __uint_as_float(sw[1]), acc1, 0, 0, 0);
+        for (int t = 0; t < 16; ++t) av[t] = tab[t * 64 + lane];
+        v16f acc0 = {}, acc1 = {};
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[2 * t]), __float_as_uint(x[2 * t + 1]),
+                                                             false, false);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], __uint_as_float(sw[0]), acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], __uint_as_float(sw[1]), acc1, 0, 0, 0);
+        }
+        unpack(acc0, acc1, 0);
+    } else {
+        v16f acc00 = {}, acc01 = {}, acc10 = {}, acc11 = {};
+#pragma unroll
+        for (int t = 0; t < D; ++t) {
+            const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[2 * t]), __float_as_uint(x[2 * t + 1]),
+                                                             false, false);
+            const float av0 = tab[t * 64 + lane], av1 = tab[(D + t) * 64 + lane];
+            acc00 = __builtin_amdgcn_mfma_f32_32x32x2f32(av0, __uint_as_float(sw[0]), acc00, 0, 0, 0);
+            acc01 = __builtin_amdgcn_mfma_f32_32x32x2f32(av0, __uint_as_float(sw[1]), acc01, 0, 0, 0);
+            acc10 = __builtin_amdgcn_mfma_f32_32x32x2f32(av1, __uint_as_float(sw[0]), acc10, 0, 0, 0);
+            acc11 = __builtin_amdgcn_mfma_f32_32x32x2f32(av1, __uint_as_float(sw[1]), acc11, 0, 0, 0);
+        }
+        unpack(acc00, acc01, 0);
+        unpack(acc10, acc11, 32);
     }
-    float y[32];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc0[r]), __float_as_uint(acc1[r]), false,
-                                                         false);
-        const int i0 = 8 * (r / 4) + (r % 4);
-        y[i0] = __uint_as_float(sw[0]);
-        y[i0 + 4] = __uint_as_float(sw[1]);
-    }
-#pragma unroll
-    for (int s = 0; s < 16; ++s) {
-        Cplx<float> n = mkf(y[s], y[16 + s]);
+    for (int s = 0; s < D; ++s) {
+        Cplx<float> n = mkf(y[s], y[D + s]);
         // keep above the band, prune below it; the FP64 row recomputation
         // runs when any lane has this output in the band (a uniform branch)
         const float m = fmaxf(fabsf(n.x), fabsf(n.y));
@@ -405,9 +434,9 @@ __device__ __forceinline__ void block_apply(Cplx<float> (&a)[16], float pr, floa
         const bool near = !keep && m >= band.lo;
         if (__builtin_amdgcn_ballot_w64(near) != 0) {
             double er = 0.0, ei = 0.0;
-            for (int k = 0; k < 16; ++k) {
-                const double mr = m64[(s * 16 + k) * 2], mi = m64[(s * 16 + k) * 2 + 1];
-                const double xr = x[k], xi = x[16 + k];
+            for (int k = 0; k < D; ++k) {
+                const double mr = m64[(s * D + k) * 2], mi = m64[(s * D + k) * 2 + 1];
+                const double xr = x[k], xi = x[D + k];
                 er += mr * xr - mi * xi;
                 ei += mr * xi + mi * xr;
             }
@@ -777,9 +806,9 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                     const bool branch = branching_handler(op.h);
                     uint32_t nzc[EPT];
                     bool counted = false;
-                    if constexpr (BLOCKS && MB == 4 && sizeof(R) == 4 && EPT == 1) {
+                    if constexpr (BLOCKS && (MB == 4 || MB == 5) && sizeof(R) == 4 && EPT == 1) {
                         if (op.h == H_SLOT_BLOCK) {
-                            block_apply(a[0], pr[0], pi[0], coef + (size_t)op.ci * 8, thr);
+                            block_apply<1 << MB>(a[0], pr[0], pi[0], coef + (size_t)op.ci * 8, thr);
                             pend += (uint32_t)op.c[0] - 1;       // the block's gates (lower bound of their nonzeros)
                         } else {
                             counted = slot_dispatch<MB, R, EPT>(op, a, pred, pr, pi, thr, coef, nzc);
@@ -865,23 +894,26 @@ __device__ __forceinline__ void wait_vector_memory() {
 // HBM rather than instruction issue) is compiled for 3 waves instead, where
 // it does not spill: at 4 waves every lane writes and rereads its spill slots
 // once per launch, extra traffic comparable to a tenth of the state's bytes.
+// BLOCKS: the FP32 16-slot sweep with fused dense blocks is compiled for 3
+// waves too (the block's 32 inputs, 32 outputs and 32 accumulators spill at
+// 4 waves; measured 1.64x faster on the layered circuit at 3).
 #ifndef SQE_OCC_F64_MB4
 #define SQE_OCC_F64_MB4 3
 #endif
 #ifndef SQE_OCC_F32_MB4
 #define SQE_OCC_F32_MB4 4
 #endif
-template <int MB, class R, bool SHORT = false>
+template <int MB, class R, bool SHORT = false, bool BLOCKS = false>
 struct SweepOcc {
     static constexpr int value =
-        MB >= 5 ? 1 : (MB == 4 ? (sizeof(R) == 8 || SHORT ? SQE_OCC_F64_MB4 : SQE_OCC_F32_MB4) : 3);
+        MB >= 5 ? 1 : (MB == 4 ? (sizeof(R) == 8 || SHORT || BLOCKS ? SQE_OCC_F64_MB4 : SQE_OCC_F32_MB4) : 3);
 };
 constexpr uint32_t SHORT_PROGRAM_OPS = 8;     // programs up to this length take the SHORT sweep
 
 // One program over the engine's store: wavefront g of the grid takes tiles
 // EPT*g .. EPT*g + EPT-1.
 template <int MB, class R, int EPT, bool BLOCKS = false, bool SHORT = false>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<MB, R, SHORT>::value))) void k_sweep(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<MB, R, SHORT, BLOCKS>::value))) void k_sweep(
     SweepArgs A) {
     const uint64_t nb = *A.nb;
     const int64_t ntiles = (int64_t)((nb + 63) >> 6);
@@ -1310,9 +1342,10 @@ struct Launch {
         const uint32_t grid = sweep_grid(mb, ntiles);
         if (a.nops == 0 && !a.store) {
             SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_count<MB_, R>), dim3(grid), dim3(BLOCK), 0, st, a));
-        } else if (mb == 4 && a.blocks) {
+        } else if ((mb == 4 || mb == 5) && a.blocks) {
             if constexpr (sizeof(R) == 4) {
-                hipLaunchKernelGGL((k_sweep<4, R, EPT, true>), dim3(grid), dim3(BLOCK), 0, st, a);
+                if (mb == 4) hipLaunchKernelGGL((k_sweep<4, R, EPT, true>), dim3(grid), dim3(BLOCK), 0, st, a);
+                else hipLaunchKernelGGL((k_sweep<5, R, EPT, true>), dim3(grid), dim3(BLOCK), 0, st, a);
             } else {
                 return hipErrorInvalidValue;
             }

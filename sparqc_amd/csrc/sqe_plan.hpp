@@ -204,16 +204,18 @@ inline MOp encode_op(const DOp& d, std::vector<double>& coef, bool f32 = false, 
 }
 
 // ---------------------------------------------------------------------------
-// Fused blocks (FP32, 16-slot buckets): a window of gates on bucket qubits
-// without controls is multiplied into one dense 16x16 complex matrix that the
-// sweep applies with MFMA.  Gates on base qubits without bucket controls
-// commute with such gates (they only scale or relabel whole buckets) and are
-// emitted ahead of the window; any other gate closes it.  A window
-// of fewer than min_gates gates runs gate by gate: a dense block costs 2048
-// MFMA FMAs per bucket whatever it holds, so it pays only for long windows.
-// Prune decisions are made once, on the block's outputs.
+// Fused blocks (FP32, 16- or 32-slot buckets): a window of gates on bucket
+// qubits without controls is multiplied into one dense D x D complex matrix
+// (D = 2^MB: 16x16 for 4 bucket qubits, 32x32 for 5) that the sweep applies
+// with MFMA.  Gates on base qubits without bucket controls commute with such
+// gates (they only scale or relabel whole buckets) and are emitted ahead of
+// the window; any other gate closes it.  A window of fewer than min_gates
+// gates runs gate by gate: a dense block costs 4 D^2 MFMA FMAs per bucket
+// whatever it holds, so it pays only for long windows.  Prune decisions are
+// made once, on the block's outputs.
 // ---------------------------------------------------------------------------
 using cx = std::complex<double>;
+using BlockMat = std::vector<cx>;                // D x D, row-major, out x in
 
 // 2x2 matrix of a gate record, row-major (the conventions of sqe.h).
 inline void gate_matrix(const sqe_gate& g, cx m[4]) {
@@ -254,9 +256,9 @@ inline bool touches_amplitudes(const DOp& d) {
     return d.tslot >= 0 || d.cmS != 0;
 }
 
-// Fused dense blocks (FP32, 16-slot buckets) built from a program's gates in
-// order.  Plain gates on bucket qubits (no controls) go into the open window,
-// whose product is one 16x16 matrix.  Base-class gates commute with the
+// Fused dense blocks (FP32, 16- or 32-slot buckets) built from a program's
+// gates in order.  Plain gates on bucket qubits (no controls) go into the
+// open window, whose product is one D x D matrix (D = 2^MB).  Base-class gates commute with the
 // window (they scale or relabel whole buckets) and are emitted ahead of it.
 // Frame gates (Pauli gates on a bucket qubit whose controls are base
 // qubits, see frame_record) act on one bucket qubit with a per-bucket
@@ -273,13 +275,13 @@ struct BlockBuilder {
     bool frames = false;                          // frame gates may be held back past the window
     uint32_t min_gates = 6;                       // smaller windows run gate by gate
     QMap qm{};
-    std::array<cx, 256> m{};                      // 16x16, row-major, out x in
+    BlockMat m;                                   // D x D, row-major, out x in
     uint32_t count = 0;
     std::vector<DOp> window;                      // the window's gates, in order
     std::vector<DOp> held;                        // frame gates held back behind the window
     uint32_t held_x = 0, held_z = 0;              // bucket slots with held X-type / Z-type gates
     uint64_t held_ctl = 0;                        // base bits the held gates' conditions read
-    std::vector<std::array<cx, 256>>* blocks = nullptr;
+    std::vector<BlockMat>* blocks = nullptr;
 
     static bool plain_slot(const sqe_gate& g, const QMap& qm) {
         return ((qm.mask >> g.target) & 1ull) && g.ctrl_mask == 0;
@@ -313,23 +315,24 @@ struct BlockBuilder {
         return 0;
     }
     void add(const sqe_gate& g) {
+        const int D = 1 << qm.mb;
         if (count == 0) {
-            for (int r = 0; r < 16; ++r)
-                for (int c = 0; c < 16; ++c) m[r * 16 + c] = r == c ? 1.0 : 0.0;
+            m.assign((size_t)D * D, cx(0.0, 0.0));
+            for (int r = 0; r < D; ++r) m[(size_t)r * D + r] = 1.0;
             window.clear();
         }
         window.push_back(plan_op(g, qm));
         const int j = slot_of(g);
         cx u[4];
         gate_matrix(g, u);
-        std::array<cx, 256> out{};
-        for (int r = 0; r < 16; ++r)
-            for (int c = 0; c < 16; ++c) {
+        BlockMat out((size_t)D * D);
+        for (int r = 0; r < D; ++r)
+            for (int c = 0; c < D; ++c) {
                 // (G M)[r][c] = sum over s of G[r][s] M[s][c]; G mixes r with r ^ 2^j
                 const int r0 = r & ~(1 << j), r1 = r | (1 << j), br = (r >> j) & 1;
-                out[r * 16 + c] = u[br * 2] * m[r0 * 16 + c] + u[br * 2 + 1] * m[r1 * 16 + c];
+                out[(size_t)r * D + c] = u[br * 2] * m[(size_t)r0 * D + c] + u[br * 2 + 1] * m[(size_t)r1 * D + c];
             }
-        m = out;
+        m.swap(out);
         ++count;
     }
     void flush(std::vector<DOp>& prog) {
@@ -384,25 +387,32 @@ struct Encoded {
     bool framed = false;
 };
 
-// Record of a block: the side table gets the 32x32 real form R = [[Mr, -Mi],
-// [Mi, Mr]] as the A operands of 16 v_mfma_f32_32x32x2f32 steps (step t, lane
-// l: R[l % 32][2t + l / 32], FP32), then M itself in FP64 (row-major re, im)
-// for the FP64 recomputation of near-threshold outputs.
-inline MOp encode_block(const DOp& d, const std::array<cx, 256>& m, std::vector<double>& coef) {
+// Record of a block (D x D complex, D = 16 or 32): the side table gets the
+// 2D x 2D real form R = [[Mr, -Mi], [Mi, Mr]] as the A operands of the
+// v_mfma_f32_32x32x2f32 chains -- D / 16 tiles of 32 output rows, D steps of
+// two input rows each (tile q, step t, lane l: R[32 q + l % 32][2t + l / 32],
+// FP32) -- then M itself in FP64 (row-major re, im) for the FP64
+// recomputation of near-threshold outputs.  The side table takes 64 D^2 / 8
+// units of 8 doubles: 128 for 16x16, 512 for 32x32.
+inline MOp encode_block(const DOp& d, const BlockMat& m, std::vector<double>& coef) {
     MOp r{};
+    const int D = m.size() == 1024 ? 32 : 16;
     const uint64_t ci = coef.size() / 8;
-    std::vector<float> tab(1024);
+    const int tiles = D / 16, steps = D;
+    std::vector<float> tab((size_t)tiles * steps * 64);
     auto R = [&](int i, int k) -> double {
-        const cx v = m[(i & 15) * 16 + (k & 15)];
-        if (i < 16) return k < 16 ? v.real() : -v.imag();
-        return k < 16 ? v.imag() : v.real();
+        const cx v = m[(size_t)(i % D) * D + (k % D)];
+        if (i < D) return k < D ? v.real() : -v.imag();
+        return k < D ? v.imag() : v.real();
     };
-    for (int t = 0; t < 16; ++t)
-        for (int l = 0; l < 64; ++l) tab[t * 64 + l] = (float)R(l % 32, 2 * t + l / 32);
+    for (int q = 0; q < tiles; ++q)
+        for (int t = 0; t < steps; ++t)
+            for (int l = 0; l < 64; ++l)
+                tab[((size_t)q * steps + t) * 64 + l] = (float)R(32 * q + l % 32, 2 * t + l / 32);
     const size_t at = coef.size();
-    coef.resize(at + 512);
-    std::memcpy(&coef[at], tab.data(), 1024 * sizeof(float));
-    for (int k = 0; k < 256; ++k) {
+    coef.resize(at + tab.size() / 2);
+    std::memcpy(&coef[at], tab.data(), tab.size() * sizeof(float));
+    for (size_t k = 0; k < m.size(); ++k) {
         coef.push_back(m[k].real());
         coef.push_back(m[k].imag());
     }
@@ -435,7 +445,7 @@ inline MOp frame_apply_record() {
 }
 
 // Encode planned ops (blocks index into `blocks`); qm non-null enables frames.
-inline void encode_program(const std::vector<DOp>& prog, const std::vector<std::array<cx, 256>>& blocks, bool f32,
+inline void encode_program(const std::vector<DOp>& prog, const std::vector<BlockMat>& blocks, bool f32,
                            const QMap* qm, Encoded& out) {
     out.ops.clear();
     out.coef.clear();

@@ -56,7 +56,7 @@ class Stats(ctypes.Structure):
                 ("reruns", ctypes.c_uint32), ("bucket_mask", ctypes.c_uint64), ("sweeps", ctypes.c_uint64),
                 ("sweep_ops", ctypes.c_uint64), ("pairs", ctypes.c_uint64), ("remaps", ctypes.c_uint64),
                 ("moved", ctypes.c_uint64), ("sweep_ms", ctypes.c_double), ("pair_ms", ctypes.c_double), ("other_ms", ctypes.c_double),
-                ("moved_pair", ctypes.c_uint64)]
+                ("moved_pair", ctypes.c_uint64), ("blocks", ctypes.c_uint64), ("block_gates", ctypes.c_uint64)]
 
 
 class BatchCircuit(ctypes.Structure):
@@ -72,7 +72,7 @@ class BatchTuning(ctypes.Structure):
 
 
 assert ctypes.sizeof(Config) == 80
-assert ctypes.sizeof(Stats) == 136
+assert ctypes.sizeof(Stats) == 152
 assert ctypes.sizeof(BatchCircuit) == 56
 assert ctypes.sizeof(BatchTuning) == 40
 

@@ -1,5 +1,6 @@
 """FP32 fused-block variant (windows of bucket-qubit gates applied as dense
-16x16 blocks with MFMA) against the FP32 oracle and the FP64 engine.
+16x16 blocks for 4 bucket qubits, 32x32 for 5, with MFMA) against the FP32
+oracle and the FP64 engine.
 
 A block prunes once, on its outputs, where gate-by-gate execution prunes after
 every branching gate; both prune at 1e-7 componentwise with FP64 decisions near
@@ -24,11 +25,12 @@ def say(*a):
     sys.stdout.flush()
 
 
-def dense_block_circuit(seed, nq=16, layers=30):
-    """Layers of H / RY / RZ / PHASE / X on 4 qubits, each layer followed by
-    diagonal, X and controlled gates elsewhere (long fusable windows)."""
+def dense_block_circuit(seed, nq=16, layers=30, bq=(1, 4, 7, 10)):
+    """Layers of H / RY / RZ / PHASE / X on the bucket qubits bq, each layer
+    followed by diagonal, X and controlled gates elsewhere (long fusable
+    windows)."""
     rng = gen.SplitMix64(seed)
-    bq = [1, 4, 7, 10]
+    bq = list(bq)
     others = [q for q in range(nq) if q not in bq]
     gl = []
     for _ in range(layers):
@@ -67,6 +69,7 @@ def test_blocks_match_fp32_oracle_and_fp64(oracle, seed):
         e.apply(gl)
         got = e.export()
         st = e.stats()
+    assert st["blocks"] > 0
     o32 = oracle.Sparse(nq, f32=True, thr=1e-7, renorm_every=100)
     o32.load(keys, re, im)
     o32.apply(gl)
@@ -76,6 +79,36 @@ def test_blocks_match_fp32_oracle_and_fp64(oracle, seed):
     o64.apply(gl)
     d64 = compare(got, o64.export(), 1e-5, max(2, got[0].size // 1000))
     say("max diff vs fp32 oracle %.2e, vs fp64 %.2e, sweeps %d" % (d32, d64, st["sweeps"]))
+    n2 = float(np.sum(got[1] ** 2 + got[2] ** 2))
+    assert abs(n2 - 1.0) < 1e-5
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_blocks_32x32_match_fp32_oracle_and_fp64(oracle, seed):
+    """Five bucket qubits: windows become 32x32 complex blocks (two 32-row
+    MFMA tiles per half-wavefront chain)."""
+    nq = 18
+    bq = (1, 4, 7, 10, 13)
+    keys, re, im = random_state(500 + seed, nq, 400)
+    gl = dense_block_circuit(seed, nq, layers=24, bq=bq)
+    say("32x32 fused blocks seed", seed, "gates", len(gl))
+    with E.Engine(nq, prec="f32", fuse_blocks=2, renorm_every=100, bucket_bits=5) as e:
+        e.load(keys, re, im)
+        e.apply(gl)
+        got = e.export()
+        st = e.stats()
+    assert st["bucket_bits"] == 5 and st["bucket_mask"] == sum(1 << q for q in bq)
+    assert st["blocks"] > 0 and st["block_gates"] >= 2 * st["blocks"]
+    o32 = oracle.Sparse(nq, f32=True, thr=1e-7, renorm_every=100)
+    o32.load(keys, re, im)
+    o32.apply(gl)
+    d32 = compare(got, o32.export(), 1e-5, max(2, got[0].size // 1000))
+    o64 = oracle.Sparse(nq)
+    o64.load(keys, re, im)
+    o64.apply(gl)
+    d64 = compare(got, o64.export(), 1e-5, max(2, got[0].size // 1000))
+    say("blocks %d holding %d gates; max diff vs fp32 oracle %.2e, vs fp64 %.2e"
+        % (st["blocks"], st["block_gates"], d32, d64))
     n2 = float(np.sum(got[1] ** 2 + got[2] ** 2))
     assert abs(n2 - 1.0) < 1e-5
 
@@ -99,7 +132,7 @@ def test_blocks_on_large_mix_close_to_fp64(oracle):
 
 
 def test_blocks_ignored_for_fp64():
-    """The flag only applies to FP32 with 16-slot buckets; FP64 runs gate by gate."""
+    """The flag only applies to FP32 (16- or 32-slot buckets); FP64 runs gate by gate."""
     nq = 12
     keys, re, im = random_state(5, nq, 64)
     gl = dense_block_circuit(3, nq=12, layers=5)

@@ -61,7 +61,7 @@ def parse():
                     help="FP32 renormalisation: 0 one launch per period, 1 one pass per run of gates (checked)")
     ap.add_argument("--tail-gates", type=int, default=200,
                     help="also run the last N gates one gate per launch (0: skip)")
-    ap.add_argument("--cpu-chunk", type=int, default=25,
+    ap.add_argument("--cpu-chunk", type=int, default=100,
                     help="gates per CPU-baseline chunk (chunks start at gates 0, 500, 1000, 1500)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
@@ -290,6 +290,37 @@ def fp32_detail(w, fp64_value, fp64_state, steps=10, warmup=2):
     return out
 
 
+def blocks_detail(seed, steps=3, warmup=1):
+    """The fused-block variant of the FP32 form on the layered circuits (long
+    windows of single-qubit gates on the branching qubits): 16x16 blocks with
+    4 branching qubits, 32x32 blocks with 5 (bucket_bits 5), each against the
+    FP64 engine on the same circuit (gate by gate, its default bucket qubits)
+    and the FP32 form gate by gate.  Updates are counted by a gate-by-gate FP32
+    run (a block does not see the nonzeros between its gates)."""
+    from sparqc_amd import engine as E
+    from sparqc_amd import gen
+    out = {}
+    for nb, bits, label in ((4, -1, "16x16_blocks_4_qubits"), (5, 5, "32x32_blocks_5_qubits")):
+        w = gen.circuit_layers(seed, nq=48, n_init=1 << 20, n_gates=2000, n_branch=nb)
+        upd64, dt64, e64, _, _ = run_timed(w, "f64", steps, warmup)
+        want = e64.export()
+        e64.close()
+        upd32, dt32, e32, _, _ = run_timed(w, "f32", steps, warmup, bucket_bits=bits)
+        e32.close()
+        _, dtb, eb, s0, s1 = run_timed(w, "f32", steps, warmup, bucket_bits=bits, fuse_blocks=6)
+        got = eb.export()
+        eb.close()
+        row = {"workload": "48 qubits, 2^20 initial keys, 2000 gates in layers on %d branching qubits" % nb,
+               "fp64_updates_per_s": upd64 / dt64, "fp32_gate_by_gate_updates_per_s": upd32 / dt32,
+               "fp32_blocks_updates_per_s": upd32 / dtb, "blocks_ratio_to_fp64": (upd32 / dtb) / (upd64 / dt64),
+               "blocks_per_step": (s1["blocks"] - s0["blocks"]) / steps,
+               "block_gates_per_step": (s1["block_gates"] - s0["block_gates"]) / steps,
+               "fp32_bucket_bits": s1["bucket_bits"], "ms_per_step_fp64": dt64 * 1e3, "ms_per_step_blocks": dtb * 1e3}
+        row.update(precision_report(got, want))
+        out[label] = row
+    return out
+
+
 def traffic_from_profiles(prec, window):
     """HBM bytes per launch measured by rocprofv3 counters, when committed."""
     p = os.path.join(ROOT, "profiles", "sweep_traffic.json")
@@ -297,6 +328,16 @@ def traffic_from_profiles(prec, window):
         with open(p) as f:
             d = json.load(f)
         return d.get("%s_w%d" % (prec, window))
+    except (OSError, ValueError):
+        return None
+
+
+def counters_source():
+    """Which build the committed counters (profiles/sweep_traffic.json) were taken on."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "sweep_traffic.json")) as f:
+            src = json.load(f).get("source", {})
+        return {"file": "profiles/sweep_traffic.json", "commit": src.get("commit")}
     except (OSError, ValueError):
         return None
 
@@ -595,6 +636,7 @@ def main():
                              "is instruction-issue bound, not HBM bound ('issue'); 'unfused_tail' runs the "
                              "same kernel one gate per launch, where HBM is the bound",
                      "issue": valu_from_profiles(a, sweep_s / a.steps),
+                     "counters_from": counters_source(),
                      "per_gate_model": {"bytes_per_update": PER_UPDATE_BYTES[a.prec],
                                         "equivalent_GBps": upd / dt * PER_UPDATE_BYTES[a.prec] / 1e9,
                                         "equivalent_frac": upd / dt * PER_UPDATE_BYTES[a.prec] / HBM_PEAK},
@@ -619,6 +661,7 @@ def main():
         # the same run (their own fixed sizes, a few seconds each)
         out["detail"]["fp32"] = fp32_detail(w, upd / dt, final)
         final = None
+        out["detail"]["fp32_blocks"] = blocks_detail(a.seed)
         out["detail"]["batch"] = {p: run_batch(a.seed, 256, p, steps=3, warmup=1) for p in ("f64", "f32")}
         out["detail"]["serve"] = run_serve(a.seed, 300, [200.0, 1000.0, 5000.0], "f64", a.step_gates)
     final = None

@@ -263,8 +263,15 @@ typedef struct sqe_batch_tuning {
 } sqe_batch_tuning;
 
 /* cfg->nq is ignored (each circuit has its own); prec, prune_thr, max_load,
- * hbm_budget_bytes (admission budget), device, bucket_bits, renorm_every and
- * timing apply to every circuit. */
+ * hbm_budget_bytes (admission budget), device, bucket_bits, renorm_every,
+ * renorm_mode and timing apply to every circuit.  FP32 circuits with
+ * renorm_every and renorm_mode 1 renormalise in one pass: an FP64 norm
+ * checkpoint every renorm_every gates and one scale at the end, so a run is
+ * one sweep launch per bucket size, a checkpoint fold and a scale launch; a
+ * checkpoint norm more than 2^-11 from 1 fails the circuit (SQE_EDEVICE from
+ * sqe_batch_info / export: admit it with renorm_mode 0, which stops at every
+ * renormalisation point).  FP64 circuits run their whole program in one
+ * sweep launch per bucket size. */
 sqe_status sqe_batch_create(const sqe_config* cfg, sqe_batch** out);
 sqe_status sqe_batch_destroy(sqe_batch* b);
 

@@ -167,6 +167,7 @@ struct Circ {
     uint64_t gates = 0;
     uint64_t bytes = 0;
     bool frames = false;                  // the program holds frame records
+    uint64_t ck0 = 0, nck = 0;            // FP32 one-pass renormalisation: checkpoint partials (nck x tiles)
 };
 
 struct Pool {
@@ -246,8 +247,8 @@ struct sqe_batch {
     hipStream_t st = nullptr;
     std::vector<Circ> c;                   // indexed by slot (= id)
     std::vector<uint32_t> free_slots;
-    RangeAlloc ta, ra, pa, ca;             // tiles, rows, program ops, coefficients
-    Pool base, amp, prog, coef;
+    RangeAlloc ta, ra, pa, ca, ka;         // tiles, rows, program ops, coefficients, checkpoint partials
+    Pool base, amp, prog, coef, ck;
     uint64_t used = 0;                     // admitted bytes
     // per-slot device arrays
     uint64_t nslot = 0;
@@ -533,6 +534,9 @@ sqe_status read_counters(sqe_batch* b, uint32_t id, unsigned long long v[4]) {
 }
 
 sqe_status load_fail(unsigned long long err) {
+    if (err & ERR_RENORM)
+        return bfail(SQE_EDEVICE, "FP32 one-pass renormalisation: a checkpoint norm moved by more than 2^-11 "
+                                  "(admit the circuit with renorm_mode 0 to renormalise step by step)");
     return bfail((err & ERR_DUP) ? SQE_EINVAL : SQE_EHIP,
                  (err & ERR_DUP) ? "duplicate key in the circuit's load" : "device-side error in the circuit's load");
 }
@@ -646,6 +650,7 @@ void release(sqe_batch* b, uint32_t id) {
     b->ra.give(c.row0, c.rows);
     if (c.nops) b->pa.give(c.prog0, c.nops);
     if (c.ncoef) b->ca.give(c.coef0, c.ncoef);
+    if (c.nck) b->ka.give(c.ck0, c.nck * c.tiles);
     b->used -= c.bytes;
     c = Circ{};
     b->free_slots.push_back(id);
@@ -657,6 +662,7 @@ void release(sqe_batch* b, uint32_t id) {
 // FP64 norm, over a task list (one task per circuit) and its work items.
 struct Group {
     bool norm = false;
+    bool fold = false;                      // norm group of one-pass circuits: fold their checkpoints
     int mb = 0;
     std::vector<BTask> tasks;
     std::vector<BItem> items;
@@ -703,6 +709,8 @@ bool plan_step(sqe_batch* b, uint64_t max_ops, std::vector<uint64_t>& pos, std::
             t.ntiles = (uint32_t)c.tiles;
             t.qm = c.qm;
             t.frames = c.frames ? 1 : 0;
+            t.ckpt = c.nck ? (double*)b->ck.p + c.ck0 : nullptr;
+            t.nck = (uint32_t)c.nck;
             t.item0 = (uint32_t)g.items.size();
             const uint32_t ti = (uint32_t)g.tasks.size();
             for (uint64_t tl = 0; tl < c.tiles; tl += b->gran) g.items.push_back(BItem{ti, (uint32_t)tl});
@@ -721,7 +729,7 @@ bool plan_step(sqe_batch* b, uint64_t max_ops, std::vector<uint64_t>& pos, std::
         g.mb = mb;
         for (uint32_t id : need_norm) {
             const Circ& c = b->c[id];
-            if (c.qm.mb != mb) continue;
+            if (c.qm.mb != mb || c.nck) continue;
             BTask t{};
             t.amp = amp_ptr(b, c);
             t.norm2 = b->norm2 + id;
@@ -734,6 +742,28 @@ bool plan_step(sqe_batch* b, uint64_t max_ops, std::vector<uint64_t>& pos, std::
             g.tasks.push_back(t);
         }
         if (!g.tasks.empty()) out.push_back(std::move(g));
+    }
+    // one-pass circuits at their final renormalisation: fold the checkpoints
+    // (one group per bucket size, so a captured run keeps each fold on the
+    // branch of the sweep that wrote its partials)
+    for (int mb = 0; mb <= 5; ++mb) {
+        Group f;
+        f.norm = true;
+        f.fold = true;
+        f.mb = mb;
+        for (uint32_t id : need_norm) {
+            const Circ& c = b->c[id];
+            if (!c.nck || c.qm.mb != mb) continue;
+            BTask t{};
+            t.norm2 = b->norm2 + id;
+            t.scale = b->scale + id;
+            t.ntiles = (uint32_t)c.tiles;
+            t.ckpt = (double*)b->ck.p + c.ck0;
+            t.nck = (uint32_t)c.nck;
+            t.err = &b->cst[id].err;
+            f.tasks.push_back(t);
+        }
+        if (!f.tasks.empty()) out.push_back(std::move(f));
     }
     return true;
 }
@@ -771,6 +801,8 @@ sqe_status launch_group(sqe_batch* b, hipStream_t st, const Group& g, const BTas
             b->sweep_ms += ms;
         }
         BHIP(launch_batch_fold(st, t, (uint32_t)g.tasks.size(), counts));
+    } else if (g.fold) {
+        BHIP(launch_ckpt_batch(st, t, (uint32_t)g.tasks.size(), 0x1p-11));
     } else {
         BatchArgs a{t, it, (uint32_t)g.items.size(), NORM_TILES, b->thr, partials, nullptr};
         const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(4096, ((uint32_t)g.items.size() + 3) / 4));
@@ -1035,7 +1067,7 @@ sqe_status sqe_batch_destroy(sqe_batch* b) {
         if (b->hrec) (void)hipHostFree(b->hrec);
         if (b->hsmall) (void)hipHostFree(b->hsmall);
         if (b->xst) (void)hipStreamDestroy(b->xst);
-        for (void* p : {(void*)b->cst, (void*)b->base.p, (void*)b->amp.p, (void*)b->prog.p, (void*)b->coef.p, (void*)b->upd,
+        for (void* p : {(void*)b->cst, (void*)b->base.p, (void*)b->amp.p, (void*)b->prog.p, (void*)b->coef.p, (void*)b->ck.p, (void*)b->upd,
                         (void*)b->nnz, (void*)b->meta, (void*)b->norm2, (void*)b->scale, (void*)b->tab,
                         (void*)b->xkeys, (void*)b->occ, (void*)b->stats,
                         (void*)b->scan_status, (void*)b->ticket, (void*)b->total, (void*)b->dtask, (void*)b->ditem, (void*)b->partials,
@@ -1076,17 +1108,31 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
                                              " crosses buckets: run this circuit with the single-circuit engine");
         // program: gates, with a RENORM op after every renorm_every-th gate
         // (Pauli gates on bucket qubits as frame records when the keys leave
-        // room for the frame)
+        // room for the frame).  FP32 with renorm_mode 1 renormalises in one
+        // pass: an FP64 norm checkpoint after every renorm_every-th gate and
+        // one RENORM op at the end, which applies the last checkpoint's scale
+        // (a global scalar, so applying it late changes nothing but the prune
+        // decisions' missing factor, checked within 2^-11 of 1).
+        const bool onepass = b->f32 && b->cfg.renorm_every && b->cfg.renorm_mode == 1;
         std::vector<DOp> dprog;
         std::vector<size_t> gate_idx, ren_idx;
         dprog.reserve(ng + 8);
+        uint32_t nck = 0;
         for (uint64_t j = 0; j < ng; ++j) {
             gate_idx.push_back(dprog.size());
             dprog.push_back(plan_op(gates[j], qm));
             if (b->cfg.renorm_every && (j + 1) % b->cfg.renorm_every == 0) {
-                ren_idx.push_back(dprog.size());
-                dprog.push_back(renorm_op());
+                if (onepass) {
+                    dprog.push_back(norm_point_op(nck++));
+                } else {
+                    ren_idx.push_back(dprog.size());
+                    dprog.push_back(renorm_op());
+                }
             }
+        }
+        if (nck) {
+            ren_idx.push_back(dprog.size());
+            dprog.push_back(renorm_op());
         }
         const bool frames = !b->cfg.frame_off && nq <= (uint32_t)FRAME_Z_BIT0 && qm.mb > 0;
         Encoded enc;
@@ -1124,7 +1170,8 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         const uint64_t S = 1ull << qm.mb;
         const uint64_t tiles = std::max<uint64_t>(1, (m + 63) / 64);
         const uint64_t rows = tiles * S;
-        const uint64_t bytes = tiles * 64 * 8 + rows * 64 * b->csize + prog.size() * sizeof(MOp) + coef.size() * 8;
+        const uint64_t bytes = tiles * 64 * 8 + rows * 64 * b->csize + prog.size() * sizeof(MOp) + coef.size() * 8 +
+                               (uint64_t)nck * tiles * 8;
         if (b->used + bytes > b->cfg.hbm_budget_bytes) {
             unreserve();
             return bfail(SQE_ETABLE_FULL, "admission: circuit needs " + std::to_string(bytes) + " bytes, " +
@@ -1146,6 +1193,7 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         c.gates = ng;
         c.bytes = bytes;
         c.frames = framed;
+        c.nck = nck;
         auto place = [&](RangeAlloc& al, Pool& pl, uint64_t units, uint64_t unit_bytes, uint64_t* at) -> sqe_status {
             if (!units) { *at = 0; return SQE_OK; }
             if (!al.take(units, at)) {
@@ -1161,10 +1209,12 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         if (st == SQE_OK && ++placed) st = place(b->ra, b->amp, rows, 64 * b->csize, &c.row0);
         if (st == SQE_OK && ++placed) st = place(b->pa, b->prog, c.nops, sizeof(MOp), &c.prog0);
         if (st == SQE_OK && ++placed) st = place(b->ca, b->coef, c.ncoef, 8, &c.coef0);
+        if (st == SQE_OK && ++placed) st = place(b->ka, b->ck, c.nck * c.tiles, 8, &c.ck0);
         if (st != SQE_OK) {
             if (placed > 0) b->ta.give(c.tile0, c.tiles);
             if (placed > 1) b->ra.give(c.row0, c.rows);
             if (placed > 2 && c.nops) b->pa.give(c.prog0, c.nops);
+            if (placed > 3 && c.ncoef) b->ca.give(c.coef0, c.ncoef);
             b->free_slots.push_back(id);
             unreserve();
             return st;

@@ -1016,7 +1016,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
         if (tile >= t.ntiles) break;
         const int64_t tl[1] = {(int64_t)tile};
         SweepCount c{0, 0, 0, 0};
-        const SweepCtx ctx{t.qm, t.frames, 0, it, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
+        // (FP32 one-pass renormalisation: norm points write one partial per tile)
+        const SweepCtx ctx{t.qm, t.frames, t.ntiles, tile, nullptr, nullptr, t.ckpt, nullptr, 0, nullptr};
         sweep_group<MB, R, 1>(t.base, t.amp, tl, t.ops, t.nops, t.coef, A.thr, t.scale, 1, c, ctx);
         upd += c.upd;
         nnz += c.nnz_out;

@@ -411,6 +411,41 @@ __global__ __launch_bounds__(BLOCK) void k_norm_batch_final(const BTask* __restr
     }
 }
 
+// FP32 one-pass renormalisation of batched circuits, one wavefront per
+// circuit: checkpoint k's norm N_k is the fixed-order sum of the circuit's
+// per-tile partials ckpt[k * ntiles + tile] (the same for the circuit alone
+// or in any mix); the scale of the last checkpoint, 1/sqrt(N), goes to the
+// circuit's scale slot for its final renormalisation record.  A checkpoint
+// whose scale lies more than tol from 1 (the prune decisions of the pass
+// would then lack a material scale) marks the circuit with ERR_RENORM.
+__global__ __launch_bounds__(BLOCK) void k_ckpt_batch(const BTask* __restrict__ tasks, uint32_t ntasks, double tol) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t ti = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    if (ti >= ntasks) return;
+    const BTask t = tasks[ti];
+    double last = 1.0;
+    bool bad = false;
+    for (uint32_t k = 0; k < t.nck; ++k) {
+        double acc = 0.0;
+        for (uint32_t i = lane; i < t.ntiles; i += 64) acc += t.ckpt[(size_t)k * t.ntiles + i];
+        for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);
+        last = acc;
+        bad = bad || !(acc > 0.0) || !(fabs(1.0 / sqrt(acc) - 1.0) < tol);
+    }
+    if (lane == 0) {
+        *t.norm2 = last;
+        *t.scale = last > 0.0 ? 1.0 / sqrt(last) : 1.0;
+        if (bad) atomicOr(t.err, ERR_RENORM);
+    }
+}
+
+hipError_t launch_ckpt_batch(hipStream_t st, const BTask* tasks, uint32_t ntasks, double tol) {
+    if (!ntasks) return hipSuccess;
+    hipLaunchKernelGGL(k_ckpt_batch, dim3((ntasks + BLOCK / 64 - 1) / (BLOCK / 64)), dim3(BLOCK), 0, st, tasks, ntasks,
+                       tol);
+    return hipGetLastError();
+}
+
 // One wavefront per circuit: fold its items' counts in a fixed order (each
 // circuit appears in one task of a launch, so the plain update is race-free).
 __global__ __launch_bounds__(BLOCK) void k_batch_fold(const BTask* __restrict__ tasks, uint32_t ntasks,

@@ -67,6 +67,10 @@ struct BTask {
     QMap qm;                     // the circuit's bucket qubits
     int32_t frames;              // its program holds frame records
     int32_t pad;
+    double* ckpt;                // FP32 one-pass renormalisation: norm partials, ntiles per checkpoint
+    uint32_t nck;                // its checkpoints
+    uint32_t pad2;
+    unsigned long long* err;     // the circuit's error word
 };
 
 struct BItem {                   // a circuit and the first of `gran` consecutive tiles
@@ -221,6 +225,7 @@ hipError_t launch_norm(int mb, bool f32, hipStream_t st, uint32_t grid, const No
                        double* scale);
 hipError_t launch_index_build(hipStream_t st, uint32_t grid, const IndexArgs& a);
 hipError_t launch_index_enumerate(hipStream_t st, uint32_t grid, const EnumArgs& e);
+hipError_t launch_ckpt_batch(hipStream_t st, const BTask* tasks, uint32_t ntasks, double tol);
 // norm2 and scale from n per-workgroup partials (of a sweep with norm_out)
 hipError_t launch_norm_final(hipStream_t st, const double* partials, uint32_t n, double* norm2, double* scale);
 // One wavefront per work item: grid = ceil(nitems / (block / 64)).

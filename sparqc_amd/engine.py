@@ -355,13 +355,17 @@ class Batch:
     """Many independent circuits on one GPU, advanced together (sqe_batch_*)."""
 
     def __init__(self, prec="f64", prune_thr=0.0, bucket_bits=-1, renorm_every=0, timing=False,
-                 device=0, hbm_budget_bytes=0, max_load=0.5, eager=False, frames=True):
+                 device=0, hbm_budget_bytes=0, max_load=0.5, eager=False, frames=True, renorm_mode=1):
         """eager: run() launches every step directly instead of replaying one
-        captured hipGraph of the whole run."""
+        captured hipGraph of the whole run.  renorm_mode (FP32 with
+        renorm_every): 1 renormalises in one pass (norm checkpoints, one
+        scale at the end), 0 stops every circuit at each renormalisation
+        point."""
         L = lib()
         cfg = Config()
         L.sqe_config_default(ctypes.byref(cfg), 1)
         cfg.batch_eager = 1 if eager else 0
+        cfg.renorm_mode = 1 if int(renorm_mode) == 1 else 0
         cfg.frame_off = 0 if frames else 1
         cfg.prec = F32 if prec in ("f32", F32) else F64
         cfg.prune_thr = prune_thr

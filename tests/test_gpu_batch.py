@@ -88,6 +88,57 @@ def test_batch_f32_renorm(oracle):
             assert abs(n2 - 1.0) < 1e-5
 
 
+def test_batch_f32_one_pass_against_step_mode(oracle):
+    """One-pass renormalisation (norm checkpoints, one scale at the end): a run
+    is two sweep launches per bucket size instead of one step per period, and
+    the states agree with the step-by-step batch and the FP32 oracle."""
+    cs = small_batch(9, 10, depth=(250, 480))
+    out = {}
+    for mode in (1, 0):
+        with E.Batch(prec="f32", renorm_every=100, renorm_mode=mode) as b:
+            ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+            sizes = {b.info(i)["bucket_bits"] for i in ids}
+            l0 = b.tuning()["launches"]
+            b.run()
+            b.sync()
+            launches = b.tuning()["launches"] - l0
+            out[mode] = ([b.export(i) for i in ids], [b.info(i)["updates"] for i in ids], launches, len(sizes))
+        say("renorm_mode %d: %d sweep launches, %d bucket sizes" % (mode, launches, len(sizes)))
+    one, step = out[1], out[0]
+    assert one[2] <= 2 * one[3] < step[2]
+    for i, c in enumerate(cs):
+        a, b = one[0][i], step[0][i]
+        common, ia, ib = np.intersect1d(a[0], b[0], return_indices=True)
+        assert common.size >= b[0].size - 2
+        d = max(np.max(np.abs(a[1][ia] - b[1][ib])), np.max(np.abs(a[2][ia] - b[2][ib])))
+        assert d <= 1e-6, "circuit %d: %.3e" % (i, d)
+        assert one[1][i] == step[1][i]
+        want, _ = oracle_of(oracle, c, f32=True, renorm_every=100)
+        common, ia, iw = np.intersect1d(a[0], want[0], return_indices=True)
+        d = max(np.max(np.abs(a[1][ia] - want[1][iw])), np.max(np.abs(a[2][ia] - want[2][iw])))
+        assert d <= 1e-6, "circuit %d vs oracle: %.3e" % (i, d)
+        assert abs(float(np.sum(a[1] ** 2 + a[2] ** 2)) - 1.0) < 1e-5
+
+
+def test_batch_f32_one_pass_reports_a_drifting_norm():
+    """A non-unitary gate (2 x identity) moves the checkpoint norm far beyond
+    2^-11: the one-pass circuit is reported failed, its neighbour is not."""
+    cs = small_batch(10, 2, depth=(120, 150))
+    bad = list(cs[0]["gates"])
+    u = G.make_gate(G.U, int(cs[0]["branch"][0]), matrix=[[2.0, 0.0], [0.0, 2.0]])
+    bad.insert(10, u)
+    with E.Batch(prec="f32", renorm_every=100) as b:
+        i0 = b.add(cs[0]["nq"], cs[0]["keys"], cs[0]["re"], cs[0]["im"], G.gate_array(bad))
+        i1 = b.add(cs[1]["nq"], cs[1]["keys"], cs[1]["re"], cs[1]["im"], cs[1]["gates"])
+        b.run()
+        b.sync()
+        with pytest.raises(E.SqeError) as ex:
+            b.info(i0)
+        assert ex.value.status == E.SQE_EDEVICE and "one-pass" in str(ex.value)
+        assert b.info(i1)["done"] == 1
+        b.export(i1)
+
+
 def test_batch_f32_matches_solo_engine():
     """FP32 batch and the single-circuit FP32 engine agree (both renormalise in FP64)."""
     cs = small_batch(8, 6, depth=(120, 220))

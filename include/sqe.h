@@ -34,7 +34,9 @@ typedef enum {
     SQE_ENOMEM = -2,       /* device or host allocation failed                    */
     SQE_ETABLE_FULL = -3,  /* the state would exceed the configured HBM budget    */
     SQE_EHIP = -4,         /* HIP runtime error or kernel-reported fault          */
-    SQE_EDEVICE = -5       /* no usable GPU                                       */
+    SQE_EDEVICE = -5,      /* no usable GPU                                       */
+    SQE_ENOTLOCAL = -6     /* batch: the circuit has cross-bucket gates (it
+                              belongs on the single-circuit engine)              */
 } sqe_status;
 
 typedef enum { SQE_F64 = 0, SQE_F32 = 1 } sqe_prec;

@@ -1104,8 +1104,8 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         const QMap qm = choose_map((int)nq, b->cfg.bucket_bits, gates, ng);
         for (uint64_t j = 0; j < ng; ++j)
             if (!gate_local(gates[j], qm))
-                return bfail(SQE_EINVAL, "gate " + std::to_string(j) +
-                                             " crosses buckets: run this circuit with the single-circuit engine");
+                return bfail(SQE_ENOTLOCAL, "gate " + std::to_string(j) +
+                                                " crosses buckets: run this circuit with the single-circuit engine");
         // program: gates, with a RENORM op after every renorm_every-th gate
         // (Pauli gates on bucket qubits as frame records when the keys leave
         // room for the frame).  FP32 with renorm_mode 1 renormalises in one

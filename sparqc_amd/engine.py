@@ -13,8 +13,8 @@ from . import gates as G
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "libsqe.so")
 
-SQE_OK, SQE_EINVAL, SQE_ENOMEM, SQE_ETABLE_FULL, SQE_EHIP, SQE_EDEVICE = 0, -1, -2, -3, -4, -5
-STATUS_NAMES = {0: "SQE_OK", -1: "SQE_EINVAL", -2: "SQE_ENOMEM", -3: "SQE_ETABLE_FULL",
+SQE_OK, SQE_EINVAL, SQE_ENOMEM, SQE_ETABLE_FULL, SQE_EHIP, SQE_EDEVICE, SQE_ENOTLOCAL = 0, -1, -2, -3, -4, -5, -6
+STATUS_NAMES = {0: "SQE_OK", -1: "SQE_EINVAL", -2: "SQE_ENOMEM", -3: "SQE_ETABLE_FULL", -6: "SQE_ENOTLOCAL",
                 -4: "SQE_EHIP", -5: "SQE_EDEVICE"}
 F64, F32 = 0, 1
 

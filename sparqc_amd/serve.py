@@ -18,6 +18,12 @@ Requests arrive at given times (seconds from the start).  The loop
 Only an idle loop waits for the device (for the oldest export, or sleeps until
 the next arrival), so admissions, steps and result copies overlap.
 
+A request the batch engine cannot hold -- one whose branching gates target
+more qubits than a bucket holds, so some of its gates cross buckets
+(SQE_ENOTLOCAL) -- is served on the single-circuit engine instead (one engine
+per qubit count and precision, kept for reuse), inside the loop at its turn
+in the arrival order; its latency counts like any other.
+
 Each circuit advances by the same gate step from its own first gate whatever
 else is in the batch, so its result does not depend on the mix: the solo and
 mixed runs of a circuit agree to the bit (checked by tests/test_gpu_serve.py).
@@ -48,6 +54,9 @@ class ServeLoop:
                             timing=timing, device=device)
         self.batch = batch
         self._pool = {}                         # capacity -> page-locked export buffers, reused
+        self._solo = {}                         # qubit count -> single-circuit engine for wide requests
+        self._renorm_every = renorm_every
+        self._device = device
         self.step_gates = int(step_gates)
         self.prec = getattr(batch, "prec", prec)
 
@@ -79,7 +88,24 @@ class ServeLoop:
             for buf in bufs:
                 buf.free()
         self._pool = {}
+        for e in self._solo.values():
+            e.close()
+        self._solo = {}
         self.batch.close()
+
+    def run_single(self, r):
+        """A request with cross-bucket gates on the single-circuit engine:
+        (state sorted by key, updates, nonzeros)."""
+        e = self._solo.get(r["nq"])
+        if e is None:
+            e = E.Engine(r["nq"], prec=self.prec, renorm_every=self._renorm_every or 0, device=self._device)
+            self._solo[r["nq"]] = e
+        u0 = e.stats()["updates"]
+        e.load(r["keys"], r["re"], r["im"])
+        e.apply(r["gates"])
+        state = e.export()
+        st = e.stats()
+        return state, st["updates"] - u0, st["nnz"]
 
     def __enter__(self):
         return self
@@ -115,6 +141,7 @@ class ServeLoop:
         peak_live = 0
         peak_bytes = 0
         peak_pending = 0
+        single = 0                              # requests served on the single-circuit engine
         t0 = clock()
 
         def collect(wait):
@@ -148,7 +175,20 @@ class ServeLoop:
             # 1. admission (FIFO, by HBM budget)
             while nxt < n and arrivals[nxt] <= now:
                 r = requests[nxt]
-                cid = b.try_add(r["nq"], r["keys"], r["re"], r["im"], r["gates"])
+                try:
+                    cid = b.try_add(r["nq"], r["keys"], r["re"], r["im"], r["gates"])
+                except E.SqeError as ex:
+                    if ex.status != E.SQE_ENOTLOCAL:
+                        raise
+                    state, upd, nnz = self.run_single(r)
+                    updates += upd
+                    single += 1
+                    latency[nxt] = clock() - t0 - arrivals[nxt]
+                    admit_t[nxt] = clock() - t0
+                    if keep_results:
+                        results[nxt] = dict(state=state, updates=upd, nnz=nnz)
+                    nxt += 1
+                    continue
                 if cid is None:
                     if not live:
                         raise E.SqeError(E.SQE_ETABLE_FULL,
@@ -186,6 +226,7 @@ class ServeLoop:
                    p50_latency_s=percentile(lat, 50), p99_latency_s=percentile(lat, 99),
                    mean_latency_s=float(np.mean(lat)) if lat else None,
                    peak_live=peak_live, peak_pool_bytes=peak_bytes, peak_exports_in_flight=peak_pending,
+                   single_engine_requests=single,
                    latency_s=latency, errors=errors,
                    queue_wait_s=[(a - arrivals[i]) if a is not None else None for i, a in enumerate(admit_t)])
         if keep_results:

@@ -226,13 +226,13 @@ def test_tuning_grid(oracle, block, gran):
 
 
 def test_batch_rejects_cross_bucket_circuit():
-    """Branching on more qubits than a bucket holds is refused with EINVAL."""
+    """Branching on more qubits than a bucket holds is refused with ENOTLOCAL."""
     nq = 12
     gl = G.gate_array([G.make_gate(G.H, q) for q in range(6)])
     with E.Batch() as b:
         with pytest.raises(E.SqeError) as ex:
             b.add(nq, np.zeros(1, dtype=np.uint64), np.ones(1), np.zeros(1), gl)
-        assert ex.value.status == E.SQE_EINVAL
+        assert ex.value.status == E.SQE_ENOTLOCAL
         assert b.usage()["used"] == 0
 
 

@@ -63,3 +63,23 @@ def test_budget_limited_serving_completes(oracle):
         o.load(r["keys"], r["re"], r["im"])
         o.apply(r["gates"])
         assert_same_state(rep["results"][i]["state"], o.export(), atol=1e-12, label="request %d" % i)
+
+
+def test_cross_bucket_requests_served_beside_the_batch(oracle):
+    """Requests branching on 6 qubits (more than a bucket holds) go to the
+    single-circuit engine; with the batched ones they all match the oracle."""
+    reqs = requests(23, 6)
+    for k in (1, 4):
+        w = gen.circuit_large(40 + k, nq=22, n_init=64, n_gates=150, n_branch=6)
+        reqs[k] = dict(nq=22, keys=w["keys"], re=w["re"], im=w["im"], gates=w["gates"])
+    arr = gen.poisson_arrivals(6, 2000.0, len(reqs))
+    with ServeLoop(prec="f64", step_gates=32, hbm_budget_bytes=1 << 30) as s:
+        rep = s.run(reqs, arr, keep_results=True)
+    say("single-engine requests:", rep["single_engine_requests"])
+    assert rep["single_engine_requests"] == 2
+    for i, r in enumerate(reqs):
+        o = oracle.Sparse(r["nq"])
+        o.load(r["keys"], r["re"], r["im"])
+        o.apply(r["gates"])
+        assert_same_state(rep["results"][i]["state"], o.export(), atol=1e-12, label="request %d" % i)
+        assert rep["results"][i]["updates"] == o.updates()

@@ -22,6 +22,8 @@ class ModelBatch:
         self.log = []
 
     def try_add(self, nq, keys, re, im, gates):
+        if nq > 40:                              # the model's stand-in for a cross-bucket circuit
+            raise E.SqeError(E.SQE_ENOTLOCAL, "gate 0 crosses buckets")
         need = int(keys.size) * 100
         if self.used + need > self.budget:
             return None
@@ -180,3 +182,28 @@ def test_failed_export_is_reported_and_the_rest_complete():
     assert list(rep["errors"]) == [1]
     assert all(x is not None for x in rep["latency_s"])
     assert rep["updates"] == 10 * 4 + 20 * 4
+
+
+def test_cross_bucket_request_served_on_single_engine():
+    """A request the batch refuses as not bucket-local (SQE_ENOTLOCAL) is
+    served on the single-circuit engine at its turn; the others are batched."""
+    clk = Clock()
+    mb = ModelBatch(budget=10**9, clock=clk)
+    s = ServeLoop(batch=mb, step_gates=10)
+    ran = []
+
+    def single(r):
+        ran.append(r["nq"])
+        clk.t += 0.005                          # the single engine takes 5 ms
+        n = int(r["keys"].size)
+        return (np.arange(n, dtype=np.uint64), np.zeros(n), np.zeros(n)), 7, n
+
+    s.run_single = single
+    wide = dict(req(6, 30), nq=48)
+    rep = s.run([req(10, 10), wide, req(10, 10)], [0.0, 0.0, 0.0], clock=clk, sleep=clk.sleep, keep_results=True)
+    assert ran == [48]
+    assert rep["single_engine_requests"] == 1
+    assert rep["latency_s"][1] == pytest.approx(0.005)
+    assert rep["results"][1]["updates"] == 7 and rep["results"][1]["nnz"] == 6
+    assert [e[1] for e in mb.log if e[0] == "add"] == [0, 1]     # requests 0 and 2 were batched
+    assert rep["updates"] == 10 * 10 + 7 + 10 * 10

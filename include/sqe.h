@@ -236,7 +236,7 @@ const char* sqe_last_error(void);
  * has no request queue; this is the batched / serving form of the same
  * gate kernels.  Every gate of an admitted circuit must be bucket-local for
  * the bucket qubits the planner picks (its branching targets, at most 4):
- * sqe_batch_add rejects a circuit with any other gate (SQE_EINVAL), which
+ * sqe_batch_add rejects a circuit with any other gate (SQE_ENOTLOCAL), which
  * then runs on the single-circuit engine.
  * ------------------------------------------------------------------------ */
 typedef struct sqe_batch sqe_batch;
@@ -285,6 +285,18 @@ sqe_status sqe_batch_destroy(sqe_batch* b);
  * sqe_batch_info, sqe_batch_export and sqe_batch_export_poll (SQE_EINVAL). */
 sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const double* re, const double* im,
                          uint64_t n, const sqe_gate* gates, uint64_t ng, int64_t* id);
+
+/* sqe_batch_add for a state already in page-locked host memory
+ * (sqe_host_alloc; a serving front end that receives requests into such
+ * buffers): the arrays are uploaded as they are, with no host-side pass over
+ * them, so they must stay unchanged until the upload has completed (after
+ * the next sqe_batch_sync, or once the circuit's export has completed).
+ * Entries with a zero amplitude load as absent; a key with bits beyond nq is
+ * reported for this circuit by sqe_batch_info / export (SQE_EINVAL), like a
+ * duplicate key.  One upload when re and im follow keys at one pitch (the
+ * three arrays of one sqe_host_alloc buffer).  No reference counterpart. */
+sqe_status sqe_batch_add_pinned(sqe_batch* b, uint32_t nq, const uint64_t* keys, const double* re, const double* im,
+                                uint64_t n, const sqe_gate* gates, uint64_t ng, int64_t* id);
 
 /* Advance every unfinished circuit by up to max_gates program records
  * (0: to its next FP64 renormalisation point or its end). */

@@ -534,6 +534,7 @@ sqe_status read_counters(sqe_batch* b, uint32_t id, unsigned long long v[4]) {
 }
 
 sqe_status load_fail(unsigned long long err) {
+    if (err & ERR_KEY) return bfail(SQE_EINVAL, "a key of the circuit's load has bits beyond nq");
     if (err & ERR_RENORM)
         return bfail(SQE_EDEVICE, "FP32 one-pass renormalisation: a checkpoint norm moved by more than 2^-11 "
                                   "(admit the circuit with renorm_mode 0 to renormalise step by step)");
@@ -1085,8 +1086,13 @@ sqe_status sqe_batch_destroy(sqe_batch* b) {
     });
 }
 
-sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const double* re, const double* im,
-                         uint64_t n, const sqe_gate* gates, uint64_t ng, int64_t* id_out) {
+// Admit a circuit (sqe_batch_add, sqe_batch_add_pinned).  Copied: the
+// entries are checked, filtered and packed into the page-locked upload ring
+// on the host.  Pinned: the caller's arrays are uploaded as they are (zero
+// amplitudes load as absent entries of live buckets, keys beyond nq are
+// reported by the load kernels in the circuit's error word).
+sqe_status admit(sqe_batch* b, uint32_t nq, const uint64_t* keys, const double* re, const double* im, uint64_t n,
+                 const sqe_gate* gates, uint64_t ng, int64_t* id_out, bool pinned) {
     return guard([&]() -> sqe_status {
         if (!b || !id_out) return bfail(SQE_EINVAL, "null argument");
         if (nq < 1 || nq > 63) return bfail(SQE_EINVAL, "nq must be in 1..63");
@@ -1144,27 +1150,28 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         for (size_t g : gate_idx) gate_at.push_back((uint32_t)(enc.start[g + 1] - 1));   // the gate's own record
         const bool framed = enc.framed;
         // the uploads go through the page-locked upload ring: program,
-        // coefficients, then the keys, real and imaginary parts of the entries
-        // with a nonzero amplitude, packed (one pass over the input: key range
-        // check, zero filter and copy); the device staging they land in is
-        // rewritten in stream order
+        // coefficients, then (copied admissions) the keys, real and imaginary
+        // parts of the entries with a nonzero amplitude, packed (one pass over
+        // the input: key range check, zero filter and copy); the device
+        // staging they land in is rewritten in stream order
         const uint64_t pc = prog.size() * sizeof(MOp) + coef.size() * 8;
         uint64_t up_off = 0;
-        BCK(ring_take(b, b->up, pc + 24 * n, UP_RING_MIN, &up_off));
+        BCK(ring_take(b, b->up, pc + (pinned ? 0 : 24 * n), UP_RING_MIN, &up_off));
         char* const hp0 = b->up.p + up_off;
         uint64_t* hk = (uint64_t*)(hp0 + pc);
-        uint64_t keybits = 0;
-        const uint64_t m = copy_entries(b, keys, re, im, n, hk, (double*)(hk + n), (double*)(hk + 2 * n), &keybits);
-        double* hr = (double*)(hk + m);
-        double* hi = hr + m;
-        if (m < n) {                             // pack at stride m
-            std::memmove(hr, hk + n, m * 8);
-            std::memmove(hi, hk + 2 * n, m * 8);
-        }
+        uint64_t m = n;
         auto unreserve = [&]() { b->up.head = up_off; };
-        if (keybits & ~lim) {
-            unreserve();
-            return bfail(SQE_EINVAL, "key has bits beyond nq");
+        if (!pinned) {
+            uint64_t keybits = 0;
+            m = copy_entries(b, keys, re, im, n, hk, (double*)(hk + n), (double*)(hk + 2 * n), &keybits);
+            if (m < n) {                             // pack at stride m
+                std::memmove(hk + m, hk + n, m * 8);
+                std::memmove(hk + 2 * m, hk + 2 * n, m * 8);
+            }
+            if (keybits & ~lim) {
+                unreserve();
+                return bfail(SQE_EINVAL, "key has bits beyond nq");
+            }
         }
         // region sizes and admission
         const uint64_t S = 1ull << qm.mb;
@@ -1226,7 +1233,7 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         ++b->layout;
         Circ& cc = b->c[id];
         BCK(stage_fit(b, std::max<uint64_t>(m, 1)));
-        const uint64_t up = pc + 24 * m;
+        const uint64_t up = pc + (pinned ? 0 : 24 * m);
         b->up.head = up_off + ring_round(up);
         char* hp = hp0;
         if (cc.nops) {
@@ -1239,8 +1246,22 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
             std::memcpy(hp, coef.data(), cc.ncoef * 8);
             BHIP(hipMemcpyAsync(b->coef.p + cc.coef0 * 8, hp, cc.ncoef * 8, hipMemcpyHostToDevice, b->st));
         }
-        if (m) BHIP(hipMemcpyAsync(b->xkeys, hk, m * 24, hipMemcpyHostToDevice, b->st));
-        BCK(ring_mark(b->up, up_off, up, b->st));
+        if (m && !pinned) BHIP(hipMemcpyAsync(b->xkeys, hk, m * 24, hipMemcpyHostToDevice, b->st));
+        if (m && pinned) {
+            // straight from the caller's arrays to the staging (packed at
+            // stride m): one pitched copy when they are equally spaced, as the
+            // three arrays of one sqe_host_alloc buffer are
+            const char* k0 = (const char*)keys;
+            const ptrdiff_t pitch = (const char*)re - k0;
+            if (pitch > 0 && (const char*)im - (const char*)re == pitch && (uint64_t)pitch >= m * 8) {
+                BHIP(hipMemcpy2DAsync(b->xkeys, m * 8, keys, (size_t)pitch, m * 8, 3, hipMemcpyHostToDevice, b->st));
+            } else {
+                BHIP(hipMemcpyAsync(b->xkeys, keys, m * 8, hipMemcpyHostToDevice, b->st));
+                BHIP(hipMemcpyAsync(b->xkeys + m, re, m * 8, hipMemcpyHostToDevice, b->st));
+                BHIP(hipMemcpyAsync(b->xkeys + 2 * m, im, m * 8, hipMemcpyHostToDevice, b->st));
+            }
+        }
+        if (up) BCK(ring_mark(b->up, up_off, up, b->st));
         // load the state into the region (insert bases, number buckets, scatter)
         const double ml = b->cfg.max_load > 0 && b->cfg.max_load < 0.95 ? b->cfg.max_load : 0.5;
         const uint64_t tneed = std::max<uint64_t>(1024, p2((uint64_t)std::ceil((double)m / ml) + 8));
@@ -1270,6 +1291,7 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         // admission never waits for the device
         LoadArgs la{};
         la.keys = b->xkeys; la.re = (const double*)(b->xkeys + m); la.im = la.re + m; la.n = m; la.qm = qm;
+        la.keymask = pinned ? ~lim : 0ull;
         la.tab = b->tab; la.tmask = tneed - 1; la.amp = amp_ptr(b, cc); la.occ = b->occ; la.stats = b->cst + id;
         EnumArgs ea{};
         ea.tab = b->tab; ea.tcap = tneed; ea.out_base = base_ptr(b, cc); ea.out_nb = b->meta + id;
@@ -1280,6 +1302,16 @@ sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const 
         *id_out = id;
         return SQE_OK;
     });
+}
+
+sqe_status sqe_batch_add(sqe_batch* b, uint32_t nq, const uint64_t* keys, const double* re, const double* im,
+                         uint64_t n, const sqe_gate* gates, uint64_t ng, int64_t* id_out) {
+    return admit(b, nq, keys, re, im, n, gates, ng, id_out, false);
+}
+
+sqe_status sqe_batch_add_pinned(sqe_batch* b, uint32_t nq, const uint64_t* keys, const double* re, const double* im,
+                                uint64_t n, const sqe_gate* gates, uint64_t ng, int64_t* id_out) {
+    return admit(b, nq, keys, re, im, n, gates, ng, id_out, true);
 }
 
 sqe_status sqe_batch_step(sqe_batch* b, uint32_t max_gates) {

@@ -162,7 +162,8 @@ struct BlockStats {
     unsigned long long pad[3];
 };
 
-enum : unsigned long long { ERR_PROBE = 1, ERR_DUP = 2, ERR_MISSING = 4, ERR_CAP = 8, ERR_SCAN = 16, ERR_RENORM = 32 };
+enum : unsigned long long { ERR_PROBE = 1, ERR_DUP = 2, ERR_MISSING = 4, ERR_CAP = 8, ERR_SCAN = 16, ERR_RENORM = 32,
+                           ERR_KEY = 64 };
 
 // Plain complex value (trivially copyable, so register arrays of it are fully
 // scalarised); 8/16-byte aligned for single 64/128-bit memory accesses.

@@ -1304,7 +1304,7 @@ template <class R>
 __global__ __launch_bounds__(BLOCK) void k_load_scatter(LoadArgs A) {
     using C = typename CT<R>::T;
     const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    const bool act = i < A.n;
+    const bool act = i < A.n && !(A.keys[i] & A.keymask);     // (a key beyond nq was reported at insert)
     const uint64_t key = act ? A.keys[i] : EMPTY_KEY;
     const uint64_t j = coop_find(A.tab, A.tmask, key & ~A.qm.mask, act, &A.stats->err);
     if (!act) return;

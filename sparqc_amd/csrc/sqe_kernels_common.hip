@@ -147,10 +147,16 @@ __global__ __launch_bounds__(BLOCK) void k_index_build(IndexArgs A) {
 }
 
 // Insert the base (key with bucket bits cleared) of every input key.
+// A key with bits in keymask (beyond the circuit's qubits) is reported and
+// left out.
 __global__ __launch_bounds__(BLOCK) void k_load_insert(LoadArgs A) {
     const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    const bool act = i < A.n;
+    bool act = i < A.n;
     const uint64_t key = act ? A.keys[i] : EMPTY_KEY;
+    if (act && (key & A.keymask)) {
+        atomicOr(&A.stats->err, ERR_KEY);
+        act = false;
+    }
     coop_insert(A.tab, A.tmask, key & ~A.qm.mask, NONE, act, &A.stats->err);
 }
 

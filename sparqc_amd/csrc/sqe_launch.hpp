@@ -188,6 +188,7 @@ struct LoadArgs {
     const double* re;
     const double* im;
     uint64_t n;
+    uint64_t keymask;            // key bits that must be clear (0: checked on the host)
     QMap qm;
     Slot* tab;
     uint64_t tmask;

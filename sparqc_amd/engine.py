@@ -24,7 +24,8 @@ EXPORTS = ["sqe_config_default", "sqe_create", "sqe_destroy", "sqe_set_basis", "
            "sqe_checkpoint_save", "sqe_checkpoint_load", "sqe_sample", "sqe_sync",
            "sqe_get_stats", "sqe_prepare", "sqe_snapshot_save", "sqe_snapshot_restore",
            "sqe_last_error",
-           "sqe_batch_create", "sqe_batch_destroy", "sqe_batch_add", "sqe_batch_step", "sqe_batch_run",
+           "sqe_batch_create", "sqe_batch_destroy", "sqe_batch_add", "sqe_batch_add_pinned", "sqe_batch_step",
+           "sqe_batch_run",
            "sqe_batch_sync", "sqe_batch_info", "sqe_batch_export", "sqe_batch_remove", "sqe_batch_finished",
            "sqe_batch_export_start", "sqe_batch_export_poll",
            "sqe_batch_usage",
@@ -136,6 +137,8 @@ def lib():
         L.sqe_batch_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
         L.sqe_batch_add.argtypes = [vp, ctypes.c_uint32, _u64p, _f64p, _f64p, ctypes.c_uint64, vp,
                                     ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64)]
+        L.sqe_batch_add_pinned.argtypes = [vp, ctypes.c_uint32, _u64p, _f64p, _f64p, ctypes.c_uint64, vp,
+                                           ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64)]
         L.sqe_batch_step.argtypes = [vp, ctypes.c_uint32]
         L.sqe_batch_info.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(BatchCircuit)]
         L.sqe_batch_export.argtypes = [vp, ctypes.c_int64, _u64p, _f64p, _f64p, ctypes.c_uint64, _u64p]
@@ -410,9 +413,25 @@ class Batch:
                                     g.ctypes.data, g.size, ctypes.byref(cid)))
         return cid.value
 
-    def try_add(self, nq, keys, re, im, gates):
-        """Like add, but returns None when the circuit does not fit the budget now."""
+    def add_pinned(self, nq, state, n, gates):
+        """Admit a circuit whose state is the first n entries of `state`, a
+        PinnedArrays (page-locked): uploaded without a host-side copy; the
+        arrays must stay unchanged until the upload has completed (after the
+        next sync, or once the circuit's export has completed)."""
+        if n > state.cap:
+            raise ValueError("n exceeds the buffer's capacity")
+        g = G.gate_array(gates)
+        cid = ctypes.c_int64()
+        _bcheck(lib().sqe_batch_add_pinned(self.h, int(nq), _ptr(state.keys, _u64p), _ptr(state.re, _f64p),
+                                           _ptr(state.im, _f64p), int(n), g.ctypes.data, g.size, ctypes.byref(cid)))
+        return cid.value
+
+    def try_add(self, nq, keys, re, im, gates, pinned=None):
+        """Like add (add_pinned with `pinned` = (PinnedArrays, n)), but returns
+        None when the circuit does not fit the budget now."""
         try:
+            if pinned is not None:
+                return self.add_pinned(nq, pinned[0], pinned[1], gates)
             return self.add(nq, keys, re, im, gates)
         except SqeError as ex:
             if ex.status == SQE_ETABLE_FULL:

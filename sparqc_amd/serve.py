@@ -15,6 +15,8 @@ Requests arrive at given times (seconds from the start).  The loop
      loop) -- and releases its pool space at once;
   4. collects the exports whose copies have completed and records each
      request's arrival -> result latency.
+A request may carry its state in a page-locked buffer (`pinned`, see
+pin_requests): it is then admitted without a host-side copy.
 Only an idle loop waits for the device (for the oldest export, or sleeps until
 the next arrival), so admissions, steps and result copies overlap.
 
@@ -176,7 +178,10 @@ class ServeLoop:
             while nxt < n and arrivals[nxt] <= now:
                 r = requests[nxt]
                 try:
-                    cid = b.try_add(r["nq"], r["keys"], r["re"], r["im"], r["gates"])
+                    if "pinned" in r:
+                        cid = b.try_add(r["nq"], r["keys"], r["re"], r["im"], r["gates"], pinned=r["pinned"])
+                    else:
+                        cid = b.try_add(r["nq"], r["keys"], r["re"], r["im"], r["gates"])
                 except E.SqeError as ex:
                     if ex.status != E.SQE_ENOTLOCAL:
                         raise
@@ -232,6 +237,23 @@ class ServeLoop:
         if keep_results:
             rep["results"] = results
         return rep
+
+
+def pin_requests(requests):
+    """Copies of the requests whose states sit in page-locked buffers
+    (engine.PinnedArrays), as a front end that receives requests into such
+    buffers would hold them: admission then uploads them without a host-side
+    copy (sqe_batch_add_pinned).  The buffers stay allocated with the
+    returned requests."""
+    out = []
+    for r in requests:
+        n = int(r["keys"].size)
+        buf = E.PinnedArrays(n)
+        buf.keys[:n] = r["keys"]
+        buf.re[:n] = r["re"]
+        buf.im[:n] = r["im"]
+        out.append(dict(r, pinned=(buf, n)))
+    return out
 
 
 def run_solo(request, prec="f64", step_gates=64, renorm_every=None):

@@ -414,3 +414,54 @@ def test_duplicate_key_reported_for_its_circuit(oracle):
             b.export_poll(tk)                    # the ticket was retired
         buf.free()
         b.sync()
+
+
+def test_add_pinned_equals_add(oracle):
+    """Admission straight from page-locked arrays gives the same states and
+    counts as the copying admission, zero amplitudes included (they load as
+    absent entries); a key beyond nq is reported for its circuit only."""
+    cs = small_batch(14, 8)
+    # give one circuit zero amplitudes
+    z = dict(cs[2])
+    z["re"] = z["re"].copy()
+    z["im"] = z["im"].copy()
+    z["re"][::3] = 0.0
+    z["im"][::3] = 0.0
+    cs[2] = z
+    bufs = []
+    with E.Batch() as a, E.Batch() as b:
+        ia = [a.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+        ib = []
+        for c in cs:
+            n = c["keys"].size
+            buf = E.PinnedArrays(n)
+            buf.keys[:n] = c["keys"]
+            buf.re[:n] = c["re"]
+            buf.im[:n] = c["im"]
+            bufs.append(buf)
+            ib.append(b.add_pinned(c["nq"], buf, n, c["gates"]))
+        # a key with a bit beyond nq, admitted pinned
+        bad = cs[0]
+        n = bad["keys"].size
+        bb = E.PinnedArrays(n)
+        bb.keys[:n] = bad["keys"]
+        bb.keys[0] = np.uint64(1) << np.uint64(bad["nq"])
+        bb.re[:n] = bad["re"]
+        bb.im[:n] = bad["im"]
+        bufs.append(bb)
+        ibad = b.add_pinned(bad["nq"], bb, n, bad["gates"])
+        a.run()
+        b.run()
+        b.sync()
+        for k, c in enumerate(cs):
+            ga, gb = a.export(ia[k]), b.export(ib[k])
+            assert_same_state(gb, ga, atol=0.0, label="circuit %d pinned vs copied" % k)
+            assert a.info(ia[k])["updates"] == b.info(ib[k])["updates"]
+            want, upd = oracle_of(oracle, c)
+            assert_same_state(gb, want, atol=1e-12, label="circuit %d vs oracle" % k)
+            assert b.info(ib[k])["updates"] == upd
+        with pytest.raises(E.SqeError) as ex:
+            b.info(ibad)
+        assert ex.value.status == E.SQE_EINVAL and "beyond nq" in str(ex.value)
+    for buf in bufs:
+        buf.free()

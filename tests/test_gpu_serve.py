@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from sparqc_amd import gen
-from sparqc_amd.serve import ServeLoop, run_solo
+from sparqc_amd.serve import ServeLoop, pin_requests, run_solo
 from tests.helpers import assert_same_state
 
 pytestmark = pytest.mark.gpu
@@ -83,3 +83,19 @@ def test_cross_bucket_requests_served_beside_the_batch(oracle):
         o.apply(r["gates"])
         assert_same_state(rep["results"][i]["state"], o.export(), atol=1e-12, label="request %d" % i)
         assert rep["results"][i]["updates"] == o.updates()
+
+
+def test_pinned_requests_mix_equals_solo(oracle):
+    """Requests held in page-locked buffers (admitted without a host copy)
+    give the same results as the same requests admitted by copy."""
+    reqs = requests(24, 10)
+    pinned = pin_requests(reqs)
+    arr = gen.poisson_arrivals(7, 3000.0, len(reqs))
+    out = []
+    for rs in (pinned, reqs):
+        with ServeLoop(prec="f64", step_gates=32, hbm_budget_bytes=1 << 30) as s:
+            out.append(s.run(rs, arr, keep_results=True))
+    for i, r in enumerate(reqs):
+        assert_same_state(out[0]["results"][i]["state"], out[1]["results"][i]["state"], atol=0.0,
+                          label="request %d pinned vs copied" % i)
+        assert out[0]["results"][i]["updates"] == out[1]["results"][i]["updates"]

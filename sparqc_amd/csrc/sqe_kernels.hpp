@@ -1094,18 +1094,22 @@ __global__ __launch_bounds__(BLOCK) void k_pair(PairArgs A) {
         constexpr int s = decltype(sc)::value;
         own[s].x = 0; own[s].y = 0; par[s].x = 0; par[s].y = 0;
     });
-    if (valid) {
+    // a bucket whose pair its partner owns is read once, by the owner: it
+    // loads nothing itself, and the owner counts the partner's nonzeros
+    const bool reads = owner || (valid && !ctrl_ok);
+    if (reads) {
         static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
             own[s] = in_amp[amp_index(b, s, S)];
         });
     }
-    const uint32_t nz_in = valid ? count_nz<MB, R>(own) : 0u;
+    uint32_t nz_in = reads ? count_nz<MB, R>(own) : 0u;
     if (owner && j != NONE) {
         static_for<S>([&](auto sc) __attribute__((always_inline)) {
             constexpr int s = decltype(sc)::value;
             par[s] = in_amp[amp_index(j, s, S)];
         });
+        nz_in += count_nz<MB, R>(par);
     }
     uint32_t count = 0;
     bool lo_nz = false, hi_nz = false;

@@ -375,11 +375,15 @@ def run_batch(seed, n_circuits, prec, steps, warmup, eager=False):
         ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
         admit_s = time.perf_counter() - t0
         tune = b.autotune()
+        sizes = sorted({b.info(i)["bucket_bits"] for i in ids})
         b.snapshot_save()
+        launches_per_run = None
         for _ in range(warmup):
             b.snapshot_restore()
+            l0 = b.tuning()["launches"]
             b.run()
             b.sync()
+            launches_per_run = b.tuning()["launches"] - l0
         t0 = time.perf_counter()
         for _ in range(steps):
             b.snapshot_restore()
@@ -393,7 +397,10 @@ def run_batch(seed, n_circuits, prec, steps, warmup, eager=False):
             "updates_per_run": upd, "nnz_final": nnz, "initial_keys": int(sum(c["keys"].size for c in cs)),
             "gates": int(sum(len(c["gates"]) for c in cs)), "pool_bytes": used, "admission_s": admit_s,
             "autotune": tune, "circuits_per_s": n_circuits / dt,
-            "launch": "eager steps" if eager else "captured hipGraph replay", "runs_timed": steps}
+            "launch": "eager steps" if eager else "captured hipGraph replay", "runs_timed": steps,
+            "bucket_sizes": sizes, "sweep_launches_per_run": launches_per_run,
+            "renormalisation": ("one pass: checkpoint fold, then a scale launch per bucket size"
+                                if prec == "f32" else None)}
 
 
 def batch_mode(a):

@@ -229,6 +229,42 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
             static_for<EPT>([&](auto ec) __attribute__((always_inline)) { nzc[decltype(ec)::value] = 1u << MB; });
         }
     };
+    // FP32 branching gate without controls: the pending factor goes into
+    // the registers first, then a dry run of the gate keeps per lane only the
+    // smallest max(|re|,|im|) of its outputs.  When every output of the
+    // wavefront is at or above the band's upper edge, the per-pair band test
+    // could not drop or re-decide any of them: the gate runs again, storing,
+    // and every slot counts as nonzero.  Otherwise the caller runs the
+    // per-pair band test on the flushed inputs (the same outputs and
+    // decisions as testing during the first pass).
+    auto fp32_clear_pass = [&](auto&& gate) __attribute__((always_inline)) -> bool {
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+            constexpr int e = decltype(ec)::value;
+            static_for<(1 << MB)>([&](auto sc) __attribute__((always_inline)) {
+                constexpr int sl = decltype(sc)::value;
+                a[e][sl] = rot_fma((R)pr[e], (R)pi[e], a[e][sl]);
+            });
+        });
+        const R hi = (R)band32(thr).hi;
+        bool low = false;
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
+            constexpr int e = decltype(ec)::value;
+            R mm = (R)__builtin_inf();
+            static_for<HALF>([&](auto qc) __attribute__((always_inline)) {
+                constexpr int s0 = PairIdx<1 << MB, J, decltype(qc)::value>::s0;
+                constexpr int s1 = PairIdx<1 << MB, J, decltype(qc)::value>::s1;
+                C n0, n1;
+                gate(a[e][s0], a[e][s1], n0, n1);
+                mm = fmin(mm, fmin(fmax(fabs(n0.x), fabs(n0.y)), fmax(fabs(n1.x), fabs(n1.y))));
+            });
+            low |= !(mm >= hi);
+        });
+        if (__builtin_amdgcn_ballot_w64(low) != 0) return false;
+        pairs([&](C x, C y, C& n0, C& n1, uint32_t&) __attribute__((always_inline)) { gate(x, y, n0, n1); },
+              std::false_type{});
+        static_for<EPT>([&](auto ec) __attribute__((always_inline)) { nzc[decltype(ec)::value] = 1u << MB; });
+        return true;
+    };
     switch (op.h) {
     case H_SLOT_DIAG: {
         const R f0r = (R)op.c[0], f0i = (R)op.c[1], f1r = (R)op.c[2], f1i = (R)op.c[3];
@@ -256,13 +292,17 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
             unprune_then_prune([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) { hpair(f, x, y, n0, n1); });
             return true;
         }
+        if constexpr (F32 && !MASK) {
+            if (fp32_clear_pass([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) { hpair(f, x, y, n0, n1); }))
+                return true;
+        }
         static_for<EPT>([&](auto ec) __attribute__((always_inline)) { nzc[decltype(ec)::value] = 0; });
         pairs([&](C x, C y, C& n0, C& n1, uint32_t& k) __attribute__((always_inline)) {
             hpair(f, x, y, n0, n1);
             prune_pair(x, y, n0, n1, k, [&](cd ex, cd ey, cd& e0, cd& e1) __attribute__((always_inline)) {
                 hpair(op.c[0], ex, ey, e0, e1);
             });
-        }, std::true_type{});
+        }, std::integral_constant<bool, !(F32 && !MASK)>{});
         return !MASK && !F32;
     }
     case H_SLOT_RY: {
@@ -273,13 +313,19 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
             });
             return true;
         }
+        if constexpr (F32 && !MASK) {
+            if (fp32_clear_pass([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
+                    ry_pair(c, sn, x, y, n0, n1);
+                }))
+                return true;
+        }
         static_for<EPT>([&](auto ec) __attribute__((always_inline)) { nzc[decltype(ec)::value] = 0; });
         pairs([&](C x, C y, C& n0, C& n1, uint32_t& k) __attribute__((always_inline)) {
             ry_pair(c, sn, x, y, n0, n1);
             prune_pair(x, y, n0, n1, k, [&](cd ex, cd ey, cd& e0, cd& e1) __attribute__((always_inline)) {
                 ry_pair(op.c[0], op.c[1], ex, ey, e0, e1);
             });
-        }, std::true_type{});
+        }, std::integral_constant<bool, !(F32 && !MASK)>{});
         return !MASK && !F32;
     }
 

@@ -75,7 +75,8 @@ def parse():
     ap.add_argument("--circuits", type=int, default=256, help="batch mode: circuits")
     ap.add_argument("--eager", action="store_true",
                     help="batch mode: launch every step directly instead of replaying a captured hipGraph")
-    ap.add_argument("--requests", type=int, default=600, help="serve mode: requests per rate")
+    ap.add_argument("--requests", type=int, default=600,
+                    help="requests per rate (serve mode, and the serving detail of the default line)")
     ap.add_argument("--rates", default="200,1000,5000", help="serve mode: arrival rates (1/s)")
     ap.add_argument("--step-gates", type=int, default=64, help="serve mode: gates per batched step")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -143,7 +144,7 @@ def cpu_model():
     return None
 
 
-def cpu_baseline(w, seconds, chunk=25, points=(0, 500, 1000, 1500)):
+def cpu_baseline(w, chunk=100, points=(0, 500, 1000, 1500)):
     """The CPU oracle (FP64 sparse engine, OpenMP) timed on a sample spread
     across the circuit's nonzero growth: `chunk` gates from each of the gate
     positions in `points`, each started from the exact state the circuit has
@@ -538,7 +539,7 @@ def wide_mode(a):
                                 "time; measured HBM bytes per kernel are in the rocprof summaries"},
            "detail": dict(kc, updates_per_step=upd, nnz_final=nnz1, bucket_bits=s1["bucket_bits"])}
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, a.cpu_chunk)
+        out["cpu_baseline"] = cpu_baseline(w, chunk=a.cpu_chunk)
     print(json.dumps(out), flush=True)
 
 
@@ -672,11 +673,11 @@ def main():
         final = None
         out["detail"]["fp32_blocks"] = blocks_detail(a.seed)
         out["detail"]["batch"] = {p: run_batch(a.seed, 256, p, steps=3, warmup=1) for p in ("f64", "f32")}
-        out["detail"]["serve"] = run_serve(a.seed, 300, [200.0, 1000.0, 5000.0], "f64", a.step_gates)
+        out["detail"]["serve"] = run_serve(a.seed, a.requests, [200.0, 1000.0, 5000.0], "f64", a.step_gates)
     final = None
     out["detail"]["stream_copy"] = stream_copy()
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, a.cpu_chunk)
+        out["cpu_baseline"] = cpu_baseline(w, chunk=a.cpu_chunk)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

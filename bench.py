@@ -254,11 +254,12 @@ def precision_report(got32, want64):
             "keys_only_one_side": int(keys.size * 2 - k32.size - k64.size)}
 
 
-def run_timed(w, prec, steps, warmup, **kw):
+def run_timed(w, prec, steps, warmup, timing=True, **kw):
     """Steps of the workload (restore the initial snapshot, apply the circuit,
-    sync) on one engine; returns (updates per step, seconds per step, engine)."""
+    sync) on one engine; returns (updates per step, seconds per step, engine).
+    timing: HIP events around every launch (per-class kernel times)."""
     from sparqc_amd import engine as E
-    eng = E.Engine(w["nq"], prec=prec, timing=True, renorm_every=100 if prec == "f32" else 0, **kw)
+    eng = E.Engine(w["nq"], prec=prec, timing=timing, renorm_every=100 if prec == "f32" else 0, **kw)
     eng.load(w["keys"], w["re"], w["im"])
     eng.prepare(w["gates"])
     eng.snapshot_save()
@@ -474,8 +475,13 @@ def c1_mode(a):
     from sparqc_amd import gen
     w = gen.circuit_small(a.seed)
     steps = max(a.steps, 50)
-    upd, dt, eng, s0, s1 = run_timed(w, "f64", steps, a.warmup)
+    # the rate without per-launch events (a step is about 120 small launches,
+    # so the timing events would be part of what is measured), then the
+    # per-class kernel times from a second run with them
+    upd, dt, eng, s0, s1 = run_timed(w, "f64", steps, a.warmup, timing=False)
     nnz1 = s1["nnz"]
+    eng.close()
+    _, dt_ev, eng, s0, s1 = run_timed(w, "f64", steps, a.warmup)
     eng.close()
     try:
         avail = len(os.sched_getaffinity(0))
@@ -503,7 +509,7 @@ def c1_mode(a):
                                   "qubits, X / CX / RZ / CZ on any, prune 1e-14", "nq": 20, "gates": 200,
                       "seed": a.seed},
            "detail": dict(kernel_classes(s0, s1, steps), updates_per_step=upd, nnz_final=nnz1,
-                          bucket_bits=s1["bucket_bits"]),
+                          bucket_bits=s1["bucket_bits"], ms_per_step_with_timing_events=dt_ev * 1e3),
            "cpu_baseline": {"value": v, "unit": "updates/s", "cores": cores, "kind": "port",
                             "sample": "FC engine (dense slots addressed by key, address book, two buffers: the "
                                       "reference's table restated on the CPU) on the whole circuit, 20 runs, %.2f s"

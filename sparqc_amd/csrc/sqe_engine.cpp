@@ -512,6 +512,17 @@ sqe_status rebuild_index(sqe_engine* e) {
     return SQE_OK;
 }
 
+// Buckets the key space allows: one per base key, 2^(nq - bucket bits).
+uint64_t max_buckets(const sqe_engine* e) {
+    const int bits = (int)e->cfg.nq - e->qm.mb;
+    return bits >= 62 ? (1ull << 62) : (1ull << std::max(bits, 0));
+}
+
+// Stores of at most this many bytes each are sized for the whole key space
+// at their first growth: a cross-bucket gate then never needs the exact
+// bucket count from the device (a host round trip per gate).
+constexpr uint64_t FULL_SPACE_STORE_BYTES = 256ull << 20;
+
 // Grow both stores so that `need` buckets fit, keeping the current state.
 // `need` = 0 asks for room for one cross-bucket gate (2 * buckets + 64).
 sqe_status grow(sqe_engine* e, uint64_t need) {
@@ -520,7 +531,14 @@ sqe_status grow(sqe_engine* e, uint64_t need) {
     e->nb_upper = nb;
     if (!need) need = 2 * nb + 64;
     if (need <= e->s[0].cap && need <= e->s[1].cap) return SQE_OK;
-    const uint64_t cap = round64(std::max<uint64_t>(need, e->s[0].cap * 2));
+    uint64_t cap = round64(std::max<uint64_t>(need, e->s[0].cap * 2));
+    if (max_buckets(e) <= (1ull << 32)) {
+        const uint64_t full = round64(2 * max_buckets(e) + 64);
+        const uint64_t full_bytes = full * 8 + e->amp_bytes(full);
+        if (full > cap && full_bytes <= FULL_SPACE_STORE_BYTES &&
+            e->bytes + 2 * full_bytes <= e->cfg.hbm_budget_bytes)
+            cap = full;
+    }
     const int other = e->cur ^ 1;
     free_store(e, e->s[other]);
     SQCK(alloc_store(e, e->s[other], cap));
@@ -619,7 +637,7 @@ sqe_status run_pair(sqe_engine* e, const DOp& op) {
     }
     note_counting_launch(e, g);
     e->cur = other;
-    e->nb_upper = std::min<uint64_t>(2 * e->nb_upper, o.cap);
+    e->nb_upper = std::min<uint64_t>(std::min<uint64_t>(2 * e->nb_upper, o.cap), max_buckets(e));
     e->tab_valid = false;
     ++e->n_pairs;
     return SQE_OK;

@@ -1218,6 +1218,7 @@ sqe_status sqe_sync(sqe_engine* e) {
             SQCK(read_stats(e, &h));
             uint64_t nb = 0;
             SQCK(read_nb(e, &nb));
+            e->nb_upper = nb;                 // exact again (cross-bucket gates only bound it)
             if (h.empty * 4 > nb && nb > 1024) {
                 SQCK(compact(e));
                 HIPCK(hipStreamSynchronize(e->st));

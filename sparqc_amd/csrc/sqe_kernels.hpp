@@ -767,8 +767,8 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
             // ---- norm point: FP64 norm of the state at this point of the
             // program (the frame and the pending factor's phase leave |a|
             // unchanged); reads the amplitude registers only ----
+            // (one partial per tile: slot ctx.wave + e; a missing tile writes none)
             const uint32_t k = __builtin_amdgcn_readfirstlane(sp[jj * 4].x) >> 13;
-            double v = 0.0;
             static_for<EPT>([&](auto ec) __attribute__((always_inline)) {
                 constexpr int e = decltype(ec)::value;
                 double t = 0.0;
@@ -776,10 +776,10 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
                     constexpr int s = decltype(sc)::value;
                     t += (double)a[e][s].x * (double)a[e][s].x + (double)a[e][s].y * (double)a[e][s].y;
                 });
-                v += t * ((double)pr[e] * (double)pr[e] + (double)pi[e] * (double)pi[e]);
+                double v = t * ((double)pr[e] * (double)pr[e] + (double)pi[e] * (double)pi[e]);
+                for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+                if (lane == 0 && ctx.ckpt && tile[e] >= 0) ctx.ckpt[(size_t)k * ctx.ckpt_stride + ctx.wave + e] = v;
             });
-            for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
-            if (lane == 0 && ctx.ckpt) ctx.ckpt[(size_t)k * ctx.ckpt_stride + ctx.wave] = v;
             ++i;
         } else {
             // ---- one slot op: amplitude registers ----
@@ -1057,10 +1057,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
     const BItem item = A.items[it];
     const BTask t = A.tasks[item.task];
     unsigned long long upd = 0, nnz = 0;
-    // FP64 8-slot buckets: two tiles per pass (each op decoded once for both;
-    // the registers of two 8-slot buckets are those of one 16-slot bucket).
-    // FP32 keeps one: its one-pass renormalisation writes a partial per tile.
-    constexpr int BEPT = (MB == 3 && sizeof(R) == 8) ? 2 : 1;
+    // 8-slot buckets: two tiles per pass (each op decoded once for both; the
+    // registers of two 8-slot buckets are those of one 16-slot bucket)
+    constexpr int BEPT = MB == 3 ? 2 : 1;
     for (uint32_t g = 0; g < A.gran; g += BEPT) {
         const uint32_t tile = item.tile + g;
         if (tile >= t.ntiles) break;

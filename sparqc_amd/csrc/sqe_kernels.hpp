@@ -1057,23 +1057,32 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<
     const BItem item = A.items[it];
     const BTask t = A.tasks[item.task];
     unsigned long long upd = 0, nnz = 0;
-    // 8-slot buckets: two tiles per pass (each op decoded once for both; the
-    // registers of two 8-slot buckets are those of one 16-slot bucket)
-    constexpr int BEPT = MB == 3 ? 2 : 1;
-    for (uint32_t g = 0; g < A.gran; g += BEPT) {
-        const uint32_t tile = item.tile + g;
-        if (tile >= t.ntiles) break;
-        int64_t tl[BEPT];
-        static_for<BEPT>([&](auto ec) __attribute__((always_inline)) {
-            constexpr uint32_t e = decltype(ec)::value;
-            tl[e] = (g + e < A.gran && tile + e < t.ntiles) ? (int64_t)(tile + e) : -1;
-        });
-        SweepCount c{0, 0, 0, 0};
-        // (FP32 one-pass renormalisation: norm points write one partial per tile)
-        const SweepCtx ctx{t.qm, t.frames, t.ntiles, tile, nullptr, nullptr, t.ckpt, nullptr, 0, nullptr};
-        sweep_group<MB, R, BEPT>(t.base, t.amp, tl, t.ops, t.nops, t.coef, A.thr, t.scale, 1, c, ctx);
-        upd += c.upd;
-        nnz += c.nnz_out;
+    auto run = [&](auto ept_c) __attribute__((always_inline)) {
+        constexpr int BEPT = decltype(ept_c)::value;
+        for (uint32_t g = 0; g < A.gran; g += BEPT) {
+            const uint32_t tile = item.tile + g;
+            if (tile >= t.ntiles) break;
+            int64_t tl[BEPT];
+            static_for<BEPT>([&](auto ec) __attribute__((always_inline)) {
+                constexpr uint32_t e = decltype(ec)::value;
+                tl[e] = (g + e < A.gran && tile + e < t.ntiles) ? (int64_t)(tile + e) : -1;
+            });
+            SweepCount c{0, 0, 0, 0};
+            // (FP32 one-pass renormalisation: norm points write one partial per tile)
+            const SweepCtx ctx{t.qm, t.frames, t.ntiles, tile, nullptr, nullptr, t.ckpt, nullptr, 0, nullptr};
+            sweep_group<MB, R, BEPT>(t.base, t.amp, tl, t.ops, t.nops, t.coef, A.thr, t.scale, 1, c, ctx);
+            upd += c.upd;
+            nnz += c.nnz_out;
+        }
+    };
+    // 8-slot buckets with two or more tiles per work item: two tiles per pass
+    // (each op decoded once for both; the registers of two 8-slot buckets are
+    // those of one 16-slot bucket)
+    if constexpr (MB == 3) {
+        if (A.gran >= 2) run(std::integral_constant<int, 2>{});
+        else run(std::integral_constant<int, 1>{});
+    } else {
+        run(std::integral_constant<int, 1>{});
     }
     for (int d = 32; d >= 1; d >>= 1) {
         upd += __shfl_xor(upd, d);

@@ -203,10 +203,16 @@ class PinnedArrays:
         self.keys = np.frombuffer(buf, dtype=np.uint64, count=cap, offset=0)
         self.re = np.frombuffer(buf, dtype=np.float64, count=cap, offset=8 * cap)
         self.im = np.frombuffer(buf, dtype=np.float64, count=cap, offset=16 * cap)
+        # the arrays' addresses as call arguments, made once (the buffer
+        # does not move until grow or free)
+        self.kp = ctypes.cast(p.value, _u64p)
+        self.rp = ctypes.cast(p.value + 8 * cap, _f64p)
+        self.ip = ctypes.cast(p.value + 16 * cap, _f64p)
 
     def free(self):
         if self._p is not None:
             self.keys = self.re = self.im = None
+            self.kp = self.rp = self.ip = None
             lib().sqe_host_free(self._p)
             self._p = None
             self.cap = 0
@@ -422,8 +428,8 @@ class Batch:
             raise ValueError("n exceeds the buffer's capacity")
         g = G.gate_array(gates)
         cid = ctypes.c_int64()
-        _bcheck(lib().sqe_batch_add_pinned(self.h, int(nq), _ptr(state.keys, _u64p), _ptr(state.re, _f64p),
-                                           _ptr(state.im, _f64p), int(n), g.ctypes.data, g.size, ctypes.byref(cid)))
+        _bcheck(lib().sqe_batch_add_pinned(self.h, int(nq), state.kp, state.rp, state.ip, int(n), g.ctypes.data,
+                                           g.size, ctypes.byref(cid)))
         return cid.value
 
     def try_add(self, nq, keys, re, im, gates, pinned=None):
@@ -508,12 +514,12 @@ class Batch:
         a ticket for export_poll.  `out` belongs to the export until then."""
         t = ctypes.c_int64()
         bound = ctypes.c_uint64()
-        rc = lib().sqe_batch_export_start(self.h, int(cid), _ptr(out.keys, _u64p), _ptr(out.re, _f64p),
-                                          _ptr(out.im, _f64p), out.cap, ctypes.byref(t), ctypes.byref(bound))
+        rc = lib().sqe_batch_export_start(self.h, int(cid), out.kp, out.rp, out.ip, out.cap, ctypes.byref(t),
+                                          ctypes.byref(bound))
         if rc == SQE_EINVAL and bound.value > out.cap:
             out.grow(bound.value)
-            rc = lib().sqe_batch_export_start(self.h, int(cid), _ptr(out.keys, _u64p), _ptr(out.re, _f64p),
-                                              _ptr(out.im, _f64p), out.cap, ctypes.byref(t), ctypes.byref(bound))
+            rc = lib().sqe_batch_export_start(self.h, int(cid), out.kp, out.rp, out.ip, out.cap, ctypes.byref(t),
+                                              ctypes.byref(bound))
         _bcheck(rc)
         return t.value
 
@@ -521,9 +527,10 @@ class Batch:
         """None while the export runs; then (n, info): the entry count (the
         entries are out.keys[:n] etc. of its PinnedArrays) and the circuit's
         counters at export."""
-        done = ctypes.c_int32()
-        n = ctypes.c_uint64()
-        c = BatchCircuit()
+        pa = self.__dict__.get("_poll_args")
+        if pa is None:                              # reused by every poll (polls run often)
+            pa = self._poll_args = (ctypes.c_int32(), ctypes.c_uint64(), BatchCircuit())
+        done, n, c = pa
         _bcheck(lib().sqe_batch_export_poll(self.h, int(ticket), 1 if wait else 0, ctypes.byref(done),
                                             ctypes.byref(n), ctypes.byref(c)))
         if not done.value:

@@ -225,6 +225,30 @@ def test_tuning_grid(oracle, block, gran):
             assert_same_state(b.export(cid), want, atol=1e-12)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_two_tiles_per_pass_equal_one(prec):
+    """Work items of two or more tiles run 8-slot buckets two tiles per pass
+    (an odd item ends with a half-empty pass): the states, update counts and,
+    in FP32, the one-pass norm checkpoints (one partial per tile) are those of
+    one tile per pass, to the bit."""
+    cs = small_batch(13, 8, depth=(150, 300))
+    runs = {}
+    for gran in (1, 2, 3):
+        with E.Batch(prec=prec, renorm_every=100 if prec == "f32" else 0) as b:
+            b.set_tuning(tiles_per_item=gran)
+            ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+            sizes = [b.info(i)["bucket_bits"] for i in ids]
+            b.run()
+            b.sync()
+            runs[gran] = ([b.export(i) for i in ids], [b.info(i)["updates"] for i in ids])
+    say("bucket bits", sizes)
+    assert 3 in sizes, "the workload should have 8-slot buckets"
+    for gran in (2, 3):
+        for i in range(len(cs)):
+            assert_same_state(runs[gran][0][i], runs[1][0][i], atol=0.0, label="gran %d circuit %d" % (gran, i))
+            assert runs[gran][1][i] == runs[1][1][i]
+
+
 def test_batch_rejects_cross_bucket_circuit():
     """Branching on more qubits than a bucket holds is refused with ENOTLOCAL."""
     nq = 12

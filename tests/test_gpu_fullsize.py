@@ -128,8 +128,9 @@ def batch_workload(oracle):
 @pytest.mark.timeout(900)
 def test_batch_256_circuits_full_size(batch_workload):
     cs, want = batch_workload
-    say("GPU FP64 batch of 256 circuits (captured run)")
+    say("GPU FP64 batch of 256 circuits (captured run, two tiles per work item as the autotuner picks)")
     with E.Batch(prec="f64") as b:
+        b.set_tuning(tiles_per_item=2)
         ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
         b.run()
         b.sync()
@@ -143,9 +144,10 @@ def test_batch_256_circuits_full_size(batch_workload):
 @pytest.mark.timeout(900)
 def test_batch_256_circuits_fp32_full_size(batch_workload):
     cs, want = batch_workload
-    say("GPU FP32 batch of 256 circuits (renorm every 100)")
+    say("GPU FP32 batch of 256 circuits (renorm every 100, two tiles per work item)")
     worst_f, worst_d = 0.0, 0.0
     with E.Batch(prec="f32", renorm_every=100) as b:
+        b.set_tuning(tiles_per_item=2)
         ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
         b.run()
         b.sync()

@@ -1447,22 +1447,27 @@ sqe_status sqe_batch_export_start(sqe_batch* b, int64_t id, uint64_t* keys, doub
         }
         if (b->rec_free.empty())
             return bfail(SQE_EINVAL, std::to_string(REC_SLOTS) + " exports in flight: poll some before starting more");
-        // device staging: header (entry count, counter snapshot), then keys,
-        // real and imaginary parts of up to `slots` entries
+        // device staging: keys, real and imaginary parts of up to `slots`
+        // entries (after one aligned spare unit)
         const uint64_t arr = ring_round(slots * 8), bytes = RING_ALIGN + 3 * arr;
         uint64_t off = 0;
         BCK(ring_take(b, b->xr, bytes, XR_RING_MIN, &off));
         char* sp = b->xr.p + off;
-        unsigned long long* hdr = (unsigned long long*)sp;
         uint64_t* sk = (uint64_t*)(sp + RING_ALIGN);
         double* sr = (double*)(sp + RING_ALIGN + arr);
         double* si = (double*)(sp + RING_ALIGN + 2 * arr);
+        // the export kernel writes the header (entry count, counter
+        // snapshot) straight into the ticket's record in page-locked host
+        // memory; the caller reads it after the ticket's event, which follows
+        // the kernel
+        const uint32_t rec = b->rec_free.back();
+        unsigned long long* rec_dev = b->drec + (uint64_t)rec * REC_WORDS;
         ExportArgs xa{};
         xa.base = base_ptr(b, c); xa.amp = amp_ptr(b, c); xa.nb = b->meta + id; xa.qm = c.qm; xa.thr = b->thr;
-        xa.filter = 1; xa.keys = sk; xa.re = sr; xa.im = si; xa.cap = slots; xa.total = hdr;
+        xa.filter = 1; xa.keys = sk; xa.re = sr; xa.im = si; xa.cap = slots; xa.total = rec_dev;
         xa.snap_src[0] = b->upd + id; xa.snap_src[1] = b->nnz + id; xa.snap_src[2] = b->meta + id;
         xa.snap_src[3] = &b->cst[id].err;
-        xa.snap = hdr + 1;
+        xa.snap = rec_dev + 1;
         const uint32_t g = std::max<uint32_t>(1, (uint32_t)((nbk + BLOCK - 1) / BLOCK));
         BCK(next_scan(b, g, &xa.scan));
         BHIP(launch_export(c.qm.mb, b->f32, b->st, g, xa));
@@ -1470,10 +1475,9 @@ sqe_status sqe_batch_export_start(sqe_batch* b, int64_t id, uint64_t* keys, doub
         // stream: the entries by DMA (one pitched copy of the three arrays when
         // the caller's arrays are equally spaced, as sqe_host_alloc buffers
         // split in three are; all `slots` staging entries, of which the
-        // header's count are valid), then the header into the ticket's record
+        // header's count are valid)
         BHIP(hipEventRecord(b->xev, b->st));
         BHIP(hipStreamWaitEvent(b->xst, b->xev, 0));
-        const uint32_t rec = b->rec_free.back();
         const uint64_t wbytes = slots * 8;
         const char* k0 = (const char*)keys;
         const ptrdiff_t pitch = (const char*)re - k0;
@@ -1484,8 +1488,6 @@ sqe_status sqe_batch_export_start(sqe_batch* b, int64_t id, uint64_t* keys, doub
             BHIP(hipMemcpyAsync(re, sr, wbytes, hipMemcpyDeviceToHost, b->xst));
             BHIP(hipMemcpyAsync(im, si, wbytes, hipMemcpyDeviceToHost, b->xst));
         }
-        BHIP(hipMemcpyAsync(b->hrec + (uint64_t)rec * REC_WORDS, hdr, (SNAP_WORDS + 1) * 8, hipMemcpyDeviceToHost,
-                            b->xst));
         BCK(ring_mark(b->xr, off, bytes, b->xst));
         hipEvent_t ev = nullptr;
         if (!b->evpool.empty()) { ev = b->evpool.back(); b->evpool.pop_back(); }

@@ -229,9 +229,10 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
             static_for<EPT>([&](auto ec) __attribute__((always_inline)) { nzc[decltype(ec)::value] = 1u << MB; });
         }
     };
-    // FP32 branching gate without controls: the pending factor goes into
-    // the registers first, then a dry run of the gate keeps per lane only the
-    // smallest max(|re|,|im|) of its outputs.  When every output of the
+    // FP32 branching gate without controls (up to 16 slots; 32-slot buckets,
+    // compiled for one wavefront per SIMD, run slower with the extra pass):
+    // the pending factor goes into the registers first, then a dry run of
+    // the gate keeps per lane only the smallest max(|re|,|im|) of its outputs.  When every output of the
     // wavefront is at or above the band's upper edge, the per-pair band test
     // could not drop or re-decide any of them: the gate runs again, storing,
     // and every slot counts as nonzero.  Otherwise the caller runs the
@@ -292,7 +293,7 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
             unprune_then_prune([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) { hpair(f, x, y, n0, n1); });
             return true;
         }
-        if constexpr (F32 && !MASK) {
+        if constexpr (F32 && !MASK && MB <= 4) {
             if (fp32_clear_pass([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) { hpair(f, x, y, n0, n1); }))
                 return true;
         }
@@ -302,7 +303,7 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
             prune_pair(x, y, n0, n1, k, [&](cd ex, cd ey, cd& e0, cd& e1) __attribute__((always_inline)) {
                 hpair(op.c[0], ex, ey, e0, e1);
             });
-        }, std::integral_constant<bool, !(F32 && !MASK)>{});
+        }, std::integral_constant<bool, !(F32 && !MASK && MB <= 4)>{});
         return !MASK && !F32;
     }
     case H_SLOT_RY: {
@@ -313,7 +314,7 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
             });
             return true;
         }
-        if constexpr (F32 && !MASK) {
+        if constexpr (F32 && !MASK && MB <= 4) {
             if (fp32_clear_pass([&](C x, C y, C& n0, C& n1) __attribute__((always_inline)) {
                     ry_pair(c, sn, x, y, n0, n1);
                 }))
@@ -325,7 +326,7 @@ __device__ __forceinline__ bool slot_handler(const SOp& op, typename CT<R>::T (&
             prune_pair(x, y, n0, n1, k, [&](cd ex, cd ey, cd& e0, cd& e1) __attribute__((always_inline)) {
                 ry_pair(op.c[0], op.c[1], ex, ey, e0, e1);
             });
-        }, std::integral_constant<bool, !(F32 && !MASK)>{});
+        }, std::integral_constant<bool, !(F32 && !MASK && MB <= 4)>{});
         return !MASK && !F32;
     }
 

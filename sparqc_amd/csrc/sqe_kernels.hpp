@@ -950,10 +950,13 @@ __device__ __forceinline__ void wait_vector_memory() {
 #ifndef SQE_OCC_F32_MB4
 #define SQE_OCC_F32_MB4 4
 #endif
+#ifndef SQE_OCC_F32_MB5
+#define SQE_OCC_F32_MB5 2
+#endif
 template <int MB, class R, bool SHORT = false, bool BLOCKS = false>
 struct SweepOcc {
     static constexpr int value =
-        MB >= 5 ? 1 : (MB == 4 ? (sizeof(R) == 8 || SHORT || BLOCKS ? SQE_OCC_F64_MB4 : SQE_OCC_F32_MB4) : 3);
+        MB >= 5 ? (BLOCKS ? 1 : SQE_OCC_F32_MB5) : (MB == 4 ? (sizeof(R) == 8 || SHORT || BLOCKS ? SQE_OCC_F64_MB4 : SQE_OCC_F32_MB4) : 3);
 };
 constexpr uint32_t SHORT_PROGRAM_OPS = 8;     // programs up to this length take the SHORT sweep
 
@@ -1049,7 +1052,7 @@ __global__ __launch_bounds__(BLOCK) void k_count(SweepArgs A) {
 // and are folded per circuit by k_batch_fold in a fixed order.
 // ---------------------------------------------------------------------------
 template <int MB, class R>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SweepOcc<MB, R>::value))) void k_sweep_batch(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MB >= 5 ? 1 : SweepOcc<MB, R>::value))) void k_sweep_batch(
     BatchArgs A) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

@@ -113,6 +113,28 @@ def test_blocks_32x32_match_fp32_oracle_and_fp64(oracle, seed):
     assert abs(n2 - 1.0) < 1e-5
 
 
+def test_fp32_32_slot_buckets_gate_by_gate(oracle):
+    """Five bucket qubits without fusion: the 32-slot FP32 sweep applies the
+    window gates one by one (per-pair band test, no fused blocks) and agrees
+    with the FP32 oracle."""
+    nq = 18
+    bq = (1, 4, 7, 10, 13)
+    keys, re, im = random_state(520, nq, 400)
+    gl = dense_block_circuit(3, nq, layers=24, bq=bq)
+    with E.Engine(nq, prec="f32", renorm_every=100, bucket_bits=5) as e:
+        e.load(keys, re, im)
+        e.apply(gl)
+        got = e.export()
+        st = e.stats()
+    assert st["bucket_bits"] == 5 and st["blocks"] == 0
+    o32 = oracle.Sparse(nq, f32=True, thr=1e-7, renorm_every=100)
+    o32.load(keys, re, im)
+    o32.apply(gl)
+    d32 = compare(got, o32.export(), 1e-5, max(2, got[0].size // 1000))
+    say("32-slot FP32 gate by gate: %d entries, max diff vs fp32 oracle %.2e" % (got[0].size, d32))
+    assert abs(float(np.sum(got[1] ** 2 + got[2] ** 2)) - 1.0) < 1e-5
+
+
 def test_blocks_on_large_mix_close_to_fp64(oracle):
     """The 48-qubit mix (2^12 keys): windows are short, results as the FP32 form."""
     w = gen.circuit_large(5, nq=48, n_init=1 << 12, n_gates=600)

@@ -943,7 +943,9 @@ __device__ __forceinline__ void wait_vector_memory() {
 // once per launch, extra traffic comparable to a tenth of the state's bytes.
 // BLOCKS: the FP32 16-slot sweep with fused dense blocks is compiled for 3
 // waves too (the block's 32 inputs, 32 outputs and 32 accumulators spill at
-// 4 waves; measured 1.64x faster on the layered circuit at 3).
+// 4 waves; measured 1.64x faster on the layered circuit at 3).  32 slots
+// (FP32 only): 2 waves (236 VGPRs, no spills; 4 % faster than 1), and 1 with
+// fused 32x32 blocks (337 VGPRs).
 #ifndef SQE_OCC_F64_MB4
 #define SQE_OCC_F64_MB4 3
 #endif

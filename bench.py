@@ -395,12 +395,29 @@ def run_batch(seed, n_circuits, prec, steps, warmup, eager=False):
         upd = sum(b.info(i)["updates"] for i in ids)
         nnz = sum(b.info(i)["nnz"] for i in ids)
         used = b.usage()["used"]
+        # both launch forms at the tuned workgroup size and tiles per item,
+        # outside the timed region: one sweep launch per step for the circuits
+        # of 0-3 bucket qubits, or one per bucket size on graph branches
+        forms = {}
+        for one in (0, 1):
+            b.set_tuning(one_launch=one)
+            b.snapshot_restore()
+            b.run()
+            b.sync()
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                b.snapshot_restore()
+                b.run()
+                b.sync()
+            forms["one_launch" if one else "launch_per_bucket_size"] = (time.perf_counter() - t1) / steps * 1e3
+        b.set_tuning(one_launch=tune["one_launch"])
     return {"updates_per_s": upd / dt, "ms_per_run": dt * 1e3, "circuits": n_circuits, "prec": prec,
             "updates_per_run": upd, "nnz_final": nnz, "initial_keys": int(sum(c["keys"].size for c in cs)),
             "gates": int(sum(len(c["gates"]) for c in cs)), "pool_bytes": used, "admission_s": admit_s,
             "autotune": tune, "circuits_per_s": n_circuits / dt,
             "launch": "eager steps" if eager else "captured hipGraph replay", "runs_timed": steps,
             "bucket_sizes": sizes, "sweep_launches_per_run": launches_per_run,
+            "ms_per_run_by_launch_form": forms,
             "renormalisation": ("one pass: checkpoint fold, then a scale launch per bucket size"
                                 if prec == "f32" else None)}
 

@@ -262,6 +262,10 @@ typedef struct sqe_batch_tuning {
     double load_ms;             /* sqe_batch_autotune: index part of the load path
                                    (clear, insert, scan numbering of every admitted
                                    circuit's buckets) at the chosen load factor        */
+    uint32_t one_launch;        /* 1 (default): the circuits of 0-3 bucket qubits
+                                   share one sweep launch per step; 0: one launch
+                                   per bucket size, on concurrent graph branches       */
+    uint32_t reserved;          /* 0 */
 } sqe_batch_tuning;
 
 /* cfg->nq is ignored (each circuit has its own); prec, prune_thr, max_load,
@@ -345,7 +349,7 @@ sqe_status sqe_batch_usage(sqe_batch* b, uint64_t* used, uint64_t* budget, uint6
 sqe_status sqe_batch_get_tuning(sqe_batch* b, sqe_batch_tuning* t);
 sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t);
 
-/* Time each (workgroup size, tiles per item) candidate on the
+/* Time each (workgroup size, tiles per item, one_launch) candidate on the
  * admitted circuits (their state is restored after every trial) and keep the
  * fastest; then time the index part of the load path (table clear, insert of
  * every admitted circuit's bucket bases, scan numbering into scratch) at each

@@ -303,6 +303,7 @@ struct sqe_batch {
     hipGraphExec_t gexec = nullptr;
     uint64_t g_layout = ~0ull;
     uint32_t g_block = 0, g_gran = 0;
+    bool g_one = false;
     std::vector<uint64_t> g_pos0, g_pos1;  // positions before / after the run
     uint64_t g_steps = 0, g_launches = 0;
     BTask* gtask = nullptr;
@@ -314,6 +315,7 @@ struct sqe_batch {
     std::vector<hipEvent_t> gev;             // fork / join events of the capture
     // tuning and counters
     uint32_t block = BLOCK, gran = 1;
+    bool one_launch = true;                  // circuits of 0..SQE_MIXED_MAX_MB bucket qubits share a launch
     double sweep_ms = 0.0;
     double load_ms = 0.0;                   // index part of the load path, last autotune
     std::unique_ptr<CopyPool> copy_pool;     // helper threads for large admission copies (made on first use)
@@ -672,12 +674,24 @@ struct Group {
 
 constexpr uint32_t NORM_TILES = 4;         // tiles per norm work item
 
+// Whether the run's sweeps go out as one launch for every circuit of
+// 0..SQE_MIXED_MAX_MB bucket qubits (k_sweep_batch_mixed): with one_launch
+// set, when circuits of two or more such bucket sizes have ops left.
+bool use_mixed(const sqe_batch* b) {
+    if (!b->one_launch) return false;
+    uint32_t seen = 0;
+    for (const Circ& c : b->c)
+        if (c.live && c.pos < c.nops && c.qm.mb <= SQE_MIXED_MAX_MB) seen |= 1u << c.qm.mb;
+    return __builtin_popcount(seen) >= 2;
+}
+
 // Plan one step from positions pos[] (advanced in place): every live circuit
 // with ops left runs to its next RENORM op (at most max_ops ops when
-// max_ops > 0), one sweep group per bucket size with the longest programs
-// first; then one norm group per bucket size for the circuits that stopped
-// before a RENORM op.  Returns false when no circuit has ops left.
-bool plan_step(sqe_batch* b, uint64_t max_ops, std::vector<uint64_t>& pos, std::vector<Group>& out) {
+// max_ops > 0), one sweep group per bucket size (with `mixed`, one group for
+// bucket sizes 0..SQE_MIXED_MAX_MB) with the longest programs first; then one
+// norm group per bucket size for the circuits that stopped before a RENORM
+// op.  Returns false when no circuit has ops left.
+bool plan_step(sqe_batch* b, uint64_t max_ops, bool mixed, std::vector<uint64_t>& pos, std::vector<Group>& out) {
     struct Seg { uint32_t id; uint64_t from, to; };
     std::vector<Seg> segs;
     for (uint32_t id = 0; id < b->c.size(); ++id) {
@@ -692,11 +706,13 @@ bool plan_step(sqe_batch* b, uint64_t max_ops, std::vector<uint64_t>& pos, std::
     std::sort(segs.begin(), segs.end(), [](const Seg& x, const Seg& y) { return x.to - x.from > y.to - y.from; });
     std::vector<uint32_t> need_norm;
     for (int mb = 0; mb <= 5; ++mb) {
+        if (mixed && mb < SQE_MIXED_MAX_MB) continue;
+        const bool mix = mixed && mb == SQE_MIXED_MAX_MB;
         Group g;
-        g.mb = mb;
+        g.mb = mix ? MB_MIXED : mb;
         for (const Seg& s : segs) {
             const Circ& c = b->c[s.id];
-            if (c.qm.mb != mb) continue;
+            if (mix ? c.qm.mb > SQE_MIXED_MAX_MB : c.qm.mb != mb) continue;
             BTask t{};
             t.base = base_ptr(b, c);
             t.amp = amp_ptr(b, c);
@@ -818,7 +834,7 @@ sqe_status step_all(sqe_batch* b, uint64_t max_gates) {
     std::vector<uint64_t> pos(b->c.size());
     for (size_t i = 0; i < b->c.size(); ++i) pos[i] = b->c[i].pos;
     std::vector<Group> gs;
-    if (!plan_step(b, max_gates, pos, gs)) return SQE_OK;
+    if (!plan_step(b, max_gates, use_mixed(b), pos, gs)) return SQE_OK;
     std::vector<BTask> T;
     std::vector<BItem> I;
     uint64_t ms = 0, mn = 0;
@@ -865,13 +881,15 @@ sqe_status run_graph(sqe_batch* b) {
     std::vector<uint64_t> pos0(b->c.size());
     for (size_t i = 0; i < b->c.size(); ++i) pos0[i] = b->c[i].pos;
     const bool hit = b->gexec && b->g_layout == b->layout && b->g_block == b->block && b->g_gran == b->gran &&
+                     b->g_one == b->one_launch &&
                      b->g_pos0 == pos0;
     if (!hit) {
         drop_graph(b);
         std::vector<Group> gs;
         std::vector<uint64_t> pos = pos0;
         uint64_t steps = 0, launches = 0;
-        while (plan_step(b, 0, pos, gs)) ++steps;
+        const bool mixed = use_mixed(b);
+        while (plan_step(b, 0, mixed, pos, gs)) ++steps;
         if (gs.empty()) return SQE_OK;
         for (const Group& g : gs) launches += g.norm ? 0 : 1;
         std::vector<BTask> T;
@@ -892,9 +910,10 @@ sqe_status run_graph(sqe_batch* b) {
         // Circuits of different bucket sizes never share work, so each bucket
         // size is its own chain of launches (steps in order) on its own
         // branch of the graph: a small chain overlaps the large one instead
-        // of waiting for its tail.
+        // of waiting for its tail.  A mixed sweep and the norms and folds of
+        // its circuits form one chain.
         std::vector<std::vector<const Group*>> chains(6);
-        for (const Group& g : gs) chains[g.mb].push_back(&g);
+        for (const Group& g : gs) chains[g.mb == MB_MIXED || (mixed && g.mb <= SQE_MIXED_MAX_MB) ? 0 : g.mb].push_back(&g);
         int nch = 0;
         for (const auto& c : chains) nch += c.empty() ? 0 : 1;
         while ((int)b->side.size() < nch - 1) {
@@ -941,6 +960,7 @@ sqe_status run_graph(sqe_batch* b) {
         b->g_layout = b->layout;
         b->g_block = b->block;
         b->g_gran = b->gran;
+        b->g_one = b->one_launch;
         b->g_pos0 = pos0;
         b->g_pos1 = pos;
         b->g_steps = steps;
@@ -1659,6 +1679,8 @@ sqe_status sqe_batch_get_tuning(sqe_batch* b, sqe_batch_tuning* t) {
         t->sweep_ms = b->sweep_ms;
         t->launches = b->launches;
         t->load_ms = b->load_ms;
+        t->one_launch = b->one_launch ? 1 : 0;
+        t->reserved = 0;
         return SQE_OK;
     });
 }
@@ -1668,14 +1690,16 @@ sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t) {
         if (!b || !t) return bfail(SQE_EINVAL, "null argument");
         if (t->block != 64 && t->block != 128 && t->block != 256) return bfail(SQE_EINVAL, "block must be 64/128/256");
         if (t->tiles_per_item < 1 || t->tiles_per_item > 64) return bfail(SQE_EINVAL, "tiles_per_item in 1..64");
+        if (t->one_launch > 1) return bfail(SQE_EINVAL, "one_launch must be 0 or 1");
         b->block = t->block;
         b->gran = t->tiles_per_item;
+        b->one_launch = t->one_launch != 0;
         if (t->load_factor > 0 && t->load_factor < 0.95) b->cfg.max_load = t->load_factor;
         return SQE_OK;
     });
 }
 
-// Time every (workgroup size, tiles per item) candidate on the
+// Time every (workgroup size, tiles per item, one_launch) candidate on the
 // batch as it stands -- state and positions are saved and restored around each
 // trial -- and keep the fastest.  With captured runs, each candidate is timed
 // on a replay of its graph.
@@ -1684,17 +1708,19 @@ sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
         if (!b) return bfail(SQE_EINVAL, "null batch");
         BHIP(hipSetDevice(b->cfg.device));
         BCK(snap_save(b));
-        struct Cand { uint32_t block, gran; };
+        struct Cand { uint32_t block, gran; bool one; };
         std::vector<Cand> cands;
-        for (uint32_t bl : {64u, 128u, 256u})
-            for (uint32_t g : {1u, 2u, 4u, 8u}) cands.push_back(Cand{bl, g});
+        for (bool one : {false, true})
+            for (uint32_t bl : {64u, 128u, 256u})
+                for (uint32_t g : {1u, 2u, 4u, 8u}) cands.push_back(Cand{bl, g, one});
         double best = 1e300;
-        Cand bc{b->block, b->gran};
+        Cand bc{b->block, b->gran, b->one_launch};
         const uint32_t timing = b->cfg.timing;
         b->cfg.timing = 0;
         for (const Cand& cd : cands) {
             b->block = cd.block;
             b->gran = cd.gran;
+            b->one_launch = cd.one;
             if (!b->cfg.batch_eager) {
                 // capture and instantiate this candidate's graph outside the timing
                 BCK(sqe_batch_run(b));
@@ -1714,6 +1740,7 @@ sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
         b->cfg.timing = timing;
         b->block = bc.block;
         b->gran = bc.gran;
+        b->one_launch = bc.one;
         BCK(tune_load_factor(b));
         if (best_out) {
             BCK(sqe_batch_get_tuning(b, best_out));

@@ -25,6 +25,12 @@ constexpr uint64_t EMPTY_KEY = ~0ull;
 constexpr uint32_t NONE = 0xffffffffu;
 constexpr int TILE = 64;          // buckets per tile (one wavefront)
 constexpr int BLOCK = 256;        // threads per workgroup for bucket kernels
+// Batched sweeps of circuits of 0..SQE_MIXED_MAX_MB bucket qubits go out as
+// one launch (k_sweep_batch_mixed, launch id MB_MIXED).
+#ifndef SQE_MIXED_MAX_MB
+#define SQE_MIXED_MAX_MB 3
+#endif
+constexpr int MB_MIXED = 8;
 
 // gate kinds (sqe_gate.kind); K_RENORM is an engine-internal op
 enum : uint32_t { KX = 0, KY, KZ, KH, KPHASE, KRZ, KRX, KRY, KU, KDIAG, KPERM, K_RENORM = 16 };

@@ -556,6 +556,19 @@ struct SweepCount {
 // the bucket's pending FP64 factor).
 __device__ __forceinline__ bool base_class(uint32_t h, uint32_t cmS) { return h <= H_BASE_PERM && cmS == 0; }
 
+// A wavefront's program staging area and the FP64 identity factor (1, 0) of
+// failed controls: one copy per workgroup, which every sweep a kernel inlines
+// shares (a kernel that dispatches on the bucket size, or that runs one- and
+// two-tile passes, still stages through 4 KB per wavefront).
+struct SweepLds {
+    uint4 prog[BLOCK / 64][64 * 4];
+    double one[BLOCK / 64][2];
+};
+__device__ __forceinline__ SweepLds& sweep_lds() {
+    __shared__ SweepLds s;
+    return s;
+}
+
 // Apply one program to EPT tiles (64 buckets each) with one wavefront: every op
 // is decoded once and applied to all of the thread's buckets.  tile[e] < 0
 // marks a missing tile (the wavefront computes on zeros and stores nothing).
@@ -619,10 +632,9 @@ __device__ __forceinline__ void sweep_group(uint64_t* __restrict__ basep, void* 
     // v_readfirstlane).  The nonzero count that every gate adds to the update
     // counter changes only at branching gates, so gates are counted in a
     // scalar and multiplied in when the count changes.
-    __shared__ uint4 s_prog[BLOCK / 64][64 * 4];
-    __shared__ double s_one[BLOCK / 64][2];   // the identity factor (1, 0) of failed FP64 controls
-    uint4* __restrict__ sp = s_prog[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
-    double* __restrict__ one = s_one[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
+    SweepLds& lds = sweep_lds();
+    uint4* __restrict__ sp = lds.prog[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
+    double* __restrict__ one = lds.one[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
     if constexpr (sizeof(R) == 8) {
         one[0] = 1.0;                         // every lane stores the same pair, so each
         one[1] = 0.0;                         // lane's later read sees its own store
@@ -962,6 +974,15 @@ struct SweepOcc {
 };
 constexpr uint32_t SHORT_PROGRAM_OPS = 8;     // programs up to this length take the SHORT sweep
 
+// The batched sweep: as the single-circuit sweep, except FP32 16-slot buckets
+// at 3 waves per SIMD (at 4 the batch kernel's per-item state pushed about
+// 190 bytes per lane of VGPRs to scratch; at 3, 28 bytes) and 32-slot buckets
+// at 1.
+template <int MB, class R>
+struct BatchOcc {
+    static constexpr int value = MB >= 5 ? 1 : (MB == 4 && sizeof(R) == 4 ? 3 : SweepOcc<MB, R>::value);
+};
+
 // One program over the engine's store: wavefront g of the grid takes tiles
 // EPT*g .. EPT*g + EPT-1.
 template <int MB, class R, int EPT, bool BLOCKS = false, bool SHORT = false>
@@ -1053,16 +1074,11 @@ __global__ __launch_bounds__(BLOCK) void k_count(SweepArgs A) {
 // sizes.  Per-item counts go to counts[2*item .. 2*item+1] (updates, nonzeros)
 // and are folded per circuit by k_batch_fold in a fixed order.
 // ---------------------------------------------------------------------------
+// One work item: its tiles of its circuit's program (upd / nnz accumulate the
+// wavefront's counts).
 template <int MB, class R>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MB >= 5 ? 1 : SweepOcc<MB, R>::value))) void k_sweep_batch(
-    BatchArgs A) {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t it = blockIdx.x * (blockDim.x >> 6) + wave;
-    if (it >= A.nitems) return;
-    const BItem item = A.items[it];
-    const BTask t = A.tasks[item.task];
-    unsigned long long upd = 0, nnz = 0;
+__device__ __forceinline__ void batch_item(const BatchArgs& A, const BItem& item, const BTask& t,
+                                           unsigned long long& upd, unsigned long long& nnz) {
     auto run = [&](auto ept_c) __attribute__((always_inline)) {
         constexpr int BEPT = decltype(ept_c)::value;
         for (uint32_t g = 0; g < A.gran; g += BEPT) {
@@ -1090,14 +1106,55 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MB >= 5 ?
     } else {
         run(std::integral_constant<int, 1>{});
     }
+}
+
+__device__ __forceinline__ void batch_counts(const BatchArgs& A, uint32_t it, unsigned long long upd,
+                                             unsigned long long nnz) {
     for (int d = 32; d >= 1; d >>= 1) {
         upd += __shfl_xor(upd, d);
         nnz += __shfl_xor(nnz, d);
     }
-    if (lane == 0) {
+    if ((threadIdx.x & 63) == 0) {
         A.counts[2 * (uint64_t)it] = upd;
         A.counts[2 * (uint64_t)it + 1] = nnz;
     }
+}
+
+template <int MB, class R>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BatchOcc<MB, R>::value))) void k_sweep_batch(
+    BatchArgs A) {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t it = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (it >= A.nitems) return;
+    const BItem item = A.items[it];
+    const BTask t = A.tasks[item.task];
+    unsigned long long upd = 0, nnz = 0;
+    batch_item<MB, R>(A, item, t, upd, nnz);
+    batch_counts(A, it, upd, nnz);
+}
+
+// The whole run's circuits of up to SQE_MIXED_MAX_MB bucket qubits in one
+// launch: each wavefront takes the item the grid order gives it (longest
+// programs first, whatever their bucket size) and applies the sweep of its
+// circuit's bucket size (a wavefront-uniform branch).  Compiled for 3 waves
+// per SIMD, the budget of the two-tile 8-slot sweep.
+template <class R>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_sweep_batch_mixed(BatchArgs A) {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t it = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (it >= A.nitems) return;
+    const BItem item = A.items[it];
+    const BTask t = A.tasks[item.task];
+    unsigned long long upd = 0, nnz = 0;
+    switch (__builtin_amdgcn_readfirstlane(t.qm.mb)) {
+    case 0: if constexpr (0 <= SQE_MIXED_MAX_MB) batch_item<0, R>(A, item, t, upd, nnz); break;
+    case 1: if constexpr (1 <= SQE_MIXED_MAX_MB) batch_item<1, R>(A, item, t, upd, nnz); break;
+    case 2: if constexpr (2 <= SQE_MIXED_MAX_MB) batch_item<2, R>(A, item, t, upd, nnz); break;
+    case 3: if constexpr (3 <= SQE_MIXED_MAX_MB) batch_item<3, R>(A, item, t, upd, nnz); break;
+    case 4: if constexpr (4 <= SQE_MIXED_MAX_MB) batch_item<4, R>(A, item, t, upd, nnz); break;
+    default: break;
+    }
+    batch_counts(A, it, upd, nnz);
 }
 
 // FP64 sum of |a|^2 of each work item's tiles -> partials[item] (FP32 renorm);
@@ -1464,6 +1521,10 @@ struct Launch {
         return hipGetLastError();
     }
     static hipError_t sweep_batch(int mb, hipStream_t st, uint32_t grid, uint32_t block, const BatchArgs& a) {
+        if (mb == MB_MIXED) {
+            hipLaunchKernelGGL(k_sweep_batch_mixed<R>, dim3(grid), dim3(block), 0, st, a);
+            return hipGetLastError();
+        }
         SQE_MB_SWITCH(mb, hipLaunchKernelGGL((k_sweep_batch<MB_, R>), dim3(grid), dim3(block), 0, st, a));
         return hipGetLastError();
     }

@@ -69,13 +69,13 @@ class BatchCircuit(ctypes.Structure):
 class BatchTuning(ctypes.Structure):
     _fields_ = [("block", ctypes.c_uint32), ("tiles_per_item", ctypes.c_uint32),
                 ("load_factor", ctypes.c_double), ("sweep_ms", ctypes.c_double), ("launches", ctypes.c_uint64),
-                ("load_ms", ctypes.c_double)]
+                ("load_ms", ctypes.c_double), ("one_launch", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 assert ctypes.sizeof(Config) == 80
 assert ctypes.sizeof(Stats) == 152
 assert ctypes.sizeof(BatchCircuit) == 56
-assert ctypes.sizeof(BatchTuning) == 40
+assert ctypes.sizeof(BatchTuning) == 48
 
 _lib = None
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -560,9 +560,11 @@ class Batch:
         _bcheck(lib().sqe_batch_get_tuning(self.h, ctypes.byref(t)))
         return {f: getattr(t, f) for f, _ in BatchTuning._fields_}
 
-    def set_tuning(self, block=None, tiles_per_item=None, load_factor=None):
+    def set_tuning(self, block=None, tiles_per_item=None, load_factor=None, one_launch=None):
         t = BatchTuning()
         _bcheck(lib().sqe_batch_get_tuning(self.h, ctypes.byref(t)))
+        if one_launch is not None:
+            t.one_launch = int(one_launch)
         if block is not None:
             t.block = block
         if tiles_per_item is not None:

@@ -249,6 +249,55 @@ def test_two_tiles_per_pass_equal_one(prec):
             assert runs[gran][1][i] == runs[1][1][i]
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_one_launch_equals_split(oracle, prec):
+    """Circuits of 0-3 bucket qubits in one sweep launch per step (each
+    wavefront branches on its circuit's bucket size) give the states and
+    update counts of one launch per bucket size, to the bit, for both the
+    captured run and direct steps; the FP64 run is also the oracle's."""
+    cs = small_batch(14, 12)
+    runs = {}
+    for one in (0, 1):
+        with E.Batch(prec=prec, renorm_every=100 if prec == "f32" else 0) as b:
+            b.set_tuning(tiles_per_item=2, one_launch=one)
+            assert b.tuning()["one_launch"] == one
+            ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+            sizes = sorted({b.info(i)["bucket_bits"] for i in ids})
+            b.run()
+            b.sync()
+            launches = b.tuning()["launches"]
+            runs[one] = ([b.export(i) for i in ids], [b.info(i)["updates"] for i in ids], launches)
+    say("bucket bits", sizes, "launches split / one", runs[0][2], runs[1][2])
+    assert len([s for s in sizes if s <= 3]) >= 2, "the workload should mix bucket sizes"
+    assert runs[1][2] < runs[0][2]
+    for i in range(len(cs)):
+        assert_same_state(runs[1][0][i], runs[0][0][i], atol=0.0, label="circuit %d" % i)
+        assert runs[1][1][i] == runs[0][1][i]
+    if prec == "f64":
+        for i, c in enumerate(cs):
+            want, upd = oracle_of(oracle, c)
+            assert_same_state(runs[1][0][i], want, atol=1e-12, label="oracle circuit %d" % i)
+            assert runs[1][1][i] == upd
+    # direct steps of 64 gates (the serving path)
+    stepped = {}
+    for one in (0, 1):
+        with E.Batch(prec=prec, renorm_every=100 if prec == "f32" else 0) as b:
+            b.set_tuning(one_launch=one)
+            ids = [b.add(c["nq"], c["keys"], c["re"], c["im"], c["gates"]) for c in cs]
+            while b.usage()["active"]:
+                b.step(64)
+            stepped[one] = [b.export(i) for i in ids]
+    for i in range(len(cs)):
+        assert_same_state(stepped[1][i], stepped[0][i], atol=0.0, label="stepped circuit %d" % i)
+
+
+def test_batch_tuning_rejects_bad_one_launch():
+    with E.Batch() as b:
+        with pytest.raises(E.SqeError) as ex:
+            b.set_tuning(one_launch=2)
+        assert ex.value.status == E.SQE_EINVAL
+
+
 def test_batch_rejects_cross_bucket_circuit():
     """Branching on more qubits than a bucket holds is refused with ENOTLOCAL."""
     nq = 12

@@ -350,7 +350,8 @@ sqe_status sqe_batch_get_tuning(sqe_batch* b, sqe_batch_tuning* t);
 sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t);
 
 /* Time each (workgroup size, tiles per item, one_launch) candidate on the
- * admitted circuits (their state is restored after every trial) and keep the
+ * admitted circuits (fastest of three runs; their state is restored after
+ * every run) and keep the
  * fastest; then time the index part of the load path (table clear, insert of
  * every admitted circuit's bucket bases, scan numbering into scratch) at each
  * candidate load factor (0.25 ... 0.75) and keep the fastest for later

@@ -1699,6 +1699,8 @@ sqe_status sqe_batch_set_tuning(sqe_batch* b, const sqe_batch_tuning* t) {
     });
 }
 
+constexpr int TUNE_REPS = 3;               // timed runs per autotuner candidate
+
 // Time every (workgroup size, tiles per item, one_launch) candidate on the
 // batch as it stands -- state and positions are saved and restored around each
 // trial -- and keep the fastest.  With captured runs, each candidate is timed
@@ -1727,15 +1729,19 @@ sqe_status sqe_batch_autotune(sqe_batch* b, sqe_batch_tuning* best_out) {
                 BCK(snap_restore(b));
                 BHIP(hipStreamSynchronize(b->st));
             }
-            BHIP(hipEventRecord(b->ev0, b->st));
-            BCK(sqe_batch_run(b));
-            BHIP(hipEventRecord(b->ev1, b->st));
-            BHIP(hipEventSynchronize(b->ev1));
-            float ms = 0.f;
-            BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-            if (ms < best) { best = ms; bc = cd; }
-            BCK(snap_restore(b));
-            BHIP(hipStreamSynchronize(b->st));
+            // the fastest of TUNE_REPS replays: one replay's time varies by a
+            // few percent, about as much as neighbouring candidates differ
+            for (int rep = 0; rep < TUNE_REPS; ++rep) {
+                BHIP(hipEventRecord(b->ev0, b->st));
+                BCK(sqe_batch_run(b));
+                BHIP(hipEventRecord(b->ev1, b->st));
+                BHIP(hipEventSynchronize(b->ev1));
+                float ms = 0.f;
+                BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                if (ms < best) { best = ms; bc = cd; }
+                BCK(snap_restore(b));
+                BHIP(hipStreamSynchronize(b->st));
+            }
         }
         b->cfg.timing = timing;
         b->block = bc.block;

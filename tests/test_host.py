@@ -93,6 +93,43 @@ def test_struct_layouts():
     assert cfg.nq == 30 and cfg.bucket_bits == -1 and cfg.max_load == 0.5
 
 
+def test_struct_mirrors_match_the_header(tmp_path):
+    """Every public struct of include/sqe.h, compiled by the host C compiler,
+    has the size and field offsets of its Python mirror (ctypes structures
+    and the gate record dtype)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    mirrors = {"sqe_config": E.Config, "sqe_stats": E.Stats, "sqe_batch_circuit": E.BatchCircuit,
+               "sqe_batch_tuning": E.BatchTuning}
+    want = {}
+    for cname, py in mirrors.items():
+        want[cname] = (ctypes.sizeof(py), {f: getattr(py, f).offset for f, _ in py._fields_})
+    want["sqe_gate"] = (G.GATE_DTYPE.itemsize, {f: G.GATE_DTYPE.fields[f][1] for f in G.GATE_DTYPE.names})
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "sqe.h"', "int main(void) {"]
+    for cname, (_, fields) in want.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f in fields:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            s, f, v = ln.split()
+            got[(s, f)] = int(v)
+    for cname, (size, fields) in want.items():
+        assert got[(cname, "size")] == size, cname
+        for f, off in fields.items():
+            assert got[(cname, f)] == off, (cname, f)
+
+
 def test_create_without_gpu_reports_no_device():
     try:
         e = E.Engine(10)
